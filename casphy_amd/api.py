@@ -1,0 +1,364 @@
+"""Host-side mirror of CasPhy's hot-path interface over the HIP library.
+
+Two layers, both backed only by libcasphy_hip.so (no CPU fallback):
+
+1. The Rcpp twins, with the reference's names and argument meaning
+   (R/RcppExports.R; src/functions.cpp):
+     greedyJoinEdgeBitReverse(mat, weight, verbose, ancestralType)
+     greedyJoinEdge2(mat, weight, tipSize, minSize, minRatio, verbose)
+     radiusWeightedParallel(i, mat, maxChar, weight, radius)
+     radiusWeightedDNAParallel(i, mat, weight, radius)
+   Matrices are numpy arrays holding the same bytes as the R matrices: an R
+   matrix with cells as columns (features x cells) is a C-order (n, W) array,
+   one with cells as rows (cells x words) is a C-order (W, n) array.
+
+2. The executables' computations (cu/*.cu main functions) on in-memory arrays:
+     join_edge_bit_reverse, join_edge_indel, radius_neighbor,
+     radius_neighbor_dna, try_radius, try_radius_dna, transfer_dis.
+
+Errors raise CasphyError (the status code and message of the C ABI).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CasphyError, check, load
+
+__all__ = [
+    "CasphyError", "score_bit_matrix", "score_indel_matrix",
+    "radius_neighbor", "radius_neighbor_dna", "try_radius", "try_radius_dna",
+    "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
+    "join_bit_batch", "transfer_dis", "join_edge_bit_reverse", "join_edge_indel",
+    "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
+    "radiusWeightedDNAParallel",
+]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _take_list(ptr, total) -> np.ndarray:
+    L = load()
+    try:
+        if total == 0:
+            return np.zeros(0, dtype=np.uint32)
+        buf = np.ctypeslib.as_array(ptr, shape=(total,))
+        return buf.copy()
+    finally:
+        L.casphy_free(C.cast(ptr, C.c_void_p))
+
+
+# ------------------------------------------------------------------ scores
+def score_bit_matrix(mat_cellmajor, weight=None) -> np.ndarray:
+    """n x n int32 shared-mutation scores of bit-packed cells (diagonal = self)."""
+    m = _u32(mat_cellmajor)
+    n, W = m.shape
+    w = _f64(weight)
+    S = np.empty((n, n), dtype=np.int32)
+    check(load().casphy_score_bit_matrix(_p(m), n, W, _p(w), _p(S)))
+    return S
+
+
+def score_indel_matrix(mat_cellmajor, weight=None) -> np.ndarray:
+    """n x n int32 indel scores; weight is (nState, ncol) state-major or None."""
+    m = _i32(mat_cellmajor)
+    n, ncol = m.shape
+    w = _f64(weight)
+    nstate = 0 if w is None else w.shape[0]
+    S = np.empty((n, n), dtype=np.int32)
+    check(load().casphy_score_indel_matrix(_p(m), n, ncol, _p(w), nstate, _p(S)))
+    return S
+
+
+# ------------------------------------------------------------------ radius
+def radius_neighbor(bmat_wordmajor, radius: int, weight=None):
+    """(nNeighbor, idx) of radiusNeighbor: .bmat payload (W, n) in, the
+    .nNeighbor counts and the concatenated ascending .idx lists out."""
+    m = _u32(bmat_wordmajor)
+    W, n = m.shape
+    w = _f64(weight)
+    counts = np.empty(n, dtype=np.uint32)
+    ptr = C.POINTER(C.c_uint32)()
+    total = C.c_uint64(0)
+    check(load().casphy_radius_bit(_p(m), W, n, _p(w), int(radius), _p(counts), C.byref(ptr), C.byref(total)))
+    return counts, _take_list(ptr, total.value)
+
+
+def radius_neighbor_dna(tbmat, radius: int):
+    """(nNeighbor, idx) of radiusNeighborDNA on fasta2bin codes (n, nwRaw)."""
+    m = _u32(tbmat)
+    n, nw = m.shape
+    counts = np.empty(n, dtype=np.uint32)
+    ptr = C.POINTER(C.c_uint32)()
+    total = C.c_uint64(0)
+    check(load().casphy_radius_dna(_p(m), n, nw, int(radius), _p(counts), C.byref(ptr), C.byref(total)))
+    return counts, _take_list(ptr, total.value)
+
+
+def try_radii(bmat_wordmajor, radii, q, weight=None) -> np.ndarray:
+    """Counts (len(radii), R) of tryRadius for every radius in one launch."""
+    m = _u32(bmat_wordmajor)
+    W, n = m.shape
+    r = np.ascontiguousarray(radii, dtype=np.int32)
+    q = _u32(q)
+    w = _f64(weight)
+    out = np.zeros((r.size, q.size), dtype=np.uint32)
+    check(load().casphy_try_radii_bit(_p(m), W, n, _p(w), _p(r), r.size, _p(q), q.size, _p(out)))
+    return out
+
+
+def try_radius(bmat_wordmajor, radius: int, q, weight=None) -> np.ndarray:
+    return try_radii(bmat_wordmajor, [radius], q, weight)[0]
+
+
+def try_radii_dna(tbmat, radii, q) -> np.ndarray:
+    m = _u32(tbmat)
+    n, nw = m.shape
+    r = np.ascontiguousarray(radii, dtype=np.int32)
+    q = _u32(q)
+    out = np.zeros((r.size, q.size), dtype=np.uint32)
+    check(load().casphy_try_radii_dna(_p(m), n, nw, _p(r), r.size, _p(q), q.size, _p(out)))
+    return out
+
+
+def try_radius_dna(tbmat, radius: int, q) -> np.ndarray:
+    return try_radii_dna(tbmat, [radius], q)[0]
+
+
+# ------------------------------------------------------------------ joins
+def join_bit(mat_cellmajor, weight=None, ancestral_type: int = 1):
+    """(edge (2n-2, 2) raw join order, node_state (2n-1, W), node_bistate)
+    indexed by node id; node_state is seq & ~bistate."""
+    m = _u32(mat_cellmajor)
+    n, W = m.shape
+    w = _f64(weight)
+    edge = np.empty((2 * n - 2, 2), dtype=np.uint32)
+    ns = np.empty((2 * n - 1, W), dtype=np.uint32)
+    nb = np.empty((2 * n - 1, W), dtype=np.uint32)
+    check(load().casphy_join_bit(_p(m), n, W, _p(w), int(ancestral_type), _p(edge), _p(ns), _p(nb)))
+    return edge, ns, nb
+
+
+def join_indel(mat_cellmajor, weight=None, tip_size=None, min_size: int = 0, min_ratio: float = 2.0):
+    """(edge (2n-2, 2), node_state (2n-1, ncol) int32 by node id, branch (2n-2,))."""
+    m = _i32(mat_cellmajor)
+    n, ncol = m.shape
+    w = _f64(weight)
+    nstate = 0 if w is None else w.shape[0]
+    ts = None if tip_size is None else _u32(tip_size)
+    edge = np.empty((2 * n - 2, 2), dtype=np.uint32)
+    ns = np.empty((2 * n - 1, ncol), dtype=np.int32)
+    br = np.empty(2 * n - 2, dtype=np.uint32)
+    check(load().casphy_join_indel(_p(m), n, ncol, _p(w), nstate, _p(ts), int(min_size), float(min_ratio),
+                                   _p(edge), _p(ns), _p(br)))
+    return edge, ns, br
+
+
+def cladewise(edge_raw, n: int):
+    """(edge_out (2n-2, 2) relabelled in DFS order, internal_order (n-1,))."""
+    e = _u32(edge_raw)
+    eo = np.empty((2 * n - 2, 2), dtype=np.uint32)
+    io = np.empty(n - 1, dtype=np.uint32)
+    check(load().casphy_cladewise(_p(e), n, _p(eo), _p(io)))
+    return eo, io
+
+
+def join_bit_batch(mats, weight=None, ancestral_type: int = 1):
+    """Independent binary joins in one launch; returns a list of raw edges."""
+    mats = [_u32(m) for m in mats]
+    W = mats[0].shape[1]
+    sizes = [m.shape[0] for m in mats]
+    offsets = np.zeros(len(mats) + 1, dtype=np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    allm = np.ascontiguousarray(np.concatenate(mats, axis=0))
+    w = _f64(weight)
+    edges = np.empty((int(sum(2 * s - 2 for s in sizes)), 2), dtype=np.uint32)
+    check(load().casphy_join_bit_batch(_p(allm), _p(offsets), len(mats), W, _p(w), int(ancestral_type),
+                                       _p(edges)))
+    out, pos = [], 0
+    for s in sizes:
+        out.append(edges[pos:pos + 2 * s - 2])
+        pos += 2 * s - 2
+    return out
+
+
+def join_edge_bit_reverse(tbmat, weight=None, ancestral_type: int = 1):
+    """The joinEdgeBitReverse executable: .tbmat payload (n, W) in,
+    (.edge2 payload (2n-2, 2), .nodeState2 payload (n-1, W)) out."""
+    m = _u32(tbmat)
+    n = m.shape[0]
+    edge, ns, _ = join_bit(m, weight, ancestral_type)
+    eo, order = cladewise(edge, n)
+    return eo, ns[order]
+
+
+def join_edge_indel(tbmat_shifted, weight, n_state: int, tip_size, min_size: int, min_ratio: float):
+    """The joinEdgeIndel executable: .tbmat payload with states +1, the .w
+    weights (nState*ncol, state-major), nState, .tipSize, minSize, minRatio ->
+    (.edge2 payload, .nodeState2 payload with states +1)."""
+    m = np.ascontiguousarray(tbmat_shifted, dtype=np.int64).astype(np.int32) - 1
+    ncol = m.shape[1]
+    w = None if weight is None else np.asarray(weight, dtype=np.float64).reshape(n_state, ncol)
+    edge, ns, _ = join_indel(m, w, tip_size, min_size, min_ratio)
+    eo, order = cladewise(edge, m.shape[0])
+    return eo, (ns[order] + 1).astype(np.uint32)
+
+
+# ------------------------------------------------------------------ transfer
+def transfer_dis(Y, X, ntip: int):
+    """(ymin, ysize, xmin, xsize) of TransferDisStream on bitset rows."""
+    Y, X = _u32(Y), _u32(X)
+    ny, ncol = Y.shape
+    nx = X.shape[0]
+    if X.shape[1] != ncol:
+        raise CasphyError(-1, "X and Y must have the same number of words per bitset")
+    ymin = np.empty(ny, np.uint32)
+    ysize = np.empty(ny, np.uint32)
+    xmin = np.empty(nx, np.uint32)
+    xsize = np.empty(nx, np.uint32)
+    check(load().casphy_transfer(_p(Y), ny, _p(X), nx, ncol, int(ntip), _p(ymin), _p(ysize), _p(xmin),
+                                 _p(xsize)))
+    return ymin, ysize, xmin, xsize
+
+
+# ------------------------------------------------------------------ Rcpp twins
+def greedyJoinEdgeBitReverse(mat, weight, verbose: bool = False, ancestralType: int = 0):
+    """src/functions.cpp:3316-3461.  mat: (n, W) bytes of the W x n R matrix.
+    Returns dict(edge (2n-2, 2), nodeState (2n-1, W), nodeBistate (2n-1, W))
+    with the columns of the R result as rows: entry c < 2n-2 is the child of
+    edge c, the last entry is the root."""
+    m = _u32(mat)
+    n, W = m.shape
+    w = None if weight is None else _f64(weight)
+    edge, ns, nb = join_bit(m, w, ancestralType)
+    order = np.concatenate([edge[:, 1], [2 * n - 2]]).astype(np.int64)
+    return {"edge": edge.astype(np.int32), "nodeState": ns[order].astype(np.int32),
+            "nodeBistate": nb[order].astype(np.int32)}
+
+
+def greedyJoinEdge2(mat, weight, tipSize, minSize: int = 0, minRatio: float = 2.0, verbose: bool = False):
+    """src/functions.cpp:1880-2017.  mat: (n, ncol) bytes of the ncol x n R
+    matrix; weight: (nState, ncol) with weight[a-1, s].  Returns the
+    (2n-1, 3+ncol) bytes of the (3+ncol) x (2n-1) R result: per column
+    parent, child, branch count, child state; the last column holds the root
+    state with zeros in rows 0..2."""
+    m = _i32(mat)
+    n, ncol = m.shape
+    edge, ns, br = join_indel(m, weight, tipSize, minSize, minRatio)
+    out = np.zeros((2 * n - 1, 3 + ncol), dtype=np.int32)
+    out[:-1, 0:2] = edge
+    out[:-1, 2] = br
+    out[:-1, 3:] = ns[edge[:, 1].astype(np.int64)]
+    out[-1, 3:] = ns[2 * n - 2]
+    return out
+
+
+def _bitmask_result(row_list, n):
+    nbin = (n + 31) // 32
+    out = np.zeros(nbin + 1, dtype=np.uint32)
+    out[0] = row_list.size
+    if row_list.size:
+        np.bitwise_or.at(out, 1 + (row_list >> 5), (np.uint32(1) << (row_list & 31).astype(np.uint32)))
+    return out.view(np.int32)
+
+
+def radiusWeightedParallel(i: int, mat, maxChar, weight, radius: float):
+    """src/functions.cpp:1825-1857.  mat: (W, n) bytes of the n x W R matrix;
+    returns IntegerVector(nbin+1) = [count, neighbour bitmask words...]."""
+    m = _u32(mat)
+    W, n = m.shape
+    r = float(radius)
+    if r != int(r):
+        raise CasphyError(-1, "radius must be an integer for the exact path")
+    counts, idx = radius_neighbor(m, int(r), weight)
+    off = np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)])
+    return _bitmask_result(idx[int(off[i]):int(off[i + 1])], n)
+
+
+def radiusWeightedDNAParallel(i: int, mat, weight, radius: float):
+    """src/functions.cpp:4021-4036 with unit weights.  mat: (n, nwRaw)."""
+    m = _u32(mat)
+    n = m.shape[0]
+    if weight is not None and not np.all(np.asarray(weight) == 1):
+        raise CasphyError(-4, "DNA weights other than 1 are outside the exact path")
+    r = float(radius)
+    counts, idx = radius_neighbor_dna(m, int(np.ceil(r)))
+    off = np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)])
+    return _bitmask_result(idx[int(off[i]):int(off[i + 1])], n)
+
+
+# ------------------------------------------------------------------ resident
+class Cells:
+    """A packed cell matrix resident in HBM (casphy_cells_* of the C ABI).
+
+    kind "bit": `mat` is the .bmat payload (W, n) (wordmajor=True) or a
+    cell-major (n, W) array; kind "dna": fasta2bin codes (n, nwRaw)."""
+
+    def __init__(self, mat, kind: str = "bit", wordmajor: bool = True, weight=None):
+        L = load()
+        m = _u32(mat)
+        if kind == "bit":
+            W, n = (m.shape if wordmajor else m.shape[::-1])
+            k = 0
+        else:
+            n, W = m.shape
+            k = 1
+            wordmajor = False
+        self.n = n
+        w = _f64(weight)
+        h = C.c_void_p()
+        check(L.casphy_cells_create(k, _p(m), n, W, int(wordmajor), _p(w), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().casphy_cells_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def radius(self, radius: int) -> int:
+        total = C.c_uint64(0)
+        check(load().casphy_cells_radius(self._h, int(radius), C.byref(total)))
+        self.total = total.value
+        return self.total
+
+    def fetch(self):
+        counts = np.empty(self.n, np.uint32)
+        idx = np.empty(max(self.total, 1), np.uint32)
+        check(load().casphy_cells_fetch(self._h, _p(counts), _p(idx)))
+        return counts, idx[: self.total]
+
+    def try_radii(self, radii, q) -> np.ndarray:
+        r = np.ascontiguousarray(radii, dtype=np.int32)
+        q = _u32(q)
+        out = np.zeros((r.size, q.size), np.uint32)
+        check(load().casphy_cells_try_radii(self._h, _p(r), r.size, _p(q), q.size, _p(out)))
+        return out
+
+    def last_ms(self):
+        a, b = C.c_float(0), C.c_float(0)
+        check(load().casphy_cells_last_ms(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+__all__.append("Cells")

@@ -1,0 +1,534 @@
+// capi.hip -- C ABI entry points: scoring, radius lists, tryRadius, transfer
+// distance, cladewise emission.  Joins live in join.hip.
+//
+// Every entry point follows the same shape: validate, upload the caller's
+// reference-layout buffers once, convert to the device layout with a kernel,
+// run the tile kernel with the matching fused epilogue, compact on device and
+// copy only the final results back.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <memory>
+#include <vector>
+
+#include "context.hpp"
+#include "kernels.hpp"
+#include "tiles.hpp"
+
+namespace casphy {
+
+namespace {
+thread_local std::string g_last_error;
+thread_local int g_device = 0;
+thread_local hipStream_t g_stream = nullptr;
+thread_local int g_stream_device = -1;
+}  // namespace
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+int thread_device() { return g_device; }
+
+hipStream_t thread_stream()
+{
+    if (!g_stream || g_stream_device != g_device) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            throw Error(CASPHY_ENODEV, "no HIP device available");
+        CASPHY_REQUIRE(g_device < count, CASPHY_ENODEV, "selected device does not exist");
+        CASPHY_HIP(hipSetDevice(g_device));
+        CASPHY_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+        g_stream_device = g_device;
+    } else {
+        CASPHY_HIP(hipSetDevice(g_device));
+    }
+    return g_stream;
+}
+
+void h2d(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (!bytes) return;
+    CASPHY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    CASPHY_HIP(hipStreamSynchronize(s));
+}
+
+void d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (!bytes) {
+        CASPHY_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    CASPHY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    CASPHY_HIP(hipStreamSynchronize(s));
+}
+
+static void check_weight(double v, u32 i)
+{
+    if (!std::isfinite(v) || v < 0.0 || v != std::floor(v))
+        throw Error(CASPHY_EWEIGHT, "weight[" + std::to_string(i) + "] = " + std::to_string(v) +
+                                        " is not a non-negative integer");
+}
+
+Planes make_planes_bits(const std::vector<double> &w, u32 ldw)
+{
+    Planes p;
+    double sum = 0.0;
+    u32 maxw = 0;
+    bool unit = true;
+    for (u32 i = 0; i < (u32)w.size(); ++i) {
+        check_weight(w[i], i);
+        sum += w[i];
+        maxw = std::max(maxw, (u32)w[i]);
+        unit &= w[i] == 1.0;
+    }
+    if (sum >= 16777216.0)
+        throw Error(CASPHY_EWEIGHT, "weights sum to >= 2^24: float accumulation is not exact");
+    if (unit) return p;
+    std::vector<int> bits;
+    for (int b = 23; b >= 0; --b) {
+        bool any = false;
+        for (double v : w) any |= (((u32)v >> b) & 1u) != 0;
+        if (any) bits.push_back(b);
+    }
+    if (bits.empty()) {  // all weights zero: one empty plane
+        bits.push_back(0);
+    }
+    p.nplanes = (u32)bits.size();
+    p.mask.assign((size_t)p.nplanes * ldw, 0u);
+    for (u32 q = 0; q < p.nplanes; ++q) {
+        for (u32 i = 0; i < (u32)w.size(); ++i)
+            if (((u32)w[i] >> bits[q]) & 1u) p.mask[(size_t)q * ldw + (i >> 5)] |= 1u << (i & 31);
+        if (q > 0) p.shift[q] = (u32)(bits[q - 1] - bits[q]);
+    }
+    p.tail = (u32)bits.back();
+    return p;
+}
+
+Planes make_planes(const double *w, u32 nsite, u32 ldw)
+{
+    if (!w) return Planes{};
+    return make_planes_bits(std::vector<double>(w, w + nsite), ldw);
+}
+
+void apply_planes(const Planes &p, TileArgs &a, DevBuf &buf, hipStream_t s)
+{
+    a.nplanes = p.nplanes;
+    if (!p.nplanes) return;
+    buf.reserve(p.mask.size() * 4);
+    h2d(buf.p, p.mask.data(), p.mask.size() * 4, s);
+    a.pmask = buf.as<u32>();
+    for (u32 q = 0; q < p.nplanes; ++q) a.pshift[q] = p.shift[q];
+    a.ptail = p.tail;
+}
+
+namespace {
+
+// Device copy of a bit matrix in cell-major rows of ldw words.
+struct BitMat {
+    DevBuf d;
+    u32 n = 0, W = 0, ldw = 0;
+};
+
+BitMat upload_bit(const u32 *host, u32 W, u32 n, bool wordmajor, hipStream_t s)
+{
+    BitMat m;
+    m.n = n; m.W = W; m.ldw = round_up(std::max<u32>(W, 1), KC_BIT);
+    DevBuf raw((size_t)W * n * 4);
+    h2d(raw.p, host, (size_t)W * n * 4, s);
+    m.d.alloc((size_t)n * m.ldw * 4);
+    if (wordmajor) launch_transpose_wordmajor(raw.as<u32>(), W, n, m.d.as<u32>(), m.ldw, s);
+    else launch_pad_rows(raw.as<u32>(), W, n, m.d.as<u32>(), m.ldw, s);
+    CASPHY_HIP(hipStreamSynchronize(s));
+    return m;
+}
+
+// DNA in the metric chosen for its content (2-bit + mask or nibbles).
+struct DnaMat {
+    DevBuf d;
+    u32 n = 0, ldw = 0;
+    int metric = M_DNA2;
+};
+
+DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s)
+{
+    DnaMat m;
+    m.n = n;
+    DevBuf raw((size_t)n * nwRaw * 4), flag(4);
+    h2d(raw.p, host, (size_t)n * nwRaw * 4, s);
+    launch_fill_u32(flag.as<u32>(), 1, 0, s);
+    launch_dna_detect(raw.as<u32>(), (u64)n * nwRaw, flag.as<u32>(), s);
+    u32 ambiguous = 0;
+    d2h(&ambiguous, flag.p, 4, s);
+    if (!ambiguous) {
+        const u32 groups = ceil_div(nwRaw, 8);
+        m.metric = M_DNA2;
+        m.ldw = round_up(std::max<u32>(3 * groups, 1), KC_DNA2);
+        m.d.alloc((size_t)n * m.ldw * 4);
+        launch_dna_pack2(raw.as<u32>(), n, nwRaw, m.d.as<u32>(), m.ldw, s);
+    } else {
+        m.metric = M_NIB;
+        m.ldw = round_up(std::max<u32>(ceil_div(nwRaw, 2), 1), KC_BIT);
+        m.d.alloc((size_t)n * m.ldw * 4);
+        launch_dna_pack_nib(raw.as<u32>(), n, nwRaw, m.d.as<u32>(), m.ldw, s);
+    }
+    CASPHY_HIP(hipStreamSynchronize(s));
+    return m;
+}
+
+// Radius lists of a device matrix: mask tiles -> counts -> scan -> compaction,
+// all kept on the device.
+struct RadiusOut {
+    DevBuf mask, counts, offsets, idx, pbuf;
+    u64 total = 0;
+    u32 n = 0;
+};
+
+void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
+                    RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+{
+    const u32 nt = ceil_div(n, TILE);
+    const u32 mstride = nt * 4;
+    o.n = n;
+    o.mask.reserve((size_t)nt * TILE * mstride * 4);
+    TileArgs a;
+    a.A = dmat; a.nA = n; a.B = dmat; a.nB = n;
+    a.ldw = ldw; a.kw = ldw; a.ntA = nt; a.ntB = nt;
+    a.triangle = 1;
+    a.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+    a.thr = radius; a.mask = o.mask.as<u32>(); a.mstride = mstride;
+    apply_planes(planes, a, o.pbuf, s);
+    if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
+    launch_tiles(metric, E_MASK, a, s);
+    if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
+    o.counts.reserve((size_t)n * 4);
+    o.offsets.reserve((size_t)(n + 1) * 8);
+    launch_row_popc(o.mask.as<u32>(), n, mstride, mstride, o.counts.as<u32>(), s);
+    launch_scan_counts(o.counts.as<u32>(), n, o.offsets.as<u64>(), s);
+    d2h(&o.total, o.offsets.as<u64>() + n, 8, s);
+    o.idx.reserve(std::max<u64>(o.total, 1) * 4);
+    launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s);
+}
+
+void radius_lists(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
+                  u32 *nneighbor, u32 **idx, uint64_t *nidx, hipStream_t s)
+{
+    RadiusOut o;
+    radius_compute(metric, dmat, n, ldw, planes, radius, o, s);
+    u32 *host = (u32 *)std::malloc(std::max<u64>(o.total, 1) * 4);
+    CASPHY_REQUIRE(host, CASPHY_ENOMEM, "host allocation of the neighbour list failed");
+    try {
+        d2h(host, o.idx.p, o.total * 4, s);
+        d2h(nneighbor, o.counts.p, (size_t)n * 4, s);
+    } catch (...) {
+        std::free(host);
+        throw;
+    }
+    *idx = host;
+    *nidx = o.total;
+}
+
+void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, const int32_t *radii,
+               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s)
+{
+    for (u32 k = 0; k < R; ++k)
+        CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
+    for (u32 k = 1; k < nradii; ++k)
+        CASPHY_REQUIRE(radii[k - 1] <= radii[k], CASPHY_EARG, "radii must be ascending");
+    DevBuf dq((size_t)R * 4), dc((size_t)R * nradii * 4), pbuf;
+    h2d(dq.p, q, (size_t)R * 4, s);
+    launch_fill_u32(dc.as<u32>(), (u64)R * nradii, 0, s);
+    TileArgs a;
+    a.A = dmat; a.Aidx = dq.as<u32>(); a.nA = R; a.B = dmat; a.nB = n;
+    a.ldw = ldw; a.kw = ldw;
+    a.ntA = ceil_div(R, TILE); a.ntB = ceil_div(n, TILE);
+    a.ntiles = a.ntA * a.ntB;
+    a.nradii = nradii;
+    for (u32 k = 0; k < nradii; ++k) a.radii[k] = radii[k];
+    a.counts = dc.as<u32>();
+    apply_planes(planes, a, pbuf, s);
+    launch_tiles(metric, E_HIST, a, s);
+    d2h(counts, dc.p, (size_t)R * nradii * 4, s);
+}
+
+void require(bool c, const char *msg) { CASPHY_REQUIRE(c, CASPHY_EARG, msg); }
+
+}  // namespace
+}  // namespace casphy
+
+using namespace casphy;
+
+extern "C" {
+
+const char *casphy_last_error(void) { return g_last_error.c_str(); }
+
+int casphy_version(void) { return 10000; }
+
+void casphy_free(void *p) { std::free(p); }
+
+int casphy_set_device(int device)
+{
+    return guarded([&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            throw Error(CASPHY_ENODEV, "no HIP device available");
+        require(device >= 0 && device < count, "device index out of range");
+        g_device = device;
+    });
+}
+
+int casphy_synchronize(void)
+{
+    return guarded([&] { CASPHY_HIP(hipStreamSynchronize(thread_stream())); });
+}
+
+int casphy_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const double *weight, int32_t *S)
+{
+    return guarded([&] {
+        require(mat && S && n > 0 && W > 0, "casphy_score_bit_matrix: bad arguments");
+        hipStream_t s = thread_stream();
+        BitMat m = upload_bit(mat, W, n, false, s);
+        Planes planes = make_planes(weight, 32 * W, m.ldw);
+        DevBuf dS((size_t)n * n * 4), pbuf;
+        TileArgs a;
+        a.A = m.d.as<u32>(); a.nA = n; a.B = a.A; a.nB = n;
+        a.ldw = m.ldw; a.kw = m.ldw;
+        a.ntA = a.ntB = ceil_div(n, TILE);
+        a.triangle = 1;
+        a.ntiles = (u32)((u64)a.ntA * (a.ntA + 1) / 2);
+        a.S = dS.as<int32_t>(); a.ldS = n;
+        apply_planes(planes, a, pbuf, s);
+        launch_tiles(M_AND, E_SCORES, a, s);
+        d2h(S, dS.p, (size_t)n * n * 4, s);
+    });
+}
+
+int casphy_radius_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n, const double *weight,
+                      int32_t radius, uint32_t *nneighbor, uint32_t **idx, uint64_t *nidx)
+{
+    return guarded([&] {
+        require(mat_wordmajor && nneighbor && idx && nidx && n > 0 && W > 0,
+                "casphy_radius_bit: bad arguments");
+        hipStream_t s = thread_stream();
+        BitMat m = upload_bit(mat_wordmajor, W, n, true, s);
+        Planes planes = make_planes(weight, 32 * W, m.ldw);
+        radius_lists(M_AND, m.d.as<u32>(), n, m.ldw, planes, radius, nneighbor, idx, nidx, s);
+    });
+}
+
+int casphy_radius_dna(const uint32_t *mat, uint32_t n, uint32_t nwRaw, int32_t radius,
+                      uint32_t *nneighbor, uint32_t **idx, uint64_t *nidx)
+{
+    return guarded([&] {
+        require(mat && nneighbor && idx && nidx && n > 0 && nwRaw > 0, "casphy_radius_dna: bad arguments");
+        hipStream_t s = thread_stream();
+        DnaMat m = upload_dna(mat, n, nwRaw, s);
+        radius_lists(m.metric, m.d.as<u32>(), n, m.ldw, Planes{}, radius, nneighbor, idx, nidx, s);
+    });
+}
+
+int casphy_try_radii_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n, const double *weight,
+                         const int32_t *radii, uint32_t nradii, const uint32_t *q, uint32_t R,
+                         uint32_t *counts)
+{
+    return guarded([&] {
+        require(mat_wordmajor && radii && q && counts && n > 0 && W > 0, "casphy_try_radii_bit: bad arguments");
+        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_try_radii_bit: 1..16 radii");
+        if (R == 0) return;
+        hipStream_t s = thread_stream();
+        BitMat m = upload_bit(mat_wordmajor, W, n, true, s);
+        Planes planes = make_planes(weight, 32 * W, m.ldw);
+        try_radii(M_AND, m.d.as<u32>(), n, m.ldw, planes, radii, nradii, q, R, counts, s);
+    });
+}
+
+int casphy_try_radius_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n, const double *weight,
+                          int32_t radius, const uint32_t *q, uint32_t R, uint32_t *counts)
+{
+    return casphy_try_radii_bit(mat_wordmajor, W, n, weight, &radius, 1, q, R, counts);
+}
+
+int casphy_try_radii_dna(const uint32_t *mat, uint32_t n, uint32_t nwRaw, const int32_t *radii,
+                         uint32_t nradii, const uint32_t *q, uint32_t R, uint32_t *counts)
+{
+    return guarded([&] {
+        require(mat && radii && q && counts && n > 0 && nwRaw > 0, "casphy_try_radii_dna: bad arguments");
+        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_try_radii_dna: 1..16 radii");
+        if (R == 0) return;
+        hipStream_t s = thread_stream();
+        DnaMat m = upload_dna(mat, n, nwRaw, s);
+        try_radii(m.metric, m.d.as<u32>(), n, m.ldw, Planes{}, radii, nradii, q, R, counts, s);
+    });
+}
+
+int casphy_try_radius_dna(const uint32_t *mat, uint32_t n, uint32_t nwRaw, int32_t radius,
+                          const uint32_t *q, uint32_t R, uint32_t *counts)
+{
+    return casphy_try_radii_dna(mat, n, nwRaw, &radius, 1, q, R, counts);
+}
+
+int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx, uint32_t ncol,
+                    uint32_t ntip, uint32_t *ymin, uint32_t *ysize, uint32_t *xmin, uint32_t *xsize)
+{
+    return guarded([&] {
+        require(Y && X && ymin && ysize && xmin && xsize && ncol > 0, "casphy_transfer: bad arguments");
+        require(ntip <= 32u * ncol, "casphy_transfer: ntip exceeds 32*ncol");
+        hipStream_t s = thread_stream();
+        const u32 init = 32u * ncol;
+        DevBuf dy, dx, dymin((size_t)std::max(ny, 1u) * 4), dxmin((size_t)std::max(nx, 1u) * 4);
+        DevBuf dys((size_t)std::max(ny, 1u) * 4), dxs((size_t)std::max(nx, 1u) * 4);
+        BitMat my, mx;
+        if (ny) my = upload_bit(Y, ncol, ny, false, s);
+        if (nx) mx = upload_bit(X, ncol, nx, false, s);
+        launch_fill_u32(dymin.as<u32>(), ny, init, s);
+        launch_fill_u32(dxmin.as<u32>(), nx, init, s);
+        if (ny && nx) {
+            TileArgs a;
+            a.A = mx.d.as<u32>(); a.nA = nx; a.B = my.d.as<u32>(); a.nB = ny;
+            a.ldw = mx.ldw; a.kw = mx.ldw;
+            a.ntA = ceil_div(nx, TILE); a.ntB = ceil_div(ny, TILE);
+            a.ntiles = a.ntA * a.ntB;
+            a.ntip = ntip; a.amin = dxmin.as<u32>(); a.bmin = dymin.as<u32>();
+            launch_tiles(M_XOR, E_TBE, a, s);
+        }
+        if (ny) launch_row_size(my.d.as<u32>(), ny, my.ldw, ntip, dys.as<u32>(), s);
+        if (nx) launch_row_size(mx.d.as<u32>(), nx, mx.ldw, ntip, dxs.as<u32>(), s);
+        d2h(ymin, dymin.p, (size_t)ny * 4, s);
+        d2h(ysize, dys.p, (size_t)ny * 4, s);
+        d2h(xmin, dxmin.p, (size_t)nx * 4, s);
+        d2h(xsize, dxs.p, (size_t)nx * 4, s);
+    });
+}
+
+int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, uint32_t *internal_order)
+{
+    return guarded([&] {
+        require(edge_raw && edge_out && n >= 2, "casphy_cladewise: bad arguments");
+        const u32 nedge = 2 * n - 2, nnode = 2 * n - 1;
+        // edge rows grouped by parent, ascending row order within a parent
+        std::vector<u32> start(nnode + 1, 0), rows(nedge);
+        for (u32 e = 0; e < nedge; ++e) {
+            const u32 p = edge_raw[2 * e], c = edge_raw[2 * e + 1];
+            require(p < nnode && c < nnode, "casphy_cladewise: node id out of range");
+            ++start[p + 1];
+        }
+        for (u32 v = 0; v < nnode; ++v) start[v + 1] += start[v];
+        std::vector<u32> fillp(start.begin(), start.end() - 1);
+        for (u32 e = 0; e < nedge; ++e) rows[fillp[edge_raw[2 * e]]++] = e;
+        std::vector<u32> relabel(nnode, nnode), cursor(start.begin(), start.end() - 1), stack;
+        for (u32 i = 0; i < n; ++i) relabel[i] = i;
+        stack.reserve(nedge + 1);
+        stack.push_back(nedge);  // the root is the last node created
+        u32 next = n, out = 0;
+        while (!stack.empty()) {
+            const u32 top = stack.back();
+            if (cursor[top] == start[top + 1]) {
+                stack.pop_back();
+                continue;
+            }
+            const u32 e = rows[cursor[top]++];
+            const u32 p = edge_raw[2 * e], c = edge_raw[2 * e + 1];
+            stack.push_back(c);
+            if (relabel[p] == nnode) { internal_order[next - n] = p; relabel[p] = next++; }
+            if (relabel[c] == nnode) { internal_order[next - n] = c; relabel[c] = next++; }
+            require(out < nedge, "casphy_cladewise: edge list is not a tree");
+            edge_out[2 * out] = relabel[p];
+            edge_out[2 * out + 1] = relabel[c];
+            ++out;
+        }
+        require(out == nedge, "casphy_cladewise: edge list is not a tree rooted at 2n-2");
+    });
+}
+
+// ---------------------------------------------------------------- resident cells
+struct casphy_cells {
+    int kind = CASPHY_KIND_BIT;
+    int metric = M_AND;
+    u32 n = 0, ldw = 0;
+    DevBuf mat;
+    Planes planes;
+    RadiusOut out;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int device = 0;
+};
+
+int casphy_cells_create(int kind, const uint32_t *mat, uint32_t n, uint32_t W, int wordmajor,
+                        const double *weight, casphy_cells **out)
+{
+    return guarded([&] {
+        require(mat && out && n > 0 && W > 0, "casphy_cells_create: bad arguments");
+        require(kind == CASPHY_KIND_BIT || kind == CASPHY_KIND_DNA, "casphy_cells_create: bad kind");
+        hipStream_t s = thread_stream();
+        auto h = std::make_unique<casphy_cells>();
+        h->kind = kind;
+        h->n = n;
+        h->device = thread_device();
+        if (kind == CASPHY_KIND_BIT) {
+            BitMat m = upload_bit(mat, W, n, wordmajor != 0, s);
+            h->ldw = m.ldw;
+            h->mat = std::move(m.d);
+            h->metric = M_AND;
+            h->planes = make_planes(weight, 32 * W, h->ldw);
+        } else {
+            require(!weight, "casphy_cells_create: DNA takes unit weights only");
+            DnaMat m = upload_dna(mat, n, W, s);
+            h->ldw = m.ldw;
+            h->mat = std::move(m.d);
+            h->metric = m.metric;
+        }
+        for (auto &e : h->ev) CASPHY_HIP(hipEventCreate(&e));
+        *out = h.release();
+    });
+}
+
+void casphy_cells_destroy(casphy_cells *h)
+{
+    if (!h) return;
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete h;
+}
+
+int casphy_cells_radius(casphy_cells *h, int32_t radius, uint64_t *nidx)
+{
+    return guarded([&] {
+        require(h && nidx, "casphy_cells_radius: bad arguments");
+        hipStream_t s = thread_stream();
+        CASPHY_HIP(hipEventRecord(h->ev[2], s));
+        radius_compute(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radius, h->out, s, h->ev[0], h->ev[1]);
+        CASPHY_HIP(hipEventRecord(h->ev[3], s));
+        CASPHY_HIP(hipEventSynchronize(h->ev[3]));
+        *nidx = h->out.total;
+    });
+}
+
+int casphy_cells_fetch(casphy_cells *h, uint32_t *nneighbor, uint32_t *idx)
+{
+    return guarded([&] {
+        require(h && nneighbor && idx && h->out.n == h->n, "casphy_cells_fetch: no radius result");
+        hipStream_t s = thread_stream();
+        d2h(nneighbor, h->out.counts.p, (size_t)h->n * 4, s);
+        d2h(idx, h->out.idx.p, h->out.total * 4, s);
+    });
+}
+
+int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
+                           uint32_t R, uint32_t *counts)
+{
+    return guarded([&] {
+        require(h && radii && q && counts, "casphy_cells_try_radii: bad arguments");
+        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_cells_try_radii: 1..16 radii");
+        if (R == 0) return;
+        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts,
+                  thread_stream());
+    });
+}
+
+int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms)
+{
+    return guarded([&] {
+        require(h && tile_ms && radius_ms, "casphy_cells_last_ms: bad arguments");
+        CASPHY_HIP(hipEventElapsedTime(tile_ms, h->ev[0], h->ev[1]));
+        CASPHY_HIP(hipEventElapsedTime(radius_ms, h->ev[2], h->ev[3]));
+    });
+}
+
+}  // extern "C"

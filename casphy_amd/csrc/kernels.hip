@@ -1,0 +1,257 @@
+// kernels.hip -- layout conversion and radius-list compaction kernels.
+//
+// All of these are HBM-streaming kernels: coalesced 4/16-byte accesses, one
+// wave per row where a row is scanned, no atomics.
+#include "kernels.hpp"
+
+namespace casphy {
+
+namespace {
+
+__global__ void k_transpose_wordmajor(const u32 *in, u32 W, u32 n, u32 *out, u32 ldw)
+{
+    // 32 x 32 tile through LDS: reads along n (contiguous in .bmat), writes along words
+    __shared__ u32 t[32][33];
+    const u32 w0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const u32 tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
+    for (u32 y = ty; y < 32; y += 8) {
+        const u32 w = w0 + y, c = c0 + tx;
+        t[y][tx] = (w < W && c < n) ? in[(size_t)w * n + c] : 0u;
+    }
+    __syncthreads();
+    for (u32 y = ty; y < 32; y += 8) {
+        const u32 c = c0 + y, w = w0 + tx;
+        if (c < n && w < ldw) out[(size_t)c * ldw + w] = t[tx][y];
+    }
+}
+
+__global__ void k_pad_rows(const u32 *in, u32 W, u32 n, u32 *out, u32 ldw)
+{
+    const u64 total = (u64)n * ldw;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 r = (u32)(x / ldw), w = (u32)(x - (u64)r * ldw);
+        out[x] = w < W ? in[(size_t)r * W + w] : 0u;
+    }
+}
+
+// A nibble with two or more bits set is an IUPAC ambiguity code.
+__device__ __forceinline__ bool has_multibit_nibble(u32 v)
+{
+    const u32 nib = v & 0x0F0F0F0Fu;
+    // x & (x - 1) per byte without borrow across bytes: nibbles are < 16
+    const u32 m = nib & ((nib | 0x10101010u) - 0x01010101u) & 0x0F0F0F0Fu;
+    return m != 0;
+}
+
+__global__ void k_dna_detect(const u32 *raw, u64 nwords, u32 *flag)
+{
+    bool any = false;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < nwords; x += (u64)gridDim.x * blockDim.x)
+        any |= has_multibit_nibble(raw[x]);
+    if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+// One thread per (row, 32-position group): reads 8 raw words.
+// Nucleotide bit -> 2-bit code: A(1)->0, T(2)->1, C(4)->2, G(8)->3.
+__global__ void k_dna_pack2(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw)
+{
+    const u32 groups = ldw / 3;
+    const u64 total = (u64)n * groups;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 r = (u32)(x / groups), g = (u32)(x - (u64)r * groups);
+        u32 hi = 0, lo = 0, m = 0;
+        for (u32 q = 0; q < 8; ++q) {
+            const u32 w = g * 8 + q;
+            const u32 v = w < nwRaw ? raw[(size_t)r * nwRaw + w] : 0u;
+            for (u32 b = 0; b < 4; ++b) {
+                const u32 nib = (v >> (8 * b)) & 0xFu;
+                const u32 pos = q * 4 + b;
+                if (nib) {
+                    const u32 code = (nib & 0xCu ? 2u : 0u) | (nib & 0xAu ? 1u : 0u);
+                    hi |= (code >> 1) << pos;
+                    lo |= (code & 1u) << pos;
+                    m |= 1u << pos;
+                }
+            }
+        }
+        u32 *o = out + (size_t)r * ldw + 3 * g;
+        o[0] = hi; o[1] = lo; o[2] = m;
+    }
+}
+
+__global__ void k_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw)
+{
+    const u64 total = (u64)n * ldw;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 r = (u32)(x / ldw), w = (u32)(x - (u64)r * ldw);
+        u32 v = 0;
+        for (u32 h = 0; h < 2; ++h) {
+            const u32 rw = 2 * w + h;
+            const u32 src = rw < nwRaw ? raw[(size_t)r * nwRaw + rw] : 0u;
+            for (u32 b = 0; b < 4; ++b) v |= ((src >> (8 * b)) & 0xFu) << (16 * h + 4 * b);
+        }
+        out[x] = v;
+    }
+}
+
+// One wave per row.
+__global__ void k_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts)
+{
+    const u32 lane = threadIdx.x & 63;
+    const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const u32 *m = mask + (size_t)row * mstride;
+    u32 c = 0;
+    for (u32 w = lane; w < words; w += 64) c += (u32)__builtin_popcount(m[w]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) counts[row] = c;
+}
+
+// Single workgroup of 1024 threads: each thread sums a contiguous chunk, the
+// chunk sums are scanned in LDS, then each thread writes its chunk's offsets.
+__global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, u64 *offsets)
+{
+    __shared__ u64 part[1024];
+    const u32 t = threadIdx.x;
+    const u32 chunk = (n + 1023) / 1024;
+    const u32 b = t * chunk, e = min(n, b + chunk);
+    u64 s = 0;
+    for (u32 i = b; i < e; ++i) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (u32 o = 1; o < 1024; o <<= 1) {
+        u64 v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    u64 run = part[t] - s;  // exclusive prefix of this chunk
+    for (u32 i = b; i < e; ++i) {
+        offsets[i] = run;
+        run += counts[i];
+    }
+    if (t == 1023) offsets[n] = part[1023];
+}
+
+// One wave per row: lanes take consecutive mask words, a wave prefix sum of the
+// word popcounts gives each lane its output position, ids are written ascending.
+__global__ void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx)
+{
+    const u32 lane = threadIdx.x & 63;
+    const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const u32 *m = mask + (size_t)row * mstride;
+    u64 base = offsets[row];
+    for (u32 w0 = 0; w0 < words; w0 += 64) {
+        const u32 w = w0 + lane;
+        u32 v = w < words ? m[w] : 0u;
+        const u32 c = (u32)__builtin_popcount(v);
+        u32 inc = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const u32 y = __shfl_up(inc, o);
+            if (lane >= (u32)o) inc += y;
+        }
+        u64 p = base + inc - c;
+        while (v) {
+            const u32 b = (u32)__builtin_ctz(v);
+            idx[p++] = w * 32 + b;
+            v &= v - 1;
+        }
+        base += __shfl(inc, 63);
+    }
+}
+
+__global__ void k_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size)
+{
+    const u32 lane = threadIdx.x & 63;
+    const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const u32 *m = mat + (size_t)row * ldw;
+    u32 c = 0;
+    for (u32 w = lane; w < ldw; w += 64) c += (u32)__builtin_popcount(m[w]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) size[row] = c > ntip - c ? ntip - c : c;
+}
+
+template <class T>
+__global__ void k_fill(T *p, u64 count, T value)
+{
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < count; x += (u64)gridDim.x * blockDim.x)
+        p[x] = value;
+}
+
+u32 grid_for(u64 items, u32 block) { return (u32)std::min<u64>((items + block - 1) / block, 65535ull * 8); }
+
+}  // namespace
+
+void launch_transpose_wordmajor(const u32 *in, u32 W, u32 n, u32 *out, u32 ldw, hipStream_t s)
+{
+    dim3 g(ceil_div(n, 32), ceil_div(ldw, 32));
+    hipLaunchKernelGGL(k_transpose_wordmajor, g, dim3(256), 0, s, in, W, n, out, ldw);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_pad_rows(const u32 *in, u32 W, u32 n, u32 *out, u32 ldw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pad_rows, dim3(grid_for((u64)n * ldw, 256)), dim3(256), 0, s, in, W, n, out, ldw);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_dna_detect(const u32 *raw, u64 nwords, u32 *flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dna_detect, dim3(grid_for(nwords, 256)), dim3(256), 0, s, raw, nwords, flag);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_dna_pack2(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dna_pack2, dim3(grid_for((u64)n * (ldw / 3), 256)), dim3(256), 0, s, raw, n, nwRaw, out, ldw);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dna_pack_nib, dim3(grid_for((u64)n * ldw, 256)), dim3(256), 0, s, raw, n, nwRaw, out, ldw);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_row_popc, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, counts);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, counts, n, offsets);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, offsets, idx);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_row_size, dim3(ceil_div(n, 4)), dim3(256), 0, s, mat, n, ldw, ntip, size);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s)
+{
+    if (!count) return;
+    hipLaunchKernelGGL(k_fill<u32>, dim3(grid_for(count, 256)), dim3(256), 0, s, p, count, value);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s)
+{
+    if (!count) return;
+    hipLaunchKernelGGL(k_fill<u64>, dim3(grid_for(count, 256)), dim3(256), 0, s, p, count, value);
+    CASPHY_HIP(hipGetLastError());
+}
+
+}  // namespace casphy

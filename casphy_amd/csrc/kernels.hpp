@@ -1,0 +1,34 @@
+// kernels.hpp -- layout conversion and list-compaction kernels (declarations).
+#pragma once
+
+#include "common.hpp"
+
+namespace casphy {
+
+// .bmat word-major [W][n] -> cell-major [n][ldw] with zero padding words.
+void launch_transpose_wordmajor(const u32 *in, u32 W, u32 n, u32 *out, u32 ldw, hipStream_t s);
+// cell-major [n][W] -> [n][ldw] with zero padding words.
+void launch_pad_rows(const u32 *in, u32 W, u32 n, u32 *out, u32 ldw, hipStream_t s);
+
+// DNA byte codes [n][nwRaw]: sets *flag (device u32, pre-zeroed) when some code
+// has more than one nucleotide bit (IUPAC ambiguity) -> 2-bit form unusable.
+void launch_dna_detect(const u32 *raw, u64 nwords, u32 *flag, hipStream_t s);
+// 2-bit + mask form: per 32 positions three words (hi, lo, mask); rows of ldw.
+void launch_dna_pack2(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hipStream_t s);
+// nibble form: 8 positions per word (low nibble of each byte code); rows of ldw.
+void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hipStream_t s);
+
+// Radius mask -> CSR lists.
+void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s);
+// Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).
+void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s);
+void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx,
+                    hipStream_t s);
+
+// size[r] = min(p, ntip - p) with p = popcount of row r (u32 arithmetic).
+void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s);
+
+void launch_fill_u32(u32 *p, u64 count, u32 value, hipStream_t s);
+void launch_fill_u64(u64 *p, u64 count, u64 value, hipStream_t s);
+
+}  // namespace casphy

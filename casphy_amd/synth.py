@@ -1,0 +1,245 @@
+"""Seeded synthetic inputs for the CasPhy hot path (numpy PCG64, never rand()).
+
+Every generator is deterministic in its seed, so the HIP path, the oracle and
+the CPU baseline see byte-identical inputs.  Layouts are the reference's
+on-disk ones:
+
+* indel   -- (n, ncol) int32, -1 missing, 0 unmutated, k>0 allele id, alleles
+             relabelled per site to 1..k in ascending order of the raw id as
+             regularize.mat does (R/CasPhy_functions.R:435-441);
+* binary  -- (n, W) uint32 cell-major, site s at bit s&31 of word s>>5
+             (src/functions.cpp:54-68);
+* DNA     -- (n, nwRaw) uint32, four fasta2bin byte codes per word, base 4w+j in
+             byte j (cu/fasta2bin.cu:12-32,189-210);
+* bipartition bitsets -- (m, ceil(ntip/32)) uint32, tip t at bit t&31 of word
+             t>>5, zero padding (the layout TransferDisStream reads,
+             cu/TransferDisStream.cu:141-147).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+DNA_CODE = {"A": 0x01, "T": 0x02, "C": 0x04, "G": 0x08, "-": 0x10}
+_BASES = np.array([0x01, 0x02, 0x04, 0x08], dtype=np.uint8)
+
+
+def pack_bits(b: np.ndarray) -> np.ndarray:
+    """(n, nsite) bool -> (n, ceil(nsite/32)) uint32, LSB first."""
+    n, nsite = b.shape
+    W = (nsite + 31) // 32
+    pad = np.zeros((n, W * 32), dtype=np.uint8)
+    pad[:, :nsite] = b
+    by = np.packbits(pad, axis=1, bitorder="little")
+    return np.ascontiguousarray(by).view("<u4").astype(np.uint32, copy=False).reshape(n, W)
+
+
+def unpack_bits(p: np.ndarray, nsite: int | None = None) -> np.ndarray:
+    p = np.ascontiguousarray(p, dtype="<u4")
+    n, W = p.shape
+    b = np.unpackbits(p.view(np.uint8).reshape(n, W * 4), axis=1, bitorder="little")
+    return b[:, : (W * 32 if nsite is None else nsite)].astype(bool)
+
+
+def pack_bytes4(codes: np.ndarray) -> np.ndarray:
+    """(n, L) uint8 byte codes -> (n, ceil(L/4)) uint32, zero padding bytes."""
+    n, L = codes.shape
+    nw = (L + 3) // 4
+    pad = np.zeros((n, nw * 4), dtype=np.uint8)
+    pad[:, :L] = codes
+    return np.ascontiguousarray(pad).view("<u4").astype(np.uint32, copy=False).reshape(n, nw)
+
+
+# --------------------------------------------------------------- trees
+def balanced_tree(depth: int):
+    """Parent array of a complete binary division tree; node 0 is the root."""
+    nn = 2 ** (depth + 1) - 1
+    parent = np.full(nn, -1, dtype=np.int64)
+    parent[1:] = (np.arange(1, nn) - 1) // 2
+    leaves = np.arange(2 ** depth - 1, nn)
+    return parent, leaves
+
+
+def split_tree(n: int, rng: np.random.Generator):
+    """Random binary division tree with n leaves grown by splitting a uniformly
+    chosen leaf (Yule / coalescent topology).  Returns (parent, leaves, birth)
+    where birth[v] is the split time of v's parent lineage under rate-1 Yule."""
+    parent = np.full(2 * n - 1, -1, dtype=np.int64)
+    birth = np.zeros(2 * n - 1, dtype=np.float64)
+    leaves = [0]
+    nxt = 1
+    t = 0.0
+    picks = rng.random(n)
+    waits = rng.exponential(1.0, n)
+    for k in range(1, n):
+        t += waits[k] / k
+        i = int(picks[k] * k)
+        v = leaves[i]
+        a, b = nxt, nxt + 1
+        nxt += 2
+        parent[a] = v
+        parent[b] = v
+        birth[a] = birth[b] = t
+        leaves[i] = a
+        leaves.append(b)
+    return parent, np.array(leaves, dtype=np.int64), birth, t
+
+
+def _levels(parent: np.ndarray):
+    depth = np.zeros(parent.size, dtype=np.int64)
+    for v in range(1, parent.size):      # parents always precede children
+        depth[v] = depth[parent[v]] + 1
+    order = np.argsort(depth, kind="stable")
+    bounds = np.searchsorted(depth[order], np.arange(depth.max() + 2))
+    return [order[bounds[d]:bounds[d + 1]] for d in range(1, depth.max() + 1)]
+
+
+# --------------------------------------------------------------- configs
+def indel_matrix(seed: int = 1, depth: int = 10, nsite: int = 200, p: float = 0.02,
+                 nallele: int = 100, zipf_s: float = 1.5, missing: float = 0.1) -> np.ndarray:
+    """Indel lineage matrix: balanced tree, irreversible per-generation
+    mutation with probability p, Zipf(s) allele ids, then `missing` entries -1."""
+    rng = np.random.default_rng(seed)
+    parent, leaves = balanced_tree(depth)
+    probs = np.arange(1, nallele + 1, dtype=np.float64) ** -zipf_s
+    probs /= probs.sum()
+    st = np.zeros((parent.size, nsite), dtype=np.int32)
+    for lev in _levels(parent):
+        s = st[parent[lev]].copy()
+        mut = (s == 0) & (rng.random(s.shape) < p)
+        s[mut] = rng.choice(nallele, size=int(mut.sum()), p=probs).astype(np.int32) + 1
+        st[lev] = s
+    mat = st[leaves]
+    mat[rng.random(mat.shape) < missing] = -1
+    return regularize(mat)
+
+
+def regularize(mat: np.ndarray) -> np.ndarray:
+    """Per-site relabel: -1 -> -1, 0 -> 0, alleles -> rank 1..k (ascending)."""
+    out = np.empty_like(mat)
+    for s in range(mat.shape[1]):
+        col = mat[:, s]
+        u = np.unique(np.concatenate([col, [-1, 0]]))
+        out[:, s] = np.searchsorted(u, col) - 1
+    return out
+
+
+def indel_weights(mat: np.ndarray) -> np.ndarray:
+    """Unit weights, nState x ncol, nState = max alleles per site
+    (R/CasPhy_functions.R:583-586)."""
+    nstate = max(1, int(mat.max()))
+    return np.ones((nstate, mat.shape[1]), dtype=np.float64)
+
+
+def binary_matrix(seed: int = 2, n: int = 10000, nsite: int = 4096, p: float = 0.005,
+                  missing: float = 0.05) -> np.ndarray:
+    """Base-editor matrix packed to (n, nsite/32) uint32."""
+    rng = np.random.default_rng(seed)
+    parent, leaves, _, _ = split_tree(n, rng)
+    st = np.zeros((parent.size, nsite), dtype=bool)
+    for lev in _levels(parent):
+        st[lev] = st[parent[lev]] | (rng.random((lev.size, nsite)) < p)
+    b = st[leaves]
+    b &= rng.random(b.shape) >= missing
+    return pack_bits(b)
+
+
+def dna_matrix(seed: int = 3, n: int = 100000, ncol: int = 1000, height: float = 0.3,
+               gap: float = 0.02, chunk: int = 8192) -> np.ndarray:
+    """JC69 alignment on a rate-1 Yule tree scaled to mean root-to-tip
+    `height` substitutions/site, `gap` random gaps, as fasta2bin byte codes."""
+    rng = np.random.default_rng(seed)
+    parent, leaves, birth, _ = split_tree(n, rng)
+    # a tip's lineage ends when the process stops: give tips the mean depth
+    depth_time = birth.copy()
+    tip_end = birth[leaves].max()
+    end = np.empty_like(birth)
+    end[:] = np.nan
+    children_end = np.full(parent.size, np.inf)
+    for v in range(1, parent.size):
+        pv = parent[v]
+        children_end[pv] = min(children_end[pv], birth[v])
+    end = np.where(np.isinf(children_end), tip_end, children_end)
+    blen = end - depth_time
+    blen[0] = 0.0
+    scale = height / max(tip_end, 1e-12)
+    blen = np.maximum(blen, 0.0) * scale
+    pchg = 0.75 * (1.0 - np.exp(-4.0 / 3.0 * blen))
+    st = np.empty((parent.size, ncol), dtype=np.uint8)
+    st[0] = rng.integers(0, 4, ncol, dtype=np.uint8)
+    for lev in _levels(parent):
+        s = st[parent[lev]].copy()
+        chg = rng.random(s.shape) < pchg[lev, None]
+        s[chg] = (s[chg] + rng.integers(1, 4, int(chg.sum()), dtype=np.uint8)) % 4
+        st[lev] = s
+    codes = _BASES[st[leaves]]
+    codes[rng.random(codes.shape) < gap] = 0x10
+    return pack_bytes4(codes)
+
+
+def query_rows(seed: int, n: int, R: int) -> np.ndarray:
+    """tryRadius query rows (the reference draws them with time-seeded rand(),
+    cu/tryRadius.cu:121,129; here they are an explicit seeded list)."""
+    return np.random.default_rng(seed).integers(0, n, R, dtype=np.int64).astype(np.uint32)
+
+
+# --------------------------------------------------------------- bipartitions
+def yule_parent(n: int, rng: np.random.Generator):
+    parent, leaves, _, _ = split_tree(n, rng)
+    return parent, leaves
+
+
+def spr(parent: np.ndarray, leaves: np.ndarray, moves: int, rng: np.random.Generator) -> np.ndarray:
+    """`moves` random subtree-prune-and-regraft moves on a rooted binary tree
+    given as a parent array; node ids and tip ids are kept."""
+    par = parent.copy()
+    nn = par.size
+    kids = [[] for _ in range(nn)]
+    for v in range(1, nn):
+        kids[par[v]].append(v)
+    root = 0
+    done = 0
+    while done < moves:
+        s = int(rng.integers(1, nn))
+        p = par[s]
+        if p == root:
+            continue
+        t = int(rng.integers(0, nn))
+        # target edge (par[t], t) must lie outside the pruned subtree and off p
+        a = t
+        bad = False
+        while a != -1:
+            if a == s:
+                bad = True
+                break
+            a = par[a]
+        if bad or t == p or t == root or par[t] == p:
+            continue
+        g = par[p]
+        sib = kids[p][0] if kids[p][1] == s else kids[p][1]
+        # remove p: sibling hangs from the grandparent
+        kids[g][kids[g].index(p)] = sib
+        par[sib] = g
+        # reuse p on edge (par[t], t)
+        q = par[t]
+        kids[q][kids[q].index(t)] = p
+        par[p] = q
+        kids[p] = [t, s]
+        par[t] = p
+        done += 1
+    return par
+
+
+def clade_bitsets(parent: np.ndarray, leaves: np.ndarray, ntip: int) -> np.ndarray:
+    """Descendant-tip bitsets of every internal non-root node, (ntip-2, W)."""
+    nn = parent.size
+    W = (ntip + 31) // 32
+    tip_of = np.full(nn, -1, dtype=np.int64)
+    tip_of[leaves] = np.arange(leaves.size)
+    bits = np.zeros((nn, W), dtype=np.uint32)
+    tl = leaves
+    bits[tl, np.arange(ntip) >> 5] = (np.uint32(1) << (np.arange(ntip) & 31).astype(np.uint32))
+    levels = _levels(parent)
+    for lev in reversed(levels):
+        np.bitwise_or.at(bits, parent[lev], bits[lev])
+    internal = np.setdiff1d(np.arange(1, nn), leaves)
+    return np.ascontiguousarray(bits[internal])

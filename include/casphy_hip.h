@@ -1,0 +1,192 @@
+/*
+ * casphy_hip.h -- C ABI of the MI355X-native CasPhy hot path (libcasphy_hip.so).
+ *
+ * The library replaces the GPU executables of CasPhy (cu/joinEdgeBitReverse.cu,
+ * cu/joinEdgeIndel.cu, cu/radiusNeighbor.cu, cu/radiusNeighborDNA.cu,
+ * cu/tryRadius.cu, cu/tryRadiusDNA.cu, cu/TransferDisStream.cu) and the Rcpp CPU
+ * twins they mirror (src/functions.cpp, registered in src/RcppExports.cpp).
+ * Every computation runs in hand-written HIP kernels for gfx950; the host side
+ * only moves buffers and launches work.  Results are integer and bit-exact with
+ * the reference's CPU path.
+ *
+ * Conventions
+ *  - Status: 0 = ok, < 0 = CASPHY_E* error; casphy_last_error() gives a
+ *    thread-local message.  No exception crosses the ABI.
+ *  - Buffers are caller-owned host memory unless a parameter says "library-
+ *    allocated" (free those with casphy_free).
+ *  - Layouts are the reference's on-disk / R-matrix layouts:
+ *      cell-major  [n][W]   (.tbmat; an R matrix with cells as columns)
+ *      word-major  [W][n]   (.bmat;  an R matrix with cells as rows)
+ *  - Weights are the .w doubles.  They must be non-negative integers whose sum
+ *    is < 2^24, the range in which the reference's float (and fp16 <= 2048 for
+ *    the indel join) accumulation is exact; other weights return CASPHY_EWEIGHT.
+ *    NULL means unit weights (R's default, R/CasPhy_functions.R:575-576).
+ *  - Thread safety: entry points are re-entrant; concurrent calls on one GPU are
+ *    serialised per call on the library's per-thread stream.
+ */
+#ifndef CASPHY_HIP_H
+#define CASPHY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CASPHY_OK 0
+#define CASPHY_EARG (-1)      /* bad argument (sizes, NULL pointers, ranges) */
+#define CASPHY_EHIP (-2)      /* HIP runtime error */
+#define CASPHY_ENOMEM (-3)    /* device or host allocation failed / over budget */
+#define CASPHY_EWEIGHT (-4)   /* weights outside the exact integer contract */
+#define CASPHY_EINTERNAL (-5)
+#define CASPHY_ENODEV (-6)    /* no usable GPU */
+
+/* Thread-local message of the last failing call on this thread. */
+const char *casphy_last_error(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int casphy_version(void);
+/* Frees a library-allocated host buffer. */
+void casphy_free(void *p);
+/* Selects the device used by this thread's later calls (default 0). */
+int casphy_set_device(int device);
+/* Blocks until this thread's stream is idle. */
+int casphy_synchronize(void);
+
+/* ------------------------------------------------------------------ scoring */
+/* Full n x n shared-mutation score matrix of bit-packed cells, S[i*n+j] =
+ * sum_s w_s b_s(i) b_s(j) (self score on the diagonal).
+ * Replaces: calcShared src/functions.cpp:3160-3170 used over all pairs at
+ * :3347-3357; calculateSharedKernel cu/joinEdgeBitReverse.cu:53-75. */
+int casphy_score_bit_matrix(const uint32_t *mat_cellmajor, uint32_t n, uint32_t W,
+                            const double *weight, int32_t *S);
+
+/* Full n x n indel score matrix: S[i*n+j] = sum_s w[a-1][s] [a_is = a_js = a > 0].
+ * mat: cell-major int32 (-1 missing, 0 unmutated, k > 0 allele); weight:
+ * nstate x ncol state-major (NULL = unit).
+ * Replaces: countApoFloat src/functions.cpp:261-270 over all pairs at
+ * :1906-1915; calculateSharedKernel cu/joinEdgeIndel.cu:67-85. */
+int casphy_score_indel_matrix(const int32_t *mat_cellmajor, uint32_t n, uint32_t ncol,
+                              const double *weight, uint32_t nstate, int32_t *S);
+
+/* ------------------------------------------------------------------ radius */
+/* Radius neighbour lists of bit-packed cells: N_r(i) = ascending {j : S(i,j) >= r}
+ * (i itself included when its self score reaches r).  nneighbor[n] receives
+ * the list lengths; *idx (library-allocated) receives the concatenated lists,
+ * *nidx their total length: exactly the .nNeighbor / .idx contents.
+ * Replaces: main of cu/radiusNeighbor.cu:162-470 (argv: prefix radius) and the
+ * radiusWeightedParallel loop of R/CasPhy_functions.R:160-208
+ * (src/functions.cpp:1825-1857). */
+int casphy_radius_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n,
+                      const double *weight, int32_t radius, uint32_t *nneighbor,
+                      uint32_t **idx, uint64_t *nidx);
+
+/* Same for DNA in fasta2bin byte codes (4 per u32, cell-major [n][nwRaw]):
+ * S(i,j) = #{p : code_i[p] & code_j[p] & 0xF != 0}.
+ * Replaces: main of cu/radiusNeighborDNA.cu:111-380 and
+ * radiusWeightedDNAParallel src/functions.cpp:4021-4036 (unit weights). */
+int casphy_radius_dna(const uint32_t *mat_cellmajor, uint32_t n, uint32_t nwRaw,
+                      int32_t radius, uint32_t *nneighbor, uint32_t **idx, uint64_t *nidx);
+
+/* tryRadius: counts[k] = |N_r(q[k])| for the R query rows q (self included).
+ * The reference draws q with time-seeded rand(); here the caller passes them.
+ * Replaces: cu/tryRadius.cu:26-50,52-189 (argv: prefix radius nReplicate). */
+int casphy_try_radius_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n,
+                          const double *weight, int32_t radius, const uint32_t *q,
+                          uint32_t R, uint32_t *counts);
+/* Replaces: cu/tryRadiusDNA.cu:45-74,76-178. */
+int casphy_try_radius_dna(const uint32_t *mat_cellmajor, uint32_t n, uint32_t nwRaw,
+                          int32_t radius, const uint32_t *q, uint32_t R, uint32_t *counts);
+/* Batched tryRadius: all radii in one launch, counts[k*R + j] for radii[k]
+ * (the autoSplit tuning loop of R/CasPhy_functions.R:2010-2021 runs one
+ * tryRadius process per radius step; this answers every step at once).
+ * nradii <= 16, radii ascending. */
+int casphy_try_radii_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n,
+                         const double *weight, const int32_t *radii, uint32_t nradii,
+                         const uint32_t *q, uint32_t R, uint32_t *counts);
+int casphy_try_radii_dna(const uint32_t *mat_cellmajor, uint32_t n, uint32_t nwRaw,
+                         const int32_t *radii, uint32_t nradii, const uint32_t *q,
+                         uint32_t R, uint32_t *counts);
+
+/* ------------------------------------------------------------------ joins */
+/* Greedy shared-mutation join of bit-packed cells (n >= 2).
+ * edge[(2n-2)*2]: (parent, child) pairs in join order, 0-based, new nodes
+ * numbered n, n+1, ... (the raw edge of greedyJoinEdgeBitReverse).
+ * node_state[(2n-1)*W] (optional): per node id, seq & ~bistate;
+ * node_bistate[(2n-1)*W] (optional): per node id.
+ * ancestral_type: 0 or 1 (calcAncestralBitReverse / ...2).
+ * Replaces: greedyJoinEdgeBitReverse src/functions.cpp:3316-3461
+ * (R signature greedyJoinEdgeBitReverse(mat, weight, verbose, ancestralType),
+ * R/RcppExports.R:264-266) and the join loop of cu/joinEdgeBitReverse.cu:285-505. */
+int casphy_join_bit(const uint32_t *mat_cellmajor, uint32_t n, uint32_t W, const double *weight,
+                    int ancestral_type, uint32_t *edge, uint32_t *node_state,
+                    uint32_t *node_bistate);
+
+/* Greedy join of indel states (n >= 2).  mat: cell-major int32 (-1/0/k).
+ * weight: nstate x ncol (NULL = unit).  tip_size: n (NULL = all 1).
+ * edge as above; node_state[(2n-1)*ncol] (optional) per node id;
+ * branch[2n-2] (optional): per edge, sites with parent 0 and child > 0.
+ * Replaces: greedyJoinEdge2 src/functions.cpp:1880-2017 (R signature
+ * greedyJoinEdge2(mat, weight, tipSize, minSize, minRatio, verbose),
+ * R/RcppExports.R:168-170) and cu/joinEdgeIndel.cu:244-468. */
+int casphy_join_indel(const int32_t *mat_cellmajor, uint32_t n, uint32_t ncol,
+                      const double *weight, uint32_t nstate, const uint32_t *tip_size,
+                      uint32_t min_size, float min_ratio, uint32_t *edge, int32_t *node_state,
+                      uint32_t *branch);
+
+/* Cladewise emission: DFS from the root 2n-2 taking the lowest unused edge row
+ * first; tips keep ids, the root becomes n, internal nodes take the next id on
+ * first discovery.  edge_out[(2n-2)*2] relabelled in DFS order (.edge2 payload);
+ * internal_order[n-1] = raw id of relabelled node n+k (row order of .nodeState2).
+ * Replaces: cu/joinEdgeBitReverse.cu:526-597 / cu/joinEdgeIndel.cu:486-558 and
+ * reOrderEdge2 src/functions.cpp:2234-2260 + R/CasPhy_functions.R:1632-1641. */
+int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out,
+                     uint32_t *internal_order);
+
+/* Batched join: `njobs` independent binary trees in one persistent work-queue
+ * launch (bootstrap replicates, split clusters).  Job k's cells are rows
+ * [offsets[k], offsets[k+1]) of mat_cellmajor (all of width W); its edges go to
+ * edge + 2*(2*offsets[k] - 2*k), i.e. 2n_k-2 rows per job in job order.
+ * No counterpart in the reference (which runs one process per tree,
+ * R/CasPhy_functions.R:2389-2392); each job's result equals casphy_join_bit. */
+int casphy_join_bit_batch(const uint32_t *mat_cellmajor, const uint32_t *offsets, uint32_t njobs,
+                          uint32_t W, const double *weight, int ancestral_type, uint32_t *edge);
+
+/* ------------------------------------------------------------------ transfer */
+/* Transfer distance between bipartition bitsets (row-major [nrow][ncol] u32,
+ * ntip tips, zero padding bits).  For each y: ymin[y] = min over x of
+ * min(H, ntip - H) with H = popcount(x ^ y) (u32 arithmetic), starting from
+ * 32*ncol; ysize[y] = min(popc(y), ntip - popc(y)); likewise xmin / xsize.
+ * Replaces: cu/TransferDisStream.cu:22-344 (argv: Yfile Xfile outprefix ntip). */
+int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx,
+                    uint32_t ncol, uint32_t ntip, uint32_t *ymin, uint32_t *ysize,
+                    uint32_t *xmin, uint32_t *xsize);
+
+/* ------------------------------------------------------------------ resident */
+/* A packed cell matrix kept in HBM across calls (the serving / tuning form: one
+ * upload, many radius and tryRadius passes).  kind: CASPHY_KIND_BIT (bit-packed
+ * sites, `mat` word-major [W][n] like .bmat when wordmajor != 0, else cell-major)
+ * or CASPHY_KIND_DNA (fasta2bin codes, cell-major [n][W=nwRaw]).
+ * No counterpart in the reference, which re-reads its input file in every
+ * tryRadius / radiusNeighbor process (R/CasPhy_functions.R:2010-2024). */
+#define CASPHY_KIND_BIT 0
+#define CASPHY_KIND_DNA 1
+typedef struct casphy_cells casphy_cells;
+int casphy_cells_create(int kind, const uint32_t *mat, uint32_t n, uint32_t W, int wordmajor,
+                        const double *weight, casphy_cells **out);
+void casphy_cells_destroy(casphy_cells *h);
+/* Radius lists computed and kept on the device; *nidx = total list length. */
+int casphy_cells_radius(casphy_cells *h, int32_t radius, uint64_t *nidx);
+/* Copies the lists of the last casphy_cells_radius call (nneighbor[n], idx[nidx]). */
+int casphy_cells_fetch(casphy_cells *h, uint32_t *nneighbor, uint32_t *idx);
+/* tryRadius counts for up to 16 ascending radii: counts[k*R + j]. */
+int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
+                           uint32_t R, uint32_t *counts);
+/* Device time (ms, hipEvents on the handle's stream) of the pair-scoring tile
+ * kernel of the last radius call and of the whole last radius call. */
+int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CASPHY_HIP_H */

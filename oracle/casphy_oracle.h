@@ -1,0 +1,90 @@
+/*
+ * casphy_oracle.h -- CPU restatement of CasPhy's bit-packed phylogeny hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * (casphy_amd/libcasphy_hip.so).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it, and only to compute expected values or a
+ * CPU timing.  The product never links, loads or calls anything here.
+ *
+ * Every function restates the reference's own algorithm (file:line cited on each
+ * definition in casphy_oracle.c), keeping its float accumulation order, its
+ * strict-compare tie rules and its output layouts.  Pinning of each function
+ * (golden vectors, closed forms, brute force) lives in tests/test_oracle_*.py.
+ */
+#ifndef CASPHY_ORACLE_H
+#define CASPHY_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- (a-1) binary shared-mutation score ---------------------------------- */
+/* Score of query row q against row t, both cell-major W words, float weights
+ * (32*W of them).  Float accumulation in ascending set-bit order of q.       */
+float orc_score_bit(const uint32_t *q, const uint32_t *t, uint32_t W, const float *w);
+/* Full n x n matrix S[i*n+j] (diagonal = self score, as in the radius path). */
+void orc_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const float *w, float *S);
+
+/* ---- (a-2) indel shared-mutation score ----------------------------------- */
+/* a, b: ncol int32 states (-1 missing, 0 unmutated, k>0 allele);
+ * w: nState x ncol, state-major ((k-1)*ncol + s).                           */
+float orc_score_indel(const int32_t *a, const int32_t *b, uint32_t ncol, const float *w);
+
+/* ---- (a-3) DNA similarity over fasta2bin byte codes ------------------------ */
+/* a, b: nwRaw u32 words, 4 byte-codes each.  Count of positions whose codes
+ * share a nucleotide bit (low nibble).                                       */
+uint32_t orc_score_dna(const uint32_t *a, const uint32_t *b, uint32_t nwRaw);
+
+/* ---- (a-7, a-8) radius neighbour lists / counts ---------------------------- */
+/* Rows [i0, i1) of the radius lists.  mat cell-major.  counts[i-i0] gets
+ * |N_r(i)|; when idx != NULL the ids are appended in ascending order at
+ * idx[*pos...] (pos advanced).  Returns 0, or -1 when idx_cap is exceeded.  */
+int orc_radius_bit_rows(const uint32_t *mat, uint32_t n, uint32_t W, const float *w, float radius,
+                        uint32_t i0, uint32_t i1, uint32_t *counts, uint32_t *idx,
+                        uint64_t idx_cap, uint64_t *pos);
+int orc_radius_dna_rows(const uint32_t *mat, uint32_t n, uint32_t nwRaw, uint32_t radius,
+                        uint32_t i0, uint32_t i1, uint32_t *counts, uint32_t *idx,
+                        uint64_t idx_cap, uint64_t *pos);
+/* tryRadius: counts[k] = |N_r(q[k])| (self included). */
+void orc_try_radius_bit(const uint32_t *mat, uint32_t n, uint32_t W, const float *w, float radius,
+                        const uint32_t *q, uint32_t R, uint32_t *counts);
+void orc_try_radius_dna(const uint32_t *mat, uint32_t n, uint32_t nwRaw, uint32_t radius,
+                        const uint32_t *q, uint32_t R, uint32_t *counts);
+
+/* ---- (a-4, a-5) greedy shared-mutation join -------------------------------- */
+/* Binary join, CPU twin greedyJoinEdgeBitReverse (O(n^2) score cache).
+ * mat: n x W cell-major.  w: 32*W float.  edge: (2n-2) x 2 (parent, child) in
+ * join order, 0-based.  node_state / node_bistate: (2n-1) x W indexed by node
+ * id (tips are the input rows with bistate 0); node_state is seq & ~bistate. */
+int orc_join_bit(const uint32_t *mat, uint32_t n, uint32_t W, const float *w, int ancestral_type,
+                 uint32_t *edge, uint32_t *node_state, uint32_t *node_bistate);
+/* The same join driven the way the GPU executable drives it: O(n) lazy
+ * local-max cache with recompute over all live nodes.                       */
+int orc_join_bit_lazy(const uint32_t *mat, uint32_t n, uint32_t W, const float *w, int ancestral_type,
+                      uint32_t *edge, uint32_t *node_state, uint32_t *node_bistate);
+/* Indel join, CPU twin greedyJoinEdge2.  mat: n x ncol int32 (-1/0/k).
+ * w: nState x ncol state-major.  tip_size: n.  edge: (2n-2) x 2.
+ * node_state: (2n-1) x ncol by node id.  branch: 2n-2 per-edge counts
+ * (row 2 of greedyJoinEdge2's output).                                       */
+int orc_join_indel(const int32_t *mat, uint32_t n, uint32_t ncol, const float *w,
+                   const uint32_t *tip_size, uint32_t min_size, float min_ratio,
+                   uint32_t *edge, int32_t *node_state, uint32_t *branch);
+int orc_join_indel_lazy(const int32_t *mat, uint32_t n, uint32_t ncol, const float *w,
+                        const uint32_t *tip_size, uint32_t min_size, float min_ratio,
+                        uint32_t *edge, int32_t *node_state, uint32_t *branch);
+
+/* ---- (a-6) cladewise emission --------------------------------------------- */
+/* edge_raw: (2n-2) x 2 in join order.  edge_out: (2n-2) x 2 relabelled, in DFS
+ * order.  internal_order[k] = raw node id of relabelled node n+k.            */
+void orc_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, uint32_t *internal_order);
+
+/* ---- (a-9) transfer distance ---------------------------------------------- */
+void orc_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx,
+                  uint32_t ncol, uint32_t ntip,
+                  uint32_t *ymin, uint32_t *ysize, uint32_t *xmin, uint32_t *xsize);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

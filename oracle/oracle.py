@@ -1,0 +1,211 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+The product (casphy_amd) never imports it.
+
+Array conventions follow the reference's on-disk layouts: binary and DNA
+matrices are cell-major (n, W) uint32, indel matrices (n, ncol) int32 with
+-1 missing / 0 unmutated / k>0 allele, weights float32 (the executables convert
+the .w doubles to float before summing, cu/radiusNeighbor.cu:240-243).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        u32, f32, i32, u64 = C.c_uint32, C.c_float, C.c_int, C.c_uint64
+        vp = C.c_void_p
+        L.orc_score_bit.restype = f32
+        L.orc_score_bit.argtypes = [u32p, u32p, u32, f32p]
+        L.orc_score_bit_matrix.argtypes = [u32p, u32, u32, f32p, f32p]
+        L.orc_score_indel.restype = f32
+        L.orc_score_indel.argtypes = [i32p, i32p, u32, f32p]
+        L.orc_score_dna.restype = u32
+        L.orc_score_dna.argtypes = [u32p, u32p, u32]
+        L.orc_radius_bit_rows.restype = i32
+        L.orc_radius_bit_rows.argtypes = [u32p, u32, u32, f32p, f32, u32, u32, u32p, vp, u64,
+                                          C.POINTER(C.c_uint64)]
+        L.orc_radius_dna_rows.restype = i32
+        L.orc_radius_dna_rows.argtypes = [u32p, u32, u32, u32, u32, u32, u32p, vp, u64,
+                                          C.POINTER(C.c_uint64)]
+        L.orc_try_radius_bit.argtypes = [u32p, u32, u32, f32p, f32, u32p, u32, u32p]
+        L.orc_try_radius_dna.argtypes = [u32p, u32, u32, u32, u32p, u32, u32p]
+        L.orc_join_bit.restype = i32
+        L.orc_join_bit.argtypes = [u32p, u32, u32, f32p, i32, u32p, u32p, u32p]
+        L.orc_join_bit_lazy.restype = i32
+        L.orc_join_bit_lazy.argtypes = [u32p, u32, u32, f32p, i32, u32p, u32p, u32p]
+        L.orc_join_indel.restype = i32
+        L.orc_join_indel.argtypes = [i32p, u32, u32, f32p, u32p, u32, f32, u32p, i32p, u32p]
+        L.orc_join_indel_lazy.restype = i32
+        L.orc_join_indel_lazy.argtypes = [i32p, u32, u32, f32p, u32p, u32, f32, u32p, i32p, u32p]
+        L.orc_cladewise.argtypes = [u32p, u32, u32p, u32p]
+        L.orc_transfer.argtypes = [u32p, u32, u32p, u32, u32, u32, u32p, u32p, u32p, u32p]
+        _lib = L
+    return _lib
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def unit_weights_bit(W: int) -> np.ndarray:
+    return np.ones(32 * W, dtype=np.float32)
+
+
+# ---------------------------------------------------------------- scoring
+def score_bit(q, t, w=None) -> float:
+    q, t = _u32(q), _u32(t)
+    w = unit_weights_bit(q.size) if w is None else _f32(w)
+    return float(lib().orc_score_bit(q, t, q.size, w))
+
+
+def score_bit_matrix(mat, w=None) -> np.ndarray:
+    mat = _u32(mat)
+    n, W = mat.shape
+    w = unit_weights_bit(W) if w is None else _f32(w)
+    S = np.empty((n, n), dtype=np.float32)
+    lib().orc_score_bit_matrix(mat, n, W, w, S)
+    return S
+
+
+def score_indel(a, b, w) -> float:
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    return float(lib().orc_score_indel(a, b, a.size, _f32(w)))
+
+
+def score_dna(a, b) -> int:
+    a, b = _u32(a), _u32(b)
+    return int(lib().orc_score_dna(a, b, a.size))
+
+
+# ---------------------------------------------------------------- radius
+def _radius_rows(kind, mat, radius, w, rows=None):
+    mat = _u32(mat)
+    n, W = mat.shape
+    i0, i1 = (0, n) if rows is None else rows
+    counts = np.zeros(i1 - i0, dtype=np.uint32)
+    pos = C.c_uint64(0)
+    L = lib()
+    if kind == "bit":
+        w = unit_weights_bit(W) if w is None else _f32(w)
+        L.orc_radius_bit_rows(mat, n, W, w, float(radius), i0, i1, counts, None, 0, C.byref(pos))
+    else:
+        L.orc_radius_dna_rows(mat, n, W, int(radius), i0, i1, counts, None, 0, C.byref(pos))
+    total = int(counts.sum(dtype=np.uint64))
+    idx = np.zeros(max(total, 1), dtype=np.uint32)
+    pos = C.c_uint64(0)
+    if kind == "bit":
+        rc = L.orc_radius_bit_rows(mat, n, W, w, float(radius), i0, i1, counts,
+                                   idx.ctypes.data_as(C.c_void_p), total, C.byref(pos))
+    else:
+        rc = L.orc_radius_dna_rows(mat, n, W, int(radius), i0, i1, counts,
+                                   idx.ctypes.data_as(C.c_void_p), total, C.byref(pos))
+    assert rc == 0
+    return counts, idx[:total]
+
+
+def radius_bit(mat, radius, w=None, rows=None):
+    """(counts, idx) = the .nNeighbor counts and the concatenated .idx lists."""
+    return _radius_rows("bit", mat, radius, w, rows)
+
+
+def radius_dna(mat, radius, rows=None):
+    return _radius_rows("dna", mat, radius, None, rows)
+
+
+def try_radius_bit(mat, radius, q, w=None) -> np.ndarray:
+    mat, q = _u32(mat), _u32(q)
+    n, W = mat.shape
+    w = unit_weights_bit(W) if w is None else _f32(w)
+    out = np.zeros(q.size, dtype=np.uint32)
+    lib().orc_try_radius_bit(mat, n, W, w, float(radius), q, q.size, out)
+    return out
+
+
+def try_radius_dna(mat, radius, q) -> np.ndarray:
+    mat, q = _u32(mat), _u32(q)
+    n, W = mat.shape
+    out = np.zeros(q.size, dtype=np.uint32)
+    lib().orc_try_radius_dna(mat, n, W, int(radius), q, q.size, out)
+    return out
+
+
+# ---------------------------------------------------------------- joins
+def join_bit(mat, w=None, ancestral_type=1, lazy=False):
+    """Returns (edge (2n-2,2) raw join order, node_state (2n-1,W), node_bistate)."""
+    mat = _u32(mat)
+    n, W = mat.shape
+    w = unit_weights_bit(W) if w is None else _f32(w)
+    edge = np.zeros((2 * n - 2, 2), dtype=np.uint32)
+    ns = np.zeros((2 * n - 1, W), dtype=np.uint32)
+    nb = np.zeros((2 * n - 1, W), dtype=np.uint32)
+    fn = lib().orc_join_bit_lazy if lazy else lib().orc_join_bit
+    rc = fn(mat, n, W, w, int(ancestral_type), edge, ns, nb)
+    if rc != 0:
+        raise RuntimeError(f"oracle join_bit rc={rc}")
+    return edge, ns, nb
+
+
+def join_indel(mat, w, tip_size=None, min_size=0, min_ratio=2.0, lazy=False):
+    """Returns (edge (2n-2,2), node_state (2n-1,ncol) int32, branch (2n-2,))."""
+    mat = np.ascontiguousarray(mat, dtype=np.int32)
+    n, ncol = mat.shape
+    ts = np.ones(n, dtype=np.uint32) if tip_size is None else _u32(tip_size)
+    edge = np.zeros((2 * n - 2, 2), dtype=np.uint32)
+    ns = np.zeros((2 * n - 1, ncol), dtype=np.int32)
+    br = np.zeros(2 * n - 2, dtype=np.uint32)
+    fn = lib().orc_join_indel_lazy if lazy else lib().orc_join_indel
+    rc = fn(mat, n, ncol, _f32(w), ts, int(min_size), float(min_ratio), edge, ns, br)
+    if rc != 0:
+        raise RuntimeError(f"oracle join_indel rc={rc}")
+    return edge, ns, br
+
+
+def cladewise(edge_raw, n):
+    """(edge_out (2n-2,2) relabelled DFS order, internal_order (n-1,) raw ids)."""
+    edge_raw = _u32(edge_raw)
+    eo = np.zeros((2 * n - 2, 2), dtype=np.uint32)
+    io = np.zeros(max(n - 1, 1), dtype=np.uint32)
+    lib().orc_cladewise(edge_raw, n, eo, io)
+    return eo, io[: n - 1]
+
+
+# ---------------------------------------------------------------- transfer
+def transfer(Y, X, ntip):
+    Y, X = _u32(Y), _u32(X)
+    ny, ncol = Y.shape
+    nx = X.shape[0]
+    ymin = np.zeros(ny, np.uint32)
+    ysize = np.zeros(ny, np.uint32)
+    xmin = np.zeros(nx, np.uint32)
+    xsize = np.zeros(nx, np.uint32)
+    lib().orc_transfer(Y, ny, X, nx, ncol, int(ntip), ymin, ysize, xmin, xsize)
+    return ymin, ysize, xmin, xsize

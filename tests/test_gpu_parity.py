@@ -1,0 +1,213 @@
+"""GPU parity: the HIP library through its C ABI vs the CPU oracle, bit-exact.
+
+Sizes are chosen so the oracle finishes in seconds; full-size properties are
+in test_gpu_fullsize.py.
+"""
+import numpy as np
+import pytest
+
+from casphy_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_bits(rng, n, W, p):
+    return synth.pack_bits(rng.random((n, 32 * W)) < p)
+
+
+# ------------------------------------------------------------------ scores
+@pytest.mark.parametrize("n,W", [(1, 1), (5, 1), (127, 2), (128, 4), (129, 3), (300, 5), (515, 32)])
+def test_score_bit_matrix(hip, orc, n, W):
+    rng = np.random.default_rng(n * 7 + W)
+    mat = rand_bits(rng, n, W, 0.15)
+    S = hip.score_bit_matrix(mat)
+    assert np.array_equal(S, orc.score_bit_matrix(mat).astype(np.int32))
+
+
+def test_score_bit_weighted(hip, orc):
+    rng = np.random.default_rng(1)
+    mat = rand_bits(rng, 200, 4, 0.2)
+    for w in (rng.integers(0, 9, 128), np.full(128, 3), np.zeros(128), rng.integers(0, 1000, 128)):
+        S = hip.score_bit_matrix(mat, w.astype(np.float64))
+        assert np.array_equal(S, orc.score_bit_matrix(mat, w.astype(np.float32)).astype(np.int32))
+
+
+def test_weights_outside_contract_rejected(hip):
+    mat = np.ones((4, 1), np.uint32)
+    for w in (np.full(32, 0.5), np.full(32, -1.0), np.full(32, 1e6)):
+        with pytest.raises(hip.CasphyError) as e:
+            hip.score_bit_matrix(mat, w)
+        assert e.value.code == -4
+
+
+def test_score_indel_matrix(hip, orc):
+    mat = synth.indel_matrix(seed=3, depth=7, nsite=60)
+    w = np.ones((mat.max(), 60), dtype=np.float32)
+    n = mat.shape[0]
+    S = hip.score_indel_matrix(mat)
+    ref = np.array([[orc.score_indel(mat[i], mat[j], w) for j in range(n)] for i in range(n)], np.int32)
+    assert np.array_equal(S, ref)
+    rng = np.random.default_rng(4)
+    wi = rng.integers(0, 7, w.shape)
+    S2 = hip.score_indel_matrix(mat, wi.astype(np.float64))
+    i, j = 5, 77
+    assert S2[i, j] == orc.score_indel(mat[i], mat[j], wi.astype(np.float32))
+    assert np.array_equal(S2, S2.T)
+
+
+# ------------------------------------------------------------------ radius
+@pytest.mark.parametrize("n,W,p,r", [(1, 1, 0.3, 1), (33, 1, 0.2, 2), (250, 4, 0.05, 3), (700, 8, 0.03, 5),
+                                     (1000, 2, 0.1, 0), (257, 3, 0.1, 1000)])
+def test_radius_bit(hip, orc, n, W, p, r):
+    rng = np.random.default_rng(n + W)
+    mat = rand_bits(rng, n, W, p)
+    counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), r)
+    c2, i2 = orc.radius_bit(mat, r)
+    assert np.array_equal(counts, c2)
+    assert np.array_equal(idx, i2)
+
+
+def test_radius_bit_weighted(hip, orc):
+    rng = np.random.default_rng(9)
+    mat = rand_bits(rng, 300, 2, 0.1)
+    w = rng.integers(0, 5, 64).astype(np.float64)
+    counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), 7, w)
+    c2, i2 = orc.radius_bit(mat, 7, w.astype(np.float32))
+    assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+def test_radius_bit_synthetic_config(hip, orc):
+    mat = synth.binary_matrix(seed=2, n=1500, nsite=1024, p=0.005)
+    for r in (1, 4, 16):
+        counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), r)
+        c2, i2 = orc.radius_bit(mat, r)
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+@pytest.mark.parametrize("n,ncol", [(2, 3), (100, 90), (600, 1000)])
+def test_radius_dna_2bit(hip, orc, n, ncol):
+    mat = synth.dna_matrix(seed=n, n=n, ncol=ncol)
+    for r in (1, ncol // 2, int(ncol * 0.9)):
+        counts, idx = hip.radius_neighbor_dna(mat, r)
+        c2, i2 = orc.radius_dna(mat, r)
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+def test_radius_dna_iupac_nibble_path(hip, orc):
+    rng = np.random.default_rng(10)
+    codes = rng.choice(np.array([1, 2, 4, 8, 3, 15, 0x10, 5], np.uint8), (300, 203))
+    mat = synth.pack_bytes4(codes)
+    for r in (1, 60, 120):
+        counts, idx = hip.radius_neighbor_dna(mat, r)
+        c2, i2 = orc.radius_dna(mat, r)
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+def test_try_radius(hip, orc):
+    mat = synth.binary_matrix(seed=4, n=900, nsite=512, p=0.01)
+    bmat = np.ascontiguousarray(mat.T)
+    q = synth.query_rows(5, 900, 300)
+    radii = np.arange(1, 17)
+    got = hip.try_radii(bmat, radii, q)
+    for k, r in enumerate(radii):
+        assert np.array_equal(got[k], orc.try_radius_bit(mat, r, q)), r
+    assert np.array_equal(hip.try_radius(bmat, 3, q), orc.try_radius_bit(mat, 3, q))
+
+
+def test_try_radius_dna(hip, orc):
+    mat = synth.dna_matrix(seed=11, n=400, ncol=300)
+    q = synth.query_rows(12, 400, 130)
+    radii = [100, 200, 250, 280, 299]
+    got = hip.try_radii_dna(mat, radii, q)
+    for k, r in enumerate(radii):
+        assert np.array_equal(got[k], orc.try_radius_dna(mat, r, q))
+
+
+# ------------------------------------------------------------------ joins
+@pytest.mark.parametrize("n,W,p", [(2, 1, 0.3), (3, 1, 0.0), (17, 1, 0.2), (130, 2, 0.1), (400, 8, 0.02)])
+@pytest.mark.parametrize("anc", [0, 1])
+def test_join_bit(hip, orc, n, W, p, anc):
+    rng = np.random.default_rng(n * 3 + anc)
+    mat = rand_bits(rng, n, W, p)
+    e, ns, nb = hip.join_bit(mat, None, anc)
+    e2, ns2, nb2 = orc.join_bit(mat, ancestral_type=anc)
+    assert np.array_equal(e, e2)
+    assert np.array_equal(ns, ns2) and np.array_equal(nb, nb2)
+
+
+def test_join_bit_synthetic(hip, orc):
+    mat = synth.binary_matrix(seed=21, n=1200, nsite=512, p=0.01)
+    e, ns, _ = hip.join_bit(mat)
+    e2, ns2, _ = orc.join_bit(mat, lazy=True)
+    assert np.array_equal(e, e2) and np.array_equal(ns, ns2)
+
+
+def test_join_bit_weighted(hip, orc):
+    rng = np.random.default_rng(22)
+    mat = rand_bits(rng, 150, 2, 0.1)
+    w = rng.integers(0, 6, 64).astype(np.float64)
+    e, _, _ = hip.join_bit(mat, w, 1)
+    e2, _, _ = orc.join_bit(mat, w.astype(np.float32), 1)
+    assert np.array_equal(e, e2)
+
+
+@pytest.mark.parametrize("min_size,ratio", [(0, 2.0), (4, 1.5)])
+def test_join_indel(hip, orc, min_size, ratio):
+    mat = synth.indel_matrix(seed=31, depth=8, nsite=120)
+    n = mat.shape[0]
+    w = np.ones((mat.max(), 120))
+    ts = np.random.default_rng(5).integers(1, 6, n).astype(np.uint32)
+    e, ns, br = hip.join_indel(mat, w, ts, min_size, ratio)
+    e2, ns2, br2 = orc.join_indel(mat, w.astype(np.float32), ts, min_size, ratio)
+    assert np.array_equal(e, e2) and np.array_equal(ns, ns2) and np.array_equal(br, br2)
+
+
+def test_join_indel_config1(hip, orc):
+    mat = synth.indel_matrix(seed=1)
+    e, ns, br = hip.join_indel(mat)
+    w = np.ones((mat.max(), mat.shape[1]), np.float32)
+    e2, ns2, br2 = orc.join_indel(mat, w)
+    assert np.array_equal(e, e2) and np.array_equal(ns, ns2) and np.array_equal(br, br2)
+
+
+def test_cladewise(hip, orc):
+    mat = synth.binary_matrix(seed=7, n=500, nsite=128, p=0.02)
+    e, _, _ = orc.join_bit(mat)
+    eo, io = hip.cladewise(e, 500)
+    eo2, io2 = orc.cladewise(e, 500)
+    assert np.array_equal(eo, eo2) and np.array_equal(io, io2)
+
+
+def test_join_batch_equals_single(hip):
+    mats = [synth.binary_matrix(seed=40 + k, n=int(nk), nsite=256, p=0.02) for k, nk in enumerate([2, 57, 300, 129])]
+    batch = hip.join_bit_batch(mats)
+    for m, e in zip(mats, batch):
+        e1, _, _ = hip.join_bit(m)
+        assert np.array_equal(e, e1)
+
+
+# ------------------------------------------------------------------ transfer
+@pytest.mark.parametrize("ntip,ny,nx", [(64, 62, 62), (75, 20, 300), (1000, 130, 129), (32, 1, 1)])
+def test_transfer(hip, orc, ntip, ny, nx):
+    rng = np.random.default_rng(ntip + ny)
+    W = (ntip + 31) // 32
+
+    def bits(m):
+        b = rng.random((m, ntip)) < rng.random()
+        return synth.pack_bits(np.pad(b, ((0, 0), (0, 32 * W - ntip))))
+    Y, X = bits(ny), bits(nx)
+    got = hip.transfer_dis(Y, X, ntip)
+    want = orc.transfer(Y, X, ntip)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_transfer_trees(hip, orc):
+    rng = np.random.default_rng(3)
+    parent, leaves = synth.yule_parent(700, rng)
+    ref = synth.clade_bitsets(parent, leaves, 700)
+    boot = synth.clade_bitsets(synth.spr(parent, leaves, 20, rng), leaves, 700)
+    got = hip.transfer_dis(ref, boot, 700)
+    want = orc.transfer(ref, boot, 700)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)

@@ -1,0 +1,263 @@
+"""Pins the CPU oracle (oracle/casphy_oracle.c) against independent checks.
+
+CPU only.  Each oracle function is compared with a restatement that works
+differently (closed forms on unpacked bits, IUPAC sets on text, O(n^3)
+lexicographic joins on 3-valued states, R's reOrderEdge2 + factor relabel,
+numpy Hamming), plus invariants and special cases.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from casphy_amd import synth
+from tests import brute
+
+
+def rand_bits(rng, n, W, p=0.1):
+    return synth.pack_bits(rng.random((n, 32 * W)) < p)
+
+
+# ------------------------------------------------------------------ scores
+def test_score_bit_closed_form(orc):
+    rng = np.random.default_rng(0)
+    mat = rand_bits(rng, 40, 3, 0.2)
+    S = orc.score_bit_matrix(mat)
+    ref = brute.score_matrix_bits(mat)
+    assert np.array_equal(S.astype(np.int64), ref)
+    assert np.array_equal(S, S.T)
+    assert np.array_equal(np.diag(S), synth.unpack_bits(mat).sum(1))
+
+
+def test_score_bit_integer_weights(orc):
+    rng = np.random.default_rng(1)
+    mat = rand_bits(rng, 24, 2, 0.3)
+    w = rng.integers(0, 9, 64).astype(np.float32)
+    S = orc.score_bit_matrix(mat, w)
+    assert np.array_equal(S.astype(np.int64), brute.score_matrix_bits(mat, w.astype(np.int64)))
+
+
+def test_score_bit_zero_rows(orc):
+    mat = np.zeros((3, 2), np.uint32)
+    mat[1, 0] = 0xFFFFFFFF
+    S = orc.score_bit_matrix(mat)
+    assert S[0].sum() == 0 and S[1, 1] == 32 and S[1, 2] == 0
+
+
+def test_score_indel_onehot(orc):
+    mat = synth.indel_matrix(seed=5, depth=4, nsite=30, p=0.1, nallele=6)
+    w = np.ones((mat.max(), mat.shape[1]), dtype=np.float32)
+    ref = brute.score_matrix_indel(mat)
+    for i in range(mat.shape[0]):
+        for j in range(mat.shape[0]):
+            assert orc.score_indel(mat[i], mat[j], w) == ref[i, j]
+    rng = np.random.default_rng(2)
+    wi = rng.integers(0, 5, w.shape).astype(np.float32)
+    refw = brute.score_matrix_indel(mat, wi.astype(np.int64))
+    assert orc.score_indel(mat[3], mat[7], wi) == refw[3, 7]
+
+
+def test_score_dna_iupac_text(orc):
+    rng = np.random.default_rng(3)
+    alphabet = "ATCGUWMRYKSHDVBN-.X"
+    code = {c: 0x10 for c in alphabet}
+    code.update({"A": 1, "T": 2, "U": 2, "C": 4, "G": 8, "W": 3, "M": 5, "R": 9, "Y": 6, "K": 10, "S": 12,
+                 "H": 7, "D": 11, "V": 13, "B": 14, "N": 15})
+    for L in (1, 7, 33, 64):
+        s1 = "".join(rng.choice(list(alphabet), L))
+        s2 = "".join(rng.choice(list(alphabet), L))
+        enc = synth.pack_bytes4(np.array([[code[c] for c in s1], [code[c] for c in s2]], dtype=np.uint8))
+        assert orc.score_dna(enc[0], enc[1]) == brute.dna_similarity_text(s1, s2)
+
+
+# ------------------------------------------------------------------ radius
+@pytest.mark.parametrize("r", [0, 1, 3, 6, 200])
+def test_radius_bit_brute(orc, r):
+    rng = np.random.default_rng(4)
+    mat = rand_bits(rng, 70, 2, 0.08)
+    counts, idx = orc.radius_bit(mat, r)
+    S = brute.score_matrix_bits(mat)
+    c2, i2 = brute.radius_lists_from_matrix(S, r)
+    assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+def test_radius_monotone_and_self(orc):
+    rng = np.random.default_rng(5)
+    mat = rand_bits(rng, 50, 1, 0.2)
+    prev = None
+    pop = synth.unpack_bits(mat).sum(1)
+    for r in range(0, 12):
+        counts, idx = orc.radius_bit(mat, r)
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        for i in range(50):
+            li = idx[off[i]:off[i + 1]]
+            assert np.all(np.diff(li.astype(np.int64)) > 0)
+            assert (i in li) == (pop[i] >= r)
+        if prev is not None:
+            assert np.all(counts <= prev)
+        prev = counts
+
+
+def test_radius_dna_and_try(orc):
+    mat = synth.dna_matrix(seed=6, n=60, ncol=90)
+    S = np.array([[orc.score_dna(mat[i], mat[j]) for j in range(60)] for i in range(60)])
+    assert np.array_equal(S, S.T)
+    for r in (40, 70, 85):
+        counts, idx = orc.radius_dna(mat, r)
+        c2, i2 = brute.radius_lists_from_matrix(S, r)
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+        q = np.array([0, 5, 5, 59], np.uint32)
+        assert np.array_equal(orc.try_radius_dna(mat, r, q), c2[q])
+
+
+def test_try_radius_bit(orc):
+    rng = np.random.default_rng(7)
+    mat = rand_bits(rng, 80, 2, 0.1)
+    S = brute.score_matrix_bits(mat)
+    q = rng.integers(0, 80, 33).astype(np.uint32)
+    for r in (1, 4, 9):
+        assert np.array_equal(orc.try_radius_bit(mat, r, q), (S[q] >= r).sum(1))
+
+
+# ------------------------------------------------------------------ ancestral rules
+@pytest.mark.parametrize("anc", [0, 1])
+def test_ancestral_rule_all_pairs_via_join(orc, anc):
+    """4 tips built so that the joins are ((0,1),(2,3)); site values at the
+    two internal nodes cover {0,1,?}^2 and the root state is checked."""
+    enc = brute.ENCODE[anc]
+    vals = [0, 1, brute.Q]
+    tips_for = {0: (0, 0), 1: (1, 1), brute.Q: (1, 0)}
+    sites = list(itertools.product(vals, vals))
+    W = 1
+    mat = np.zeros((4, W), np.uint32)
+    for k, (u, v) in enumerate(sites):
+        a, b = tips_for[u]
+        c, d = tips_for[v]
+        for t, bit in zip(range(4), (a, b, c, d)):
+            mat[t, 0] |= np.uint32(bit) << np.uint32(k)
+    # anchor bits: 20 bits shared by tips 0,1 and 14 by tips 2,3 make the join
+    # order (0,1), (2,3), root regardless of the test sites
+    for k in range(10, 30):
+        mat[0, 0] |= np.uint32(1) << np.uint32(k)
+        mat[1, 0] |= np.uint32(1) << np.uint32(k)
+    for k in range(30, 32):
+        mat[2, 0] |= np.uint32(1) << np.uint32(k)
+        mat[3, 0] |= np.uint32(1) << np.uint32(k)
+    edge, ns, nb = orc.join_bit(mat, ancestral_type=anc)
+    assert [tuple(e) for e in edge[:2]] == [(4, 0), (4, 1)]
+    assert [tuple(e) for e in edge[2:4]] == [(5, 2), (5, 3)]
+    for k, (u, v) in enumerate(sites):
+        for node, val in ((4, u), (5, v)):
+            assert ((ns[node, 0] >> k) & 1) == (1 if val == 1 else 0)
+            assert ((nb[node, 0] >> k) & 1) == enc[val][1]
+        want = brute.RULE[anc][(u, v)]
+        assert ((ns[6, 0] >> k) & 1) == (1 if want == 1 else 0), (u, v)
+        assert ((nb[6, 0] >> k) & 1) == enc[want][1], (u, v)
+
+
+# ------------------------------------------------------------------ joins
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("anc", [0, 1])
+def test_join_bit_vs_lexicographic(orc, seed, anc):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2, 24))
+    mat = rand_bits(rng, n, 1, float(rng.choice([0.05, 0.2, 0.5])))
+    edges, state = brute.bin_join(mat, anc)
+    e1, ns1, _ = orc.join_bit(mat, ancestral_type=anc)
+    e2, ns2, _ = orc.join_bit(mat, ancestral_type=anc, lazy=True)
+    assert np.array_equal(e1, edges)
+    assert np.array_equal(e2, edges)
+    assert np.array_equal(synth.unpack_bits(ns1), state)
+    assert np.array_equal(ns1, ns2)
+
+
+def test_join_bit_weighted_vs_lexicographic(orc):
+    rng = np.random.default_rng(11)
+    mat = rand_bits(rng, 14, 1, 0.3)
+    w = rng.integers(0, 4, 32)
+    edges, _ = brute.bin_join(mat, 1, w)
+    e1, _, _ = orc.join_bit(mat, w.astype(np.float32), 1)
+    e2, _, _ = orc.join_bit(mat, w.astype(np.float32), 1, lazy=True)
+    assert np.array_equal(e1, edges) and np.array_equal(e2, edges)
+
+
+def test_join_bit_invariants(orc):
+    mat = synth.binary_matrix(seed=12, n=300, nsite=256, p=0.02)
+    e1, ns1, _ = orc.join_bit(mat)
+    e2, ns2, _ = orc.join_bit(mat, lazy=True)
+    n = 300
+    assert np.array_equal(e1, e2) and np.array_equal(ns1, ns2)
+    child = e1[:, 1]
+    assert sorted(child.tolist()) == list(range(2 * n - 2))
+    assert np.all(e1[0::2, 0] == np.arange(n, 2 * n - 1)) and np.all(e1[1::2, 0] == e1[0::2, 0])
+    assert np.all(e1[0::2, 1] < e1[1::2, 1])  # first child is the smaller id
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_join_indel_vs_lexicographic(orc, seed):
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(2, 20))
+    mat = rng.integers(-1, 4, (n, 12)).astype(np.int32)
+    w = np.ones((max(1, mat.max()), 12))
+    ts = rng.integers(1, 5, n).astype(np.uint32)
+    for min_size, ratio in ((0, 2.0), (3, 1.5)):
+        edges, states = brute.indel_join(mat, None, ts, min_size, ratio)
+        e1, ns1, br1 = orc.join_indel(mat, w, ts, min_size, ratio)
+        e2, ns2, br2 = orc.join_indel(mat, w, ts, min_size, ratio, lazy=True)
+        assert np.array_equal(e1, edges) and np.array_equal(e2, edges)
+        assert np.array_equal(ns1, states) and np.array_equal(ns2, states)
+        br = [int(((states[p] == 0) & (states[c] > 0)).sum()) for p, c in edges]
+        assert np.array_equal(br1, br)
+
+
+def test_join_indel_weighted(orc):
+    rng = np.random.default_rng(13)
+    mat = rng.integers(-1, 3, (12, 10)).astype(np.int32)
+    w = rng.integers(0, 4, (2, 10))
+    edges, _ = brute.indel_join(mat, w)
+    e1, _, _ = orc.join_indel(mat, w.astype(np.float32))
+    assert np.array_equal(e1, edges)
+
+
+# ------------------------------------------------------------------ emission
+@pytest.mark.parametrize("seed", range(4))
+def test_cladewise_vs_R(orc, seed):
+    rng = np.random.default_rng(300 + seed)
+    n = int(rng.integers(2, 60))
+    mat = rand_bits(rng, n, 1, 0.2)
+    edge, _, _ = orc.join_bit(mat)
+    eo, io = orc.cladewise(edge, n)
+    e2, io2 = brute.cladewise_R(edge, n)
+    assert np.array_equal(eo, e2) and np.array_equal(io, io2)
+    assert eo[0, 0] == n and io[0] == 2 * n - 2
+    # preorder: every internal parent appears (as a child) before its own edges
+    seen = {n}
+    for p, c in eo:
+        assert p in seen
+        seen.add(int(c))
+
+
+# ------------------------------------------------------------------ transfer
+def test_transfer_brute(orc):
+    rng = np.random.default_rng(14)
+    ntip = 75
+    W = 3
+    def bits(m):
+        b = rng.random((m, ntip)) < 0.4
+        return synth.pack_bits(np.pad(b, ((0, 0), (0, 32 * W - ntip))))
+    Y, X = bits(20), bits(33)
+    got = orc.transfer(Y, X, ntip)
+    want = brute.transfer_brute(Y, X, ntip)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_transfer_identical_trees_and_complement(orc):
+    rng = np.random.default_rng(15)
+    parent, leaves = synth.yule_parent(40, rng)
+    B = synth.clade_bitsets(parent, leaves, 40)
+    ymin, ysize, xmin, xsize = orc.transfer(B, B, 40)
+    assert np.all(ymin == 0) and np.all(xmin == 0)
+    comp = synth.pack_bits(~synth.unpack_bits(B, 40))
+    ymin2, ysize2, _, _ = orc.transfer(B, comp, 40)
+    assert np.all(ymin2 == 0) and np.array_equal(ysize2, ysize)
