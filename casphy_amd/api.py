@@ -30,7 +30,7 @@ __all__ = [
     "CasphyError", "score_bit_matrix", "score_indel_matrix",
     "radius_neighbor", "radius_neighbor_dna", "try_radius", "try_radius_dna",
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
-    "join_bit_batch", "transfer_dis", "join_edge_bit_reverse", "join_edge_indel",
+    "join_bit_batch", "join_indel_batch", "transfer_dis", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
     "radiusWeightedDNAParallel",
 ]
@@ -190,6 +190,27 @@ def join_bit_batch(mats, weight=None, ancestral_type: int = 1):
     edges = np.empty((int(sum(2 * s - 2 for s in sizes)), 2), dtype=np.uint32)
     check(load().casphy_join_bit_batch(_p(allm), _p(offsets), len(mats), W, _p(w), int(ancestral_type),
                                        _p(edges)))
+    out, pos = [], 0
+    for s in sizes:
+        out.append(edges[pos:pos + 2 * s - 2])
+        pos += 2 * s - 2
+    return out
+
+
+def join_indel_batch(mats, weight=None, tip_sizes=None, min_size: int = 0, min_ratio: float = 2.0):
+    """Independent indel joins (same site count) in one persistent launch."""
+    mats = [_i32(m) for m in mats]
+    ncol = mats[0].shape[1]
+    sizes = [m.shape[0] for m in mats]
+    offsets = np.zeros(len(mats) + 1, dtype=np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    allm = np.ascontiguousarray(np.concatenate(mats, axis=0))
+    w = _f64(weight)
+    nstate = 0 if w is None else w.shape[0]
+    ts = None if tip_sizes is None else _u32(np.concatenate(tip_sizes))
+    edges = np.empty((int(sum(2 * s - 2 for s in sizes)), 2), dtype=np.uint32)
+    check(load().casphy_join_indel_batch(_p(allm), _p(offsets), len(mats), ncol, _p(w), nstate, _p(ts),
+                                         int(min_size), float(min_ratio), _p(edges)))
     out, pos = [], 0
     for s in sizes:
         out.append(edges[pos:pos + 2 * s - 2])

@@ -334,6 +334,209 @@ __global__ void k_indel_decode(const u32 *rows, u32 count, u32 ldw, const u32 *l
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched form: one persistent workgroup per tree, jobs taken from an atomic
+// work queue, the whole join (initial local maxima, argmax, stale recompute,
+// ancestral merge, rescoring) inside the workgroup with LDS broadcasts and
+// barriers only.  Job k owns rows [off_k, off_k + n_k) of every per-slot array;
+// its node arrays start at 2*off_k - k, its edges at 2*(2*off_k - 2*k), its
+// internal states at (off_k - k) rows.
+struct BatchArgs {
+    u32 *rows;
+    int32_t *lmax;
+    u32 *lidx, *slot2node, *node2slot, *size, *edge, *nstate;
+    const u32 *offsets;
+    const u32 *tip_size;  // sum n_k or nullptr
+    u32 njobs, ldw;
+    int rule;
+    u32 min_size;
+    float min_ratio;
+    u32 *next_job;
+    u32 *status;          // per job: 1 ok, 2 no live pair
+    u32 nplanes;
+    const u32 *pmask;
+    u32 pshift[MAX_PLANES];
+    u32 ptail;
+};
+
+// Score of rows q and r (ldw words each, plane 0), by 16 cooperating lanes;
+// every lane of the group returns the total.
+__device__ __forceinline__ u32 group16_score(const BatchArgs &b, const u32 *q, const u32 *r, u32 l16)
+{
+    const u32 np = b.nplanes ? b.nplanes : 1;
+    u32 S = 0;
+    for (u32 p = 0; p < np; ++p) {
+        if (p) S <<= b.pshift[p];
+        u32 part = 0;
+        for (u32 w = 4 * l16; w < b.ldw; w += 64) {
+            uint4 x = *reinterpret_cast<const uint4 *>(q + w);
+            const uint4 y = *reinterpret_cast<const uint4 *>(r + w);
+            if (b.nplanes) {
+                const uint4 m = *reinterpret_cast<const uint4 *>(b.pmask + (size_t)p * b.ldw + w);
+                x.x &= m.x; x.y &= m.y; x.z &= m.z; x.w &= m.w;
+            }
+            part += __builtin_popcount(x.x & y.x) + __builtin_popcount(x.y & y.y) +
+                    __builtin_popcount(x.z & y.z) + __builtin_popcount(x.w & y.w);
+        }
+        for (int o = 1; o < 16; o <<= 1) part += __shfl_xor(part, o);
+        S += part;
+    }
+    if (b.nplanes) S <<= b.ptail;
+    return S;
+}
+
+// Workgroup-wide max of a u64 key (1024 threads).
+__device__ __forceinline__ unsigned long long wg_max(unsigned long long k, unsigned long long *red)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(k, o);
+        k = x > k ? x : k;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = k;
+    __syncthreads();
+    unsigned long long m = red[0];
+    for (int w = 1; w < 16; ++w) m = red[w] > m ? red[w] : m;
+    return m;
+}
+
+__global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
+{
+    __shared__ unsigned long long red[16];
+    __shared__ u32 sh[8];
+    const u32 tid = threadIdx.x, l16 = tid & 15, grp = tid >> 4;  // 64 groups of 16 lanes
+    const u32 ldw = b.ldw;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) sh[0] = atomicAdd(b.next_job, 1u);
+        __syncthreads();
+        const u32 job = sh[0];
+        if (job >= b.njobs) return;
+        const u32 off = b.offsets[job], n = b.offsets[job + 1] - off, nnode = 2 * n - 1;
+        u32 *rows = b.rows + (size_t)off * 2 * ldw;
+        int32_t *lmax = b.lmax + off;
+        u32 *lidx = b.lidx + off, *s2n = b.slot2node + off;
+        u32 *n2s = b.node2slot + (2 * (size_t)off - job);
+        u32 *size = b.size + (2 * (size_t)off - job);
+        u32 *edge = b.edge + 2 * (2 * (size_t)off - 2 * (size_t)job);
+        u32 *nst = b.nstate + ((size_t)off - job) * 2 * ldw;
+
+        // maps and sizes
+        for (u32 i = tid; i < nnode; i += 1024) {
+            if (i < n) {
+                s2n[i] = i;
+                n2s[i] = i;
+                size[i] = b.tip_size ? b.tip_size[off + i] : 1u;
+            } else {
+                n2s[i] = n;
+            }
+        }
+        // initial local maxima: group g scores rows i = g, g+64, ... against all j > i
+        for (u32 i = grp; i < n; i += 64) {
+            unsigned long long best = 0;
+            const u32 *qi = rows + (size_t)i * 2 * ldw;
+            for (u32 j = i + 1; j < n; ++j) {
+                const u32 S = group16_score(b, qi, rows + (size_t)j * 2 * ldw, l16);
+                const unsigned long long key = ((unsigned long long)S << 32) | (0xFFFFFFFFu - j);
+                best = key > best ? key : best;
+            }
+            if (l16 == 0) {
+                lmax[i] = best ? (int32_t)(best >> 32) : -1;
+                lidx[i] = best ? 0xFFFFFFFFu - (u32)best : 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
+        u32 nodeCount = n;
+        u32 status = 1;
+        while (nodeCount < nnode) {
+            // argmax of (lmax, -node id) from (-1, 0)
+            unsigned long long best = 0xFFFFFFFFull;
+            for (u32 s = tid; s < n; s += 1024) {
+                const unsigned long long k =
+                    ((unsigned long long)(u32)(lmax[s] + 1) << 32) | (0xFFFFFFFFu - s2n[s]);
+                best = k > best ? k : best;
+            }
+            best = wg_max(best, red);
+            if (tid == 0) {
+                const u32 j1 = 0xFFFFFFFFu - (u32)best;
+                const int32_t v = (int32_t)(u32)(best >> 32) - 1;
+                const u32 c1 = j1 < nnode ? n2s[j1] : n;
+                u32 verdict = 0, j2 = 0xFFFFFFFFu, c2 = n;
+                if (v >= 0 && c1 < n) {
+                    j2 = lidx[c1];
+                    c2 = j2 < nnode ? n2s[j2] : n;
+                    verdict = c2 < n ? 2 : 1;
+                }
+                sh[1] = verdict; sh[2] = c1; sh[3] = c2; sh[4] = j1; sh[5] = j2;
+            }
+            __syncthreads();
+            const u32 verdict = sh[1], c1 = sh[2], c2 = sh[3], j1 = sh[4], j2 = sh[5];
+            if (verdict == 0) { status = 2; break; }
+            const u32 *q = rows + (size_t)c1 * 2 * ldw;
+            if (verdict == 1) {
+                // recompute row c1 over every live slot
+                unsigned long long kb = 0;
+                for (u32 k = grp; k < n; k += 64) {
+                    const u32 id = s2n[k];
+                    if (k == c1 || id >= nnode) continue;
+                    const u32 S = group16_score(b, q, rows + (size_t)k * 2 * ldw, l16);
+                    const unsigned long long key = ((unsigned long long)S << 32) | (0xFFFFFFFFu - id);
+                    kb = key > kb ? key : kb;
+                }
+                kb = wg_max(kb, red);
+                if (tid == 0) {
+                    lmax[c1] = kb ? (int32_t)(kb >> 32) : -1;
+                    lidx[c1] = kb ? 0xFFFFFFFFu - (u32)kb : 0xFFFFFFFFu;
+                }
+                __syncthreads();
+                continue;
+            }
+            // join: size rule, merge into slot c1, record
+            u32 choice = 0;
+            if (b.rule == 2) {
+                const u32 sa = size[j1], sb = size[j2];
+                if (sa < b.min_size && (float)sa * b.min_ratio < (float)sb) choice = 1;
+                else if (sb < b.min_size && (float)sb * b.min_ratio < (float)sa) choice = 2;
+            }
+            u32 *ra = rows + (size_t)c1 * 2 * ldw;
+            const u32 *rb = rows + (size_t)c2 * 2 * ldw;
+            u32 *out = nst + (size_t)(nodeCount - n) * 2 * ldw;
+            for (u32 w = tid; w < ldw; w += 1024) {
+                const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
+                u32 r0, r1;
+                if (choice == 1) { r0 = b0; r1 = b1; }
+                else if (choice == 2) { r0 = a0; r1 = a1; }
+                else merge_words(b.rule, a0, a1, b0, b1, r0, r1);
+                ra[w] = r0; ra[ldw + w] = r1;
+                out[w] = r0; out[ldw + w] = r1;
+            }
+            if (tid == 0) {
+                const u32 k = nodeCount - n;
+                edge[4 * k + 0] = nodeCount; edge[4 * k + 1] = j1;
+                edge[4 * k + 2] = nodeCount; edge[4 * k + 3] = j2;
+                n2s[j1] = n; n2s[j2] = n; n2s[nodeCount] = c1;
+                s2n[c1] = nodeCount; s2n[c2] = nnode;
+                lmax[c1] = -1; lmax[c2] = -1;
+                size[nodeCount] = size[j1] + size[j2];
+            }
+            __syncthreads();
+            // rescore the new node against every live slot (strict '>')
+            for (u32 k = grp; k < n; k += 64) {
+                if (k == c1 || s2n[k] >= nnode) continue;
+                const u32 S = group16_score(b, q, rows + (size_t)k * 2 * ldw, l16);
+                if (l16 == 0 && (int32_t)S > lmax[k]) {
+                    lmax[k] = (int32_t)S;
+                    lidx[k] = nodeCount;
+                }
+            }
+            ++nodeCount;
+            __syncthreads();
+        }
+        if (tid == 0) b.status[job] = status;
+    }
+}
+
 u32 grid_for(u64 items) { return (u32)std::min<u64>((items + 255) / 256, 65535ull * 8); }
 
 // Host-side layout of one-hot indel encoding.
@@ -495,6 +698,47 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     CASPHY_REQUIRE(!h.err, CASPHY_EINTERNAL, "join found no live pair");
 }
 
+
+// Runs the persistent batch kernel on encoded rows (sum n_k slots, 2 planes).
+void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, const Planes &planes,
+               const u32 *dtip, u32 min_size, float min_ratio, u32 *edge_host, hipStream_t s)
+{
+    const u32 njobs = (u32)offs.size() - 1;
+    const u64 total = offs.back();
+    const u64 nodes = 2 * total - njobs, edges = 2 * total - 2 * (u64)njobs, internal = total - njobs;
+    DevBuf lmax(total * 4), lidx(total * 4), s2n(total * 4), n2s(nodes * 4), size(nodes * 4),
+        dedge(std::max<u64>(edges, 1) * 8), nst(std::max<u64>(internal, 1) * 2 * ldw * 4),
+        doff(offs.size() * 4), ctr(4), status((size_t)njobs * 4), pm;
+    h2d(doff.p, offs.data(), offs.size() * 4, s);
+    launch_fill_u32(ctr.as<u32>(), 1, 0, s);
+    launch_fill_u32(status.as<u32>(), njobs, 0, s);
+    BatchArgs b{};
+    b.rows = rows.as<u32>(); b.lmax = lmax.as<int32_t>(); b.lidx = lidx.as<u32>();
+    b.slot2node = s2n.as<u32>(); b.node2slot = n2s.as<u32>(); b.size = size.as<u32>();
+    b.edge = dedge.as<u32>(); b.nstate = nst.as<u32>(); b.offsets = doff.as<u32>();
+    b.tip_size = dtip; b.njobs = njobs; b.ldw = ldw; b.rule = rule;
+    b.min_size = min_size; b.min_ratio = min_ratio; b.next_job = ctr.as<u32>(); b.status = status.as<u32>();
+    b.nplanes = planes.nplanes;
+    if (planes.nplanes) {
+        pm.alloc(planes.mask.size() * 4);
+        h2d(pm.p, planes.mask.data(), planes.mask.size() * 4, s);
+        b.pmask = pm.as<u32>();
+        for (u32 q = 0; q < planes.nplanes; ++q) b.pshift[q] = planes.shift[q];
+        b.ptail = planes.tail;
+    }
+    int dev = 0, cus = 0;
+    CASPHY_HIP(hipGetDevice(&dev));
+    CASPHY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const u32 grid = std::min<u32>(njobs, (u32)std::max(cus, 1) * 2);
+    hipLaunchKernelGGL(k_join_batch, dim3(grid), dim3(1024), 0, s, b);
+    CASPHY_HIP(hipGetLastError());
+    std::vector<u32> st(njobs);
+    d2h(st.data(), status.p, (size_t)njobs * 4, s);
+    for (u32 k = 0; k < njobs; ++k)
+        CASPHY_REQUIRE(st[k] == 1, CASPHY_EINTERNAL, "batched join " + std::to_string(k) + " did not complete");
+    d2h(edge_host, dedge.p, edges * 8, s);
+}
+
 void require(bool c, const char *msg) { CASPHY_REQUIRE(c, CASPHY_EARG, msg); }
 
 }  // namespace
@@ -621,15 +865,53 @@ int casphy_join_bit_batch(const uint32_t *mat, const uint32_t *offsets, uint32_t
     return guarded([&] {
         require(mat && offsets && edge && W > 0, "casphy_join_bit_batch: bad arguments");
         require(ancestral_type == 0 || ancestral_type == 1, "casphy_join_bit_batch: ancestralType must be 0 or 1");
-        size_t eo = 0;
-        for (u32 k = 0; k < njobs; ++k) {
-            const u32 nk = offsets[k + 1] - offsets[k];
-            require(offsets[k + 1] >= offsets[k] && nk >= 2, "casphy_join_bit_batch: every job needs >= 2 cells");
-            int rc = casphy_join_bit(mat + (size_t)offsets[k] * W, nk, W, weight, ancestral_type, edge + eo, nullptr,
-                                     nullptr);
-            if (rc) throw Error(rc, casphy_last_error());
-            eo += (size_t)(2 * nk - 2) * 2;
+        if (njobs == 0) return;
+        std::vector<u32> offs(offsets, offsets + njobs + 1);
+        for (u32 k = 0; k < njobs; ++k)
+            require(offs[k + 1] >= offs[k] + 2, "casphy_join_bit_batch: every job needs >= 2 cells");
+        require(offs[0] == 0, "casphy_join_bit_batch: offsets[0] must be 0");
+        hipStream_t s = thread_stream();
+        const u32 ldw = round_up(W, KC_BIT);
+        const u32 total = offs.back();
+        Planes planes = make_planes(weight, 32 * W, ldw);
+        DevBuf raw((size_t)total * W * 4), rows((size_t)total * 2 * ldw * 4);
+        h2d(raw.p, mat, (size_t)total * W * 4, s);
+        hipLaunchKernelGGL(k_bit_slots, dim3(grid_for((u64)total * 2 * ldw)), dim3(256), 0, s, raw.as<u32>(), total,
+                           W, rows.as<u32>(), ldw);
+        CASPHY_HIP(hipGetLastError());
+        batch_run(rows, offs, ldw, ancestral_type, planes, nullptr, 0, 2.0f, edge, s);
+    });
+}
+
+int casphy_join_indel_batch(const int32_t *mat, const uint32_t *offsets, uint32_t njobs, uint32_t ncol,
+                            const double *weight, uint32_t nstate, const uint32_t *tip_size, uint32_t min_size,
+                            float min_ratio, uint32_t *edge)
+{
+    return guarded([&] {
+        require(mat && offsets && edge && ncol > 0, "casphy_join_indel_batch: bad arguments");
+        require(!weight || nstate > 0, "casphy_join_indel_batch: weights need nState > 0");
+        if (njobs == 0) return;
+        std::vector<u32> offs(offsets, offsets + njobs + 1);
+        require(offs[0] == 0, "casphy_join_indel_batch: offsets[0] must be 0");
+        for (u32 k = 0; k < njobs; ++k)
+            require(offs[k + 1] >= offs[k] + 2, "casphy_join_indel_batch: every job needs >= 2 cells");
+        hipStream_t s = thread_stream();
+        const u32 total = offs.back();
+        DevBuf dmat((size_t)total * ncol * 4);
+        h2d(dmat.p, mat, (size_t)total * ncol * 4, s);
+        // one layout for every job: per-site allele groups sized by the maximum over all jobs
+        IndelLayout L = indel_layout(dmat.as<int32_t>(), total, ncol, s);
+        Planes planes = indel_planes(L, weight, nstate, ncol);
+        DevBuf rows((size_t)total * 2 * L.ldw * 4), dts;
+        launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
+        hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, dmat.as<int32_t>(),
+                           total, ncol, L.dloci.as<u32>(), L.dwidth.as<u32>(), rows.as<u32>(), L.ldw);
+        CASPHY_HIP(hipGetLastError());
+        if (tip_size) {
+            dts.alloc((size_t)total * 4);
+            h2d(dts.p, tip_size, (size_t)total * 4, s);
         }
+        batch_run(rows, offs, L.ldw, 2, planes, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge, s);
     });
 }
 

@@ -146,11 +146,19 @@ int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out,
 /* Batched join: `njobs` independent binary trees in one persistent work-queue
  * launch (bootstrap replicates, split clusters).  Job k's cells are rows
  * [offsets[k], offsets[k+1]) of mat_cellmajor (all of width W); its edges go to
- * edge + 2*(2*offsets[k] - 2*k), i.e. 2n_k-2 rows per job in job order.
+ * edge + 2*(2*offsets[k] - 2*k), i.e. 2n_k-2 rows per job in job order.  offsets[0]
+ * must be 0.  One workgroup runs each job from an atomic work queue.
  * No counterpart in the reference (which runs one process per tree,
  * R/CasPhy_functions.R:2389-2392); each job's result equals casphy_join_bit. */
 int casphy_join_bit_batch(const uint32_t *mat_cellmajor, const uint32_t *offsets, uint32_t njobs,
                           uint32_t W, const double *weight, int ancestral_type, uint32_t *edge);
+/* Batched indel joins (same job layout; tip_size holds offsets[njobs] sizes or
+ * is NULL).  No counterpart in the reference (one joinEdgeIndel process per
+ * cluster, R/CasPhy_functions.R:2199-2210); each job equals casphy_join_indel. */
+int casphy_join_indel_batch(const int32_t *mat_cellmajor, const uint32_t *offsets, uint32_t njobs,
+                            uint32_t ncol, const double *weight, uint32_t nstate,
+                            const uint32_t *tip_size, uint32_t min_size, float min_ratio,
+                            uint32_t *edge);
 
 /* ------------------------------------------------------------------ transfer */
 /* Transfer distance between bipartition bitsets (row-major [nrow][ncol] u32,

@@ -186,6 +186,32 @@ def test_join_batch_equals_single(hip):
         assert np.array_equal(e, e1)
 
 
+def test_join_batch_weighted_and_types(hip):
+    rng = np.random.default_rng(41)
+    mats = [rand_bits(rng, int(k), 3, 0.1) for k in (5, 64, 200)]
+    w = rng.integers(0, 4, 96).astype(np.float64)
+    for anc in (0, 1):
+        batch = hip.join_bit_batch(mats, w, anc)
+        for m, e in zip(mats, batch):
+            e1, _, _ = hip.join_bit(m, w, anc)
+            assert np.array_equal(e, e1)
+
+
+def test_join_indel_batch_bootstrap(hip, orc):
+    """Bootstrap replicates of the indel configuration (sites resampled with
+    replacement) in one persistent launch, each equal to its own oracle join."""
+    base = synth.indel_matrix(seed=1, depth=8)
+    rng = np.random.default_rng(42)
+    reps = [base[:, rng.integers(0, base.shape[1], base.shape[1])] for _ in range(6)]
+    reps.append(base[:3])
+    ts = [np.ones(r.shape[0], np.uint32) for r in reps]
+    batch = hip.join_indel_batch(reps, None, ts, 0, 2.0)
+    for r, e in zip(reps, batch):
+        w = np.ones((max(1, r.max()), r.shape[1]), np.float32)
+        e2, _, _ = orc.join_indel(r, w)
+        assert np.array_equal(e, e2)
+
+
 # ------------------------------------------------------------------ transfer
 @pytest.mark.parametrize("ntip,ny,nx", [(64, 62, 62), (75, 20, 300), (1000, 130, 129), (32, 1, 1)])
 def test_transfer(hip, orc, ntip, ny, nx):
