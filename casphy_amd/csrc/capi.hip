@@ -13,6 +13,7 @@
 
 #include "context.hpp"
 #include "kernels.hpp"
+#include "mfma.hpp"
 #include "tiles.hpp"
 
 namespace casphy {
@@ -177,27 +178,56 @@ DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s)
 // Radius lists of a device matrix: mask tiles -> counts -> scan -> compaction,
 // all kept on the device.
 struct RadiusOut {
-    DevBuf mask, counts, offsets, idx, pbuf;
+    DevBuf mask, counts, offsets, idx, pbuf, stage;
     u64 total = 0;
     u32 n = 0;
 };
 
+// The int8 matrix-core path serves unit-weight bit scores; CASPHY_MFMA=0
+// selects the VALU popcount tiles instead (both are exact).
+bool mfma_enabled()
+{
+    const char *e = std::getenv("CASPHY_MFMA");
+    return !(e && e[0] == '0');
+}
+
+// Stage-major copy of a cell-major matrix for the MFMA tiles.
+void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
+{
+    P.reserve((size_t)ceil_div(n, TILE) * TILE * ldw * 4);
+    launch_to_stage_major(dmat, n, ldw, P.as<u32>(), s);
+}
+
 void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
-                    RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+                    RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                    const u32 *stage = nullptr)
 {
     const u32 nt = ceil_div(n, TILE);
     const u32 mstride = nt * 4;
     o.n = n;
     o.mask.reserve((size_t)nt * TILE * mstride * 4);
-    TileArgs a;
-    a.A = dmat; a.nA = n; a.B = dmat; a.nB = n;
-    a.ldw = ldw; a.kw = ldw; a.ntA = nt; a.ntB = nt;
-    a.triangle = 1;
-    a.ntiles = (u32)((u64)nt * (nt + 1) / 2);
-    a.thr = radius; a.mask = o.mask.as<u32>(); a.mstride = mstride;
-    apply_planes(planes, a, o.pbuf, s);
+    const bool mfma = metric == M_AND && planes.nplanes == 0 && mfma_enabled();
+    if (mfma && !stage) {
+        make_stage(dmat, n, ldw, o.stage, s);
+        stage = o.stage.as<u32>();
+    }
     if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
-    launch_tiles(metric, E_MASK, a, s);
+    if (mfma) {
+        MfmaArgs m;
+        m.P = stage; m.nrows_pad = nt * TILE; m.stages = ldw / 4;
+        m.nA = n; m.nB = n; m.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+        m.thr = radius; m.mask = o.mask.as<u32>(); m.mstride = mstride;
+        launch_mfma_tiles(E_MASK, m, s);
+    } else {
+        TileArgs a;
+        a.A = dmat; a.nA = n; a.B = dmat; a.nB = n;
+        a.ldw = ldw; a.kw = ldw; a.ntA = nt; a.ntB = nt;
+        a.triangle = 1;
+        a.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+        a.thr = radius; a.mask = o.mask.as<u32>(); a.mstride = mstride;
+        apply_planes(planes, a, o.pbuf, s);
+        launch_tiles(metric, E_MASK, a, s);
+    }
     if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
     o.counts.reserve((size_t)n * 4);
     o.offsets.reserve((size_t)(n + 1) * 8);
@@ -287,16 +317,26 @@ int casphy_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const d
         hipStream_t s = thread_stream();
         BitMat m = upload_bit(mat, W, n, false, s);
         Planes planes = make_planes(weight, 32 * W, m.ldw);
-        DevBuf dS((size_t)n * n * 4), pbuf;
-        TileArgs a;
-        a.A = m.d.as<u32>(); a.nA = n; a.B = a.A; a.nB = n;
-        a.ldw = m.ldw; a.kw = m.ldw;
-        a.ntA = a.ntB = ceil_div(n, TILE);
-        a.triangle = 1;
-        a.ntiles = (u32)((u64)a.ntA * (a.ntA + 1) / 2);
-        a.S = dS.as<int32_t>(); a.ldS = n;
-        apply_planes(planes, a, pbuf, s);
-        launch_tiles(M_AND, E_SCORES, a, s);
+        DevBuf dS((size_t)n * n * 4), pbuf, stage;
+        const u32 nt = ceil_div(n, TILE);
+        if (planes.nplanes == 0 && mfma_enabled()) {
+            make_stage(m.d.as<u32>(), n, m.ldw, stage, s);
+            MfmaArgs q;
+            q.P = stage.as<u32>(); q.nrows_pad = nt * TILE; q.stages = m.ldw / 4;
+            q.nA = n; q.nB = n; q.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+            q.S = dS.as<int32_t>(); q.ldS = n;
+            launch_mfma_tiles(E_SCORES, q, s);
+        } else {
+            TileArgs a;
+            a.A = m.d.as<u32>(); a.nA = n; a.B = a.A; a.nB = n;
+            a.ldw = m.ldw; a.kw = m.ldw;
+            a.ntA = a.ntB = nt;
+            a.triangle = 1;
+            a.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+            a.S = dS.as<int32_t>(); a.ldS = n;
+            apply_planes(planes, a, pbuf, s);
+            launch_tiles(M_AND, E_SCORES, a, s);
+        }
         d2h(S, dS.p, (size_t)n * n * 4, s);
     });
 }
@@ -445,6 +485,7 @@ struct casphy_cells {
     u32 n = 0, ldw = 0;
     DevBuf mat;
     Planes planes;
+    DevBuf stage;  // stage-major copy for the MFMA tiles (bit kind, unit weights)
     RadiusOut out;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int device = 0;
@@ -492,8 +533,16 @@ int casphy_cells_radius(casphy_cells *h, int32_t radius, uint64_t *nidx)
     return guarded([&] {
         require(h && nidx, "casphy_cells_radius: bad arguments");
         hipStream_t s = thread_stream();
+        if (h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled() && !h->stage.p)
+            make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
         CASPHY_HIP(hipEventRecord(h->ev[2], s));
-        radius_compute(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radius, h->out, s, h->ev[0], h->ev[1]);
+        const u32 *stage = nullptr;
+        if (h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled()) {
+            if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
+            stage = h->stage.as<u32>();
+        }
+        radius_compute(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radius, h->out, s, h->ev[0], h->ev[1],
+                       stage);
         CASPHY_HIP(hipEventRecord(h->ev[3], s));
         CASPHY_HIP(hipEventSynchronize(h->ev[3]));
         *nidx = h->out.total;

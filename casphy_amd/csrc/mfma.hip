@@ -1,0 +1,192 @@
+// mfma.hip -- exact AND-count on the int8 matrix cores (gfx950).
+//
+// popcount(a & b) over bit-packed sites equals the dot product of the 0/1
+// byte vectors, and v_mfma_i32_32x32x32_i8 accumulates such dot products in
+// int32 exactly (at most 2^31 / 1 = every realistic site count).  The kernel:
+//   * reads the packed bits in a stage-major layout P[stage][row][4 u32]
+//     (128 sites = 16 bytes per row per stage, 2 KB contiguous per 128 rows),
+//     so every global load is a fully coalesced 8-byte load;
+//   * expands each 32-bit word into 32 bytes of 0/1 in registers
+//     ((nibble * 0x00204081) & 0x01010101 per output word) and stores them in
+//     LDS rows of 128 + 16 bytes (the pad spreads rows over the LDS banks);
+//   * 4 waves per workgroup, each owning a 64 x 64 block of the 128 x 128 pair
+//     tile as 2 x 2 MFMA 32x32 blocks (64 int32 accumulators per lane);
+//   * fuses the same epilogues as the VALU tiles: the radius bitmask words are
+//     assembled with one 64-lane ballot per accumulator register (the two
+//     halves of the wave hold rows m and m+4 of one 32-column word), and the
+//     mirrored words from the per-lane column bits and one cross-half shuffle.
+// Accumulator layout of a 32x32 MFMA block: lane l, register v holds row
+// 8*(v/4) + 4*(l/32) + v%4 and column l%32.  The A and B operands use the same
+// (lane, byte) -> k mapping, so the product does not depend on how the
+// hardware orders k inside an instruction.
+#include "mfma.hpp"
+
+namespace casphy {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int NTHR = 256;
+constexpr int KB = 128;        // sites per stage
+constexpr int LROW = KB + 16;  // LDS bytes per row (padded)
+
+__global__ void k_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 nrows_pad, u32 *P)
+{
+    // P[s][r][q] = mat[r][4s + q]; rows >= n are zero
+    const u32 stages = ldw / 4;
+    const u64 total = (u64)stages * nrows_pad;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 s = (u32)(x / nrows_pad), r = (u32)(x - (u64)s * nrows_pad);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (r < n) v = *reinterpret_cast<const uint4 *>(mat + (size_t)r * ldw + 4 * s);
+        reinterpret_cast<uint4 *>(P)[x] = v;
+    }
+}
+
+__device__ __forceinline__ u32 spread4(u32 nib) { return (nib * 0x00204081u) & 0x01010101u; }
+
+template <int EPI>
+__global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LROW];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][TILE * LROW];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+
+    // loader: thread -> (row tid/2, words 2*(tid&1) .. +1 of the 4-word stage chunk)
+    const u32 lr = tid >> 1, lh = tid & 1;
+    const uint2 *PA = reinterpret_cast<const uint2 *>(a.P) + (size_t)I * TILE * 2 + tid;
+    const uint2 *PB = reinterpret_cast<const uint2 *>(a.P) + (size_t)J * TILE * 2 + tid;
+    const size_t sstride = (size_t)a.nrows_pad * 2;  // uint2 per stage
+    uint2 ga, gb;
+    auto gload = [&](u32 s) {
+        ga = PA[s * sstride];
+        gb = PB[s * sstride];
+    };
+    auto expand_store = [&](unsigned char *dst, uint2 w) {
+        uint4 *d = reinterpret_cast<uint4 *>(dst + lr * LROW + lh * 64);
+        d[0] = make_uint4(spread4(w.x & 15), spread4((w.x >> 4) & 15), spread4((w.x >> 8) & 15), spread4((w.x >> 12) & 15));
+        d[1] = make_uint4(spread4((w.x >> 16) & 15), spread4((w.x >> 20) & 15), spread4((w.x >> 24) & 15), spread4(w.x >> 28));
+        d[2] = make_uint4(spread4(w.y & 15), spread4((w.y >> 4) & 15), spread4((w.y >> 8) & 15), spread4((w.y >> 12) & 15));
+        d[3] = make_uint4(spread4((w.y >> 16) & 15), spread4((w.y >> 20) & 15), spread4((w.y >> 24) & 15), spread4(w.y >> 28));
+    };
+
+    v16i acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
+
+    const u32 nst = a.stages;
+    gload(0);
+    expand_store(As[0], ga);
+    expand_store(Bs[0], gb);
+    __syncthreads();
+    const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
+    for (u32 s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nst) gload(s + 1);
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 32) {
+            v4i fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LROW + kk + koff]);
+                fb[x] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LROW + kk + koff]);
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+        }
+        if (s + 1 < nst) {
+            expand_store(As[buf ^ 1], ga);
+            expand_store(Bs[buf ^ 1], gb);
+        }
+        __syncthreads();
+    }
+
+    const u32 half = lane >> 5, col = lane & 31;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const u32 r0 = I * TILE + wm * 64 + 32 * x;  // block row base
+            const u32 c0 = J * TILE + wn * 64 + 32 * y;  // block column base
+            const u32 j = c0 + col;
+            if (EPI == E_MASK) {
+                u32 colbits = 0;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
+                    const bool hit = acc[x][y][v] >= a.thr;
+                    const unsigned long long b = __ballot(hit && j < a.nB);
+                    // low half: row 8*(v/4)+v%4, high half: that row + 4
+                    if (lane == (u32)v || lane == (u32)(v + 32)) {
+                        const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
+                        const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
+                        const u32 i = r0 + m_out;
+                        if (i < a.nA) a.mask[(size_t)i * a.mstride + c0 / 32] = word;
+                    }
+                    if (hit && r0 + m < a.nA) colbits |= 1u << m;
+                }
+                if (I != J) {
+                    colbits |= __shfl_xor(colbits, 32);
+                    if (half == 0 && j < a.nB) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
+                }
+            } else if (EPI == E_SCORES) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                    if (i < a.nA && j < a.nB) {
+                        a.S[(size_t)i * a.ldS + j] = acc[x][y][v];
+                        if (I != J) a.S[(size_t)j * a.ldS + i] = acc[x][y][v];
+                    }
+                }
+            }
+        }
+}
+
+u32 grid_for(u64 items) { return (u32)std::min<u64>((items + 255) / 256, 65535ull * 8); }
+
+}  // namespace
+
+void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 *P, hipStream_t s)
+{
+    const u32 nrows_pad = ceil_div(n, TILE) * TILE;
+    hipLaunchKernelGGL(k_to_stage_major, dim3(grid_for((u64)(ldw / 4) * nrows_pad)), dim3(256), 0, s, mat, n, ldw,
+                       nrows_pad, P);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    if (a.ntiles == 0) return;
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    if (epi == E_MASK)
+        hipLaunchKernelGGL(k_mfma_tiles<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_SCORES)
+        hipLaunchKernelGGL(k_mfma_tiles<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
+    else
+        throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
+    CASPHY_HIP(hipGetLastError());
+}
+
+}  // namespace casphy

@@ -16,8 +16,15 @@ def rand_bits(rng, n, W, p):
 
 
 # ------------------------------------------------------------------ scores
+@pytest.fixture(params=["mfma", "valu"])
+def bitpath(request, monkeypatch):
+    """Runs a test on the int8 matrix-core tiles and on the VALU popcount tiles."""
+    monkeypatch.setenv("CASPHY_MFMA", "1" if request.param == "mfma" else "0")
+    return request.param
+
+
 @pytest.mark.parametrize("n,W", [(1, 1), (5, 1), (127, 2), (128, 4), (129, 3), (300, 5), (515, 32)])
-def test_score_bit_matrix(hip, orc, n, W):
+def test_score_bit_matrix(hip, orc, bitpath, n, W):
     rng = np.random.default_rng(n * 7 + W)
     mat = rand_bits(rng, n, W, 0.15)
     S = hip.score_bit_matrix(mat)
@@ -58,7 +65,7 @@ def test_score_indel_matrix(hip, orc):
 # ------------------------------------------------------------------ radius
 @pytest.mark.parametrize("n,W,p,r", [(1, 1, 0.3, 1), (33, 1, 0.2, 2), (250, 4, 0.05, 3), (700, 8, 0.03, 5),
                                      (1000, 2, 0.1, 0), (257, 3, 0.1, 1000)])
-def test_radius_bit(hip, orc, n, W, p, r):
+def test_radius_bit(hip, orc, bitpath, n, W, p, r):
     rng = np.random.default_rng(n + W)
     mat = rand_bits(rng, n, W, p)
     counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), r)
@@ -76,7 +83,7 @@ def test_radius_bit_weighted(hip, orc):
     assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
-def test_radius_bit_synthetic_config(hip, orc):
+def test_radius_bit_synthetic_config(hip, orc, bitpath):
     mat = synth.binary_matrix(seed=2, n=1500, nsite=1024, p=0.005)
     for r in (1, 4, 16):
         counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), r)
