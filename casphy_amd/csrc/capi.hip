@@ -110,6 +110,14 @@ Planes make_planes(const double *w, u32 nsite, u32 ldw)
     return make_planes_bits(std::vector<double>(w, w + nsite), ldw);
 }
 
+// The int8 matrix-core path serves unit-weight bit scores; CASPHY_MFMA=0
+// selects the VALU popcount tiles instead (both are exact).
+bool mfma_enabled()
+{
+    const char *e = std::getenv("CASPHY_MFMA");
+    return !(e && e[0] == '0');
+}
+
 void apply_planes(const Planes &p, TileArgs &a, DevBuf &buf, hipStream_t s)
 {
     a.nplanes = p.nplanes;
@@ -183,19 +191,12 @@ struct RadiusOut {
     u32 n = 0;
 };
 
-// The int8 matrix-core path serves unit-weight bit scores; CASPHY_MFMA=0
-// selects the VALU popcount tiles instead (both are exact).
-bool mfma_enabled()
-{
-    const char *e = std::getenv("CASPHY_MFMA");
-    return !(e && e[0] == '0');
-}
 
 // Stage-major copy of a cell-major matrix for the MFMA tiles.
 void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
 {
     P.reserve((size_t)ceil_div(n, TILE) * TILE * ldw * 4);
-    launch_to_stage_major(dmat, n, ldw, P.as<u32>(), s);
+    launch_to_stage_major(dmat, n, ldw, ldw, P.as<u32>(), s);
 }
 
 void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
@@ -214,7 +215,7 @@ void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &p
     if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
     if (mfma) {
         MfmaArgs m;
-        m.P = stage; m.nrows_pad = nt * TILE; m.stages = ldw / 4;
+        m.PA = m.PB = stage; m.padA = m.padB = nt * TILE; m.stages = ldw / 4;
         m.nA = n; m.nB = n; m.ntiles = (u32)((u64)nt * (nt + 1) / 2);
         m.thr = radius; m.mask = o.mask.as<u32>(); m.mstride = mstride;
         launch_mfma_tiles(E_MASK, m, s);
@@ -256,26 +257,44 @@ void radius_lists(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &pla
     *nidx = o.total;
 }
 
+// tryRadius: query x cell scores (matrix-core tiles for unit-weight bits,
+// VALU tiles otherwise) followed by one counting workgroup per query.
 void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, const int32_t *radii,
-               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s)
+               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage = nullptr)
 {
     for (u32 k = 0; k < R; ++k)
         CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
     for (u32 k = 1; k < nradii; ++k)
         CASPHY_REQUIRE(radii[k - 1] <= radii[k], CASPHY_EARG, "radii must be ascending");
-    DevBuf dq((size_t)R * 4), dc((size_t)R * nradii * 4), pbuf;
+    DevBuf dq((size_t)R * 4), dc((size_t)R * nradii * 4), dr((size_t)nradii * 4), dS((size_t)R * n * 4), pbuf;
     h2d(dq.p, q, (size_t)R * 4, s);
-    launch_fill_u32(dc.as<u32>(), (u64)R * nradii, 0, s);
+    h2d(dr.p, radii, (size_t)nradii * 4, s);
+    const u32 ntA = ceil_div(R, TILE), ntB = ceil_div(n, TILE);
+    if (metric == M_AND && planes.nplanes == 0 && mfma_enabled()) {
+        DevBuf own, sq((size_t)ntA * TILE * ldw * 4);
+        if (!stage) {
+            make_stage(dmat, n, ldw, own, s);
+            stage = own.as<u32>();
+        }
+        launch_to_stage_major(dmat, R, ldw, ldw, sq.as<u32>(), s, dq.as<u32>());
+        MfmaArgs m;
+        m.PA = sq.as<u32>(); m.PB = stage; m.padA = ntA * TILE; m.padB = ntB * TILE; m.stages = ldw / 4;
+        m.nA = R; m.nB = n; m.ntB = ntB; m.ntiles = ntA * ntB; m.triangle = 0;
+        m.S = dS.as<int32_t>(); m.ldS = n;
+        launch_mfma_tiles(E_SCORES, m, s);
+        launch_count_radii(dS.as<int32_t>(), R, n, n, dr.as<int32_t>(), nradii, dc.as<u32>(), s);
+        d2h(counts, dc.p, (size_t)R * nradii * 4, s);
+        return;
+    }
     TileArgs a;
     a.A = dmat; a.Aidx = dq.as<u32>(); a.nA = R; a.B = dmat; a.nB = n;
     a.ldw = ldw; a.kw = ldw;
-    a.ntA = ceil_div(R, TILE); a.ntB = ceil_div(n, TILE);
-    a.ntiles = a.ntA * a.ntB;
-    a.nradii = nradii;
-    for (u32 k = 0; k < nradii; ++k) a.radii[k] = radii[k];
-    a.counts = dc.as<u32>();
+    a.ntA = ntA; a.ntB = ntB;
+    a.ntiles = ntA * ntB;
+    a.S = dS.as<int32_t>(); a.ldS = n;
     apply_planes(planes, a, pbuf, s);
-    launch_tiles(metric, E_HIST, a, s);
+    launch_tiles(metric, E_SCORES, a, s);
+    launch_count_radii(dS.as<int32_t>(), R, n, n, dr.as<int32_t>(), nradii, dc.as<u32>(), s);
     d2h(counts, dc.p, (size_t)R * nradii * 4, s);
 }
 
@@ -322,7 +341,7 @@ int casphy_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const d
         if (planes.nplanes == 0 && mfma_enabled()) {
             make_stage(m.d.as<u32>(), n, m.ldw, stage, s);
             MfmaArgs q;
-            q.P = stage.as<u32>(); q.nrows_pad = nt * TILE; q.stages = m.ldw / 4;
+            q.PA = q.PB = stage.as<u32>(); q.padA = q.padB = nt * TILE; q.stages = m.ldw / 4;
             q.nA = n; q.nB = n; q.ntiles = (u32)((u64)nt * (nt + 1) / 2);
             q.S = dS.as<int32_t>(); q.ldS = n;
             launch_mfma_tiles(E_SCORES, q, s);
@@ -420,7 +439,23 @@ int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t 
         if (nx) mx = upload_bit(X, ncol, nx, false, s);
         launch_fill_u32(dymin.as<u32>(), ny, init, s);
         launch_fill_u32(dxmin.as<u32>(), nx, init, s);
-        if (ny && nx) {
+        if (ny && nx && mfma_enabled()) {
+            // H = popc(x) + popc(y) - 2 <x, y> with the dot product on the matrix cores
+            DevBuf sx, sy, px((size_t)nx * 4), py((size_t)ny * 4);
+            make_stage(mx.d.as<u32>(), nx, mx.ldw, sx, s);
+            make_stage(my.d.as<u32>(), ny, my.ldw, sy, s);
+            launch_row_popc(mx.d.as<u32>(), nx, mx.ldw, mx.ldw, px.as<u32>(), s);
+            launch_row_popc(my.d.as<u32>(), ny, my.ldw, my.ldw, py.as<u32>(), s);
+            MfmaArgs q;
+            q.PA = sx.as<u32>(); q.PB = sy.as<u32>();
+            q.padA = ceil_div(nx, TILE) * TILE; q.padB = ceil_div(ny, TILE) * TILE;
+            q.stages = mx.ldw / 4; q.nA = nx; q.nB = ny;
+            q.ntB = ceil_div(ny, TILE); q.ntiles = ceil_div(nx, TILE) * q.ntB; q.triangle = 0;
+            q.ntip = ntip; q.popA = px.as<u32>(); q.popB = py.as<u32>();
+            q.amin = dxmin.as<u32>(); q.bmin = dymin.as<u32>();
+            launch_mfma_tiles(E_TBE, q, s);
+            CASPHY_HIP(hipStreamSynchronize(s));
+        } else if (ny && nx) {
             TileArgs a;
             a.A = mx.d.as<u32>(); a.nA = nx; a.B = my.d.as<u32>(); a.nB = ny;
             a.ldw = mx.ldw; a.kw = mx.ldw;
@@ -566,8 +601,13 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
         require(h && radii && q && counts, "casphy_cells_try_radii: bad arguments");
         require(nradii >= 1 && nradii <= MAX_RADII, "casphy_cells_try_radii: 1..16 radii");
         if (R == 0) return;
-        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts,
-                  thread_stream());
+        hipStream_t s = thread_stream();
+        const u32 *stage = nullptr;
+        if (h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled()) {
+            if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
+            stage = h->stage.as<u32>();
+        }
+        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s, stage);
     });
 }
 

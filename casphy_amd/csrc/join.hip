@@ -30,6 +30,7 @@
 
 #include "context.hpp"
 #include "kernels.hpp"
+#include "mfma.hpp"
 #include "tiles.hpp"
 
 namespace casphy {
@@ -638,6 +639,17 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     }
 
     launch_fill_u64(t.keys.as<u64>(), n, 0, s);
+    if (planes.nplanes == 0 && mfma_enabled()) {
+        const u32 nt = ceil_div(n, TILE);
+        DevBuf stage((size_t)nt * TILE * ldw * 4);
+        launch_to_stage_major(a.rows, n, 2 * ldw, ldw, stage.as<u32>(), s);
+        MfmaArgs q;
+        q.PA = q.PB = stage.as<u32>(); q.padA = q.padB = nt * TILE; q.stages = ldw / 4;
+        q.nA = n; q.nB = n; q.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+        q.keys = t.keys.as<u64>();
+        launch_mfma_tiles(E_LOCALMAX, q, s);
+        CASPHY_HIP(hipStreamSynchronize(s));
+    } else {
     TileArgs ta;
     ta.A = a.rows; ta.nA = n; ta.B = a.rows; ta.nB = n;
     ta.ldw = 2 * ldw; ta.kw = ldw;
@@ -650,6 +662,7 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     for (u32 q = 0; q < planes.nplanes; ++q) ta.pshift[q] = planes.shift[q];
     ta.ptail = planes.tail;
     launch_tiles(M_AND, E_LOCALMAX, ta, s);
+    }
     hipLaunchKernelGGL(k_join_init, dim3(grid_for(2 * (u64)n)), dim3(256), 0, s, a, t.keys.as<u64>(), dtip_size);
     CASPHY_HIP(hipGetLastError());
 

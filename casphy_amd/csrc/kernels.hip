@@ -133,13 +133,19 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, 
     if (t == 1023) offsets[n] = part[1023];
 }
 
-// One wave per row: lanes take consecutive mask words, a wave prefix sum of the
-// word popcounts gives each lane its output position, ids are written ascending.
-__global__ void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx)
+// One wave per row.  Per chunk of 64 mask words: a wave prefix sum of the
+// word popcounts places every lane's ids in an LDS staging buffer (full words
+// as four 16-byte stores of consecutive ids), then the wave streams the chunk's
+// ids to HBM with consecutive lanes writing consecutive addresses.
+constexpr int CWAVES = 4;
+__global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words,
+                                                        const u64 *offsets, u32 *idx)
 {
-    const u32 lane = threadIdx.x & 63;
-    const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    __shared__ __attribute__((aligned(16))) u32 stage[CWAVES][64 * 32];
+    const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const u32 row = blockIdx.x * CWAVES + wv;
     if (row >= n) return;
+    u32 *buf = stage[wv];
     const u32 *m = mask + (size_t)row * mstride;
     u64 base = offsets[row];
     for (u32 w0 = 0; w0 < words; w0 += 64) {
@@ -151,14 +157,57 @@ __global__ void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const 
             const u32 y = __shfl_up(inc, o);
             if (lane >= (u32)o) inc += y;
         }
-        u64 p = base + inc - c;
-        while (v) {
-            const u32 b = (u32)__builtin_ctz(v);
-            idx[p++] = w * 32 + b;
-            v &= v - 1;
+        const u32 total = __shfl(inc, 63);
+        u32 p = inc - c;
+        if (v == 0xFFFFFFFFu && (p & 3) == 0) {
+            const u32 id0 = w * 32;
+            uint4 *d = reinterpret_cast<uint4 *>(buf + p);
+            for (u32 q = 0; q < 8; ++q) d[q] = make_uint4(id0 + 4 * q, id0 + 4 * q + 1, id0 + 4 * q + 2, id0 + 4 * q + 3);
+        } else {
+            while (v) {
+                const u32 b = (u32)__builtin_ctz(v);
+                buf[p++] = w * 32 + b;
+                v &= v - 1;
+            }
         }
-        base += __shfl(inc, 63);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (u32 k = lane; k < total; k += 64) idx[base + k] = buf[k];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        base += total;
     }
+}
+
+// tryRadius counts from a query x cell score matrix: one workgroup per query,
+// per-thread counters for every radius, then a workgroup reduction.
+__global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii,
+                                                     u32 nradii, u32 *counts)
+{
+    __shared__ u32 red[16][MAX_RADII_K];
+    const u32 q = blockIdx.x, tid = threadIdx.x;
+    if (q >= R) return;
+    int32_t rr[MAX_RADII_K];
+    u32 c[MAX_RADII_K];
+#pragma unroll
+    for (int k = 0; k < MAX_RADII_K; ++k) {
+        rr[k] = k < (int)nradii ? radii[k] : 0x7FFFFFFF;
+        c[k] = 0;
+    }
+    const int32_t *row = S + (size_t)q * ldS;
+    for (u32 j = tid; j < n; j += 256) {
+        const int32_t v = row[j];
+#pragma unroll
+        for (int k = 0; k < MAX_RADII_K; ++k) c[k] += v >= rr[k];
+    }
+#pragma unroll
+    for (int k = 0; k < MAX_RADII_K; ++k) {
+        u32 x = c[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if ((tid & 63) == 0) red[tid >> 6][k] = x;
+    }
+    __syncthreads();
+    if (tid < nradii) counts[(size_t)tid * R + q] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
 __global__ void k_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size)
@@ -229,7 +278,16 @@ void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s)
 
 void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, offsets, idx);
+    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n, CWAVES)), dim3(64 * CWAVES), 0, s, mask, n, mstride, words, offsets,
+                       idx);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
+                        hipStream_t s)
+{
+    if (!R) return;
+    hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii, nradii, counts);
     CASPHY_HIP(hipGetLastError());
 }
 

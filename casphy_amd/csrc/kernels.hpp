@@ -25,6 +25,11 @@ void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s);
 void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx,
                     hipStream_t s);
 
+// counts[k*R + q] = #{j < n : S[q*ldS + j] >= radii[k]} for k < nradii (<= 16).
+constexpr int MAX_RADII_K = 16;
+void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
+                        hipStream_t s);
+
 // size[r] = min(p, ntip - p) with p = popcount of row r (u32 arithmetic).
 void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s);
 

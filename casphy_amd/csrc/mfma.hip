@@ -32,15 +32,15 @@ constexpr int NTHR = 256;
 constexpr int KB = 128;        // sites per stage
 constexpr int LROW = KB + 16;  // LDS bytes per row (padded)
 
-__global__ void k_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 nrows_pad, u32 *P)
+__global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 ldw, u32 kw, u32 nrows_pad, u32 *P)
 {
-    // P[s][r][q] = mat[r][4s + q]; rows >= n are zero
-    const u32 stages = ldw / 4;
+    // P[s][r][q] = mat[r][4s + q] for the first kw words of rows of stride ldw; rows >= n are zero
+    const u32 stages = kw / 4;
     const u64 total = (u64)stages * nrows_pad;
     for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
         const u32 s = (u32)(x / nrows_pad), r = (u32)(x - (u64)s * nrows_pad);
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (r < n) v = *reinterpret_cast<const uint4 *>(mat + (size_t)r * ldw + 4 * s);
+        if (r < n) v = *reinterpret_cast<const uint4 *>(mat + (size_t)(rowidx ? rowidx[r] : r) * ldw + 4 * s);
         reinterpret_cast<uint4 *>(P)[x] = v;
     }
 }
@@ -57,25 +57,28 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    {
+    if (a.triangle) {
         u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
         while ((u64)j * (j + 1) / 2 > t) --j;
         while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
         J = j;
         I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
     }
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
     // loader: thread -> (row tid/2, words 2*(tid&1) .. +1 of the 4-word stage chunk)
     const u32 lr = tid >> 1, lh = tid & 1;
-    const uint2 *PA = reinterpret_cast<const uint2 *>(a.P) + (size_t)I * TILE * 2 + tid;
-    const uint2 *PB = reinterpret_cast<const uint2 *>(a.P) + (size_t)J * TILE * 2 + tid;
-    const size_t sstride = (size_t)a.nrows_pad * 2;  // uint2 per stage
+    const uint2 *PA = reinterpret_cast<const uint2 *>(a.PA) + (size_t)I * TILE * 2 + tid;
+    const uint2 *PB = reinterpret_cast<const uint2 *>(a.PB) + (size_t)J * TILE * 2 + tid;
+    const size_t sa = (size_t)a.padA * 2, sb = (size_t)a.padB * 2;  // uint2 per stage
     uint2 ga, gb;
     auto gload = [&](u32 s) {
-        ga = PA[s * sstride];
-        gb = PB[s * sstride];
+        ga = PA[s * sa];
+        gb = PB[s * sb];
     };
     auto expand_store = [&](unsigned char *dst, uint2 w) {
         uint4 *d = reinterpret_cast<uint4 *>(dst + lr * LROW + lh * 64);
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
                     }
                     if (hit && r0 + m < a.nA) colbits |= 1u << m;
                 }
-                if (I != J) {
+                if (a.triangle && I != J) {
                     colbits |= __shfl_xor(colbits, 32);
                     if (half == 0 && j < a.nB) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
                 }
@@ -157,9 +160,44 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
                     const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
                     if (i < a.nA && j < a.nB) {
                         a.S[(size_t)i * a.ldS + j] = acc[x][y][v];
-                        if (I != J) a.S[(size_t)j * a.ldS + i] = acc[x][y][v];
+                        if (a.triangle && I != J) a.S[(size_t)j * a.ldS + i] = acc[x][y][v];
                     }
                 }
+            } else if (EPI == E_LOCALMAX) {
+                // per row: max over j > i of (S, smallest j), reduced over the 32 lanes of a half
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                    unsigned long long key = 0;
+                    if (j < a.nB && j > i) key = ((unsigned long long)(u32)acc[x][y][v] << 32) | (0xFFFFFFFFu - j);
+#pragma unroll
+                    for (int o = 1; o < 32; o <<= 1) {
+                        const unsigned long long q = __shfl_xor(key, o);
+                        key = q > key ? q : key;
+                    }
+                    if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
+                }
+            } else if (EPI == E_TBE) {
+                const u32 pb = j < a.nB ? a.popB[j] : 0u;
+                u32 cmin = 0xFFFFFFFFu;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                    const bool valid = i < a.nA && j < a.nB;
+                    const u32 h = (valid ? a.popA[i] : 0u) + pb - 2u * (u32)acc[x][y][v];
+                    const u32 hc = a.ntip - h;
+                    u32 d = valid ? (h < hc ? h : hc) : 0xFFFFFFFFu;
+                    cmin = d < cmin ? d : cmin;
+#pragma unroll
+                    for (int o = 1; o < 32; o <<= 1) {
+                        const u32 q = __shfl_xor(d, o);
+                        d = q < d ? q : d;
+                    }
+                    if (col == 0 && i < a.nA) atomicMin(&a.amin[i], d);
+                }
+                const u32 q = __shfl_xor(cmin, 32);
+                cmin = q < cmin ? q : cmin;
+                if (half == 0 && j < a.nB) atomicMin(&a.bmin[j], cmin);
             }
         }
 }
@@ -168,11 +206,11 @@ u32 grid_for(u64 items) { return (u32)std::min<u64>((items + 255) / 256, 65535ul
 
 }  // namespace
 
-void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 *P, hipStream_t s)
+void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s, const u32 *rowidx)
 {
     const u32 nrows_pad = ceil_div(n, TILE) * TILE;
-    hipLaunchKernelGGL(k_to_stage_major, dim3(grid_for((u64)(ldw / 4) * nrows_pad)), dim3(256), 0, s, mat, n, ldw,
-                       nrows_pad, P);
+    hipLaunchKernelGGL(k_to_stage_major, dim3(grid_for((u64)(kw / 4) * nrows_pad)), dim3(256), 0, s, mat, rowidx, n,
+                       ldw, kw, nrows_pad, P);
     CASPHY_HIP(hipGetLastError());
 }
 
@@ -184,6 +222,10 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
         hipLaunchKernelGGL(k_mfma_tiles<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
     else if (epi == E_SCORES)
         hipLaunchKernelGGL(k_mfma_tiles<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_LOCALMAX)
+        hipLaunchKernelGGL(k_mfma_tiles<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_TBE)
+        hipLaunchKernelGGL(k_mfma_tiles<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
     else
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
     CASPHY_HIP(hipGetLastError());

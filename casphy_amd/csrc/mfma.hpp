@@ -6,20 +6,31 @@
 namespace casphy {
 
 struct MfmaArgs {
-    const u32 *P = nullptr;  // stage-major packed bits: [stages][nrows_pad][4]
-    u32 nrows_pad = 0;
-    u32 stages = 0;          // 128 sites each
-    u32 nA = 0, nB = 0;      // valid rows (A == B: self scoring, upper-triangle tiles)
+    const u32 *PA = nullptr;  // stage-major packed bits of the A rows: [stages][nrows_padA][4]
+    const u32 *PB = nullptr;  // and of the B rows (== PA for self scoring)
+    u32 padA = 0, padB = 0;   // padded row counts (multiples of 128)
+    u32 stages = 0;           // 128 sites each
+    u32 nA = 0, nB = 0;       // valid rows
+    u32 ntB = 0;              // tiles along B (full grids)
     u32 ntiles = 0;
-    int32_t thr = 0;         // E_MASK
+    int triangle = 1;         // A == B, tiles I <= J
+    int32_t thr = 0;          // E_MASK
     u32 *mask = nullptr;
     u32 mstride = 0;
-    int32_t *S = nullptr;    // E_SCORES
+    int32_t *S = nullptr;     // E_SCORES
     u64 ldS = 0;
+    u64 *keys = nullptr;      // E_LOCALMAX
+    u32 ntip = 0;             // E_TBE: H = popA + popB - 2 <a, b>
+    const u32 *popA = nullptr, *popB = nullptr;
+    u32 *amin = nullptr, *bmin = nullptr;
 };
 
-// cell-major [n][ldw] -> stage-major P (rows padded to a multiple of 128).
-void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 *P, hipStream_t s);
+// First kw words of cell-major rows of stride ldw -> stage-major P (rows padded
+// to a multiple of 128; kw a multiple of 32).  With rowidx, row r of P is
+// row rowidx[r] of mat.
+void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s,
+                           const u32 *rowidx = nullptr);
+bool mfma_enabled();
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
 
 }  // namespace casphy

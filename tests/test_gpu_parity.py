@@ -133,7 +133,7 @@ def test_try_radius_dna(hip, orc):
 # ------------------------------------------------------------------ joins
 @pytest.mark.parametrize("n,W,p", [(2, 1, 0.3), (3, 1, 0.0), (17, 1, 0.2), (130, 2, 0.1), (400, 8, 0.02)])
 @pytest.mark.parametrize("anc", [0, 1])
-def test_join_bit(hip, orc, n, W, p, anc):
+def test_join_bit(hip, orc, bitpath, n, W, p, anc):
     rng = np.random.default_rng(n * 3 + anc)
     mat = rand_bits(rng, n, W, p)
     e, ns, nb = hip.join_bit(mat, None, anc)
@@ -159,7 +159,7 @@ def test_join_bit_weighted(hip, orc):
 
 
 @pytest.mark.parametrize("min_size,ratio", [(0, 2.0), (4, 1.5)])
-def test_join_indel(hip, orc, min_size, ratio):
+def test_join_indel(hip, orc, bitpath, min_size, ratio):
     mat = synth.indel_matrix(seed=31, depth=8, nsite=120)
     n = mat.shape[0]
     w = np.ones((mat.max(), 120))
@@ -221,7 +221,7 @@ def test_join_indel_batch_bootstrap(hip, orc):
 
 # ------------------------------------------------------------------ transfer
 @pytest.mark.parametrize("ntip,ny,nx", [(64, 62, 62), (75, 20, 300), (1000, 130, 129), (32, 1, 1)])
-def test_transfer(hip, orc, ntip, ny, nx):
+def test_transfer(hip, orc, bitpath, ntip, ny, nx):
     rng = np.random.default_rng(ntip + ny)
     W = (ntip + 31) // 32
 
@@ -235,7 +235,7 @@ def test_transfer(hip, orc, ntip, ny, nx):
         assert np.array_equal(g, w)
 
 
-def test_transfer_trees(hip, orc):
+def test_transfer_trees(hip, orc, bitpath):
     rng = np.random.default_rng(3)
     parent, leaves = synth.yule_parent(700, rng)
     ref = synth.clade_bitsets(parent, leaves, 700)
