@@ -36,6 +36,7 @@ SIGNATURES = [
     ("casphy_join_bit_batch", C.c_int, [vp, vp, u32, u32, vp, C.c_int, vp]),
     ("casphy_join_indel_batch", C.c_int, [vp, vp, u32, u32, vp, u32, vp, u32, f32, vp]),
     ("casphy_transfer", C.c_int, [vp, u32, vp, u32, u32, u32, vp, vp, vp, vp]),
+    ("casphy_transfer_batch", C.c_int, [vp, u32, vp, vp, u32, u32, u32, vp, vp, vp, vp]),
     ("casphy_cells_create", C.c_int, [C.c_int, vp, u32, u32, C.c_int, vp, C.POINTER(vp)]),
     ("casphy_cells_destroy", None, [vp]),
     ("casphy_cells_radius", C.c_int, [vp, i32, C.POINTER(u64)]),

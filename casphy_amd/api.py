@@ -30,7 +30,7 @@ __all__ = [
     "CasphyError", "score_bit_matrix", "score_indel_matrix",
     "radius_neighbor", "radius_neighbor_dna", "try_radius", "try_radius_dna",
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
-    "join_bit_batch", "join_indel_batch", "transfer_dis", "join_edge_bit_reverse", "join_edge_indel",
+    "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
     "radiusWeightedDNAParallel",
 ]
@@ -255,6 +255,26 @@ def transfer_dis(Y, X, ntip: int):
     check(load().casphy_transfer(_p(Y), ny, _p(X), nx, ncol, int(ntip), _p(ymin), _p(ysize), _p(xmin),
                                  _p(xsize)))
     return ymin, ysize, xmin, xsize
+
+
+def transfer_dis_batch(Y, Xs, ntip: int):
+    """Transfer distances of reference bitsets Y against several replicate
+    sets in one launch: (ymin (nrep, ny), ysize, [xmin_b], [xsize_b])."""
+    Y = _u32(Y)
+    ny, ncol = Y.shape
+    Xs = [_u32(x) for x in Xs]
+    offs = np.zeros(len(Xs) + 1, dtype=np.uint32)
+    offs[1:] = np.cumsum([x.shape[0] for x in Xs])
+    X = np.ascontiguousarray(np.concatenate(Xs, axis=0)) if Xs else np.zeros((0, ncol), np.uint32)
+    ymin = np.empty((len(Xs), ny), np.uint32)
+    ysize = np.empty(ny, np.uint32)
+    xmin = np.empty(max(int(offs[-1]), 1), np.uint32)
+    xsize = np.empty(max(int(offs[-1]), 1), np.uint32)
+    check(load().casphy_transfer_batch(_p(Y), ny, _p(X), _p(offs), len(Xs), ncol, int(ntip), _p(ymin), _p(ysize),
+                                       _p(xmin), _p(xsize)))
+    xm = [xmin[offs[b]:offs[b + 1]] for b in range(len(Xs))]
+    xs = [xsize[offs[b]:offs[b + 1]] for b in range(len(Xs))]
+    return ymin, ysize, xm, xs
 
 
 # ------------------------------------------------------------------ Rcpp twins

@@ -473,6 +473,67 @@ int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t 
     });
 }
 
+int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, const uint32_t *xoffsets, uint32_t nrep,
+                          uint32_t ncol, uint32_t ntip, uint32_t *ymin, uint32_t *ysize, uint32_t *xmin,
+                          uint32_t *xsize)
+{
+    return guarded([&] {
+        require(Y && X && xoffsets && ymin && ysize && ncol > 0 && ny > 0, "casphy_transfer_batch: bad arguments");
+        require(ntip <= 32u * ncol, "casphy_transfer_batch: ntip exceeds 32*ncol");
+        require(xoffsets[0] == 0, "casphy_transfer_batch: xoffsets[0] must be 0");
+        for (u32 b = 0; b < nrep; ++b)
+            require(xoffsets[b + 1] >= xoffsets[b], "casphy_transfer_batch: xoffsets must be ascending");
+        const u32 nx = nrep ? xoffsets[nrep] : 0;
+        require(nx == 0 || (xmin && xsize), "casphy_transfer_batch: bad arguments");
+        hipStream_t s = thread_stream();
+        const u32 init = 32u * ncol;
+        // every replicate starts on a fresh 128-row tile so that a tile belongs to one replicate
+        std::vector<u32> rowidx, rowmap, rep_of_tile;
+        for (u32 b = 0; b < nrep; ++b) {
+            const u32 cnt = xoffsets[b + 1] - xoffsets[b];
+            const u32 padded = ceil_div(cnt, TILE) * TILE;
+            for (u32 r = 0; r < padded; ++r) {
+                rowidx.push_back(r < cnt ? xoffsets[b] + r : 0xFFFFFFFFu);
+                rowmap.push_back(r < cnt ? xoffsets[b] + r : 0xFFFFFFFFu);
+            }
+            for (u32 t = 0; t < padded / TILE; ++t) rep_of_tile.push_back(b);
+        }
+        const u32 npad = (u32)rowidx.size();
+        BitMat my = upload_bit(Y, ncol, ny, false, s);
+        DevBuf dymin((size_t)nrep * ny * 4 + 4), dys((size_t)ny * 4), dxmin((size_t)std::max(nx, 1u) * 4),
+            dxs((size_t)std::max(nx, 1u) * 4), py((size_t)ny * 4), sy;
+        launch_fill_u32(dymin.as<u32>(), (u64)nrep * ny, init, s);
+        launch_fill_u32(dxmin.as<u32>(), nx, init, s);
+        launch_row_size(my.d.as<u32>(), ny, my.ldw, ntip, dys.as<u32>(), s);
+        if (nx) {
+            BitMat mx = upload_bit(X, ncol, nx, false, s);
+            launch_row_size(mx.d.as<u32>(), nx, mx.ldw, ntip, dxs.as<u32>(), s);
+            DevBuf didx((size_t)npad * 4), dmap((size_t)npad * 4), drep(rep_of_tile.size() * 4),
+                sx((size_t)npad * mx.ldw * 4), px((size_t)npad * 4);
+            h2d(didx.p, rowidx.data(), (size_t)npad * 4, s);
+            h2d(dmap.p, rowmap.data(), (size_t)npad * 4, s);
+            h2d(drep.p, rep_of_tile.data(), rep_of_tile.size() * 4, s);
+            launch_to_stage_major(mx.d.as<u32>(), npad, mx.ldw, mx.ldw, sx.as<u32>(), s, didx.as<u32>());
+            make_stage(my.d.as<u32>(), ny, my.ldw, sy, s);
+            launch_stage_popc(sx.as<u32>(), npad, mx.ldw, px.as<u32>(), s);  // padding rows count 0
+            launch_row_popc(my.d.as<u32>(), ny, my.ldw, my.ldw, py.as<u32>(), s);
+            MfmaArgs q;
+            q.PA = sx.as<u32>(); q.PB = sy.as<u32>();
+            q.padA = npad; q.padB = ceil_div(ny, TILE) * TILE;
+            q.stages = mx.ldw / 4; q.nA = npad; q.nB = ny;
+            q.ntB = ceil_div(ny, TILE); q.ntiles = (npad / TILE) * q.ntB; q.triangle = 0;
+            q.ntip = ntip; q.popA = px.as<u32>(); q.popB = py.as<u32>();
+            q.amin = dxmin.as<u32>(); q.bmin = dymin.as<u32>();
+            q.arow_map = dmap.as<u32>(); q.tile_rep = drep.as<u32>(); q.bstride = ny;
+            launch_mfma_tiles(E_TBE, q, s);
+            d2h(xmin, dxmin.p, (size_t)nx * 4, s);
+            d2h(xsize, dxs.p, (size_t)nx * 4, s);
+        }
+        d2h(ymin, dymin.p, (size_t)nrep * ny * 4, s);
+        d2h(ysize, dys.p, (size_t)ny * 4, s);
+    });
+}
+
 int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, uint32_t *internal_order)
 {
     return guarded([&] {

@@ -40,7 +40,8 @@ __global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 l
     for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
         const u32 s = (u32)(x / nrows_pad), r = (u32)(x - (u64)s * nrows_pad);
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (r < n) v = *reinterpret_cast<const uint4 *>(mat + (size_t)(rowidx ? rowidx[r] : r) * ldw + 4 * s);
+        const u32 src = r < n ? (rowidx ? rowidx[r] : r) : 0xFFFFFFFFu;
+        if (src != 0xFFFFFFFFu) v = *reinterpret_cast<const uint4 *>(mat + (size_t)src * ldw + 4 * s);
         reinterpret_cast<uint4 *>(P)[x] = v;
     }
 }
@@ -179,11 +180,13 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
                 }
             } else if (EPI == E_TBE) {
                 const u32 pb = j < a.nB ? a.popB[j] : 0u;
+                const u32 bbase = a.tile_rep ? a.tile_rep[I] * a.bstride : 0u;
                 u32 cmin = 0xFFFFFFFFu;
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    const bool valid = i < a.nA && j < a.nB;
+                    const u32 out = i < a.nA ? (a.arow_map ? a.arow_map[i] : i) : 0xFFFFFFFFu;
+                    const bool valid = out != 0xFFFFFFFFu && j < a.nB;
                     const u32 h = (valid ? a.popA[i] : 0u) + pb - 2u * (u32)acc[x][y][v];
                     const u32 hc = a.ntip - h;
                     u32 d = valid ? (h < hc ? h : hc) : 0xFFFFFFFFu;
@@ -193,13 +196,25 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
                         const u32 q = __shfl_xor(d, o);
                         d = q < d ? q : d;
                     }
-                    if (col == 0 && i < a.nA) atomicMin(&a.amin[i], d);
+                    if (col == 0 && out != 0xFFFFFFFFu) atomicMin(&a.amin[out], d);
                 }
                 const u32 q = __shfl_xor(cmin, 32);
                 cmin = q < cmin ? q : cmin;
-                if (half == 0 && j < a.nB) atomicMin(&a.bmin[j], cmin);
+                if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
             }
         }
+}
+
+__global__ void k_stage_popc(const u32 *P, u32 n, u32 stages, u32 *pop)
+{
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    u32 c = 0;
+    for (u32 st = 0; st < stages; ++st) {
+        const uint4 v = reinterpret_cast<const uint4 *>(P)[(size_t)st * n + r];
+        c += __builtin_popcount(v.x) + __builtin_popcount(v.y) + __builtin_popcount(v.z) + __builtin_popcount(v.w);
+    }
+    pop[r] = c;
 }
 
 u32 grid_for(u64 items) { return (u32)std::min<u64>((items + 255) / 256, 65535ull * 8); }
@@ -211,6 +226,12 @@ void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipSt
     const u32 nrows_pad = ceil_div(n, TILE) * TILE;
     hipLaunchKernelGGL(k_to_stage_major, dim3(grid_for((u64)(kw / 4) * nrows_pad)), dim3(256), 0, s, mat, rowidx, n,
                        ldw, kw, nrows_pad, P);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_stage_popc(const u32 *P, u32 n, u32 ldw, u32 *pop, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stage_popc, dim3(ceil_div(n, 256)), dim3(256), 0, s, P, n, ldw / 4, pop);
     CASPHY_HIP(hipGetLastError());
 }
 

@@ -23,6 +23,9 @@ struct MfmaArgs {
     u32 ntip = 0;             // E_TBE: H = popA + popB - 2 <a, b>
     const u32 *popA = nullptr, *popB = nullptr;
     u32 *amin = nullptr, *bmin = nullptr;
+    const u32 *arow_map = nullptr;  // E_TBE batches: padded A row -> output row (~0u = padding)
+    const u32 *tile_rep = nullptr;  // E_TBE batches: A tile -> replicate (bmin row block)
+    u32 bstride = 0;                // E_TBE batches: bmin entries per replicate
 };
 
 // First kw words of cell-major rows of stride ldw -> stage-major P (rows padded
@@ -31,6 +34,8 @@ struct MfmaArgs {
 void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s,
                            const u32 *rowidx = nullptr);
 bool mfma_enabled();
+// Row popcounts of a stage-major matrix with n rows (n a multiple of 128).
+void launch_stage_popc(const u32 *P, u32 n, u32 ldw, u32 *pop, hipStream_t s);
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
 
 }  // namespace casphy

@@ -230,16 +230,26 @@ def spr(parent: np.ndarray, leaves: np.ndarray, moves: int, rng: np.random.Gener
 
 
 def clade_bitsets(parent: np.ndarray, leaves: np.ndarray, ntip: int) -> np.ndarray:
-    """Descendant-tip bitsets of every internal non-root node, (ntip-2, W)."""
+    """Descendant-tip bitsets of every internal non-root node, (ntip-2, W),
+    ordered by node id.  Built level by level from the deepest nodes up."""
     nn = parent.size
     W = (ntip + 31) // 32
-    tip_of = np.full(nn, -1, dtype=np.int64)
-    tip_of[leaves] = np.arange(leaves.size)
+    kids = np.full((nn, 2), -1, dtype=np.int64)
+    order = np.argsort(parent[1:], kind="stable") + 1
+    ps = parent[order]
+    first = np.ones(order.size, bool)
+    first[1:] = ps[1:] != ps[:-1]
+    kids[ps[first], 0] = order[first]
+    kids[ps[~first], 1] = order[~first]
     bits = np.zeros((nn, W), dtype=np.uint32)
-    tl = leaves
-    bits[tl, np.arange(ntip) >> 5] = (np.uint32(1) << (np.arange(ntip) & 31).astype(np.uint32))
+    t = np.arange(ntip)
+    bits[leaves, t >> 5] = np.uint32(1) << (t & 31).astype(np.uint32)
     levels = _levels(parent)
-    for lev in reversed(levels):
-        np.bitwise_or.at(bits, parent[lev], bits[lev])
-    internal = np.setdiff1d(np.arange(1, nn), leaves)
+    internal_mask = kids[:, 0] >= 0
+    for lev in reversed([np.array([0])] + levels):
+        iv = lev[internal_mask[lev]]
+        if iv.size:
+            bits[iv] = bits[kids[iv, 0]] | bits[kids[iv, 1]]
+    internal = np.nonzero(internal_mask)[0]
+    internal = internal[internal != 0]
     return np.ascontiguousarray(bits[internal])

@@ -169,6 +169,14 @@ int casphy_join_indel_batch(const int32_t *mat_cellmajor, const uint32_t *offset
 int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx,
                     uint32_t ncol, uint32_t ntip, uint32_t *ymin, uint32_t *ysize,
                     uint32_t *xmin, uint32_t *xsize);
+/* Batched transfer distance: one reference set Y against nrep replicate sets
+ * whose rows are X[xoffsets[b] .. xoffsets[b+1]), all in one launch.
+ * ymin[b*ny + y] = min over x of replicate b; xmin/xsize per X row; ysize[ny].
+ * No counterpart in the reference (one TransferDisStream process per pair of
+ * files); replicate b's results equal casphy_transfer(Y, X_b). */
+int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X,
+                          const uint32_t *xoffsets, uint32_t nrep, uint32_t ncol, uint32_t ntip,
+                          uint32_t *ymin, uint32_t *ysize, uint32_t *xmin, uint32_t *xsize);
 
 /* ------------------------------------------------------------------ resident */
 /* A packed cell matrix kept in HBM across calls (the serving / tuning form: one

@@ -244,3 +244,17 @@ def test_transfer_trees(hip, orc, bitpath):
     want = orc.transfer(ref, boot, 700)
     for g, w in zip(got, want):
         assert np.array_equal(g, w)
+
+
+def test_transfer_batch(hip, orc):
+    rng = np.random.default_rng(77)
+    ntip = 500
+    parent, leaves = synth.yule_parent(ntip, rng)
+    ref = synth.clade_bitsets(parent, leaves, ntip)
+    boots = [synth.clade_bitsets(synth.spr(parent, leaves, k, rng), leaves, ntip) for k in (1, 10, 40)]
+    boots.append(boots[0][:7])
+    ymin, ysize, xm, xs = hip.transfer_dis_batch(ref, boots, ntip)
+    for b, X in enumerate(boots):
+        y2, ys2, x2, xs2 = orc.transfer(ref, X, ntip)
+        assert np.array_equal(ymin[b], y2) and np.array_equal(ysize, ys2)
+        assert np.array_equal(xm[b], x2) and np.array_equal(xs[b], xs2)
