@@ -74,12 +74,100 @@ def cpu_baseline(mat: np.ndarray, radius: int, gpu_counts: np.ndarray, budget_s:
                       f"(radiusWeightedParallel per-query loop, {threads} threads, {el:.1f} s)"}
 
 
+def timed(fn, reps=1):
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(reps):
+        out = fn()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def extras(cp, synth, cells, q) -> dict:
+    """The other configurations, plain vs batched, each a device-resident or
+    host-API pass as a user would run it; every batched result is compared
+    with its plain counterpart."""
+    ex = {}
+    # tryRadius: one launch per radius (the autoSplit loop) vs all 16 radii at once
+    plain_s, per = timed(lambda: [cells.try_radii([r], q) for r in RADII], 3)
+    batch_s, allr = timed(lambda: cells.try_radii(RADII, q), 3)
+    assert all(np.array_equal(per[k][0], allr[k]) for k in range(len(RADII)))
+    ex["tryradius_16_radii"] = {"plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
+                                "speedup": plain_s / batch_s}
+    print("# tryRadius done", file=sys.stderr, flush=True)
+
+    # indel tree builds, 1,024 cells x 200 sites: one join vs bootstrap replicates in one launch
+    base = synth.indel_matrix(seed=1)
+    rng = np.random.default_rng(11)
+    reps = [base[:, rng.integers(0, base.shape[1], base.shape[1])] for _ in range(256)]
+    cp.join_indel(reps[0])
+    plain_s, e_plain = timed(lambda: cp.join_indel(reps[0]), 3)
+    batch_s, e_batch = timed(lambda: cp.join_indel_batch(reps), 1)
+    assert np.array_equal(e_batch[0], e_plain[0])
+    ex["indel_join_1024x200"] = {"plain_jobs_per_s": 1 / plain_s, "batched_jobs_per_s": len(reps) / batch_s,
+                                 "batched_jobs": len(reps), "speedup": plain_s * len(reps) / batch_s}
+    print("# indel joins done", file=sys.stderr, flush=True)
+
+    # binary tree build of the 10,000-cell matrix
+    mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)
+    join_s, _ = timed(lambda: cp.join_bit(mat), 1)
+    ex["binary_join_10000x4096"] = {"plain_jobs_per_s": 1 / join_s, "ms_per_job": join_s * 1e3}
+    print("# binary join done", file=sys.stderr, flush=True)
+
+    # DNA 100,000 x 1,000: tuning in chunks of 16 radii from radius_initial 100, step 10, then the lists
+    t = time.perf_counter()
+    dna = synth.dna_matrix(seed=3, n=100000, ncol=1000)
+    gen = time.perf_counter() - t
+    dcells = cp.Cells(dna, "dna")
+    dq = synth.query_rows(3, dna.shape[0], R)
+    bound = 9.5 - np.log10(dna.shape[0])
+    start, radius, tries = 100, None, 0
+    t0 = time.perf_counter()
+    while radius is None and start <= 1000:
+        radii = np.arange(start, start + 160, 10, dtype=np.int32)
+        cnt = dcells.try_radii(radii, dq)
+        tries += 1
+        for k, r in enumerate(radii):
+            if np.log10(max(cnt[k].mean(), 1e-30)) <= bound:
+                radius = int(r)
+                break
+        start += 160
+    tune_s = time.perf_counter() - t0
+    rad_s, total = timed(lambda: dcells.radius(radius), 1)
+    rad_s, total = timed(lambda: dcells.radius(radius), 2)
+    tile_ms, _ = dcells.last_ms()
+    npairs = dna.shape[0] * (dna.shape[0] + 1) // 2
+    ex["dna_radius_100000x1000"] = {"radius": radius, "tuning_launches": tries, "tuning_ms": tune_s * 1e3,
+                                   "list_pass_ms": rad_s * 1e3, "pair_tile_ms": tile_ms, "neighbour_ids": int(total),
+                                   "cell_pair_scores_per_s": npairs / rad_s, "input_gen_s": gen}
+    dcells.close()
+    print("# DNA done", file=sys.stderr, flush=True)
+
+    # transfer distance: 65,536-tip Yule reference vs SPR-perturbed bootstrap trees
+    rng = np.random.default_rng(4)
+    ntip = 65536
+    parent, leaves = synth.yule_parent(ntip, rng)
+    ref = synth.clade_bitsets(parent, leaves, ntip)
+    moves = int(np.ceil(0.05 * (ntip - 3)))
+    boots = [synth.clade_bitsets(synth.spr(parent, leaves, moves, rng), leaves, ntip) for _ in range(2)]
+    plain_s, pl = timed(lambda: [cp.transfer_dis(ref, b, ntip) for b in boots], 1)
+    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, boots, ntip), 1)
+    for k in range(len(boots)):
+        assert np.array_equal(bt[0][k], pl[k][0]) and np.array_equal(bt[2][k], pl[k][2])
+    pairs = ref.shape[0] * sum(b.shape[0] for b in boots)
+    ex["tbe_65536_tips"] = {"replicates": len(boots), "plain_jobs_per_s": len(boots) / plain_s,
+                            "batched_jobs_per_s": len(boots) / batch_s, "bipartition_pairs_per_s": pairs / batch_s,
+                            "mfma_tops": 2 * ntip * pairs / batch_s / 1e12}
+    print("# TBE done", file=sys.stderr, flush=True)
+    return ex
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -134,7 +222,7 @@ def main():
         "metric": "cell-pair scores/s + tree-build & TBE jobs/s on 1 GPU, batched vs plain, bit-exact",
         "value": value, "unit": "cell-pair scores/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "vs_baseline": None, "dtype": "int8" if roof["bound"] == "mfma" else "u32", "data": "synthetic",
         "config": {"workload": "base-editor binary 10,000 cells x 4,096 sites (128 u32/cell): tryRadius "
                                "1,024 queries x radii 1..16 + radiusNeighbor lists at the autoSplit radius",
                    "cells": N, "sites": NSITE, "queries": R, "radius": r, "neighbour_ids": int(total)},
@@ -144,6 +232,8 @@ def main():
         cells.radius(r)
         gc, _ = cells.fetch()
         out["cpu_baseline"] = cpu_baseline(mat, r, gc)
+    if not args.no_extra:
+        out["extra"] = extras(cp, synth, cells, q)
     print(json.dumps(out))
 
 
