@@ -74,6 +74,21 @@ def cpu_baseline(mat: np.ndarray, radius: int, gpu_counts: np.ndarray, budget_s:
                       f"(radiusWeightedParallel per-query loop, {threads} threads, {el:.1f} s)"}
 
 
+def calibration_copy(nbytes: int = 256 << 20) -> None:
+    """One device-to-device copy of `nbytes` (the HIP runtime's copy kernel),
+    the known-size reference against which FETCH_SIZE / WRITE_SIZE counters of
+    the pair-tile kernel are scaled."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def timed(fn, reps=1):
     t0 = time.perf_counter()
     out = None
@@ -168,6 +183,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
+    ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -175,6 +191,9 @@ def main():
 
     import casphy_amd as cp
     from casphy_amd import synth
+
+    if args.calibrate:
+        calibration_copy()
 
     t = time.perf_counter()
     mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)

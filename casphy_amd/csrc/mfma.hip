@@ -21,6 +21,9 @@
 // hardware orders k inside an instruction.
 #include "mfma.hpp"
 
+#include <cstdlib>
+#include <cstring>
+
 namespace casphy {
 
 namespace {
@@ -29,8 +32,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int NTHR = 256;
-constexpr int KB = 128;        // sites per stage
-constexpr int LROW = KB + 16;  // LDS bytes per row (padded)
 
 __global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 ldw, u32 kw, u32 nrows_pad, u32 *P)
 {
@@ -48,11 +49,51 @@ __global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 l
 
 __device__ __forceinline__ u32 spread4(u32 nib) { return (nib * 0x00204081u) & 0x01010101u; }
 
-template <int EPI>
-__global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
+__device__ __forceinline__ uint4 spread16(u32 w)  // bits 0..15 -> 16 bytes
 {
-    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LROW];
-    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][TILE * LROW];
+    return make_uint4(spread4(w & 15), spread4((w >> 4) & 15), spread4((w >> 8) & 15), spread4((w >> 12) & 15));
+}
+
+// Per-thread slice of one stage of one operand: KB/16 bytes of packed bits.
+template <int KB> struct Slice;
+template <> struct Slice<128> {
+    uint2 v;
+    __device__ __forceinline__ void load(const u32 *P, size_t pad, u32 tile, u32 s, u32 tid)
+    {
+        v = reinterpret_cast<const uint2 *>(P)[(size_t)s * pad * 2 + (size_t)tile * TILE * 2 + tid];
+    }
+    __device__ __forceinline__ void store(unsigned char *dst, u32 tid) const  // row tid/2, bytes (tid&1)*64
+    {
+        uint4 *d = reinterpret_cast<uint4 *>(dst + (tid >> 1) * (128 + 16) + (tid & 1) * 64);
+        d[0] = spread16(v.x);
+        d[1] = spread16(v.x >> 16);
+        d[2] = spread16(v.y);
+        d[3] = spread16(v.y >> 16);
+    }
+};
+template <> struct Slice<64> {
+    u32 v;
+    __device__ __forceinline__ void load(const u32 *P, size_t pad, u32 tile, u32 s, u32 tid)
+    {
+        // 64-site stage s = half (s & 1) of 128-site stage s >> 1
+        v = P[(size_t)(s >> 1) * pad * 4 + ((size_t)tile * TILE + (tid >> 1)) * 4 + 2 * (s & 1) + (tid & 1)];
+    }
+    __device__ __forceinline__ void store(unsigned char *dst, u32 tid) const  // row tid/2, bytes (tid&1)*32
+    {
+        uint4 *d = reinterpret_cast<uint4 *>(dst + (tid >> 1) * (64 + 16) + (tid & 1) * 32);
+        d[0] = spread16(v);
+        d[1] = spread16(v >> 16);
+    }
+};
+
+// KB sites per LDS stage, PF stages of global prefetch in registers, OCC
+// workgroups per CU the launch bounds ask for.
+template <int EPI, int KB, int PF, int OCC>
+__global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
+{
+    constexpr int LR = KB + 16;  // LDS bytes per row (the pad spreads rows over the banks)
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LR];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][TILE * LR];
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -71,24 +112,6 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
-    // loader: thread -> (row tid/2, words 2*(tid&1) .. +1 of the 4-word stage chunk)
-    const u32 lr = tid >> 1, lh = tid & 1;
-    const uint2 *PA = reinterpret_cast<const uint2 *>(a.PA) + (size_t)I * TILE * 2 + tid;
-    const uint2 *PB = reinterpret_cast<const uint2 *>(a.PB) + (size_t)J * TILE * 2 + tid;
-    const size_t sa = (size_t)a.padA * 2, sb = (size_t)a.padB * 2;  // uint2 per stage
-    uint2 ga, gb;
-    auto gload = [&](u32 s) {
-        ga = PA[s * sa];
-        gb = PB[s * sb];
-    };
-    auto expand_store = [&](unsigned char *dst, uint2 w) {
-        uint4 *d = reinterpret_cast<uint4 *>(dst + lr * LROW + lh * 64);
-        d[0] = make_uint4(spread4(w.x & 15), spread4((w.x >> 4) & 15), spread4((w.x >> 8) & 15), spread4((w.x >> 12) & 15));
-        d[1] = make_uint4(spread4((w.x >> 16) & 15), spread4((w.x >> 20) & 15), spread4((w.x >> 24) & 15), spread4(w.x >> 28));
-        d[2] = make_uint4(spread4(w.y & 15), spread4((w.y >> 4) & 15), spread4((w.y >> 8) & 15), spread4((w.y >> 12) & 15));
-        d[3] = make_uint4(spread4((w.y >> 16) & 15), spread4((w.y >> 20) & 15), spread4((w.y >> 24) & 15), spread4(w.y >> 28));
-    };
-
     v16i acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -97,34 +120,51 @@ __global__ __launch_bounds__(NTHR, 2) void k_mfma_tiles(MfmaArgs a)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
 
-    const u32 nst = a.stages;
-    gload(0);
-    expand_store(As[0], ga);
-    expand_store(Bs[0], gb);
+    const u32 nst = a.stages * (128 / KB);  // a multiple of 8: rows are padded to 1,024 sites
+    Slice<KB> ga[PF], gb[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        ga[u].load(a.PA, a.padA, I, u, tid);
+        gb[u].load(a.PB, a.padB, J, u, tid);
+    }
+    ga[0].store(As[0], tid);
+    gb[0].store(Bs[0], tid);
+    if (PF < (int)nst) {
+        ga[0].load(a.PA, a.padA, I, PF, tid);
+        gb[0].load(a.PB, a.padB, J, PF, tid);
+    }
     __syncthreads();
     const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
-    for (u32 s = 0; s < nst; ++s) {
-        const int buf = s & 1;
-        if (s + 1 < nst) gload(s + 1);
+    for (u32 s0 = 0; s0 < nst; s0 += PF) {
 #pragma unroll
-        for (int kk = 0; kk < KB; kk += 32) {
-            v4i fa[2], fb[2];
+        for (int u = 0; u < PF; ++u) {
+            const u32 s = s0 + u;
+            const int buf = s & 1;
 #pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LROW + kk + koff]);
-                fb[x] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LROW + kk + koff]);
+            for (int kk = 0; kk < KB; kk += 32) {
+                v4i fa[2], fb[2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + kk + koff]);
+                    fb[x] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LR + kk + koff]);
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
             }
-#pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int y = 0; y < 2; ++y)
-                    acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+            const int nx = (u + 1) % PF;  // static after unrolling
+            if (s + 1 < nst) {
+                ga[nx].store(As[buf ^ 1], tid);
+                gb[nx].store(Bs[buf ^ 1], tid);
+                if (s + 1 + PF < nst) {
+                    ga[nx].load(a.PA, a.padA, I, s + 1 + PF, tid);
+                    gb[nx].load(a.PB, a.padB, J, s + 1 + PF, tid);
+                }
+            }
+            __syncthreads();
         }
-        if (s + 1 < nst) {
-            expand_store(As[buf ^ 1], ga);
-            expand_store(Bs[buf ^ 1], gb);
-        }
-        __syncthreads();
     }
 
     const u32 half = lane >> 5, col = lane & 31;
@@ -235,20 +275,45 @@ void launch_stage_popc(const u32 *P, u32 n, u32 ldw, u32 *pop, hipStream_t s)
     CASPHY_HIP(hipGetLastError());
 }
 
+template <int KB, int PF, int OCC>
+void launch_variant(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    if (epi == E_MASK)
+        hipLaunchKernelGGL((k_mfma_tiles<E_MASK, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_SCORES)
+        hipLaunchKernelGGL((k_mfma_tiles<E_SCORES, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_LOCALMAX)
+        hipLaunchKernelGGL((k_mfma_tiles<E_LOCALMAX, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_TBE)
+        hipLaunchKernelGGL((k_mfma_tiles<E_TBE, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
+    else
+        throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
+}
+
+// Stage depth / prefetch variants (CASPHY_MFMA_VARIANT), picked by measurement.
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8"};
+
+int mfma_variant()
+{
+    const char *e = std::getenv("CASPHY_MFMA_VARIANT");
+    if (e)
+        for (int k = 0; k < 6; ++k)
+            if (!std::strcmp(e, kVariants[k])) return k;
+    return 0;
+}
+
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
 {
     if (a.ntiles == 0) return;
-    const u32 grid = 8 * ((a.ntiles + 7) / 8);
-    if (epi == E_MASK)
-        hipLaunchKernelGGL(k_mfma_tiles<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
-    else if (epi == E_SCORES)
-        hipLaunchKernelGGL(k_mfma_tiles<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
-    else if (epi == E_LOCALMAX)
-        hipLaunchKernelGGL(k_mfma_tiles<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
-    else if (epi == E_TBE)
-        hipLaunchKernelGGL(k_mfma_tiles<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
-    else
-        throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
+    switch (mfma_variant()) {
+    case 0: launch_variant<128, 1, 2>(epi, a, s); break;
+    case 1: launch_variant<128, 2, 2>(epi, a, s); break;
+    case 2: launch_variant<128, 4, 2>(epi, a, s); break;
+    case 3: launch_variant<64, 2, 4>(epi, a, s); break;
+    case 4: launch_variant<64, 4, 4>(epi, a, s); break;
+    default: launch_variant<64, 8, 4>(epi, a, s); break;
+    }
     CASPHY_HIP(hipGetLastError());
 }
 
