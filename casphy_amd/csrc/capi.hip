@@ -195,7 +195,7 @@ struct RadiusOut {
 // Stage-major copy of a cell-major matrix for the MFMA tiles.
 void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
 {
-    P.reserve((size_t)ceil_div(n, TILE) * TILE * ldw * 4);
+    P.reserve((size_t)stage_rows(n) * ldw * 4);
     launch_to_stage_major(dmat, n, ldw, ldw, P.as<u32>(), s);
 }
 
@@ -215,8 +215,8 @@ void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &p
     if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
     if (mfma) {
         MfmaArgs m;
-        m.PA = m.PB = stage; m.padA = m.padB = nt * TILE; m.stages = ldw / 4;
-        m.nA = n; m.nB = n; m.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+        m.PA = m.PB = stage; m.padA = m.padB = stage_rows(n); m.stages = ldw / 4;
+        m.nA = n; m.nB = n;
         m.thr = radius; m.mask = o.mask.as<u32>(); m.mstride = mstride;
         launch_mfma_tiles(E_MASK, m, s);
     } else {
@@ -271,15 +271,15 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
     h2d(dr.p, radii, (size_t)nradii * 4, s);
     const u32 ntA = ceil_div(R, TILE), ntB = ceil_div(n, TILE);
     if (metric == M_AND && planes.nplanes == 0 && mfma_enabled()) {
-        DevBuf own, sq((size_t)ntA * TILE * ldw * 4);
+        DevBuf own, sq((size_t)stage_rows(R) * ldw * 4);
         if (!stage) {
             make_stage(dmat, n, ldw, own, s);
             stage = own.as<u32>();
         }
         launch_to_stage_major(dmat, R, ldw, ldw, sq.as<u32>(), s, dq.as<u32>());
         MfmaArgs m;
-        m.PA = sq.as<u32>(); m.PB = stage; m.padA = ntA * TILE; m.padB = ntB * TILE; m.stages = ldw / 4;
-        m.nA = R; m.nB = n; m.ntB = ntB; m.ntiles = ntA * ntB; m.triangle = 0;
+        m.PA = sq.as<u32>(); m.PB = stage; m.padA = stage_rows(R); m.padB = stage_rows(n); m.stages = ldw / 4;
+        m.nA = R; m.nB = n; m.triangle = 0;
         m.S = dS.as<int32_t>(); m.ldS = n;
         launch_mfma_tiles(E_SCORES, m, s);
         launch_count_radii(dS.as<int32_t>(), R, n, n, dr.as<int32_t>(), nradii, dc.as<u32>(), s);
@@ -341,8 +341,8 @@ int casphy_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const d
         if (planes.nplanes == 0 && mfma_enabled()) {
             make_stage(m.d.as<u32>(), n, m.ldw, stage, s);
             MfmaArgs q;
-            q.PA = q.PB = stage.as<u32>(); q.padA = q.padB = nt * TILE; q.stages = m.ldw / 4;
-            q.nA = n; q.nB = n; q.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+            q.PA = q.PB = stage.as<u32>(); q.padA = q.padB = stage_rows(n); q.stages = m.ldw / 4;
+            q.nA = n; q.nB = n;
             q.S = dS.as<int32_t>(); q.ldS = n;
             launch_mfma_tiles(E_SCORES, q, s);
         } else {
@@ -448,9 +448,9 @@ int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t 
             launch_row_popc(my.d.as<u32>(), ny, my.ldw, my.ldw, py.as<u32>(), s);
             MfmaArgs q;
             q.PA = sx.as<u32>(); q.PB = sy.as<u32>();
-            q.padA = ceil_div(nx, TILE) * TILE; q.padB = ceil_div(ny, TILE) * TILE;
+            q.padA = stage_rows(nx); q.padB = stage_rows(ny);
             q.stages = mx.ldw / 4; q.nA = nx; q.nB = ny;
-            q.ntB = ceil_div(ny, TILE); q.ntiles = ceil_div(nx, TILE) * q.ntB; q.triangle = 0;
+            q.triangle = 0;
             q.ntip = ntip; q.popA = px.as<u32>(); q.popB = py.as<u32>();
             q.amin = dxmin.as<u32>(); q.bmin = dymin.as<u32>();
             launch_mfma_tiles(E_TBE, q, s);
@@ -509,19 +509,19 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, con
             BitMat mx = upload_bit(X, ncol, nx, false, s);
             launch_row_size(mx.d.as<u32>(), nx, mx.ldw, ntip, dxs.as<u32>(), s);
             DevBuf didx((size_t)npad * 4), dmap((size_t)npad * 4), drep(rep_of_tile.size() * 4),
-                sx((size_t)npad * mx.ldw * 4), px((size_t)npad * 4);
+                sx((size_t)stage_rows(npad) * mx.ldw * 4), px((size_t)npad * 4);
             h2d(didx.p, rowidx.data(), (size_t)npad * 4, s);
             h2d(dmap.p, rowmap.data(), (size_t)npad * 4, s);
             h2d(drep.p, rep_of_tile.data(), rep_of_tile.size() * 4, s);
             launch_to_stage_major(mx.d.as<u32>(), npad, mx.ldw, mx.ldw, sx.as<u32>(), s, didx.as<u32>());
             make_stage(my.d.as<u32>(), ny, my.ldw, sy, s);
-            launch_stage_popc(sx.as<u32>(), npad, mx.ldw, px.as<u32>(), s);  // padding rows count 0
+            launch_stage_popc(sx.as<u32>(), npad, stage_rows(npad), mx.ldw, px.as<u32>(), s);  // padding rows: 0
             launch_row_popc(my.d.as<u32>(), ny, my.ldw, my.ldw, py.as<u32>(), s);
             MfmaArgs q;
             q.PA = sx.as<u32>(); q.PB = sy.as<u32>();
-            q.padA = npad; q.padB = ceil_div(ny, TILE) * TILE;
+            q.padA = stage_rows(npad); q.padB = stage_rows(ny);
             q.stages = mx.ldw / 4; q.nA = npad; q.nB = ny;
-            q.ntB = ceil_div(ny, TILE); q.ntiles = (npad / TILE) * q.ntB; q.triangle = 0;
+            q.triangle = 0;
             q.ntip = ntip; q.popA = px.as<u32>(); q.popB = py.as<u32>();
             q.amin = dxmin.as<u32>(); q.bmin = dymin.as<u32>();
             q.arow_map = dmap.as<u32>(); q.tile_rep = drep.as<u32>(); q.bstride = ny;

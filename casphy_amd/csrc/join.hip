@@ -47,7 +47,7 @@ struct Ctl {
     u32 done;
     u32 err;
     u32 steps;
-    u32 pad;
+    u32 arrive;  // k_step workgroups that finished their rescoring
     unsigned long long key;
 };
 
@@ -88,11 +88,11 @@ __device__ __forceinline__ void merge_words(int rule, u32 a0, u32 a1, u32 b0, u3
     }
 }
 
-__global__ __launch_bounds__(1024) void k_select(JoinArgs a)
+// Selection by one workgroup of NT threads (see the file comment); red, sh and
+// fin are LDS scratch of the calling kernel.
+template <int NT>
+__device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long long *red, u32 *sh, u32 *fin)
 {
-    __shared__ unsigned long long red[16];
-    __shared__ u32 sh[5];
-    __shared__ u32 fin[3];  // recompute result: slot, value, partner (LDS copy)
     Ctl *ctl = a.ctl;
     if (ctl->done) return;
     const u32 tid = threadIdx.x, n = a.n, nnode = 2 * n - 1;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
     const u32 fslot = fin[0];
     // argmax of (lmax, -node id) from the candidate (-1, 0)
     unsigned long long best = 0xFFFFFFFFull;  // value -1 (+1 bias = 0), id 0
-    for (u32 s = tid; s < n; s += 1024) {
+    for (u32 s = tid; s < n; s += NT) {
         const int32_t v = s == fslot ? (int32_t)fin[1] : a.lmax[s];
         const u32 id = a.slot2node[s];
         const unsigned long long k = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - id);
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
     __syncthreads();
     if (tid == 0) {
         unsigned long long b = red[0];
-        for (int w = 1; w < 16; ++w) b = red[w] > b ? red[w] : b;
+        for (int w = 1; w < NT / 64; ++w) b = red[w] > b ? red[w] : b;
         const u32 j1 = 0xFFFFFFFFu - (u32)b;
         const int32_t v = (int32_t)(u32)(b >> 32) - 1;
         u32 c1 = j1 < nnode ? a.node2slot[j1] : n;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
     u32 *ra = a.rows + (size_t)c1 * 2 * ldw;
     const u32 *rb = a.rows + (size_t)c2 * 2 * ldw;
     u32 *out = a.nstate + (size_t)(nc - n) * 2 * ldw;
-    for (u32 w = tid; w < ldw; w += 1024) {
+    for (u32 w = tid; w < ldw; w += NT) {
         const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
         u32 r0, r1;
         if (choice == 1) { r0 = b0; r1 = b1; }
@@ -195,13 +195,20 @@ __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
     }
 }
 
-// 16 lanes per target slot, 16 slots per 256-thread workgroup.
-__global__ __launch_bounds__(256) void k_rescore(JoinArgs a)
+__global__ __launch_bounds__(1024) void k_select(JoinArgs a)
 {
-    __shared__ unsigned long long red[4];
+    __shared__ unsigned long long red[16];
+    __shared__ u32 sh[5];
+    __shared__ u32 fin[3];  // recompute result: slot, value, partner (LDS copy)
+    select_phase<1024>(a, red, sh, fin);
+}
+
+// 16 lanes per target slot, 16 slots per 256-thread workgroup.
+__device__ __forceinline__ void rescore_phase(const JoinArgs &a, unsigned long long *red)
+{
     const Ctl *ctl = a.ctl;
     const u32 mode = ctl->mode;
-    if (mode != MODE_RESCORE && mode != MODE_RECOMPUTE) return;
+    if (mode != MODE_RESCORE && mode != MODE_RECOMPUTE) return;  // uniform over the grid
     const u32 c1 = ctl->c1, newnode = ctl->newnode, n = a.n, nnode = 2 * n - 1, ldw = a.ldw;
     const u32 tid = threadIdx.x, l16 = tid & 15;
     const u32 k = blockIdx.x * 16 + (tid >> 4);
@@ -251,6 +258,33 @@ __global__ __launch_bounds__(256) void k_rescore(JoinArgs a)
     }
 }
 
+__global__ __launch_bounds__(256) void k_rescore(JoinArgs a)
+{
+    __shared__ unsigned long long red[4];
+    rescore_phase(a, red);
+}
+
+// One join step in one launch: every workgroup rescores its 16 slots, then the
+// last workgroup to arrive (device-scope counter after a release fence) runs
+// the selection for the next step.  No workgroup waits on another.
+__global__ __launch_bounds__(256) void k_step(JoinArgs a)
+{
+    __shared__ unsigned long long red[16];
+    __shared__ u32 sh[5];
+    __shared__ u32 fin[3];
+    __shared__ u32 last;
+    if (a.ctl->done) return;
+    rescore_phase(a, red);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&a.ctl->arrive, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) a.ctl->arrive = 0;
+    select_phase<256>(a, red, sh, fin);
+}
+
 // keys -> lmax/lidx, maps, sizes, control block.
 __global__ void k_join_init(JoinArgs a, const u64 *keys, const u32 *tip_size)
 {
@@ -271,7 +305,7 @@ __global__ void k_join_init(JoinArgs a, const u64 *keys, const u32 *tip_size)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Ctl *c = a.ctl;
         c->nodeCount = n; c->mode = MODE_NONE; c->c1 = 0; c->newnode = 0;
-        c->done = n < 2 ? 1u : 0u; c->err = 0; c->steps = 0; c->pad = 0; c->key = 0;
+        c->done = n < 2 ? 1u : 0u; c->err = 0; c->steps = 0; c->arrive = 0; c->key = 0;
     }
 }
 
@@ -640,12 +674,11 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
 
     launch_fill_u64(t.keys.as<u64>(), n, 0, s);
     if (planes.nplanes == 0 && mfma_enabled()) {
-        const u32 nt = ceil_div(n, TILE);
-        DevBuf stage((size_t)nt * TILE * ldw * 4);
+        DevBuf stage((size_t)stage_rows(n) * ldw * 4);
         launch_to_stage_major(a.rows, n, 2 * ldw, ldw, stage.as<u32>(), s);
         MfmaArgs q;
-        q.PA = q.PB = stage.as<u32>(); q.padA = q.padB = nt * TILE; q.stages = ldw / 4;
-        q.nA = n; q.nB = n; q.ntiles = (u32)((u64)nt * (nt + 1) / 2);
+        q.PA = q.PB = stage.as<u32>(); q.padA = q.padB = stage_rows(n); q.stages = ldw / 4;
+        q.nA = n; q.nB = n;
         q.keys = t.keys.as<u64>();
         launch_mfma_tiles(E_LOCALMAX, q, s);
         CASPHY_HIP(hipStreamSynchronize(s));
@@ -673,10 +706,22 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     const bool use_graph = !(env && env[0] == '0');
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
+    // first selection, then fused steps (rescore + next selection) per launch;
+    // CASPHY_JOIN_FUSED=0 keeps the two-kernel step
+    const char *fenv = std::getenv("CASPHY_JOIN_FUSED");
+    const bool fused = !(fenv && fenv[0] == '0');
+    if (fused) {
+        hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, a);
+        CASPHY_HIP(hipGetLastError());
+    }
     auto enqueue_chunk = [&]() {
         for (u32 i = 0; i < CH; ++i) {
-            hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, a);
-            hipLaunchKernelGGL(k_rescore, dim3(rescore_grid), dim3(256), 0, s, a);
+            if (fused) {
+                hipLaunchKernelGGL(k_step, dim3(rescore_grid), dim3(256), 0, s, a);
+            } else {
+                hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, a);
+                hipLaunchKernelGGL(k_rescore, dim3(rescore_grid), dim3(256), 0, s, a);
+            }
         }
     };
     if (use_graph) {

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
                 }
             } else if (EPI == E_TBE) {
                 const u32 pb = j < a.nB ? a.popB[j] : 0u;
-                const u32 bbase = a.tile_rep ? a.tile_rep[I] * a.bstride : 0u;
+                const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
                 u32 cmin = 0xFFFFFFFFu;
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
@@ -245,13 +245,13 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
         }
 }
 
-__global__ void k_stage_popc(const u32 *P, u32 n, u32 stages, u32 *pop)
+__global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     u32 c = 0;
     for (u32 st = 0; st < stages; ++st) {
-        const uint4 v = reinterpret_cast<const uint4 *>(P)[(size_t)st * n + r];
+        const uint4 v = reinterpret_cast<const uint4 *>(P)[(size_t)st * stride + r];
         c += __builtin_popcount(v.x) + __builtin_popcount(v.y) + __builtin_popcount(v.z) + __builtin_popcount(v.w);
     }
     pop[r] = c;
@@ -263,16 +263,174 @@ u32 grid_for(u64 items) { return (u32)std::min<u64>((items + 255) / 256, 65535ul
 
 void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s, const u32 *rowidx)
 {
-    const u32 nrows_pad = ceil_div(n, TILE) * TILE;
+    const u32 nrows_pad = stage_rows(n);
     hipLaunchKernelGGL(k_to_stage_major, dim3(grid_for((u64)(kw / 4) * nrows_pad)), dim3(256), 0, s, mat, rowidx, n,
                        ldw, kw, nrows_pad, P);
     CASPHY_HIP(hipGetLastError());
 }
 
-void launch_stage_popc(const u32 *P, u32 n, u32 ldw, u32 *pop, hipStream_t s)
+void launch_stage_popc(const u32 *P, u32 n, u32 stride, u32 ldw, u32 *pop, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_stage_popc, dim3(ceil_div(n, 256)), dim3(256), 0, s, P, n, ldw / 4, pop);
+    hipLaunchKernelGGL(k_stage_popc, dim3(ceil_div(n, 256)), dim3(256), 0, s, P, n, stride, ldw / 4, pop);
     CASPHY_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident variant: no LDS.  A workgroup of 4 independent waves owns a
+// 256 x 256 pair tile; each wave owns 128 x 128 pairs as 4 x 4 MFMA blocks
+// (256 int32 accumulators per lane).  Per 128-site stage a lane loads one
+// 16-byte stage-major chunk for each of its 8 row blocks (32 consecutive rows
+// per block: one 512-byte coalesced piece; the two lane halves share it) and
+// expands only the 16 bits its MFMA operand needs, right before the 16 MFMAs
+// that use them.  Each expanded fragment feeds 4 MFMAs.
+constexpr u32 RT = 256;  // workgroup tile
+
+template <int EPI>
+__global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
+{
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (a.triangle) {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+    const u32 half = lane >> 5, col = lane & 31;
+    const u32 arow0 = I * RT + wm * 128, brow0 = J * RT + wn * 128;
+    const uint4 *PA = reinterpret_cast<const uint4 *>(a.PA) + arow0 + col;
+    const uint4 *PB = reinterpret_cast<const uint4 *>(a.PB) + brow0 + col;
+
+    v16i acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
+
+    uint4 ra[4], rb[4], na[4], nb[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        ra[x] = PA[32 * x];
+        rb[x] = PB[32 * x];
+    }
+    const u32 nst = a.stages;
+    for (u32 s = 0; s < nst; ++s) {
+        if (s + 1 < nst) {
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                na[x] = PA[(size_t)(s + 1) * a.padA + 32 * x];
+                nb[x] = PB[(size_t)(s + 1) * a.padB + 32 * x];
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            v4i fa[4], fb[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const u32 wa = kk == 0 ? ra[x].x : kk == 1 ? ra[x].y : kk == 2 ? ra[x].z : ra[x].w;
+                const u32 wb = kk == 0 ? rb[x].x : kk == 1 ? rb[x].y : kk == 2 ? rb[x].z : rb[x].w;
+                const uint4 ea = spread16(half ? wa >> 16 : wa);
+                const uint4 eb = spread16(half ? wb >> 16 : wb);
+                fa[x] = v4i{(int)ea.x, (int)ea.y, (int)ea.z, (int)ea.w};
+                fb[x] = v4i{(int)eb.x, (int)eb.y, (int)eb.z, (int)eb.w};
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            ra[x] = na[x];
+            rb[x] = nb[x];
+        }
+    }
+
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const u32 r0 = arow0 + 32 * x;  // block row base
+            const u32 c0 = brow0 + 32 * y;  // block column base
+            const u32 j = c0 + col;
+            if (EPI == E_MASK) {
+                u32 colbits = 0;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
+                    const bool hit = acc[x][y][v] >= a.thr;
+                    const unsigned long long b = __ballot(hit && j < a.nB);
+                    if ((lane == (u32)v || lane == (u32)(v + 32)) && c0 < a.mstride * 32) {
+                        const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
+                        const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
+                        const u32 i = r0 + m_out;
+                        if (i < a.nA) a.mask[(size_t)i * a.mstride + c0 / 32] = word;
+                    }
+                    if (hit && r0 + m < a.nA) colbits |= 1u << m;
+                }
+                if (a.triangle && I != J) {
+                    colbits |= __shfl_xor(colbits, 32);
+                    if (half == 0 && j < a.nB && r0 < a.mstride * 32)
+                        a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
+                }
+            } else if (EPI == E_SCORES) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                    if (i < a.nA && j < a.nB) {
+                        a.S[(size_t)i * a.ldS + j] = acc[x][y][v];
+                        if (a.triangle && I != J) a.S[(size_t)j * a.ldS + i] = acc[x][y][v];
+                    }
+                }
+            } else if (EPI == E_LOCALMAX) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                    unsigned long long key = 0;
+                    if (j < a.nB && j > i) key = ((unsigned long long)(u32)acc[x][y][v] << 32) | (0xFFFFFFFFu - j);
+#pragma unroll
+                    for (int o = 1; o < 32; o <<= 1) {
+                        const unsigned long long q = __shfl_xor(key, o);
+                        key = q > key ? q : key;
+                    }
+                    if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
+                }
+            } else if (EPI == E_TBE) {
+                const u32 pb = j < a.nB ? a.popB[j] : 0u;
+                const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
+                u32 cmin = 0xFFFFFFFFu;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                    const u32 out = i < a.nA ? (a.arow_map ? a.arow_map[i] : i) : 0xFFFFFFFFu;
+                    const bool valid = out != 0xFFFFFFFFu && j < a.nB;
+                    const u32 h = (valid ? a.popA[i] : 0u) + pb - 2u * (u32)acc[x][y][v];
+                    const u32 hc = a.ntip - h;
+                    u32 d = valid ? (h < hc ? h : hc) : 0xFFFFFFFFu;
+                    cmin = d < cmin ? d : cmin;
+#pragma unroll
+                    for (int o = 1; o < 32; o <<= 1) {
+                        const u32 q = __shfl_xor(d, o);
+                        d = q < d ? q : d;
+                    }
+                    if (col == 0 && out != 0xFFFFFFFFu) atomicMin(&a.amin[out], d);
+                }
+                const u32 q = __shfl_xor(cmin, 32);
+                cmin = q < cmin ? q : cmin;
+                if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
+            }
+        }
 }
 
 template <int KB, int PF, int OCC>
@@ -291,28 +449,52 @@ void launch_variant(int epi, const MfmaArgs &a, hipStream_t s)
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }
 
-// Stage depth / prefetch variants (CASPHY_MFMA_VARIANT), picked by measurement.
-const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8"};
+void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    if (epi == E_MASK)
+        hipLaunchKernelGGL(k_mfma_reg<E_MASK>, dim3(grid), dim3(256), 0, s, a);
+    else if (epi == E_SCORES)
+        hipLaunchKernelGGL(k_mfma_reg<E_SCORES>, dim3(grid), dim3(256), 0, s, a);
+    else if (epi == E_LOCALMAX)
+        hipLaunchKernelGGL(k_mfma_reg<E_LOCALMAX>, dim3(grid), dim3(256), 0, s, a);
+    else if (epi == E_TBE)
+        hipLaunchKernelGGL(k_mfma_reg<E_TBE>, dim3(grid), dim3(256), 0, s, a);
+    else
+        throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
+}
+
+// Variants (CASPHY_MFMA_VARIANT): LDS-staged with stage depth x prefetch, or
+// register-resident ("reg"); picked by measurement.
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg"};
+constexpr int kDefaultVariant = 3;
 
 int mfma_variant()
 {
     const char *e = std::getenv("CASPHY_MFMA_VARIANT");
     if (e)
-        for (int k = 0; k < 6; ++k)
+        for (int k = 0; k < 7; ++k)
             if (!std::strcmp(e, kVariants[k])) return k;
-    return 0;
+    return kDefaultVariant;
 }
 
-void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
+void launch_mfma_tiles(int epi, const MfmaArgs &in, hipStream_t s)
 {
+    MfmaArgs a = in;
+    const int v = mfma_variant();
+    const u32 T = v == 6 ? RT : TILE;
+    const u32 ntA = ceil_div(a.nA, T);
+    a.ntB = ceil_div(a.nB, T);
+    a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1) / 2) : ntA * a.ntB;
     if (a.ntiles == 0) return;
-    switch (mfma_variant()) {
+    switch (v) {
     case 0: launch_variant<128, 1, 2>(epi, a, s); break;
     case 1: launch_variant<128, 2, 2>(epi, a, s); break;
     case 2: launch_variant<128, 4, 2>(epi, a, s); break;
     case 3: launch_variant<64, 2, 4>(epi, a, s); break;
     case 4: launch_variant<64, 4, 4>(epi, a, s); break;
-    default: launch_variant<64, 8, 4>(epi, a, s); break;
+    case 5: launch_variant<64, 8, 4>(epi, a, s); break;
+    default: launch_reg(epi, a, s); break;
     }
     CASPHY_HIP(hipGetLastError());
 }

@@ -5,14 +5,18 @@
 
 namespace casphy {
 
+// Rows of every stage-major copy are padded to a multiple of this.
+constexpr u32 STAGE_PAD = 256;
+inline u32 stage_rows(u32 n) { return ceil_div(n, STAGE_PAD) * STAGE_PAD; }
+
 struct MfmaArgs {
     const u32 *PA = nullptr;  // stage-major packed bits of the A rows: [stages][nrows_padA][4]
     const u32 *PB = nullptr;  // and of the B rows (== PA for self scoring)
     u32 padA = 0, padB = 0;   // padded row counts (multiples of 128)
     u32 stages = 0;           // 128 sites each
     u32 nA = 0, nB = 0;       // valid rows
-    u32 ntB = 0;              // tiles along B (full grids)
-    u32 ntiles = 0;
+    u32 ntB = 0;              // set by the launcher (tile grid depends on the variant)
+    u32 ntiles = 0;           // set by the launcher
     int triangle = 1;         // A == B, tiles I <= J
     int32_t thr = 0;          // E_MASK
     u32 *mask = nullptr;
@@ -24,18 +28,19 @@ struct MfmaArgs {
     const u32 *popA = nullptr, *popB = nullptr;
     u32 *amin = nullptr, *bmin = nullptr;
     const u32 *arow_map = nullptr;  // E_TBE batches: padded A row -> output row (~0u = padding)
-    const u32 *tile_rep = nullptr;  // E_TBE batches: A tile -> replicate (bmin row block)
+    const u32 *tile_rep = nullptr;  // E_TBE batches: 128-row block of A -> replicate (bmin row block)
     u32 bstride = 0;                // E_TBE batches: bmin entries per replicate
 };
 
 // First kw words of cell-major rows of stride ldw -> stage-major P (rows padded
-// to a multiple of 128; kw a multiple of 32).  With rowidx, row r of P is
+// to a multiple of STAGE_PAD; kw a multiple of 32).  With rowidx, row r of P is
 // row rowidx[r] of mat.
 void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s,
                            const u32 *rowidx = nullptr);
 bool mfma_enabled();
-// Row popcounts of a stage-major matrix with n rows (n a multiple of 128).
-void launch_stage_popc(const u32 *P, u32 n, u32 ldw, u32 *pop, hipStream_t s);
+// Row popcounts of the first n rows of a stage-major matrix whose stages hold
+// `stride` rows.
+void launch_stage_popc(const u32 *P, u32 n, u32 stride, u32 ldw, u32 *pop, hipStream_t s);
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
 
 }  // namespace casphy

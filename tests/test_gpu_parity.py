@@ -16,10 +16,12 @@ def rand_bits(rng, n, W, p):
 
 
 # ------------------------------------------------------------------ scores
-@pytest.fixture(params=["mfma", "valu"])
+@pytest.fixture(params=["mfma-lds", "mfma-reg", "valu"])
 def bitpath(request, monkeypatch):
-    """Runs a test on the int8 matrix-core tiles and on the VALU popcount tiles."""
-    monkeypatch.setenv("CASPHY_MFMA", "1" if request.param == "mfma" else "0")
+    """Runs a test on both int8 matrix-core tile variants (LDS-staged bytes,
+    register-resident bits) and on the VALU popcount tiles."""
+    monkeypatch.setenv("CASPHY_MFMA", "0" if request.param == "valu" else "1")
+    monkeypatch.setenv("CASPHY_MFMA_VARIANT", "reg" if request.param == "mfma-reg" else "64x2")
     return request.param
 
 
@@ -110,7 +112,7 @@ def test_radius_dna_iupac_nibble_path(hip, orc):
         assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
-def test_try_radius(hip, orc):
+def test_try_radius(hip, orc, bitpath):
     mat = synth.binary_matrix(seed=4, n=900, nsite=512, p=0.01)
     bmat = np.ascontiguousarray(mat.T)
     q = synth.query_rows(5, 900, 300)
@@ -142,7 +144,14 @@ def test_join_bit(hip, orc, bitpath, n, W, p, anc):
     assert np.array_equal(ns, ns2) and np.array_equal(nb, nb2)
 
 
-def test_join_bit_synthetic(hip, orc):
+@pytest.fixture(params=["fused", "two-kernel"])
+def joinstep(request, monkeypatch):
+    """Single-tree join with one launch per step or select + rescore launches."""
+    monkeypatch.setenv("CASPHY_JOIN_FUSED", "1" if request.param == "fused" else "0")
+    return request.param
+
+
+def test_join_bit_synthetic(hip, orc, joinstep):
     mat = synth.binary_matrix(seed=21, n=1200, nsite=512, p=0.01)
     e, ns, _ = hip.join_bit(mat)
     e2, ns2, _ = orc.join_bit(mat, lazy=True)
@@ -169,7 +178,7 @@ def test_join_indel(hip, orc, bitpath, min_size, ratio):
     assert np.array_equal(e, e2) and np.array_equal(ns, ns2) and np.array_equal(br, br2)
 
 
-def test_join_indel_config1(hip, orc):
+def test_join_indel_config1(hip, orc, joinstep):
     mat = synth.indel_matrix(seed=1)
     e, ns, br = hip.join_indel(mat)
     w = np.ones((mat.max(), mat.shape[1]), np.float32)
@@ -246,7 +255,7 @@ def test_transfer_trees(hip, orc, bitpath):
         assert np.array_equal(g, w)
 
 
-def test_transfer_batch(hip, orc):
+def test_transfer_batch(hip, orc, bitpath):
     rng = np.random.default_rng(77)
     ntip = 500
     parent, leaves = synth.yule_parent(ntip, rng)
