@@ -173,7 +173,84 @@ def extras(cp, synth, cells, q) -> dict:
                             "batched_jobs_per_s": len(boots) / batch_s, "bipartition_pairs_per_s": pairs / batch_s,
                             "mfma_tops": 2 * ntip * pairs / batch_s / 1e12}
     print("# TBE done", file=sys.stderr, flush=True)
+    ex["serving_mix"] = serving_mix(cp, synth)
+    print("# serving done", file=sys.stderr, flush=True)
     return ex
+
+
+def serving_mix(cp, synth, njobs: int = 200, seed: int = 5) -> dict:
+    """Reduced form of the serving configuration: a Poisson stream of tree-build
+    (50/50 indel 200 sites / binary 4,096 sites) and TBE jobs, cells log-uniform
+    in [1k, 16k], run through the request loop twice with the same arrival
+    schedule: queue-only (one worker, one job per launch) and continuous
+    batching (4 workers, small joins batched).  Every result of the batched
+    run is compared with the queue-only run."""
+    import threading
+    rng = np.random.default_rng(seed)
+    # pools the jobs draw their cells from
+    bin_pool = synth.binary_matrix(seed=seed, n=16384, nsite=4096, p=0.005, missing=0.05)
+    ind_pool = synth.indel_matrix(seed=seed, depth=14, nsite=200)
+    trees = {}
+    jobs = []
+    for k in range(njobs):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(16000))))
+        u = rng.random()
+        if u < 0.4:
+            rows = np.sort(rng.choice(bin_pool.shape[0], n, replace=False))
+            jobs.append(("bit", bin_pool[rows]))
+        elif u < 0.8:
+            rows = np.sort(rng.choice(ind_pool.shape[0], n, replace=False))
+            jobs.append(("indel", ind_pool[rows]))
+        else:
+            ntip = int(2 ** int(np.clip(np.round(np.log2(n)), 10, 14)))
+            if ntip not in trees:
+                par, lv = synth.yule_parent(ntip, rng)
+                trees[ntip] = (synth.clade_bitsets(par, lv, ntip), par, lv)
+            Y, par, lv = trees[ntip]
+            X = synth.clade_bitsets(synth.spr(par, lv, int(np.ceil(0.05 * ntip)), rng), lv, ntip)
+            jobs.append(("tbe", (Y, X, ntip)))
+
+    def run(workers, batch, rate):
+        os.environ["CASPHY_SERVE_BATCH"] = "1" if batch else "0"
+        srv = cp.Server(hbm_budget_bytes=32 << 30, workers=workers)
+        gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, len(jobs)) if rate else np.zeros(len(jobs))
+        ids = [None] * len(jobs)
+
+        def feeder():
+            t0 = time.perf_counter()
+            due = 0.0
+            for k, (kind, data) in enumerate(jobs):
+                due += gaps[k]
+                d = t0 + due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[k] = srv.submit_transfer(*data) if kind == "tbe" else srv.submit_join(data, kind)
+        t0 = time.perf_counter()
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        outs = [srv.wait(ids[k])[0] for k in range(len(jobs))]
+        el = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        os.environ.pop("CASPHY_SERVE_BATCH", None)
+        return outs, el, st
+
+    outs_q, el_burst, _ = run(1, False, 0)             # capacity of the queue-only form
+    rate = 0.8 * len(jobs) / el_burst
+    outs_q, el_q, st_q = run(1, False, rate)
+    outs_b, el_b, st_b = run(4, True, rate)
+    for a, b in zip(outs_q, outs_b):
+        if isinstance(a, tuple):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
+        else:
+            assert np.array_equal(a, b)
+    return {"jobs": len(jobs), "arrival_rate_per_s": rate, "cells": "log-uniform 1k..16k",
+            "queue_only": {"jobs_per_s": st_q["jobs_per_s"], "p50_ms": st_q["p50_ms"], "p99_ms": st_q["p99_ms"],
+                           "launches": st_q["launches"]},
+            "continuous_batching": {"jobs_per_s": st_b["jobs_per_s"], "p50_ms": st_b["p50_ms"],
+                                    "p99_ms": st_b["p99_ms"], "launches": st_b["launches"]},
+            "bit_exact_between_forms": True}
 
 
 def main():

@@ -43,6 +43,13 @@ SIGNATURES = [
     ("casphy_cells_fetch", C.c_int, [vp, vp, vp]),
     ("casphy_cells_try_radii", C.c_int, [vp, vp, u32, vp, u32, vp]),
     ("casphy_cells_last_ms", C.c_int, [vp, C.POINTER(f32), C.POINTER(f32)]),
+    ("casphy_server_create", C.c_int, [C.c_int, u64, u32, C.POINTER(vp)]),
+    ("casphy_server_destroy", None, [vp]),
+    ("casphy_server_submit_join", C.c_int, [vp, C.c_int, vp, u32, u32, C.c_int, C.POINTER(u64)]),
+    ("casphy_server_submit_transfer", C.c_int, [vp, vp, u32, vp, u32, u32, u32, C.POINTER(u64)]),
+    ("casphy_server_wait", C.c_int, [vp, u64, vp, u64, C.POINTER(C.c_double)]),
+    ("casphy_server_stats", C.c_int, [vp, C.POINTER(u64), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(u64)]),
 ]
 
 _lib = None

@@ -403,3 +403,67 @@ class Cells:
 
 
 __all__.append("Cells")
+
+
+# ------------------------------------------------------------------ serving
+class Server:
+    """Request loop over one GPU (casphy_server_* of the C ABI): FIFO admission
+    under an HBM budget, `workers` concurrent streams, small joins batched into
+    one persistent launch.  submit_* return a job id; wait(job) returns the
+    job's output (raw edges for joins, (ymin, ysize, xmin, xsize) for
+    transfers) and its latency in ms."""
+
+    def __init__(self, hbm_budget_bytes: int = 64 << 30, workers: int = 4, device: int = 0):
+        h = C.c_void_p()
+        check(load().casphy_server_create(device, int(hbm_budget_bytes), int(workers), C.byref(h)))
+        self._h = h
+        self._shape = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().casphy_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_join(self, mat, kind: str = "bit", ancestral_type: int = 1) -> int:
+        m = _u32(mat) if kind == "bit" else _i32(mat)
+        n, width = m.shape
+        job = C.c_uint64(0)
+        check(load().casphy_server_submit_join(self._h, 0 if kind == "bit" else 2, _p(m), n, width,
+                                               int(ancestral_type), C.byref(job)))
+        self._shape[job.value] = ("join", n)
+        return job.value
+
+    def submit_transfer(self, Y, X, ntip: int) -> int:
+        Y, X = _u32(Y), _u32(X)
+        job = C.c_uint64(0)
+        check(load().casphy_server_submit_transfer(self._h, _p(Y), Y.shape[0], _p(X), X.shape[0], Y.shape[1],
+                                                   int(ntip), C.byref(job)))
+        self._shape[job.value] = ("transfer", Y.shape[0], X.shape[0])
+        return job.value
+
+    def wait(self, job: int):
+        kind = self._shape.pop(job)
+        size = (2 * kind[1] - 2) * 2 if kind[0] == "join" else 2 * kind[1] + 2 * kind[2]
+        out = np.empty(max(size, 1), np.uint32)
+        lat = C.c_double(0)
+        check(load().casphy_server_wait(self._h, int(job), _p(out), size, C.byref(lat)))
+        if kind[0] == "join":
+            return out[:size].reshape(-1, 2), lat.value
+        ny, nx = kind[1], kind[2]
+        return (out[:ny], out[ny:2 * ny], out[2 * ny:2 * ny + nx], out[2 * ny + nx:2 * ny + 2 * nx]), lat.value
+
+    def stats(self) -> dict:
+        c, l = C.c_uint64(0), C.c_uint64(0)
+        p50, p99, jps = C.c_double(0), C.c_double(0), C.c_double(0)
+        check(load().casphy_server_stats(self._h, C.byref(c), C.byref(p50), C.byref(p99), C.byref(jps), C.byref(l)))
+        return {"completed": c.value, "p50_ms": p50.value, "p99_ms": p99.value, "jobs_per_s": jps.value,
+                "launches": l.value}
+
+
+__all__.append("Server")

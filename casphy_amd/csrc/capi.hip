@@ -39,6 +39,12 @@ hipStream_t thread_stream()
         CASPHY_HIP(hipSetDevice(g_device));
         CASPHY_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
         g_stream_device = g_device;
+        // keep freed stream-ordered memory in the pool instead of returning it
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, g_device) == hipSuccess && pool) {
+            uint64_t keep = ~0ull;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
     } else {
         CASPHY_HIP(hipSetDevice(g_device));
     }

@@ -61,36 +61,51 @@ int guarded(F &&body)
 
 inline u32 ceil_div(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 
-// Device buffer owned by the library (RAII).
+// Per-thread stream on the thread's selected device (defined in capi.hip).
+hipStream_t thread_stream();
+
+// Device buffer owned by the library (RAII).  Allocation and release are
+// stream-ordered (hipMallocAsync / hipFreeAsync on the allocating thread's
+// stream, from a pool that keeps freed memory), so concurrent callers on
+// different streams never wait on a device-wide free.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    hipStream_t st = nullptr;
     DevBuf() = default;
     explicit DevBuf(size_t b) { alloc(b); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), st(o.st) { o.p = nullptr; o.bytes = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept
     {
         if (this != &o) {
-            if (p) (void)hipFree(p);
-            p = o.p; bytes = o.bytes;
+            release();
+            p = o.p; bytes = o.bytes; st = o.st;
             o.p = nullptr; o.bytes = 0;
         }
         return *this;
     }
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    ~DevBuf() { release(); }
+    void release() noexcept
+    {
+        if (p) (void)hipFreeAsync(p, st);
+        p = nullptr;
+        bytes = 0;
+    }
     void alloc(size_t b)
     {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        bytes = b;
+        release();
         if (b) {
-            hipError_t e = hipMalloc(&p, b);
+            st = thread_stream();
+            hipError_t e = hipMallocAsync(&p, b, st);
             if (e != hipSuccess) {
                 p = nullptr;
-                throw Error(CASPHY_ENOMEM, "hipMalloc(" + std::to_string(b) + " B) failed: " +
+                (void)hipGetLastError();
+                throw Error(CASPHY_ENOMEM, "hipMallocAsync(" + std::to_string(b) + " B) failed: " +
                                                 hipGetErrorString(e));
             }
+            bytes = b;
         }
     }
     // grow-only reuse

@@ -202,6 +202,38 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
  * kernel of the last radius call and of the whole last radius call. */
 int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms);
 
+/* ------------------------------------------------------------------ serving */
+/* A request loop over one GPU: jobs of mixed sizes are submitted from any
+ * thread, admitted in FIFO order while their estimated device footprint fits
+ * `hbm_budget_bytes`, and run by `workers` worker threads (one stream each, so
+ * their launches overlap on the device).  Joins of at most 2,048 cells that are
+ * waiting when a worker frees up are batched into one persistent launch.  Every
+ * job's output equals the plain entry point's (casphy_join_bit / _indel with
+ * unit weights, casphy_transfer).  No counterpart in the reference (R forks
+ * one executable per job, R/CasPhy_functions.R:2389-2392). */
+#define CASPHY_KIND_INDEL 2
+#define CASPHY_JOB_TRANSFER 3
+typedef struct casphy_server casphy_server;
+int casphy_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, casphy_server **out);
+/* Stops the workers after the running launches; queued jobs are dropped. */
+void casphy_server_destroy(casphy_server *s);
+/* kind CASPHY_KIND_BIT: mat = n x width u32 cell-major bits;
+ * kind CASPHY_KIND_INDEL: mat = n x width int32 states (-1/0/k).  The input is
+ * copied before the call returns. */
+int casphy_server_submit_join(casphy_server *s, int kind, const void *mat, uint32_t n, uint32_t width,
+                              int ancestral_type, uint64_t *job);
+int casphy_server_submit_transfer(casphy_server *s, const uint32_t *Y, uint32_t ny, const uint32_t *X,
+                                  uint32_t nx, uint32_t ncol, uint32_t ntip, uint64_t *job);
+/* Blocks until `job` is done and copies its output: joins (2n-2)*2 raw edge
+ * values, transfers ymin[ny] ysize[ny] xmin[nx] xsize[nx].  A job is waited
+ * once. */
+int casphy_server_wait(casphy_server *s, uint64_t job, uint32_t *out, uint64_t out_len,
+                       double *latency_ms);
+/* Completed jobs, p50 / p99 latency (submit to completion) and throughput
+ * since the first submission; `launches` counts worker launches. */
+int casphy_server_stats(casphy_server *s, uint64_t *completed, double *p50_ms, double *p99_ms,
+                        double *jobs_per_s, uint64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif
