@@ -178,10 +178,10 @@ def extras(cp, synth, cells, q) -> dict:
     return ex
 
 
-def serving_mix(cp, synth, njobs: int = 200, seed: int = 5) -> dict:
+def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 8000) -> dict:
     """Reduced form of the serving configuration: a Poisson stream of tree-build
     (50/50 indel 200 sites / binary 4,096 sites) and TBE jobs, cells log-uniform
-    in [1k, 16k], run through the request loop twice with the same arrival
+    in [1k, max_cells], run through the request loop twice with the same arrival
     schedule: queue-only (one worker, one job per launch) and continuous
     batching (4 workers, small joins batched).  Every result of the batched
     run is compared with the queue-only run."""
@@ -193,7 +193,7 @@ def serving_mix(cp, synth, njobs: int = 200, seed: int = 5) -> dict:
     trees = {}
     jobs = []
     for k in range(njobs):
-        n = int(np.exp(rng.uniform(np.log(1000), np.log(16000))))
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(max_cells))))
         u = rng.random()
         if u < 0.4:
             rows = np.sort(rng.choice(bin_pool.shape[0], n, replace=False))
@@ -234,6 +234,8 @@ def serving_mix(cp, synth, njobs: int = 200, seed: int = 5) -> dict:
         st = srv.stats()
         srv.close()
         os.environ.pop("CASPHY_SERVE_BATCH", None)
+        print(f"# serving run: workers={workers} batching={batch} rate={rate:.1f}/s {el:.1f} s", file=sys.stderr,
+              flush=True)
         return outs, el, st
 
     outs_q, el_burst, _ = run(1, False, 0)             # capacity of the queue-only form
@@ -245,7 +247,7 @@ def serving_mix(cp, synth, njobs: int = 200, seed: int = 5) -> dict:
             assert all(np.array_equal(x, y) for x, y in zip(a, b))
         else:
             assert np.array_equal(a, b)
-    return {"jobs": len(jobs), "arrival_rate_per_s": rate, "cells": "log-uniform 1k..16k",
+    return {"jobs": len(jobs), "arrival_rate_per_s": rate, "cells": f"log-uniform 1k..{max_cells // 1000}k",
             "queue_only": {"jobs_per_s": st_q["jobs_per_s"], "p50_ms": st_q["p50_ms"], "p99_ms": st_q["p99_ms"],
                            "launches": st_q["launches"]},
             "continuous_batching": {"jobs_per_s": st_b["jobs_per_s"], "p50_ms": st_b["p50_ms"],
@@ -310,6 +312,10 @@ def main():
         # VALU: AND + popcount-accumulate per pair-word
         roof = {"bound": "valu", "achieved": 2 * (NSITE // 32) * pairs_tile / tile_s / 1e12,
                 "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s", "kernel": "k_tiles<AND, MASK>"}
+    tuning = cp.mfma_tuning() if roof["bound"] == "mfma" else []
+    chosen = [e["chosen"] for e in tuning if e["epilogue"] == 0 and e["triangle"] == 1]
+    if chosen:
+        roof["variant"] = chosen[0]
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
     roof["kernel_ms"] = tile_s * 1000
@@ -323,6 +329,7 @@ def main():
                                "1,024 queries x radii 1..16 + radiusNeighbor lists at the autoSplit radius",
                    "cells": N, "sites": NSITE, "queries": R, "radius": r, "neighbour_ids": int(total)},
         "roofline": roof,
+        "mfma_tuning": tuning,
     }
     if not args.no_cpu:
         cells.radius(r)

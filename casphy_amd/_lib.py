@@ -22,6 +22,7 @@ SIGNATURES = [
     ("casphy_free", None, [vp]),
     ("casphy_set_device", C.c_int, [C.c_int]),
     ("casphy_synchronize", C.c_int, []),
+    ("casphy_mfma_tuning", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("casphy_score_bit_matrix", C.c_int, [vp, u32, u32, vp, vp]),
     ("casphy_score_indel_matrix", C.c_int, [vp, u32, u32, vp, u32, vp]),
     ("casphy_radius_bit", C.c_int, [vp, u32, u32, vp, i32, vp, C.POINTER(C.POINTER(u32)), C.POINTER(u64)]),

@@ -32,7 +32,7 @@ __all__ = [
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
-    "radiusWeightedDNAParallel",
+    "radiusWeightedDNAParallel", "mfma_tuning",
 ]
 
 
@@ -344,6 +344,20 @@ def radiusWeightedDNAParallel(i: int, mat, weight, radius: float):
 
 
 # ------------------------------------------------------------------ resident
+def mfma_tuning() -> list:
+    """Choices of the MFMA tile autotuner so far: one dict per shape class
+    (epilogue, triangle, log2 row counts, sites) with the chosen variant and
+    the time in ms of every candidate.  CASPHY_MFMA_VARIANT forces a variant
+    and CASPHY_AUTOTUNE=0 keeps the default one."""
+    import json
+    lib = load()
+    n = C.c_size_t(0)
+    check(lib.casphy_mfma_tuning(None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib.casphy_mfma_tuning(buf, n.value + 1, C.byref(n)))
+    return json.loads(buf.value.decode())
+
+
 class Cells:
     """A packed cell matrix resident in HBM (casphy_cells_* of the C ABI).
 

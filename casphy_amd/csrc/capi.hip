@@ -335,6 +335,20 @@ int casphy_synchronize(void)
     return guarded([&] { CASPHY_HIP(hipStreamSynchronize(thread_stream())); });
 }
 
+int casphy_mfma_tuning(char *buf, size_t cap, size_t *len)
+{
+    return guarded([&] {
+        require(len && (buf || cap == 0), "casphy_mfma_tuning: bad arguments");
+        const std::string r = mfma_tuning_report();
+        *len = r.size();
+        if (cap) {
+            const size_t k = std::min(cap - 1, r.size());
+            std::memcpy(buf, r.data(), k);
+            buf[k] = 0;
+        }
+    });
+}
+
 int casphy_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const double *weight, int32_t *S)
 {
     return guarded([&] {

@@ -23,6 +23,10 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 namespace casphy {
 
@@ -464,24 +468,26 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }
 
-// Variants (CASPHY_MFMA_VARIANT): LDS-staged with stage depth x prefetch, or
-// register-resident ("reg"); picked by measurement.
-const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg"};
-constexpr int kDefaultVariant = 3;
+namespace {
 
-int mfma_variant()
+// Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg"};
+constexpr int kNumVariants = 7;
+constexpr int kDefaultVariant = 3;
+constexpr int kCandidates[] = {3, 0, 4, 6};  // what the autotuner times
+
+int variant_from_env()
 {
     const char *e = std::getenv("CASPHY_MFMA_VARIANT");
     if (e)
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < kNumVariants; ++k)
             if (!std::strcmp(e, kVariants[k])) return k;
-    return kDefaultVariant;
+    return -1;
 }
 
-void launch_mfma_tiles(int epi, const MfmaArgs &in, hipStream_t s)
+void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
-    const int v = mfma_variant();
     const u32 T = v == 6 ? RT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntB = ceil_div(a.nB, T);
@@ -497,6 +503,92 @@ void launch_mfma_tiles(int epi, const MfmaArgs &in, hipStream_t s)
     default: launch_reg(epi, a, s); break;
     }
     CASPHY_HIP(hipGetLastError());
+}
+
+// Autotuner: the first launch of each shape class (epilogue, triangle,
+// log2 of both row counts, sites) runs every candidate variant on the real
+// data -- all epilogues are idempotent (mask / score stores, atomic max / min
+// of identical values) -- times them with events and keeps the fastest.
+struct TuneKey {
+    int epi, tri;
+    u32 la, lb, stages;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(epi, tri, la, lb, stages) < std::tie(o.epi, o.tri, o.la, o.lb, o.stages);
+    }
+};
+struct TuneEntry {
+    int variant;
+    float ms[kNumVariants];
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+
+u32 log2_ceil(u32 x)
+{
+    u32 l = 0;
+    while ((1u << l) < x && l < 31) ++l;
+    return l;
+}
+
+}  // namespace
+
+void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    const int forced = variant_from_env();
+    if (forced >= 0) return launch_one(forced, epi, a, s);
+    const char *at = std::getenv("CASPHY_AUTOTUNE");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    CASPHY_HIP(hipStreamIsCapturing(s, &cap));
+    if ((at && at[0] == '0') || cap != hipStreamCaptureStatusNone) return launch_one(kDefaultVariant, epi, a, s);
+    const TuneKey key{epi, a.triangle, log2_ceil(a.nA), log2_ceil(a.nB), a.stages};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) return launch_one(it->second.variant, epi, a, s);
+    }
+    TuneEntry e{};
+    for (float &m : e.ms) m = -1.0f;
+    hipEvent_t ev[2];
+    CASPHY_HIP(hipEventCreate(&ev[0]));
+    CASPHY_HIP(hipEventCreate(&ev[1]));
+    float best = 1e30f;
+    e.variant = kDefaultVariant;
+    for (int v : kCandidates) {
+        CASPHY_HIP(hipEventRecord(ev[0], s));
+        launch_one(v, epi, a, s);
+        CASPHY_HIP(hipEventRecord(ev[1], s));
+        CASPHY_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0;
+        CASPHY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        e.ms[v] = ms;
+        if (ms < best) { best = ms; e.variant = v; }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.emplace(key, e);
+}
+
+std::string mfma_tuning_report()
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out = "[";
+    for (auto &kv : g_tune) {
+        if (out.size() > 1) out += ",";
+        out += "{\"epilogue\":" + std::to_string(kv.first.epi) + ",\"triangle\":" + std::to_string(kv.first.tri) +
+               ",\"log2_rows_a\":" + std::to_string(kv.first.la) + ",\"log2_rows_b\":" + std::to_string(kv.first.lb) +
+               ",\"sites\":" + std::to_string(kv.first.stages * 128) + ",\"chosen\":\"" + kVariants[kv.second.variant] +
+               "\",\"ms\":{";
+        bool first = true;
+        for (int v : kCandidates) {
+            if (!first) out += ",";
+            first = false;
+            out += "\"" + std::string(kVariants[v]) + "\":" + std::to_string(kv.second.ms[v]);
+        }
+        out += "}}";
+    }
+    return out + "]";
 }
 
 }  // namespace casphy

@@ -1,6 +1,8 @@
 // mfma.hpp -- exact AND-count tiles on the int8 matrix cores (declarations).
 #pragma once
 
+#include <string>
+
 #include "tiles.hpp"
 
 namespace casphy {
@@ -42,5 +44,7 @@ bool mfma_enabled();
 // `stride` rows.
 void launch_stage_popc(const u32 *P, u32 n, u32 stride, u32 ldw, u32 *pop, hipStream_t s);
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
+// JSON report of the autotuner's choices (shape class -> variant, timings).
+std::string mfma_tuning_report();
 
 }  // namespace casphy

@@ -52,6 +52,11 @@ void casphy_free(void *p);
 int casphy_set_device(int device);
 /* Blocks until this thread's stream is idle. */
 int casphy_synchronize(void);
+/* JSON list of the MFMA tile variants the autotuner chose so far (one entry
+ * per shape class, with the time of every candidate).  Writes at most cap
+ * bytes including the terminating NUL and stores the full length in *len.
+ * No counterpart in the reference, which has one kernel shape per path. */
+int casphy_mfma_tuning(char *buf, size_t cap, size_t *len);
 
 /* ------------------------------------------------------------------ scoring */
 /* Full n x n shared-mutation score matrix of bit-packed cells, S[i*n+j] =

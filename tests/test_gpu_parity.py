@@ -16,12 +16,17 @@ def rand_bits(rng, n, W, p):
 
 
 # ------------------------------------------------------------------ scores
-@pytest.fixture(params=["mfma-lds", "mfma-reg", "valu"])
+@pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-reg", "valu"])
 def bitpath(request, monkeypatch):
-    """Runs a test on both int8 matrix-core tile variants (LDS-staged bytes,
-    register-resident bits) and on the VALU popcount tiles."""
+    """Runs a test on the autotuned int8 matrix-core path (every candidate
+    variant runs on the first call of a shape class), on both fixed variants
+    (LDS-staged bytes, register-resident bits) and on the VALU popcount tiles."""
     monkeypatch.setenv("CASPHY_MFMA", "0" if request.param == "valu" else "1")
-    monkeypatch.setenv("CASPHY_MFMA_VARIANT", "reg" if request.param == "mfma-reg" else "64x2")
+    if request.param == "mfma-auto":
+        monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
+        monkeypatch.delenv("CASPHY_AUTOTUNE", raising=False)
+    else:
+        monkeypatch.setenv("CASPHY_MFMA_VARIANT", "reg" if request.param == "mfma-reg" else "64x2")
     return request.param
 
 
@@ -31,6 +36,19 @@ def test_score_bit_matrix(hip, orc, bitpath, n, W):
     mat = rand_bits(rng, n, W, 0.15)
     S = hip.score_bit_matrix(mat)
     assert np.array_equal(S, orc.score_bit_matrix(mat).astype(np.int32))
+
+
+def test_autotuner_report(hip, monkeypatch):
+    monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
+    monkeypatch.setenv("CASPHY_MFMA", "1")
+    rng = np.random.default_rng(11)
+    mat = rand_bits(rng, 700, 8, 0.1)
+    S1 = hip.score_bit_matrix(mat)     # first call of this shape class: tuning
+    S2 = hip.score_bit_matrix(mat)     # cached choice
+    assert np.array_equal(S1, S2)
+    rep = [e for e in hip.mfma_tuning() if e["sites"] == 256 and e["log2_rows_a"] == 10]
+    assert rep and all(e["chosen"] in e["ms"] for e in rep)
+    assert all(t > 0 for e in rep for t in e["ms"].values())
 
 
 def test_score_bit_weighted(hip, orc):
