@@ -124,6 +124,7 @@ def extras(cp, synth, cells, q) -> dict:
 
     # binary tree build of the 10,000-cell matrix
     mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)
+    cp.join_bit(mat)  # first call of the shape class runs the tile autotuner
     join_s, _ = timed(lambda: cp.join_bit(mat), 1)
     ex["binary_join_10000x4096"] = {"plain_jobs_per_s": 1 / join_s, "ms_per_job": join_s * 1e3}
     print("# binary join done", file=sys.stderr, flush=True)
@@ -164,6 +165,8 @@ def extras(cp, synth, cells, q) -> dict:
     ref = synth.clade_bitsets(parent, leaves, ntip)
     moves = int(np.ceil(0.05 * (ntip - 3)))
     boots = [synth.clade_bitsets(synth.spr(parent, leaves, moves, rng), leaves, ntip) for _ in range(2)]
+    cp.transfer_dis_batch(ref, boots, ntip)  # autotuner's first call of the shape class
+    cp.transfer_dis(ref, boots[0], ntip)
     plain_s, pl = timed(lambda: [cp.transfer_dis(ref, b, ntip) for b in boots], 1)
     batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, boots, ntip), 1)
     for k in range(len(boots)):
@@ -172,6 +175,22 @@ def extras(cp, synth, cells, q) -> dict:
     ex["tbe_65536_tips"] = {"replicates": len(boots), "plain_jobs_per_s": len(boots) / plain_s,
                             "batched_jobs_per_s": len(boots) / batch_s, "bipartition_pairs_per_s": pairs / batch_s,
                             "mfma_tops": 2 * ntip * pairs / batch_s / 1e12}
+    del boots
+    # small bootstrap trees, where one replicate does not fill the GPU: 4,096 tips x 64 replicates
+    ntip = 4096
+    parent, leaves = synth.yule_parent(ntip, rng)
+    ref = synth.clade_bitsets(parent, leaves, ntip)
+    moves = int(np.ceil(0.05 * (ntip - 3)))
+    boots = [synth.clade_bitsets(synth.spr(parent, leaves, moves, rng), leaves, ntip) for _ in range(64)]
+    cp.transfer_dis_batch(ref, boots, ntip)
+    cp.transfer_dis(ref, boots[0], ntip)
+    plain_s, pl = timed(lambda: [cp.transfer_dis(ref, b, ntip) for b in boots], 1)
+    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, boots, ntip), 1)
+    for k in range(len(boots)):
+        assert np.array_equal(bt[0][k], pl[k][0]) and np.array_equal(bt[2][k], pl[k][2])
+    ex["tbe_4096_tips_64_replicates"] = {"plain_jobs_per_s": len(boots) / plain_s,
+                                         "batched_jobs_per_s": len(boots) / batch_s,
+                                         "speedup": plain_s / batch_s}
     print("# TBE done", file=sys.stderr, flush=True)
     ex["serving_mix"] = serving_mix(cp, synth)
     print("# serving done", file=sys.stderr, flush=True)

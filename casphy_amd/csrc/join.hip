@@ -62,6 +62,7 @@ struct JoinArgs {
     u32 *edge;
     u32 *nstate;
     Ctl *ctl;
+    unsigned long long *part;  // per-workgroup argmax keys of the fused step
     u32 n, ldw;
     int rule;
     u32 min_size;
@@ -90,8 +91,12 @@ __device__ __forceinline__ void merge_words(int rule, u32 a0, u32 a1, u32 b0, u3
 
 // Selection by one workgroup of NT threads (see the file comment); red, sh and
 // fin are LDS scratch of the calling kernel.
-template <int NT>
-__device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long long *red, u32 *sh, u32 *fin)
+// PARTS: the argmax runs over nparts per-workgroup maxima (written by the
+// rescoring workgroups of the same launch, which leave out a slot under
+// recompute) plus the recomputed slot, instead of over every slot.
+template <int NT, bool PARTS>
+__device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long long *red, u32 *sh, u32 *fin,
+                                             u32 nparts)
 {
     Ctl *ctl = a.ctl;
     if (ctl->done) return;
@@ -112,11 +117,22 @@ __device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long lo
     const u32 fslot = fin[0];
     // argmax of (lmax, -node id) from the candidate (-1, 0)
     unsigned long long best = 0xFFFFFFFFull;  // value -1 (+1 bias = 0), id 0
-    for (u32 s = tid; s < n; s += NT) {
-        const int32_t v = s == fslot ? (int32_t)fin[1] : a.lmax[s];
-        const u32 id = a.slot2node[s];
-        const unsigned long long k = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - id);
-        best = k > best ? k : best;
+    if (PARTS) {
+        for (u32 b = tid; b < nparts; b += NT) {
+            const unsigned long long k = a.part[b];
+            best = k > best ? k : best;
+        }
+        if (tid == 0 && fslot != 0xFFFFFFFFu) {
+            const unsigned long long k = ((unsigned long long)(fin[1] + 1u) << 32) | (0xFFFFFFFFu - a.slot2node[fslot]);
+            best = k > best ? k : best;
+        }
+    } else {
+        for (u32 s = tid; s < n; s += NT) {
+            const int32_t v = s == fslot ? (int32_t)fin[1] : a.lmax[s];
+            const u32 id = a.slot2node[s];
+            const unsigned long long k = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - id);
+            best = k > best ? k : best;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long x = __shfl_xor(best, o);
@@ -200,19 +216,23 @@ __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
     __shared__ unsigned long long red[16];
     __shared__ u32 sh[5];
     __shared__ u32 fin[3];  // recompute result: slot, value, partner (LDS copy)
-    select_phase<1024>(a, red, sh, fin);
+    select_phase<1024, false>(a, red, sh, fin, 0);
 }
 
-// 16 lanes per target slot, 16 slots per 256-thread workgroup.
-__device__ __forceinline__ void rescore_phase(const JoinArgs &a, unsigned long long *red)
+// 16 lanes per target slot, 16 slots per 256-thread workgroup.  Returns (to
+// every thread) the workgroup's argmax key (lmax + 1, smallest node id) over
+// its live slots after the update, leaving out a slot under recompute.
+__device__ __forceinline__ unsigned long long rescore_phase(const JoinArgs &a, unsigned long long *red)
 {
     const Ctl *ctl = a.ctl;
-    const u32 mode = ctl->mode;
-    if (mode != MODE_RESCORE && mode != MODE_RECOMPUTE) return;  // uniform over the grid
+    const u32 mode = ctl->mode;  // uniform over the grid
     const u32 c1 = ctl->c1, newnode = ctl->newnode, n = a.n, nnode = 2 * n - 1, ldw = a.ldw;
     const u32 tid = threadIdx.x, l16 = tid & 15;
     const u32 k = blockIdx.x * 16 + (tid >> 4);
-    const bool live = k < n && k != c1 && a.slot2node[k] < nnode;
+    const u32 node = k < n ? a.slot2node[k] : nnode;
+    const bool alive = node < nnode;
+    const bool active = mode == MODE_RESCORE || mode == MODE_RECOMPUTE;
+    const bool live = active && alive && k != c1;
     u32 S = 0;
     if (live) {
         const u32 *q = a.rows + (size_t)c1 * 2 * ldw;
@@ -236,32 +256,37 @@ __device__ __forceinline__ void rescore_phase(const JoinArgs &a, unsigned long l
         }
         if (a.nplanes) S <<= a.ptail;
     }
-    if (mode == MODE_RESCORE) {
-        if (live && l16 == 0 && (int32_t)S > a.lmax[k]) {
-            a.lmax[k] = (int32_t)S;
+    unsigned long long pk = 0, rk = 0;
+    if (l16 == 0 && alive && !(mode == MODE_RECOMPUTE && k == c1)) {
+        int32_t v = a.lmax[k];
+        if (mode == MODE_RESCORE && live && (int32_t)S > v) {
+            v = (int32_t)S;
+            a.lmax[k] = v;
             a.lidx[k] = newnode;
         }
-        return;
+        pk = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - node);
     }
-    unsigned long long key = 0;
-    if (live) key = ((unsigned long long)S << 32) | (0xFFFFFFFFu - a.slot2node[k]);
+    if (mode == MODE_RECOMPUTE && live) rk = ((unsigned long long)S << 32) | (0xFFFFFFFFu - node);
     for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long x = __shfl_xor(key, o);
-        key = x > key ? x : key;
+        const unsigned long long x = __shfl_xor(pk, o), y = __shfl_xor(rk, o);
+        pk = x > pk ? x : pk;
+        rk = y > rk ? y : rk;
     }
-    if ((tid & 63) == 0) red[tid >> 6] = key;
+    if ((tid & 63) == 0) { red[tid >> 6] = pk; red[4 + (tid >> 6)] = rk; }
     __syncthreads();
-    if (tid == 0) {
-        unsigned long long b = red[0];
-        for (int w = 1; w < 4; ++w) b = red[w] > b ? red[w] : b;
-        if (b) atomicMax((unsigned long long *)&((Ctl *)ctl)->key, b);
+    pk = red[0]; rk = red[4];
+    for (int w = 1; w < 4; ++w) {
+        pk = red[w] > pk ? red[w] : pk;
+        rk = red[4 + w] > rk ? red[4 + w] : rk;
     }
+    if (mode == MODE_RECOMPUTE && tid == 0 && rk) atomicMax((unsigned long long *)&((Ctl *)ctl)->key, rk);
+    return pk;
 }
 
 __global__ __launch_bounds__(256) void k_rescore(JoinArgs a)
 {
-    __shared__ unsigned long long red[4];
-    rescore_phase(a, red);
+    __shared__ unsigned long long red[8];
+    (void)rescore_phase(a, red);
 }
 
 // One join step in one launch: every workgroup rescores its 16 slots, then the
@@ -274,7 +299,8 @@ __global__ __launch_bounds__(256) void k_step(JoinArgs a)
     __shared__ u32 fin[3];
     __shared__ u32 last;
     if (a.ctl->done) return;
-    rescore_phase(a, red);
+    const unsigned long long pk = rescore_phase(a, red);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = pk;
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(&a.ctl->arrive, 1u) == gridDim.x - 1;
@@ -282,7 +308,7 @@ __global__ __launch_bounds__(256) void k_step(JoinArgs a)
     if (!last) return;
     __threadfence();
     if (threadIdx.x == 0) a.ctl->arrive = 0;
-    select_phase<256>(a, red, sh, fin);
+    select_phase<256, true>(a, red, sh, fin, gridDim.x);
 }
 
 // keys -> lmax/lidx, maps, sizes, control block.
@@ -632,7 +658,7 @@ Planes indel_planes(const IndelLayout &L, const double *weight, u32 nstate, u32 
 struct Tree {
     u32 n = 0, ldw = 0;
     int rule = 1;
-    DevBuf rows, lmax, lidx, s2n, n2s, size, edge, nstate, ctl, keys, pmask;
+    DevBuf rows, lmax, lidx, s2n, n2s, size, edge, nstate, ctl, keys, pmask, part;
     JoinArgs args{};
 };
 
@@ -650,6 +676,7 @@ void tree_alloc(Tree &t, u32 n, u32 ldw, int rule)
     t.nstate.alloc((size_t)(n - 1) * 2 * ldw * 4);
     t.ctl.alloc(sizeof(Ctl));
     t.keys.alloc((size_t)n * 8);
+    t.part.alloc((size_t)ceil_div(n, 16) * 8);
 }
 
 // Initial local maxima (all-pairs upper triangle with the LOCALMAX epilogue)
@@ -662,6 +689,7 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     a.rows = t.rows.as<u32>(); a.lmax = t.lmax.as<int32_t>(); a.lidx = t.lidx.as<u32>();
     a.slot2node = t.s2n.as<u32>(); a.node2slot = t.n2s.as<u32>(); a.size = t.size.as<u32>();
     a.edge = t.edge.as<u32>(); a.nstate = t.nstate.as<u32>(); a.ctl = t.ctl.as<Ctl>();
+    a.part = t.part.as<unsigned long long>();
     a.n = n; a.ldw = ldw; a.rule = t.rule; a.min_size = min_size; a.min_ratio = min_ratio;
     a.nplanes = planes.nplanes;
     if (planes.nplanes) {
