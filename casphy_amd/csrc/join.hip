@@ -10,13 +10,15 @@
 //   edge      raw (parent, child) pairs, nstate = state of each internal node
 //   ctl       control block read and written only by the kernels
 // A join step is two launches with fixed arguments:
-//   k_select   (one workgroup) finalises a pending recompute, takes the global
-//              (score, smallest node id) argmax, and either requests a
-//              recompute of a stale row or performs the join: ancestral update
-//              into the first child's slot, bookkeeping, edge record;
 //   k_rescore  (all slots) scores the new node against every live slot and
 //              raises lmax where strictly larger, or computes the stale row's
-//              (score, smallest node id) maximum for the recompute.
+//              (score, smallest node id) maximum for the recompute; each
+//              workgroup also leaves the argmax key of its 16 slots;
+//   k_select   (one workgroup) finalises a pending recompute, takes the global
+//              (score, smallest node id) argmax over those keys, and either
+//              requests a recompute of a stale row or performs the join:
+//              ancestral update into the first child's slot, bookkeeping, edge
+//              record.
 // The host only enqueues steps (captured once into a hipGraph and replayed)
 // and polls the done flag every chunk, so no per-join round trip exists.
 // Semantics follow cu/joinEdgeBitReverse.cu:347-505 / cu/joinEdgeIndel.cu:
@@ -62,7 +64,8 @@ struct JoinArgs {
     u32 *edge;
     u32 *nstate;
     Ctl *ctl;
-    unsigned long long *part;  // per-workgroup argmax keys of the fused step
+    unsigned long long *part;  // per-workgroup argmax keys left by the rescoring launch
+    u32 nparts;                // = rescoring workgroups, ceil(n / 16)
     u32 n, ldw;
     int rule;
     u32 min_size;
@@ -211,12 +214,15 @@ __device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long lo
     }
 }
 
+// PARTS = false: full scan over the slots (first selection after the initial
+// maxima); PARTS = true: over the maxima k_rescore left per workgroup.
+template <bool PARTS>
 __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
 {
     __shared__ unsigned long long red[16];
     __shared__ u32 sh[5];
     __shared__ u32 fin[3];  // recompute result: slot, value, partner (LDS copy)
-    select_phase<1024, false>(a, red, sh, fin, 0);
+    select_phase<1024, PARTS>(a, red, sh, fin, a.nparts);
 }
 
 // 16 lanes per target slot, 16 slots per 256-thread workgroup.  Returns (to
@@ -286,7 +292,9 @@ __device__ __forceinline__ unsigned long long rescore_phase(const JoinArgs &a, u
 __global__ __launch_bounds__(256) void k_rescore(JoinArgs a)
 {
     __shared__ unsigned long long red[8];
-    (void)rescore_phase(a, red);
+    if (a.ctl->done) return;
+    const unsigned long long pk = rescore_phase(a, red);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = pk;
 }
 
 // One join step in one launch: every workgroup rescores its 16 slots, then the
@@ -730,25 +738,27 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     // Join loop: CH steps per captured graph, poll the done flag per chunk.
     const u32 CH = 64;
     const u32 rescore_grid = ceil_div(n, 16);
+    a.nparts = rescore_grid;
     const char *env = std::getenv("CASPHY_JOIN_GRAPH");
     const bool use_graph = !(env && env[0] == '0');
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    // first selection, then fused steps (rescore + next selection) per launch;
-    // CASPHY_JOIN_FUSED=0 keeps the two-kernel step
+    // First selection by a full scan, then per step a rescoring launch that
+    // leaves per-workgroup maxima and a one-workgroup selection over them.
+    // CASPHY_JOIN_FUSED=1 runs both in one launch (the last workgroup to
+    // arrive selects); its per-workgroup device-scope release fences cost more
+    // than the second launch on this part, so it is not the default.
     const char *fenv = std::getenv("CASPHY_JOIN_FUSED");
-    const bool fused = !(fenv && fenv[0] == '0');
-    if (fused) {
-        hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, a);
-        CASPHY_HIP(hipGetLastError());
-    }
+    const bool fused = fenv && fenv[0] == '1';
+    hipLaunchKernelGGL(k_select<false>, dim3(1), dim3(1024), 0, s, a);
+    CASPHY_HIP(hipGetLastError());
     auto enqueue_chunk = [&]() {
         for (u32 i = 0; i < CH; ++i) {
             if (fused) {
                 hipLaunchKernelGGL(k_step, dim3(rescore_grid), dim3(256), 0, s, a);
             } else {
-                hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, s, a);
                 hipLaunchKernelGGL(k_rescore, dim3(rescore_grid), dim3(256), 0, s, a);
+                hipLaunchKernelGGL(k_select<true>, dim3(1), dim3(1024), 0, s, a);
             }
         }
     };
@@ -782,6 +792,10 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     if (exec) CASPHY_HIP(hipGraphExecDestroy(exec));
     if (graph) CASPHY_HIP(hipGraphDestroy(graph));
     CASPHY_REQUIRE(!h.err, CASPHY_EINTERNAL, "join found no live pair");
+    const char *st = std::getenv("CASPHY_JOIN_STATS");
+    if (st && st[0] == '1')
+        std::fprintf(stderr, "join stats: n=%u selections=%u launched=%llu\n", n, h.steps,
+                     (unsigned long long)launched);
 }
 
 
