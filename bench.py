@@ -8,13 +8,17 @@ one plain pass of the radius hot path over the resident matrix:
   1. tryRadius for 1,024 seeded query rows at radii 1..16 (one launch);
   2. the radius that autoSplit's rule takes from those counts
      (R/CasPhy_functions.R:2004-2021 with radius_initial = 1, step = 1:
-     the first r whose log10(mean count) <= 9.5 - log10(n));
+     the first r whose log10(mean count) <= 9.5 - log10(n)), chosen on the
+     device;
   3. radiusNeighbor lists at that radius for all 10,000 cells (upper-triangle
      pair tiles -> neighbour bitmask -> counts -> scan -> on-device compaction).
+All three run as one device round (casphy_cells_autosplit) with one host
+synchronisation; before timing, its result is checked against the host-driven
+loop (tryRadius counts back to the host, host rule, radiusNeighbor).
 
 value = cell-pair scores per second: (n(n+1)/2 + 1024 n) scores per step.
-The matrix is resident in HBM before timing; per step only the query list
-(4 KB) goes up and the counts (64 KB) and the list length (8 B) come back.
+The matrix is resident in HBM before timing; per step only the radii and the
+query list (4 KB) go up and the chosen radius and list length (16 B) come back.
 """
 from __future__ import annotations
 
@@ -31,16 +35,18 @@ sys.path.insert(0, ROOT)
 
 N, NSITE, R = 10000, 4096, 1024
 RADII = np.arange(1, 17, dtype=np.int32)
+AUTOSPLIT_BOUND = 9.5  # nNeighborBound default of the autoSplit loop, R/CasPhy_functions.R:1967
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
 MFMA_I8_PEAK_TOPS = 5000.0                # MI355X dense int8 matrix-core peak, 5.0 POP/s
 
 
 def autosplit_radius(counts: np.ndarray, n: int) -> int:
-    bound = 9.5 - np.log10(n)
+    """Host form of the autoSplit stopping rule over the tried radii."""
+    bound = AUTOSPLIT_BOUND - np.log10(n)
     for k, r in enumerate(RADII):
         if np.log10(max(counts[k].mean(), 1e-30)) <= bound:
             return int(r)
-    return int(RADII[-1])
+    return -1
 
 
 def cpu_baseline(mat: np.ndarray, radius: int, gpu_counts: np.ndarray, budget_s: float = 12.0) -> dict:
@@ -300,19 +306,29 @@ def main():
     bmat = np.ascontiguousarray(mat.T)  # .bmat payload (word-major)
     cells = cp.Cells(bmat, "bit", wordmajor=True)
 
-    def step():
-        counts = cells.try_radii(RADII, q)
-        r = autosplit_radius(counts, N)
-        total = cells.radius(r)
-        return r, total, counts
+    bound = AUTOSPLIT_BOUND - np.log10(N)
 
+    def step():
+        # tryRadius at 16 radii, radius choice and radius lists: one device round
+        return cells.autosplit(RADII, q, bound)
+
+    # the device round equals the host-driven loop (tryRadius, host rule, radiusNeighbor)
+    counts = cells.try_radii(RADII, q)
+    r_host = autosplit_radius(counts, N)
+    t_host = cells.radius(r_host)
+    c_host, i_host = cells.fetch()
+    r_dev, t_dev, c_dev = cells.autosplit(RADII, q, bound, counts=True)
+    c_d, i_d = cells.fetch()
+    assert r_dev == r_host and t_dev == t_host and np.array_equal(c_dev, counts)
+    assert np.array_equal(c_d, c_host) and np.array_equal(i_d, i_host)
+    del c_host, i_host, c_d, i_d
     for _ in range(args.warmup):
         step()
     cp._lib.check(cp._lib.load().casphy_synchronize())
     tile_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r, total, counts = step()
+        r, total = step()
         tile_ms.append(cells.last_ms()[0])
     cp._lib.check(cp._lib.load().casphy_synchronize())
     el = time.perf_counter() - t0

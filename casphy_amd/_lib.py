@@ -43,6 +43,7 @@ SIGNATURES = [
     ("casphy_cells_radius", C.c_int, [vp, i32, C.POINTER(u64)]),
     ("casphy_cells_fetch", C.c_int, [vp, vp, vp]),
     ("casphy_cells_try_radii", C.c_int, [vp, vp, u32, vp, u32, vp]),
+    ("casphy_cells_autosplit", C.c_int, [vp, vp, u32, vp, u32, C.c_double, C.POINTER(i32), C.POINTER(u64), vp]),
     ("casphy_cells_last_ms", C.c_int, [vp, C.POINTER(f32), C.POINTER(f32)]),
     ("casphy_server_create", C.c_int, [C.c_int, u64, u32, C.POINTER(vp)]),
     ("casphy_server_destroy", None, [vp]),

@@ -410,6 +410,19 @@ class Cells:
         check(load().casphy_cells_try_radii(self._h, _p(r), r.size, _p(q), q.size, _p(out)))
         return out
 
+    def autosplit(self, radii, q, bound: float, counts: bool = False):
+        """One autoSplit round on the device: returns (radius, total list
+        length) -- radius -1 when no radius qualifies -- plus the tryRadius
+        counts (nradii, R) when counts=True.  The lists stay resident (fetch)."""
+        r = np.ascontiguousarray(radii, dtype=np.int32)
+        q = _u32(q)
+        out = np.zeros((r.size, q.size), np.uint32) if counts else None
+        rad, total = C.c_int32(0), C.c_uint64(0)
+        check(load().casphy_cells_autosplit(self._h, _p(r), r.size, _p(q), q.size, float(bound), C.byref(rad),
+                                            C.byref(total), _p(out)))
+        self.total = total.value
+        return (rad.value, self.total, out) if counts else (rad.value, self.total)
+
     def last_ms(self):
         a, b = C.c_float(0), C.c_float(0)
         check(load().casphy_cells_last_ms(self._h, C.byref(a), C.byref(b)))

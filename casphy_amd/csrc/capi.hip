@@ -205,9 +205,11 @@ void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
     launch_to_stage_major(dmat, n, ldw, ldw, P.as<u32>(), s);
 }
 
-void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
-                    RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                    const u32 *stage = nullptr)
+// Radius mask (threshold `radius`, or *thr_dev when given), per-row counts and
+// u64 offsets (offsets[n] = total) on the stream; nothing is read back.
+void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
+                 const int32_t *thr_dev, RadiusOut &o, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                 const u32 *stage)
 {
     const u32 nt = ceil_div(n, TILE);
     const u32 mstride = nt * 4;
@@ -223,7 +225,7 @@ void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &p
         MfmaArgs m;
         m.PA = m.PB = stage; m.padA = m.padB = stage_rows(n); m.stages = ldw / 4;
         m.nA = n; m.nB = n;
-        m.thr = radius; m.mask = o.mask.as<u32>(); m.mstride = mstride;
+        m.thr = radius; m.thr_dev = thr_dev; m.mask = o.mask.as<u32>(); m.mstride = mstride;
         launch_mfma_tiles(E_MASK, m, s);
     } else {
         TileArgs a;
@@ -231,7 +233,7 @@ void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &p
         a.ldw = ldw; a.kw = ldw; a.ntA = nt; a.ntB = nt;
         a.triangle = 1;
         a.ntiles = (u32)((u64)nt * (nt + 1) / 2);
-        a.thr = radius; a.mask = o.mask.as<u32>(); a.mstride = mstride;
+        a.thr = radius; a.thr_dev = thr_dev; a.mask = o.mask.as<u32>(); a.mstride = mstride;
         apply_planes(planes, a, o.pbuf, s);
         launch_tiles(metric, E_MASK, a, s);
     }
@@ -240,9 +242,40 @@ void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &p
     o.offsets.reserve((size_t)(n + 1) * 8);
     launch_row_popc(o.mask.as<u32>(), n, mstride, mstride, o.counts.as<u32>(), s);
     launch_scan_counts(o.counts.as<u32>(), n, o.offsets.as<u64>(), s);
+}
+
+void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
+                    RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                    const u32 *stage = nullptr)
+{
+    radius_mask(metric, dmat, n, ldw, planes, radius, nullptr, o, s, ev0, ev1, stage);
+    const u32 mstride = ceil_div(n, TILE) * 4;
     d2h(&o.total, o.offsets.as<u64>() + n, 8, s);
     o.idx.reserve(std::max<u64>(o.total, 1) * 4);
     launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s);
+}
+
+// Largest sum S of R tryRadius counts (each <= n) whose mean passes the
+// autoSplit stopping rule log10(max(S / R, 1e-30)) <= bound, evaluated with
+// the host's log10 (monotone in S) so the device comparison sums[k] <= smax
+// decides exactly as the host rule would; any = 0 when even S = 0 fails.
+void autosplit_limit(u32 R, u32 n, double bound, u64 &smax, int &any)
+{
+    auto pass = [&](u64 S) { return std::log10(std::max((double)S / (double)R, 1e-30)) <= bound; };
+    const u64 top = (u64)R * n;
+    smax = 0;
+    any = pass(0) ? 1 : 0;
+    if (!any) return;
+    if (pass(top)) {
+        smax = top;
+        return;
+    }
+    u64 lo = 0, hi = top;  // pass(lo), !pass(hi)
+    while (hi - lo > 1) {
+        const u64 mid = lo + (hi - lo) / 2;
+        (pass(mid) ? lo : hi) = mid;
+    }
+    smax = lo;
 }
 
 void radius_lists(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
@@ -605,6 +638,21 @@ struct casphy_cells {
     RadiusOut out;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int device = 0;
+    // autoSplit round: uploads (radii | query rows), query stage copy, scores,
+    // counts, per-radius sums + choice; pinned host staging for one upload and
+    // the read-back of (choice, total)
+    DevBuf a_in, a_sq, a_S, a_counts, a_misc, a_pbuf;
+    void *pin = nullptr;
+    size_t pin_bytes = 0;
+    void pinned(size_t b)
+    {
+        if (b <= pin_bytes) return;
+        if (pin) CASPHY_HIP(hipHostFree(pin));
+        pin = nullptr;
+        pin_bytes = 0;
+        CASPHY_HIP(hipHostMalloc(&pin, b, hipHostMallocDefault));
+        pin_bytes = b;
+    }
 };
 
 int casphy_cells_create(int kind, const uint32_t *mat, uint32_t n, uint32_t W, int wordmajor,
@@ -641,6 +689,7 @@ void casphy_cells_destroy(casphy_cells *h)
     if (!h) return;
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    if (h->pin) (void)hipHostFree(h->pin);
     delete h;
 }
 
@@ -689,6 +738,96 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
             stage = h->stage.as<u32>();
         }
         try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s, stage);
+    });
+}
+
+int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q, uint32_t R,
+                           double bound, int32_t *radius, uint64_t *nidx, uint32_t *counts)
+{
+    return guarded([&] {
+        require(h && radii && q && radius && nidx && R > 0, "casphy_cells_autosplit: bad arguments");
+        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_cells_autosplit: 1..16 radii");
+        const u32 n = h->n, ldw = h->ldw;
+        for (u32 k = 0; k < R; ++k)
+            CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
+        for (u32 k = 1; k < nradii; ++k) require(radii[k - 1] <= radii[k], "radii must be ascending");
+        u64 smax;
+        int any;
+        autosplit_limit(R, n, bound, smax, any);
+        hipStream_t s = thread_stream();
+        const bool mfma = h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled();
+        if (mfma && !h->stage.p) make_stage(h->mat.as<u32>(), n, ldw, h->stage, s);
+        CASPHY_HIP(hipEventRecord(h->ev[2], s));
+
+        // one upload: radii (padded to 16) | query rows; results land after them
+        const size_t up = (MAX_RADII + (size_t)R) * 4, res_off = (up + 15) / 16 * 16;
+        h->pinned(res_off + 16);
+        int32_t *pr = static_cast<int32_t *>(h->pin);
+        for (u32 k = 0; k < MAX_RADII; ++k) pr[k] = k < nradii ? radii[k] : 0x7FFFFFFF;
+        std::memcpy(pr + MAX_RADII, q, (size_t)R * 4);
+        h->a_in.reserve(up);
+        CASPHY_HIP(hipMemcpyAsync(h->a_in.p, h->pin, up, hipMemcpyHostToDevice, s));
+        const int32_t *dr = h->a_in.as<int32_t>();
+        const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
+        h->a_S.reserve((size_t)R * n * 4);
+        h->a_counts.reserve((size_t)R * nradii * 4);
+        h->a_misc.reserve(MAX_RADII * 8 + 16);
+        u64 *sums = h->a_misc.as<u64>();
+        int32_t *choice = reinterpret_cast<int32_t *>(sums + MAX_RADII);
+        launch_fill_u64(sums, MAX_RADII, 0, s);
+
+        // tryRadius counts of the query rows, per-radius sums, device-side choice
+        int32_t *dS = h->a_S.as<int32_t>();
+        if (mfma) {
+            h->a_sq.reserve((size_t)stage_rows(R) * ldw * 4);
+            launch_to_stage_major(h->mat.as<u32>(), R, ldw, ldw, h->a_sq.as<u32>(), s, dq);
+            MfmaArgs m;
+            m.PA = h->a_sq.as<u32>(); m.PB = h->stage.as<u32>(); m.padA = stage_rows(R); m.padB = stage_rows(n);
+            m.stages = ldw / 4; m.nA = R; m.nB = n; m.triangle = 0;
+            m.S = dS; m.ldS = n;
+            launch_mfma_tiles(E_SCORES, m, s);
+        } else {
+            TileArgs a;
+            a.A = h->mat.as<u32>(); a.Aidx = dq; a.nA = R; a.B = h->mat.as<u32>(); a.nB = n;
+            a.ldw = ldw; a.kw = ldw;
+            a.ntA = ceil_div(R, TILE); a.ntB = ceil_div(n, TILE);
+            a.ntiles = a.ntA * a.ntB;
+            a.S = dS; a.ldS = n;
+            apply_planes(h->planes, a, h->a_pbuf, s);
+            launch_tiles(h->metric, E_SCORES, a, s);
+        }
+        launch_count_radii(dS, R, n, n, dr, nradii, h->a_counts.as<u32>(), s, sums);
+        launch_autosplit_choose(sums, dr, nradii, smax, any, choice, s);
+
+        // radiusNeighbor lists at the chosen radius (nothing qualifies: threshold
+        // INT32_MAX, empty lists); ids beyond the current capacity are dropped and
+        // the compaction reruns once the total is known
+        RadiusOut &o = h->out;
+        radius_mask(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
+                    mfma ? h->stage.as<u32>() : nullptr);
+        const u32 mstride = ceil_div(n, TILE) * 4;
+        const u64 cap = o.idx.bytes / 4;
+        if (cap) launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, cap);
+        CASPHY_HIP(hipEventRecord(h->ev[3], s));
+        char *res = static_cast<char *>(h->pin) + res_off;
+        CASPHY_HIP(hipMemcpyAsync(res, choice, 8, hipMemcpyDeviceToHost, s));
+        CASPHY_HIP(hipMemcpyAsync(res + 8, o.offsets.as<u64>() + n, 8, hipMemcpyDeviceToHost, s));
+        if (counts) CASPHY_HIP(hipMemcpyAsync(counts, h->a_counts.p, (size_t)R * nradii * 4, hipMemcpyDeviceToHost, s));
+        CASPHY_HIP(hipStreamSynchronize(s));
+        int32_t ch[2];
+        u64 total;
+        std::memcpy(ch, res, 8);
+        std::memcpy(&total, res + 8, 8);
+        if (total > cap) {
+            o.idx.reserve(total * 4);
+            launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, total);
+            CASPHY_HIP(hipEventRecord(h->ev[3], s));
+            CASPHY_HIP(hipStreamSynchronize(s));
+        }
+        o.total = total;
+        o.n = n;
+        *radius = ch[1] < 0 ? -1 : ch[0];
+        *nidx = total;
     });
 }
 

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, 
 // ids to HBM with consecutive lanes writing consecutive addresses.
 constexpr int CWAVES = 4;
 __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words,
-                                                        const u64 *offsets, u32 *idx)
+                                                        const u64 *offsets, u32 *idx, u64 cap)
 {
     __shared__ __attribute__((aligned(16))) u32 stage[CWAVES][64 * 32];
     const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -172,7 +172,8 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        for (u32 k = lane; k < total; k += 64) idx[base + k] = buf[k];
+        for (u32 k = lane; k < total; k += 64)
+            if (base + k < cap) idx[base + k] = buf[k];
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         base += total;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
 // tryRadius counts from a query x cell score matrix: one workgroup per query,
 // per-thread counters for every radius, then a workgroup reduction.
 __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii,
-                                                     u32 nradii, u32 *counts)
+                                                     u32 nradii, u32 *counts, u64 *sums)
 {
     __shared__ u32 red[16][MAX_RADII_K];
     const u32 q = blockIdx.x, tid = threadIdx.x;
@@ -207,7 +208,27 @@ __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u3
         if ((tid & 63) == 0) red[tid >> 6][k] = x;
     }
     __syncthreads();
-    if (tid < nradii) counts[(size_t)tid * R + q] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    if (tid < nradii) {
+        const u32 c = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        counts[(size_t)tid * R + q] = c;
+        if (sums) atomicAdd((unsigned long long *)&sums[tid], (unsigned long long)c);
+    }
+}
+
+__global__ void k_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, u64 smax, int any,
+                                   int32_t *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int32_t r = 0x7FFFFFFF, k = -1;
+    if (any)
+        for (u32 j = 0; j < nradii; ++j)
+            if (sums[j] <= smax) {
+                r = radii[j];
+                k = (int32_t)j;
+                break;
+            }
+    out[0] = r;
+    out[1] = k;
 }
 
 __global__ void k_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size)
@@ -276,18 +297,26 @@ void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s)
     CASPHY_HIP(hipGetLastError());
 }
 
-void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx, hipStream_t s)
+void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx, hipStream_t s,
+                    u64 cap)
 {
     hipLaunchKernelGGL(k_compact, dim3(ceil_div(n, CWAVES)), dim3(64 * CWAVES), 0, s, mask, n, mstride, words, offsets,
-                       idx);
+                       idx, cap);
     CASPHY_HIP(hipGetLastError());
 }
 
 void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
-                        hipStream_t s)
+                        hipStream_t s, u64 *sums)
 {
     if (!R) return;
-    hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii, nradii, counts);
+    hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii, nradii, counts, sums);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, u64 smax, int any, int32_t *out,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_autosplit_choose, dim3(1), dim3(64), 0, s, sums, radii, nradii, smax, any, out);
     CASPHY_HIP(hipGetLastError());
 }
 

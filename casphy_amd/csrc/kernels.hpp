@@ -22,13 +22,20 @@ void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hi
 void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s);
 // Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).
 void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s);
+// Ids at positions >= cap are not written (cap = capacity of idx).
 void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx,
-                    hipStream_t s);
+                    hipStream_t s, u64 cap = ~0ull);
 
-// counts[k*R + q] = #{j < n : S[q*ldS + j] >= radii[k]} for k < nradii (<= 16).
+// counts[k*R + q] = #{j < n : S[q*ldS + j] >= radii[k]} for k < nradii (<= 16);
+// with sums, also sums[k] += sum over q of counts[k*R + q] (sums zeroed by the caller).
 constexpr int MAX_RADII_K = 16;
 void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
-                        hipStream_t s);
+                        hipStream_t s, u64 *sums = nullptr);
+
+// autoSplit choice: out[0] = radii[k] and out[1] = k for the first k with
+// sums[k] <= smax, or out[0] = INT32_MAX, out[1] = -1 when none (or !any).
+void launch_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, u64 smax, int any, int32_t *out,
+                             hipStream_t s);
 
 // size[r] = min(p, ntip - p) with p = popcount of row r (u32 arithmetic).
 void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s);

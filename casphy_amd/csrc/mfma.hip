@@ -172,6 +172,7 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
     }
 
     const u32 half = lane >> 5, col = lane & 31;
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
-                    const bool hit = acc[x][y][v] >= a.thr;
+                    const bool hit = acc[x][y][v] >= thr;
                     const unsigned long long b = __ballot(hit && j < a.nB);
                     // low half: row 8*(v/4)+v%4, high half: that row + 4
                     if (lane == (u32)v || lane == (u32)(v + 32)) {
@@ -309,6 +310,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
     const u32 half = lane >> 5, col = lane & 31;
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const u32 arow0 = I * RT + wm * 128, brow0 = J * RT + wn * 128;
     const uint4 *PA = reinterpret_cast<const uint4 *>(a.PA) + arow0 + col;
     const uint4 *PB = reinterpret_cast<const uint4 *>(a.PB) + brow0 + col;
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
-                    const bool hit = acc[x][y][v] >= a.thr;
+                    const bool hit = acc[x][y][v] >= thr;
                     const unsigned long long b = __ballot(hit && j < a.nB);
                     if ((lane == (u32)v || lane == (u32)(v + 32)) && c0 < a.mstride * 32) {
                         const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);

@@ -21,6 +21,7 @@ struct MfmaArgs {
     u32 ntiles = 0;           // set by the launcher
     int triangle = 1;         // A == B, tiles I <= J
     int32_t thr = 0;          // E_MASK
+    const int32_t *thr_dev = nullptr;  // E_MASK: threshold read from device memory when set
     u32 *mask = nullptr;
     u32 mstride = 0;
     int32_t *S = nullptr;     // E_SCORES

@@ -180,13 +180,14 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
     const u32 i0 = I * TILE + ty * RB, j0 = J * TILE + tx * RB;
 
     if (EPI == E_MASK) {
+        const int32_t thr = a.thr_dev ? *a.thr_dev : a.thr;
         // direct words: row i, columns J*128 + 32w .. +31
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
             u32 byte = 0;
 #pragma unroll
             for (int c = 0; c < RB; ++c)
-                if ((int32_t)acc[r][c] >= a.thr && j0 + c < a.nB) byte |= 1u << c;
+                if ((int32_t)acc[r][c] >= thr && j0 + c < a.nB) byte |= 1u << c;
             u32 piece = byte << ((tx & 3) * 8);
             piece |= __shfl_xor(piece, 1);
             piece |= __shfl_xor(piece, 2);
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
                 u32 byte = 0;
 #pragma unroll
                 for (int r = 0; r < RB; ++r)
-                    if ((int32_t)acc[r][c] >= a.thr && i0 + r < a.nA) byte |= 1u << r;
+                    if ((int32_t)acc[r][c] >= thr && i0 + r < a.nA) byte |= 1u << r;
                 u32 piece = byte << ((ty & 3) * 8);
                 piece |= __shfl_xor(piece, 16);
                 piece |= __shfl_xor(piece, 32);

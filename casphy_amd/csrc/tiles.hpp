@@ -44,6 +44,7 @@ struct TileArgs {
     u32 ptail = 0;                // shift applied after the last plane
     // epilogue parameters
     int32_t thr = 0;             // E_MASK
+    const int32_t *thr_dev = nullptr;  // E_MASK: threshold read from device memory when set
     u32 *mask = nullptr;
     u32 mstride = 0;
     int32_t *S = nullptr;        // E_SCORES

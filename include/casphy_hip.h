@@ -203,6 +203,19 @@ int casphy_cells_fetch(casphy_cells *h, uint32_t *nneighbor, uint32_t *idx);
 /* tryRadius counts for up to 16 ascending radii: counts[k*R + j]. */
 int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
                            uint32_t R, uint32_t *counts);
+/* One autoSplit round with a single host synchronisation: tryRadius counts of
+ * the R query rows at each of up to 16 ascending radii, then the first radius
+ * whose mean count c (= sum / R in double) satisfies
+ * log10(max(c, 1e-30)) <= bound -- the stopping rule of the autoSplit loop,
+ * R/CasPhy_functions.R:2007-2021 (bound = nNeighborBound - log10(ntip)) --
+ * chosen on the device, and the radius lists at that radius kept on the
+ * device as after casphy_cells_radius (fetch / last_ms apply).  *radius = -1
+ * and *nidx = 0 when no radius qualifies.  counts (R x nradii, as
+ * casphy_cells_try_radii) may be NULL.
+ * Replaces: the tryRadius / radiusNeighbor executable round trips of that
+ * loop (R/CasPhy_functions.R:2012-2025). */
+int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
+                           uint32_t R, double bound, int32_t *radius, uint64_t *nidx, uint32_t *counts);
 /* Device time (ms, hipEvents on the handle's stream) of the pair-scoring tile
  * kernel of the last radius call and of the whole last radius call. */
 int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms);

@@ -141,6 +141,72 @@ def test_try_radius(hip, orc, bitpath):
     assert np.array_equal(hip.try_radius(bmat, 3, q), orc.try_radius_bit(mat, 3, q))
 
 
+def autosplit_rule(counts, radii, bound):
+    """The autoSplit stopping rule on host: first radius whose mean count c has
+    log10(max(c, 1e-30)) <= bound, else -1."""
+    for k, r in enumerate(radii):
+        if np.log10(max(counts[k].mean(), 1e-30)) <= bound:
+            return int(r)
+    return -1
+
+
+def test_cells_autosplit_bit(hip, orc, bitpath):
+    mat = synth.binary_matrix(seed=23, n=900, nsite=256, p=0.01)
+    cells = hip.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
+    q = synth.query_rows(5, 900, 200)
+    radii = np.arange(1, 17) * 2
+    ref = np.stack([orc.try_radius_bit(mat, int(r), q) for r in radii])
+    seen = set()
+    # larger, then smaller radii: the resident list buffer grows between calls
+    for bound in (0.5, -40.0, 3.0, 1.5, 0.5):
+        want = autosplit_rule(ref, radii, bound)
+        r, total, cnt = cells.autosplit(radii, q, bound, counts=True)
+        assert np.array_equal(cnt, ref)
+        assert r == want, bound
+        if want < 0:
+            assert total == 0
+            continue
+        seen.add(want)
+        counts, idx = cells.fetch()
+        c2, i2 = orc.radius_bit(mat, want)
+        assert total == idx.size and np.array_equal(counts, c2) and np.array_equal(idx, i2)
+        assert cells.autosplit(radii, q, bound) == (r, total)
+    assert len(seen) >= 2
+
+
+def test_cells_autosplit_dna(hip, orc):
+    mat = synth.dna_matrix(seed=11, n=400, ncol=300)
+    cells = hip.Cells(mat, "dna")
+    q = synth.query_rows(12, 400, 130)
+    radii = np.array([100, 200, 250, 280, 290, 299])
+    ref = np.stack([orc.try_radius_dna(mat, int(r), q) for r in radii])
+    for bound in (1.0, 2.0, 2.7):
+        want = autosplit_rule(ref, radii, bound)
+        r, total, cnt = cells.autosplit(radii, q, bound, counts=True)
+        assert np.array_equal(cnt, ref) and r == want
+        if want >= 0:
+            counts, idx = cells.fetch()
+            c2, i2 = orc.radius_dna(mat, want)
+            assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+def test_cells_autosplit_limit_exact(hip):
+    """The device comparison equals the host rule at the boundary: bounds set to
+    log10 of the exact mean count of one radius."""
+    mat = synth.binary_matrix(seed=24, n=700, nsite=256, p=0.01)
+    cells = hip.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
+    q = synth.query_rows(6, 700, 333)
+    radii = np.arange(1, 17)
+    cnt = cells.try_radii(radii, q)
+    for k in (2, 5, 9):
+        m = cnt[k].mean()
+        if m <= 0:
+            continue
+        for bound in (np.log10(m), np.nextafter(np.log10(m), -np.inf), np.nextafter(np.log10(m), np.inf)):
+            r, _ = cells.autosplit(radii, q, float(bound))
+            assert r == autosplit_rule(cnt, radii, float(bound)), (k, bound)
+
+
 def test_try_radius_dna(hip, orc):
     mat = synth.dna_matrix(seed=11, n=400, ncol=300)
     q = synth.query_rows(12, 400, 130)
