@@ -250,6 +250,144 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
         }
 }
 
+// ---------------------------------------------------------------------------
+// Wide LDS variant: a workgroup of 4 waves (2 x 2) owns a 256 x 256 pair tile
+// and each wave 128 x 128 pairs as 4 x 4 MFMA blocks, so every 16-byte operand
+// read from LDS feeds 4 MFMAs (twice the 2 x 2 blocking): per 64-site stage
+// the CU moves 32 KB of expanded bytes into LDS and 64 KB out of it for 128
+// MFMAs.  One workgroup per CU (256 accumulators per lane); the next stage's
+// packed bits are prefetched into registers while the current one is used.
+constexpr u32 WT = 256;  // workgroup tile
+
+// Radius-mask / score epilogue of one 32 x 32 block whose rows start at r0 and
+// columns at c0 (mirror: also write the transposed words).
+template <int EPI>
+__device__ __forceinline__ void wide_block_epilogue(const MfmaArgs &a, const v16i &acc, u32 r0, u32 c0, u32 lane,
+                                                    bool mirror, int32_t thr)
+{
+    const u32 half = lane >> 5, col = lane & 31, j = c0 + col;
+    if (EPI == E_MASK) {
+        u32 colbits = 0;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
+            const bool hit = acc[v] >= thr;
+            const unsigned long long b = __ballot(hit && j < a.nB);
+            if ((lane == (u32)v || lane == (u32)(v + 32)) && c0 / 32 < a.mstride) {
+                const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
+                const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
+                const u32 i = r0 + m_out;
+                if (i < a.nA) a.mask[(size_t)i * a.mstride + c0 / 32] = word;
+            }
+            if (hit && r0 + m < a.nA) colbits |= 1u << m;
+        }
+        if (mirror) {
+            colbits |= __shfl_xor(colbits, 32);
+            if (half == 0 && j < a.nB && r0 / 32 < a.mstride) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+            if (i < a.nA && j < a.nB) {
+                a.S[(size_t)i * a.ldS + j] = acc[v];
+                if (mirror) a.S[(size_t)j * a.ldS + i] = acc[v];
+            }
+        }
+    }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256, 1) void k_mfma_wide(MfmaArgs a)
+{
+    constexpr int KB = 64, LR = KB + 16;  // sites per LDS stage; LDS bytes per row
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][WT * LR];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][WT * LR];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (a.triangle) {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+
+    v16i acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
+
+    // thread tid stages row tid of the A tile and of the B tile: 64 sites =
+    // half (s & 1) of 128-site stage s >> 1, 8 bytes each
+    const u32 nst = a.stages * 2;
+    const uint2 *PA = reinterpret_cast<const uint2 *>(a.PA) + ((size_t)I * WT + tid) * 2;
+    const uint2 *PB = reinterpret_cast<const uint2 *>(a.PB) + ((size_t)J * WT + tid) * 2;
+    auto ld = [&](const uint2 *P, size_t pad, u32 s) { return P[(size_t)(s >> 1) * pad * 2 + (s & 1)]; };
+    auto st = [&](unsigned char *dst, uint2 v) {
+        uint4 *d = reinterpret_cast<uint4 *>(dst + tid * LR);
+        d[0] = spread16(v.x);
+        d[1] = spread16(v.x >> 16);
+        d[2] = spread16(v.y);
+        d[3] = spread16(v.y >> 16);
+    };
+    uint2 ga = ld(PA, a.padA, 0), gb = ld(PB, a.padB, 0);
+    st(As[0], ga);
+    st(Bs[0], gb);
+    if (nst > 1) {
+        ga = ld(PA, a.padA, 1);
+        gb = ld(PB, a.padB, 1);
+    }
+    __syncthreads();
+    const u32 arow = wm * 128 + (lane & 31), brow = wn * 128 + (lane & 31), koff = 16 * (lane >> 5);
+    for (u32 s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 32) {
+            v4i fa[4], fb[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + kk + koff]);
+                fb[x] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LR + kk + koff]);
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+        }
+        if (s + 1 < nst) {
+            st(As[buf ^ 1], ga);
+            st(Bs[buf ^ 1], gb);
+            if (s + 2 < nst) {
+                ga = ld(PA, a.padA, s + 2);
+                gb = ld(PB, a.padB, s + 2);
+            }
+        }
+        __syncthreads();
+    }
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            wide_block_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 128 + 32 * y, lane,
+                                     mirror, thr);
+}
+
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -455,6 +593,17 @@ void launch_variant(int epi, const MfmaArgs &a, hipStream_t s)
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }
 
+void launch_wide(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    if (epi == E_MASK)
+        hipLaunchKernelGGL(k_mfma_wide<E_MASK>, dim3(grid), dim3(256), 0, s, a);
+    else if (epi == E_SCORES)
+        hipLaunchKernelGGL(k_mfma_wide<E_SCORES>, dim3(grid), dim3(256), 0, s, a);
+    else
+        throw Error(CASPHY_EINTERNAL, "MFMA wide tiles: unsupported epilogue");
+}
+
 void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 {
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
@@ -473,10 +622,13 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
-const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg"};
-constexpr int kNumVariants = 7;
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide"};
+constexpr int kNumVariants = 8;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7};  // what the autotuner times
+constexpr int kWide = 7;  // radius-mask and score epilogues only
+
+bool variant_supports(int v, int epi) { return v != kWide || epi == E_MASK || epi == E_SCORES; }
 
 int variant_from_env()
 {
@@ -490,7 +642,8 @@ int variant_from_env()
 void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
-    const u32 T = v == 6 ? RT : TILE;
+    if (!variant_supports(v, epi)) v = kDefaultVariant;
+    const u32 T = v == 6 ? RT : v == kWide ? WT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntB = ceil_div(a.nB, T);
     a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1) / 2) : ntA * a.ntB;
@@ -502,7 +655,8 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case 3: launch_variant<64, 2, 4>(epi, a, s); break;
     case 4: launch_variant<64, 4, 4>(epi, a, s); break;
     case 5: launch_variant<64, 8, 4>(epi, a, s); break;
-    default: launch_reg(epi, a, s); break;
+    case 6: launch_reg(epi, a, s); break;
+    default: launch_wide(epi, a, s); break;
     }
     CASPHY_HIP(hipGetLastError());
 }
@@ -557,6 +711,7 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
     float best = 1e30f;
     e.variant = kDefaultVariant;
     for (int v : kCandidates) {
+        if (!variant_supports(v, epi)) continue;
         CASPHY_HIP(hipEventRecord(ev[0], s));
         launch_one(v, epi, a, s);
         CASPHY_HIP(hipEventRecord(ev[1], s));
@@ -584,6 +739,7 @@ std::string mfma_tuning_report()
                "\",\"ms\":{";
         bool first = true;
         for (int v : kCandidates) {
+            if (kv.second.ms[v] < 0) continue;
             if (!first) out += ",";
             first = false;
             out += "\"" + std::string(kVariants[v]) + "\":" + std::to_string(kv.second.ms[v]);

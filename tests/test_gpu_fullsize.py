@@ -55,11 +55,16 @@ def test_config2_radius_properties(hip, orc, cfg2, monkeypatch):
             c2, i2 = orc.radius_bit(cfg2, r, None, (i0, i0 + 8))
             assert np.array_equal(counts[i0:i0 + 8], c2)
             assert np.array_equal(idx[off[i0]:off[i0 + 8]], i2)
-        # the VALU tiles give the same lists
+        # the VALU tiles and every matrix-core variant give the same lists
         monkeypatch.setenv("CASPHY_MFMA", "0")
         c3, i3 = hip.radius_neighbor(bmat, r)
         monkeypatch.setenv("CASPHY_MFMA", "1")
         assert np.array_equal(c3, counts) and np.array_equal(i3, idx)
+        for variant in ("64x2", "wide", "reg"):
+            monkeypatch.setenv("CASPHY_MFMA_VARIANT", variant)
+            c3, i3 = hip.radius_neighbor(bmat, r)
+            assert np.array_equal(c3, counts) and np.array_equal(i3, idx), variant
+        monkeypatch.delenv("CASPHY_MFMA_VARIANT")
     q = synth.query_rows(2, n, 1024)
     tr = cells.try_radii(np.arange(1, 17), q)
     total = cells.radius(16)
