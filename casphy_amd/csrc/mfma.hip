@@ -58,35 +58,45 @@ __device__ __forceinline__ uint4 spread16(u32 w)  // bits 0..15 -> 16 bytes
     return make_uint4(spread4(w & 15), spread4((w >> 4) & 15), spread4((w >> 8) & 15), spread4((w >> 12) & 15));
 }
 
-// Per-thread slice of one stage of one operand: KB/16 bytes of packed bits.
+// Per-thread slice of one stage: one row of one operand (threads 0..127 the A
+// rows, 128..255 the B rows), KB sites of packed bits.  Consecutive lanes own
+// consecutive rows, so the global load is contiguous and the expanded bytes
+// go to LDS rows LR bytes apart (LR = KB + 16), which keeps the 16-byte
+// stores of 8 consecutive lanes on distinct banks.
 template <int KB> struct Slice;
 template <> struct Slice<128> {
-    uint2 v;
-    __device__ __forceinline__ void load(const u32 *P, size_t pad, u32 tile, u32 s, u32 tid)
+    uint4 v;
+    __device__ __forceinline__ void load(const u32 *P, size_t pad, u32 tile, u32 s, u32 row)
     {
-        v = reinterpret_cast<const uint2 *>(P)[(size_t)s * pad * 2 + (size_t)tile * TILE * 2 + tid];
+        v = reinterpret_cast<const uint4 *>(P)[(size_t)s * pad + (size_t)tile * TILE + row];
     }
-    __device__ __forceinline__ void store(unsigned char *dst, u32 tid) const  // row tid/2, bytes (tid&1)*64
+    __device__ __forceinline__ void store(unsigned char *dst, u32 row) const
     {
-        uint4 *d = reinterpret_cast<uint4 *>(dst + (tid >> 1) * (128 + 16) + (tid & 1) * 64);
+        uint4 *d = reinterpret_cast<uint4 *>(dst + row * (128 + 16));
         d[0] = spread16(v.x);
         d[1] = spread16(v.x >> 16);
         d[2] = spread16(v.y);
         d[3] = spread16(v.y >> 16);
+        d[4] = spread16(v.z);
+        d[5] = spread16(v.z >> 16);
+        d[6] = spread16(v.w);
+        d[7] = spread16(v.w >> 16);
     }
 };
 template <> struct Slice<64> {
-    u32 v;
-    __device__ __forceinline__ void load(const u32 *P, size_t pad, u32 tile, u32 s, u32 tid)
+    uint2 v;
+    __device__ __forceinline__ void load(const u32 *P, size_t pad, u32 tile, u32 s, u32 row)
     {
         // 64-site stage s = half (s & 1) of 128-site stage s >> 1
-        v = P[(size_t)(s >> 1) * pad * 4 + ((size_t)tile * TILE + (tid >> 1)) * 4 + 2 * (s & 1) + (tid & 1)];
+        v = reinterpret_cast<const uint2 *>(P)[((size_t)(s >> 1) * pad + (size_t)tile * TILE + row) * 2 + (s & 1)];
     }
-    __device__ __forceinline__ void store(unsigned char *dst, u32 tid) const  // row tid/2, bytes (tid&1)*32
+    __device__ __forceinline__ void store(unsigned char *dst, u32 row) const
     {
-        uint4 *d = reinterpret_cast<uint4 *>(dst + (tid >> 1) * (64 + 16) + (tid & 1) * 32);
-        d[0] = spread16(v);
-        d[1] = spread16(v >> 16);
+        uint4 *d = reinterpret_cast<uint4 *>(dst + row * (64 + 16));
+        d[0] = spread16(v.x);
+        d[1] = spread16(v.x >> 16);
+        d[2] = spread16(v.y);
+        d[3] = spread16(v.y >> 16);
     }
 };
 
@@ -125,18 +135,17 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
             for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
 
     const u32 nst = a.stages * (128 / KB);  // a multiple of 8: rows are padded to 1,024 sites
-    Slice<KB> ga[PF], gb[PF];
+    const bool opB = tid >= TILE;  // wave-uniform: waves 0-1 stage A rows, waves 2-3 B rows
+    const u32 row = tid & (TILE - 1);
+    const u32 *P = opB ? a.PB : a.PA;
+    const size_t pad = opB ? a.padB : a.padA;
+    const u32 tile = opB ? J : I;
+    unsigned char(*Ls)[TILE * LR] = opB ? Bs : As;
+    Slice<KB> g[PF];
 #pragma unroll
-    for (int u = 0; u < PF; ++u) {
-        ga[u].load(a.PA, a.padA, I, u, tid);
-        gb[u].load(a.PB, a.padB, J, u, tid);
-    }
-    ga[0].store(As[0], tid);
-    gb[0].store(Bs[0], tid);
-    if (PF < (int)nst) {
-        ga[0].load(a.PA, a.padA, I, PF, tid);
-        gb[0].load(a.PB, a.padB, J, PF, tid);
-    }
+    for (int u = 0; u < PF; ++u) g[u].load(P, pad, tile, u, row);
+    g[0].store(Ls[0], row);
+    if (PF < (int)nst) g[0].load(P, pad, tile, PF, row);
     __syncthreads();
     const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
     for (u32 s0 = 0; s0 < nst; s0 += PF) {
@@ -160,12 +169,8 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
             }
             const int nx = (u + 1) % PF;  // static after unrolling
             if (s + 1 < nst) {
-                ga[nx].store(As[buf ^ 1], tid);
-                gb[nx].store(Bs[buf ^ 1], tid);
-                if (s + 1 + PF < nst) {
-                    ga[nx].load(a.PA, a.padA, I, s + 1 + PF, tid);
-                    gb[nx].load(a.PB, a.padB, J, s + 1 + PF, tid);
-                }
+                g[nx].store(Ls[buf ^ 1], row);
+                if (s + 1 + PF < nst) g[nx].load(P, pad, tile, s + 1 + PF, row);
             }
             __syncthreads();
         }
@@ -388,6 +393,99 @@ __global__ __launch_bounds__(256, 1) void k_mfma_wide(MfmaArgs a)
                                      mirror, thr);
 }
 
+// Wide tile with two waves per SIMD: 8 waves (2 x 4) per 256 x 256 tile, each
+// 128 x 64 pairs as 4 x 2 MFMA blocks (128 accumulators per lane).  Per
+// 64-site stage the CU moves 32 KB into LDS and 96 KB out of it for 128 MFMAs;
+// threads 0..255 stage A rows and 256..511 B rows, one row each.
+template <int EPI>
+__global__ __launch_bounds__(512, 1) void k_mfma_wide8(MfmaArgs a)
+{
+    constexpr int KB = 64, LR = KB + 16;
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][WT * LR];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][WT * LR];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (a.triangle) {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 2, wn = wave & 3;
+
+    v16i acc[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
+
+    const u32 nst = a.stages * 2;
+    const bool opB = tid >= WT;  // wave-uniform
+    const u32 row = tid & (WT - 1);
+    const uint2 *P = reinterpret_cast<const uint2 *>(opB ? a.PB : a.PA) + ((size_t)(opB ? J : I) * WT + row) * 2;
+    const size_t pad2 = (size_t)(opB ? a.padB : a.padA) * 2;
+    unsigned char(*Ls)[WT * LR] = opB ? Bs : As;
+    auto ld = [&](u32 s) { return P[(size_t)(s >> 1) * pad2 + (s & 1)]; };
+    auto st = [&](unsigned char *dst, uint2 v) {
+        uint4 *d = reinterpret_cast<uint4 *>(dst + row * LR);
+        d[0] = spread16(v.x);
+        d[1] = spread16(v.x >> 16);
+        d[2] = spread16(v.y);
+        d[3] = spread16(v.y >> 16);
+    };
+    uint2 g0 = ld(0), g1 = nst > 1 ? ld(1) : make_uint2(0, 0);
+    st(Ls[0], g0);
+    if (nst > 2) g0 = ld(2);
+    __syncthreads();
+    const u32 arow = wm * 128 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
+    for (u32 s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 32) {
+            v4i fa[4], fb[2];
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+                fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + kk + koff]);
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+                fb[y] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * y) * LR + kk + koff]);
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+        }
+        if (s + 1 < nst) {
+            // stage s + 1 was loaded two iterations ago (g1 for odd s + 1, g0 for even)
+            st(Ls[buf ^ 1], (s & 1) ? g0 : g1);
+            if (s + 3 < nst) {
+                if (s & 1) g0 = ld(s + 3);
+                else g1 = ld(s + 3);
+            }
+        }
+        __syncthreads();
+    }
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+            wide_block_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 64 + 32 * y, lane,
+                                     mirror, thr);
+}
+
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -604,6 +702,17 @@ void launch_wide(int epi, const MfmaArgs &a, hipStream_t s)
         throw Error(CASPHY_EINTERNAL, "MFMA wide tiles: unsupported epilogue");
 }
 
+void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    if (epi == E_MASK)
+        hipLaunchKernelGGL(k_mfma_wide8<E_MASK>, dim3(grid), dim3(512), 0, s, a);
+    else if (epi == E_SCORES)
+        hipLaunchKernelGGL(k_mfma_wide8<E_SCORES>, dim3(grid), dim3(512), 0, s, a);
+    else
+        throw Error(CASPHY_EINTERNAL, "MFMA wide tiles: unsupported epilogue");
+}
+
 void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 {
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
@@ -622,13 +731,13 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
-const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide"};
-constexpr int kNumVariants = 8;
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8"};
+constexpr int kNumVariants = 9;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7};  // what the autotuner times
-constexpr int kWide = 7;  // radius-mask and score epilogues only
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8};  // what the autotuner times
+constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
 
-bool variant_supports(int v, int epi) { return v != kWide || epi == E_MASK || epi == E_SCORES; }
+bool variant_supports(int v, int epi) { return v < kWide || epi == E_MASK || epi == E_SCORES; }
 
 int variant_from_env()
 {
@@ -643,7 +752,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
     if (!variant_supports(v, epi)) v = kDefaultVariant;
-    const u32 T = v == 6 ? RT : v == kWide ? WT : TILE;
+    const u32 T = v == 6 ? RT : v >= kWide ? WT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntB = ceil_div(a.nB, T);
     a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1) / 2) : ntA * a.ntB;
@@ -656,7 +765,8 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case 4: launch_variant<64, 4, 4>(epi, a, s); break;
     case 5: launch_variant<64, 8, 4>(epi, a, s); break;
     case 6: launch_reg(epi, a, s); break;
-    default: launch_wide(epi, a, s); break;
+    case kWide: launch_wide(epi, a, s); break;
+    default: launch_wide8(epi, a, s); break;
     }
     CASPHY_HIP(hipGetLastError());
 }
