@@ -95,6 +95,27 @@ def calibration_copy(nbytes: int = 256 << 20) -> None:
     hip.hipFree(b)
 
 
+# kernel behind each matrix-core tile variant with the radius-mask epilogue
+VARIANT_KERNEL = {"128x1": "k_mfma_tiles<0, 128, 1, 2>", "128x2": "k_mfma_tiles<0, 128, 2, 2>",
+                  "128x4": "k_mfma_tiles<0, 128, 4, 2>", "64x2": "k_mfma_tiles<0, 64, 2, 4>",
+                  "64x4": "k_mfma_tiles<0, 64, 4, 4>", "64x8": "k_mfma_tiles<0, 64, 8, 4>",
+                  "reg": "k_mfma_reg<0>", "wide": "k_mfma_wide<0>", "wide8": "k_mfma_wide8<0>"}
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_mfma_tiles_mask_10000x4096_pmc.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    summary of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled
+    by what the counter reads for a 256 MiB copy); None when not measured."""
+    if not os.path.exists(TRAFFIC_TABLE):
+        return None
+    for line in open(TRAFFIC_TABLE):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == kernel:
+            return int(cells[5].replace(",", ""))
+    return None
+
+
 def timed(fn, reps=1):
     t0 = time.perf_counter()
     out = None
@@ -349,10 +370,13 @@ def main():
                 "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s", "kernel": "k_tiles<AND, MASK>"}
     tuning = cp.mfma_tuning() if roof["bound"] == "mfma" else []
     chosen = [e["chosen"] for e in tuning if e["epilogue"] == 0 and e["triangle"] == 1]
-    if chosen:
-        roof["variant"] = chosen[0]
+    forced = os.environ.get("CASPHY_MFMA_VARIANT")
+    variant = forced if forced in VARIANT_KERNEL else (chosen[0] if chosen else "64x2")
+    if roof["bound"] == "mfma":
+        roof["variant"] = variant
+        roof["kernel"] = VARIANT_KERNEL[variant]
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(roof["kernel"])
     roof["kernel_ms"] = tile_s * 1000
     roof["algorithmic"] = f"{2 * NSITE} ops per cell pair x {pairs_tile} pairs per launch"
     out = {
