@@ -219,9 +219,46 @@ def extras(cp, synth, cells, q) -> dict:
                                          "batched_jobs_per_s": len(boots) / batch_s,
                                          "speedup": plain_s / batch_s}
     print("# TBE done", file=sys.stderr, flush=True)
+    ex["scan_100000x81920_bit"] = scan_form(cp, synth)
+    print("# scan form done", file=sys.stderr, flush=True)
     ex["serving_mix"] = serving_mix(cp, synth)
     print("# serving done", file=sys.stderr, flush=True)
     return ex
+
+
+def scan_form(cp, synth, n: int = 100000, W: int = 2560, R: int = 4) -> dict:
+    """Scan form of the query x cell scores: tryRadius for 4 query rows over
+    100,000 cells x 81,920 sites (1.02 GB, four times the 256 MB MALL, so each
+    pass streams from HBM), radii 1200..1350.  Bits are i.i.d. with density
+    1/8 (the pass is bandwidth bound, the values do not matter); the counts
+    are compared with the tile path on the same resident cells."""
+    rng = np.random.default_rng(6)
+    big = np.empty((n, W), np.uint32)
+    for i in range(0, n, 10000):
+        shape = (min(10000, n - i), W)
+        blk = rng.integers(0, 2 ** 32, size=shape, dtype=np.uint32)
+        blk &= rng.integers(0, 2 ** 32, size=shape, dtype=np.uint32)
+        blk &= rng.integers(0, 2 ** 32, size=shape, dtype=np.uint32)
+        big[i:i + shape[0]] = blk
+    cells = cp.Cells(big, "bit", wordmajor=False)
+    del big
+    q = synth.query_rows(7, n, R)
+    radii = (1200 + 10 * np.arange(16)).astype(np.int32)
+    cells.try_radii(radii, q)
+    ms = []
+    for _ in range(10):
+        counts = cells.try_radii(radii, q)
+        ms.append(cells.last_ms()[0])
+    os.environ["CASPHY_SCAN"] = "0"
+    tiles = cells.try_radii(radii, q)
+    os.environ.pop("CASPHY_SCAN")
+    assert np.array_equal(counts, tiles)
+    cells.close()
+    pass_ms = float(np.median(ms))
+    nbytes = n * W * 4
+    return {"queries": R, "radii": 16, "bytes_per_pass": nbytes, "pass_ms": pass_ms,
+            "GB_per_s": nbytes / (pass_ms / 1e3) / 1e9, "hbm_frac": nbytes / (pass_ms / 1e3) / 8e12,
+            "kernel": "k_scan_bit", "bit_exact_vs_tiles": True}
 
 
 def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 8000) -> dict:

@@ -299,7 +299,8 @@ void radius_lists(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &pla
 // tryRadius: query x cell scores (matrix-core tiles for unit-weight bits,
 // VALU tiles otherwise) followed by one counting workgroup per query.
 void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, const int32_t *radii,
-               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage = nullptr)
+               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage = nullptr,
+               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     for (u32 k = 0; k < R; ++k)
         CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
@@ -309,6 +310,20 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
     h2d(dq.p, q, (size_t)R * 4, s);
     h2d(dr.p, radii, (size_t)nradii * 4, s);
     const u32 ntA = ceil_div(R, TILE), ntB = ceil_div(n, TILE);
+    const char *scan_env = std::getenv("CASPHY_SCAN");
+    if (metric == M_AND && planes.nplanes == 0 && R <= SCAN_Q && scan_fits(ldw) && !(scan_env && scan_env[0] == '0')) {
+        // few queries over wide rows: one HBM-bound pass over the cells
+        DevBuf qr((size_t)R * ldw * 4);
+        for (u32 k = 0; k < R; ++k)
+            CASPHY_HIP(hipMemcpyAsync(qr.as<u32>() + (size_t)k * ldw, dmat + (size_t)q[k] * ldw, (size_t)ldw * 4,
+                                      hipMemcpyDeviceToDevice, s));
+        launch_fill_u32(dc.as<u32>(), (u64)R * nradii, 0, s);
+        if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
+        launch_scan_bit(dmat, n, ldw, qr.as<u32>(), R, dr.as<int32_t>(), nradii, dc.as<u32>(), R, nullptr, s);
+        if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
+        d2h(counts, dc.p, (size_t)R * nradii * 4, s);
+        return;
+    }
     if (metric == M_AND && planes.nplanes == 0 && mfma_enabled()) {
         DevBuf own, sq((size_t)stage_rows(R) * ldw * 4);
         if (!stage) {
@@ -320,7 +335,9 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
         m.PA = sq.as<u32>(); m.PB = stage; m.padA = stage_rows(R); m.padB = stage_rows(n); m.stages = ldw / 4;
         m.nA = R; m.nB = n; m.triangle = 0;
         m.S = dS.as<int32_t>(); m.ldS = n;
+        if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
         launch_mfma_tiles(E_SCORES, m, s);
+        if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
         launch_count_radii(dS.as<int32_t>(), R, n, n, dr.as<int32_t>(), nradii, dc.as<u32>(), s);
         d2h(counts, dc.p, (size_t)R * nradii * 4, s);
         return;
@@ -332,7 +349,9 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
     a.ntiles = ntA * ntB;
     a.S = dS.as<int32_t>(); a.ldS = n;
     apply_planes(planes, a, pbuf, s);
+    if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
     launch_tiles(metric, E_SCORES, a, s);
+    if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
     launch_count_radii(dS.as<int32_t>(), R, n, n, dr.as<int32_t>(), nradii, dc.as<u32>(), s);
     d2h(counts, dc.p, (size_t)R * nradii * 4, s);
 }
@@ -737,7 +756,8 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
             if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
             stage = h->stage.as<u32>();
         }
-        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s, stage);
+        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s, stage,
+                  h->ev[0], h->ev[1]);
     });
 }
 

@@ -37,6 +37,14 @@ void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *
 void launch_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, u64 smax, int any, int32_t *out,
                              hipStream_t s);
 
+// Scan form of tryRadius for up to SCAN_Q query rows (qrows: Q x ldw, cell-major):
+// counts[k*R + q] += #{r < n : <mat_r, q-th query> >= radii[k]} (counts zeroed by the
+// caller); S (Q x n, may be null) receives the scores.  For rows of 256..4,096 words.
+constexpr u32 SCAN_Q = 4;
+bool scan_fits(u32 ldw);
+void launch_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q, const int32_t *radii, u32 nradii,
+                     u32 *counts, u32 R, int32_t *S, hipStream_t s);
+
 // size[r] = min(p, ntip - p) with p = popcount of row r (u32 arithmetic).
 void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s);
 

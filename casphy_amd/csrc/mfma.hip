@@ -766,7 +766,8 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case 5: launch_variant<64, 8, 4>(epi, a, s); break;
     case 6: launch_reg(epi, a, s); break;
     case kWide: launch_wide(epi, a, s); break;
-    default: launch_wide8(epi, a, s); break;
+    case kWide8: launch_wide8(epi, a, s); break;
+    default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());
 }

@@ -216,8 +216,9 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
  * loop (R/CasPhy_functions.R:2012-2025). */
 int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
                            uint32_t R, double bound, int32_t *radius, uint64_t *nidx, uint32_t *counts);
-/* Device time (ms, hipEvents on the handle's stream) of the pair-scoring tile
- * kernel of the last radius call and of the whole last radius call. */
+/* Device time (ms, hipEvents on the handle's stream): tile_ms = the scoring
+ * kernel of the last radius, try_radii or autosplit call (autosplit: the
+ * radius-mask tiles); radius_ms = the whole last radius or autosplit call. */
 int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms);
 
 /* ------------------------------------------------------------------ serving */

@@ -353,3 +353,22 @@ def test_transfer_batch(hip, orc, bitpath):
         y2, ys2, x2, xs2 = orc.transfer(ref, X, ntip)
         assert np.array_equal(ymin[b], y2) and np.array_equal(ysize, ys2)
         assert np.array_equal(xm[b], x2) and np.array_equal(xs[b], xs2)
+
+
+# ------------------------------------------------------------------ scan form
+@pytest.mark.parametrize("n,W", [(700, 256), (333, 300), (1025, 1024)])
+def test_try_radius_scan_form(hip, orc, monkeypatch, n, W):
+    """Up to four queries over wide rows take one HBM pass (scan form); its
+    counts equal the oracle and the tile path."""
+    rng = np.random.default_rng(n + W)
+    mat = rand_bits(rng, n, W, 0.02)
+    bmat = np.ascontiguousarray(mat.T)
+    radii = np.arange(1, 17) * max(1, W // 64)
+    for R in (1, 3, 4):
+        q = synth.query_rows(R, n, R)
+        got = hip.try_radii(bmat, radii, q)
+        for k, r in enumerate(radii):
+            assert np.array_equal(got[k], orc.try_radius_bit(mat, int(r), q)), (R, r)
+        monkeypatch.setenv("CASPHY_SCAN", "0")
+        assert np.array_equal(hip.try_radii(bmat, radii, q), got)
+        monkeypatch.delenv("CASPHY_SCAN")
