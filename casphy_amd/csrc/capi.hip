@@ -656,6 +656,7 @@ struct casphy_cells {
     DevBuf stage;  // stage-major copy for the MFMA tiles (bit kind, unit weights)
     RadiusOut out;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timed[2] = {false, false};  // ev[0..1] / ev[2..3] recorded by some call
     int device = 0;
     // autoSplit round: uploads (radii | query rows), query stage copy, scores,
     // counts, per-radius sums + choice; pinned host staging for one upload and
@@ -725,6 +726,7 @@ int casphy_cells_radius(casphy_cells *h, int32_t radius, uint64_t *nidx)
             if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
             stage = h->stage.as<u32>();
         }
+        h->timed[0] = h->timed[1] = true;
         radius_compute(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radius, h->out, s, h->ev[0], h->ev[1],
                        stage);
         CASPHY_HIP(hipEventRecord(h->ev[3], s));
@@ -756,6 +758,7 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
             if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
             stage = h->stage.as<u32>();
         }
+        h->timed[0] = true;
         try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s, stage,
                   h->ev[0], h->ev[1]);
     });
@@ -823,6 +826,7 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         // INT32_MAX, empty lists); ids beyond the current capacity are dropped and
         // the compaction reruns once the total is known
         RadiusOut &o = h->out;
+        h->timed[0] = h->timed[1] = true;
         radius_mask(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
                     mfma ? h->stage.as<u32>() : nullptr);
         const u32 mstride = ceil_div(n, TILE) * 4;
@@ -855,8 +859,9 @@ int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms)
 {
     return guarded([&] {
         require(h && tile_ms && radius_ms, "casphy_cells_last_ms: bad arguments");
-        CASPHY_HIP(hipEventElapsedTime(tile_ms, h->ev[0], h->ev[1]));
-        CASPHY_HIP(hipEventElapsedTime(radius_ms, h->ev[2], h->ev[3]));
+        *tile_ms = *radius_ms = 0.0f;
+        if (h->timed[0]) CASPHY_HIP(hipEventElapsedTime(tile_ms, h->ev[0], h->ev[1]));
+        if (h->timed[1]) CASPHY_HIP(hipEventElapsedTime(radius_ms, h->ev[2], h->ev[3]));
     });
 }
 

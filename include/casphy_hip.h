@@ -218,7 +218,8 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
                            uint32_t R, double bound, int32_t *radius, uint64_t *nidx, uint32_t *counts);
 /* Device time (ms, hipEvents on the handle's stream): tile_ms = the scoring
  * kernel of the last radius, try_radii or autosplit call (autosplit: the
- * radius-mask tiles); radius_ms = the whole last radius or autosplit call. */
+ * radius-mask tiles); radius_ms = the whole last radius or autosplit call;
+ * 0 for a time no call has measured yet. */
 int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms);
 
 /* ------------------------------------------------------------------ serving */
