@@ -317,10 +317,9 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
         for (u32 k = 0; k < R; ++k)
             CASPHY_HIP(hipMemcpyAsync(qr.as<u32>() + (size_t)k * ldw, dmat + (size_t)q[k] * ldw, (size_t)ldw * 4,
                                       hipMemcpyDeviceToDevice, s));
-        launch_fill_u32(dc.as<u32>(), (u64)R * nradii, 0, s);
-        if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
-        launch_scan_bit(dmat, n, ldw, qr.as<u32>(), R, dr.as<int32_t>(), nradii, dc.as<u32>(), R, nullptr, s);
-        if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
+        DevBuf part((size_t)scan_grid(n) * 64 * 4);
+        launch_scan_bit(dmat, n, ldw, qr.as<u32>(), R, dr.as<int32_t>(), nradii, dc.as<u32>(), R, nullptr,
+                        part.as<u32>(), s, ev0, ev1);
         d2h(counts, dc.p, (size_t)R * nradii * 4, s);
         return;
     }

@@ -38,12 +38,16 @@ void launch_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, 
                              hipStream_t s);
 
 // Scan form of tryRadius for up to SCAN_Q query rows (qrows: Q x ldw, cell-major):
-// counts[k*R + q] += #{r < n : <mat_r, q-th query> >= radii[k]} (counts zeroed by the
-// caller); S (Q x n, may be null) receives the scores.  For rows of 256..4,096 words.
+// counts[k*R + q] = #{r < n : <mat_r, q-th query> >= radii[k]} for q < Q, k < nradii;
+// S (Q x n, may be null) receives the scores; part holds scan_grid(n) x 64 u32 of
+// per-workgroup counts.  For rows of 256..4,096 words.  ev0/ev1 (may be null)
+// bracket the scoring kernel.
 constexpr u32 SCAN_Q = 4;
 bool scan_fits(u32 ldw);
+u32 scan_grid(u32 n);
 void launch_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q, const int32_t *radii, u32 nradii,
-                     u32 *counts, u32 R, int32_t *S, hipStream_t s);
+                     u32 *counts, u32 R, int32_t *S, u32 *part, hipStream_t s, hipEvent_t ev0 = nullptr,
+                     hipEvent_t ev1 = nullptr);
 
 // size[r] = min(p, ntip - p) with p = popcount of row r (u32 arithmetic).
 void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s);

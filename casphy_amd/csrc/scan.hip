@@ -6,14 +6,15 @@
 // held in LDS, so a pass is bound by HBM bandwidth: per 16 bytes of a cell row
 // a lane does SCAN_Q x 4 AND + popcount pairs (at SCAN_Q = 4, about 40% of
 // the VALU rate at full HBM bandwidth).
-//   * one wave per cell row at a time, 64 lanes x 16 bytes per step, rows
-//     read with non-temporal loads (each byte is used once);
-//   * the SCAN_Q per-lane partial sums are reduced with a transposed
-//     butterfly: after the exchanges over lane bits 5 and 4 every 16-lane
-//     group holds one query's partial sums, which the remaining four steps
-//     reduce, so lane l ends with the score of query (l >> 4) & 3;
-//   * lane (q, k = l & 15) counts the rows whose score reaches radius k, and
-//     each wave adds its counts to the global counters once at the end.
+//   * one wave per cell row at a time, 64 lanes x 16 bytes per load and up to
+//     eight loads in flight per lane, rows read with non-temporal loads (each
+//     byte is used once);
+//   * the SCAN_Q per-lane partial sums are reduced with DPP row shifts and
+//     row broadcasts (no LDS traffic) and read out of lane 63;
+//   * lane (q, k = l & 15) counts the rows whose score reaches radius k; the
+//     workgroup's counts go to a per-workgroup slot and one small launch sums
+//     them (atomics from every workgroup on the same 64 counters would
+//     serialise on two cache lines).
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -24,10 +25,27 @@ namespace {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q,
-                                                  const int32_t *radii, u32 nradii, u32 *counts, u32 R,
-                                                  int32_t *S)
+// Wave sum with DPP moves only (no LDS crossbar, which the query reads keep
+// busy): inclusive scans within each 16-lane row (row_shr 1, 2, 4, 8), then
+// row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then rows
+// 0-1's total into rows 2-3 (row_bcast:31).  Lane 63 ends with the sum.
+__device__ __forceinline__ u32 wave_sum_lane63(u32 v)
 {
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
+constexpr u32 SCAN_WAVES = 4;  // waves per workgroup share one copy of the queries in LDS
+
+__global__ __launch_bounds__(64 * SCAN_WAVES) void k_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q,
+                                                  const int32_t *radii, u32 nradii, u32 *part, int32_t *S)
+{
+    __shared__ u32 red[SCAN_WAVES][64];
     extern __shared__ uint4 qs[];  // SCAN_Q x ldw/4, zero rows for q >= Q
     const u32 v4 = ldw / 4;
     for (u32 x = threadIdx.x; x < SCAN_Q * v4; x += blockDim.x) {
@@ -39,62 +57,97 @@ __global__ __launch_bounds__(256) void k_scan_bit(const u32 *mat, u32 n, u32 ldw
     const u32 myq = (lane >> 4) & 3, myk = lane & 15;
     const int32_t rk = myk < nradii ? radii[myk] : 0x7FFFFFFF;
     u32 cnt = 0;
-    for (u32 r = blockIdx.x * 4 + wv; r < n; r += gridDim.x * 4) {
+    // eight 16-byte loads in flight per lane; past the row end a lane loads
+    // nothing, scores zeros and reads a valid query word.  The first chunk of
+    // the next row is loaded before the current row's reduction.
+    u32x4 x[8];
+    auto load_chunk = [&](u32 r, u32 w0) {
         const u32x4 *row = reinterpret_cast<const u32x4 *>(mat + (size_t)r * ldw);
-        u32 p0 = 0, p1 = 0, p2 = 0, p3 = 0;
-        auto acc = [&](const u32x4 x, u32 w) {
-            const uint4 a = qs[w], b = qs[v4 + w], c = qs[2 * v4 + w], d = qs[3 * v4 + w];
-            p0 += __builtin_popcount(x.x & a.x) + __builtin_popcount(x.y & a.y) + __builtin_popcount(x.z & a.z) +
-                  __builtin_popcount(x.w & a.w);
-            p1 += __builtin_popcount(x.x & b.x) + __builtin_popcount(x.y & b.y) + __builtin_popcount(x.z & b.z) +
-                  __builtin_popcount(x.w & b.w);
-            p2 += __builtin_popcount(x.x & c.x) + __builtin_popcount(x.y & c.y) + __builtin_popcount(x.z & c.z) +
-                  __builtin_popcount(x.w & c.w);
-            p3 += __builtin_popcount(x.x & d.x) + __builtin_popcount(x.y & d.y) + __builtin_popcount(x.z & d.z) +
-                  __builtin_popcount(x.w & d.w);
-        };
-        u32 w = lane;
-        for (; w + 192 < v4; w += 256) {  // four 16-byte loads in flight per lane
-            const u32x4 x0 = __builtin_nontemporal_load(row + w), x1 = __builtin_nontemporal_load(row + w + 64),
-                        x2 = __builtin_nontemporal_load(row + w + 128), x3 = __builtin_nontemporal_load(row + w + 192);
-            acc(x0, w);
-            acc(x1, w + 64);
-            acc(x2, w + 128);
-            acc(x3, w + 192);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const u32 w = w0 + 64 * u + lane;
+            x[u] = w < v4 ? __builtin_nontemporal_load(row + w) : u32x4{0, 0, 0, 0};
         }
-        for (; w < v4; w += 64) acc(__builtin_nontemporal_load(row + w), w);
-        // lane bit 5 picks queries {0,1} or {2,3}; lane bit 4 then one of the pair
-        const bool h5 = lane & 32, h4 = lane & 16;
-        u32 k0 = h5 ? p2 : p0, k1 = h5 ? p3 : p1;
-        const u32 s0 = h5 ? p0 : p2, s1 = h5 ? p1 : p3;
-        k0 += __shfl_xor(s0, 32);
-        k1 += __shfl_xor(s1, 32);
-        u32 v = h4 ? k1 : k0;
-        v += __shfl_xor(h4 ? k0 : k1, 16);
-        v += __shfl_xor(v, 8);
-        v += __shfl_xor(v, 4);
-        v += __shfl_xor(v, 2);
-        v += __shfl_xor(v, 1);
+    };
+    const u32 stride = gridDim.x * SCAN_WAVES;
+    u32 r = blockIdx.x * SCAN_WAVES + wv;
+    if (r < n) load_chunk(r, 0);
+    for (; r < n; r += stride) {
+        u32 p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+        for (u32 w0 = 0; w0 < v4; w0 += 512) {
+            if (w0) load_chunk(r, w0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const u32 w = w0 + 64 * u + lane;
+                const u32 wq = w < v4 ? w : 0;
+                const uint4 a = qs[wq], b = qs[v4 + wq], c = qs[2 * v4 + wq], d = qs[3 * v4 + wq];
+                const u32x4 y = x[u];
+                p0 += __builtin_popcount(y.x & a.x) + __builtin_popcount(y.y & a.y) +
+                      __builtin_popcount(y.z & a.z) + __builtin_popcount(y.w & a.w);
+                p1 += __builtin_popcount(y.x & b.x) + __builtin_popcount(y.y & b.y) +
+                      __builtin_popcount(y.z & b.z) + __builtin_popcount(y.w & b.w);
+                p2 += __builtin_popcount(y.x & c.x) + __builtin_popcount(y.y & c.y) +
+                      __builtin_popcount(y.z & c.z) + __builtin_popcount(y.w & c.w);
+                p3 += __builtin_popcount(y.x & d.x) + __builtin_popcount(y.y & d.y) +
+                      __builtin_popcount(y.z & d.z) + __builtin_popcount(y.w & d.w);
+            }
+        }
+        if (r + stride < n) load_chunk(r + stride, 0);
+        const u32 t0 = __builtin_amdgcn_readlane(wave_sum_lane63(p0), 63);
+        const u32 t1 = __builtin_amdgcn_readlane(wave_sum_lane63(p1), 63);
+        const u32 t2 = __builtin_amdgcn_readlane(wave_sum_lane63(p2), 63);
+        const u32 t3 = __builtin_amdgcn_readlane(wave_sum_lane63(p3), 63);
+        const u32 v = myq == 0 ? t0 : myq == 1 ? t1 : myq == 2 ? t2 : t3;
         if (myq < Q) {
             cnt += (int32_t)v >= rk;
             if (S && myk == 0) S[(size_t)myq * n + r] = (int32_t)v;
         }
     }
-    if (myq < Q && myk < nradii && cnt) atomicAdd(&counts[(size_t)myk * R + myq], cnt);
+    // per-workgroup counts (summed by k_scan_sum: one atomic per workgroup and
+    // counter would serialise 2,048 workgroups on the same two cache lines)
+    red[wv][lane] = cnt;
+    __syncthreads();
+    if (wv == 0) {
+        u32 c = 0;
+#pragma unroll
+        for (u32 k = 0; k < SCAN_WAVES; ++k) c += red[k][lane];
+        part[(size_t)blockIdx.x * 64 + lane] = c;
+    }
+}
+
+// counts[k*R + q] = sum over workgroups of lane (q, k)'s partial count.
+__global__ __launch_bounds__(256) void k_scan_sum(const u32 *part, u32 nblocks, u32 Q, u32 nradii, u32 *counts,
+                                                  u32 R)
+{
+    __shared__ u32 red[4][64];
+    const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    u32 c = 0;
+    for (u32 b = wv; b < nblocks; b += 4) c += part[(size_t)b * 64 + lane];
+    red[wv][lane] = c;
+    __syncthreads();
+    const u32 q = (lane >> 4) & 3, k = lane & 15;
+    if (wv == 0 && q < Q && k < nradii) counts[(size_t)k * R + q] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
 }  // namespace
 
 bool scan_fits(u32 ldw) { return ldw >= 256 && (size_t)SCAN_Q * ldw * 4 <= 64 * 1024; }
 
+u32 scan_grid(u32 n) { return std::min<u32>(ceil_div(n, SCAN_WAVES), 2048); }
+
 void launch_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q, const int32_t *radii, u32 nradii,
-                     u32 *counts, u32 R, int32_t *S, hipStream_t s)
+                     u32 *counts, u32 R, int32_t *S, u32 *part, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (!n || !Q) return;
-    CASPHY_REQUIRE(Q <= SCAN_Q && scan_fits(ldw) && nradii <= 16, CASPHY_EINTERNAL, "scan form: bad shape");
-    const u32 grid = std::min<u32>(ceil_div(n, 4), 2048);
-    hipLaunchKernelGGL(k_scan_bit, dim3(grid), dim3(256), (size_t)SCAN_Q * ldw * 4, s, mat, n, ldw, qrows, Q, radii,
-                       nradii, counts, R, S);
+    CASPHY_REQUIRE(Q <= SCAN_Q && Q <= R && scan_fits(ldw) && nradii <= 16, CASPHY_EINTERNAL,
+                   "scan form: bad shape");
+    const u32 grid = scan_grid(n);
+    if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
+    hipLaunchKernelGGL(k_scan_bit, dim3(grid), dim3(64 * SCAN_WAVES), (size_t)SCAN_Q * ldw * 4, s, mat, n, ldw, qrows,
+                       Q, radii, nradii, part, S);
+    CASPHY_HIP(hipGetLastError());
+    if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
+    hipLaunchKernelGGL(k_scan_sum, dim3(1), dim3(256), 0, s, part, grid, Q, nradii, counts, R);
     CASPHY_HIP(hipGetLastError());
 }
 
