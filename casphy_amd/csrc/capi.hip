@@ -658,7 +658,7 @@ struct casphy_cells {
     bool timed[2] = {false, false};  // ev[0..1] / ev[2..3] recorded by some call
     int device = 0;
     // autoSplit round: uploads (radii | query rows), query stage copy, scores,
-    // counts, per-radius sums + choice; pinned host staging for one upload and
+    // counts and the choice; pinned host staging for one upload and
     // the read-back of (choice, total)
     DevBuf a_in, a_sq, a_S, a_counts, a_misc, a_pbuf;
     void *pin = nullptr;
@@ -793,10 +793,8 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
         h->a_S.reserve((size_t)R * n * 4);
         h->a_counts.reserve((size_t)R * nradii * 4);
-        h->a_misc.reserve(MAX_RADII * 8 + 16);
-        u64 *sums = h->a_misc.as<u64>();
-        int32_t *choice = reinterpret_cast<int32_t *>(sums + MAX_RADII);
-        launch_fill_u64(sums, MAX_RADII, 0, s);
+        h->a_misc.reserve(16);
+        int32_t *choice = h->a_misc.as<int32_t>();
 
         // tryRadius counts of the query rows, per-radius sums, device-side choice
         int32_t *dS = h->a_S.as<int32_t>();
@@ -818,8 +816,8 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
             apply_planes(h->planes, a, h->a_pbuf, s);
             launch_tiles(h->metric, E_SCORES, a, s);
         }
-        launch_count_radii(dS, R, n, n, dr, nradii, h->a_counts.as<u32>(), s, sums);
-        launch_autosplit_choose(sums, dr, nradii, smax, any, choice, s);
+        launch_count_radii(dS, R, n, n, dr, nradii, h->a_counts.as<u32>(), s);
+        launch_autosplit_choose(h->a_counts.as<u32>(), R, dr, nradii, smax, any, choice, s);
 
         // radiusNeighbor lists at the chosen radius (nothing qualifies: threshold
         // INT32_MAX, empty lists); ids beyond the current capacity are dropped and

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
 // tryRadius counts from a query x cell score matrix: one workgroup per query,
 // per-thread counters for every radius, then a workgroup reduction.
 __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii,
-                                                     u32 nradii, u32 *counts, u64 *sums)
+                                                     u32 nradii, u32 *counts)
 {
     __shared__ u32 red[16][MAX_RADII_K];
     const u32 q = blockIdx.x, tid = threadIdx.x;
@@ -208,27 +208,33 @@ __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u3
         if ((tid & 63) == 0) red[tid >> 6][k] = x;
     }
     __syncthreads();
-    if (tid < nradii) {
-        const u32 c = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-        counts[(size_t)tid * R + q] = c;
-        if (sums) atomicAdd((unsigned long long *)&sums[tid], (unsigned long long)c);
-    }
+    if (tid < nradii) counts[(size_t)tid * R + q] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
-__global__ void k_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, u64 smax, int any,
-                                   int32_t *out)
+// One workgroup of 16 waves: wave k sums the R counts of radius k, then the
+// first radius whose sum is at most smax is chosen.
+__global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii,
+                                                           u32 nradii, u64 smax, int any, int32_t *out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int32_t r = 0x7FFFFFFF, k = -1;
+    __shared__ u64 sums[MAX_RADII_K];
+    const u32 lane = threadIdx.x & 63, k = threadIdx.x >> 6;
+    u64 c = 0;
+    if (k < nradii)
+        for (u32 q = lane; q < R; q += 64) c += counts[(size_t)k * R + q];
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) sums[k] = c;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int32_t r = 0x7FFFFFFF, kk = -1;
     if (any)
         for (u32 j = 0; j < nradii; ++j)
             if (sums[j] <= smax) {
                 r = radii[j];
-                k = (int32_t)j;
+                kk = (int32_t)j;
                 break;
             }
     out[0] = r;
-    out[1] = k;
+    out[1] = kk;
 }
 
 __global__ void k_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size)
@@ -306,17 +312,17 @@ void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *o
 }
 
 void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
-                        hipStream_t s, u64 *sums)
+                        hipStream_t s)
 {
     if (!R) return;
-    hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii, nradii, counts, sums);
+    hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii, nradii, counts);
     CASPHY_HIP(hipGetLastError());
 }
 
-void launch_autosplit_choose(const u64 *sums, const int32_t *radii, u32 nradii, u64 smax, int any, int32_t *out,
-                             hipStream_t s)
+void launch_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii, u32 nradii, u64 smax, int any,
+                             int32_t *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_autosplit_choose, dim3(1), dim3(64), 0, s, sums, radii, nradii, smax, any, out);
+    hipLaunchKernelGGL(k_autosplit_choose, dim3(1), dim3(1024), 0, s, counts, R, radii, nradii, smax, any, out);
     CASPHY_HIP(hipGetLastError());
 }
 
