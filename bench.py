@@ -264,10 +264,12 @@ def scan_form(cp, synth, n: int = 100000, W: int = 2560, R: int = 4) -> dict:
 def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 8000) -> dict:
     """Reduced form of the serving configuration: a Poisson stream of tree-build
     (50/50 indel 200 sites / binary 4,096 sites) and TBE jobs, cells log-uniform
-    in [1k, max_cells], run through the request loop twice with the same arrival
-    schedule: queue-only (one worker, one job per launch) and continuous
-    batching (4 workers, small joins batched).  Every result of the batched
-    run is compared with the queue-only run."""
+    in [1k, max_cells], through the request loop in two forms: queue-only (one
+    worker, one job per launch) and continuous batching (4 workers whose
+    launches overlap on the device, small joins batched).  Each form runs
+    once with every job submitted at once (capacity) and once with Poisson
+    arrivals at 80% of the queue-only capacity (latency).  Every result is
+    compared with the first queue-only run."""
     import threading
     rng = np.random.default_rng(seed)
     # pools the jobs draw their cells from
@@ -321,21 +323,31 @@ def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 800
               flush=True)
         return outs, el, st
 
-    outs_q, el_burst, _ = run(1, False, 0)             # capacity of the queue-only form
-    rate = 0.8 * len(jobs) / el_burst
-    outs_q, el_q, st_q = run(1, False, rate)
-    outs_b, el_b, st_b = run(4, True, rate)
-    for a, b in zip(outs_q, outs_b):
-        if isinstance(a, tuple):
-            assert all(np.array_equal(x, y) for x, y in zip(a, b))
-        else:
-            assert np.array_equal(a, b)
+    def same(xs, ys):
+        for a, b in zip(xs, ys):
+            if isinstance(a, tuple):
+                assert all(np.array_equal(x, y) for x, y in zip(a, b))
+            else:
+                assert np.array_equal(a, b)
+
+    # capacity: every job submitted at once
+    outs_q, el_qb, st_qb = run(1, False, 0)
+    outs_b, el_bb, st_bb = run(4, True, 0)
+    same(outs_q, outs_b)
+    # Poisson arrivals at 80% of the queue-only capacity
+    rate = 0.8 * len(jobs) / el_qb
+    outs_q2, _, st_q = run(1, False, rate)
+    outs_b2, _, st_b = run(4, True, rate)
+    same(outs_q, outs_q2)
+    same(outs_q, outs_b2)
+
+    def form(burst, el, poisson):
+        return {"saturated_jobs_per_s": len(jobs) / el, "saturated_launches": burst["launches"],
+                "poisson_jobs_per_s": poisson["jobs_per_s"], "p50_ms": poisson["p50_ms"], "p99_ms": poisson["p99_ms"],
+                "launches": poisson["launches"]}
     return {"jobs": len(jobs), "arrival_rate_per_s": rate, "cells": f"log-uniform 1k..{max_cells // 1000}k",
-            "queue_only": {"jobs_per_s": st_q["jobs_per_s"], "p50_ms": st_q["p50_ms"], "p99_ms": st_q["p99_ms"],
-                           "launches": st_q["launches"]},
-            "continuous_batching": {"jobs_per_s": st_b["jobs_per_s"], "p50_ms": st_b["p50_ms"],
-                                    "p99_ms": st_b["p99_ms"], "launches": st_b["launches"]},
-            "bit_exact_between_forms": True}
+            "queue_only": form(st_qb, el_qb, st_q), "continuous_batching": form(st_bb, el_bb, st_b),
+            "saturated_speedup": el_qb / el_bb, "bit_exact_between_forms": True}
 
 
 def main():
