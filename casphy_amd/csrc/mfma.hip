@@ -264,11 +264,13 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
 // packed bits are prefetched into registers while the current one is used.
 constexpr u32 WT = 256;  // workgroup tile
 
-// Radius-mask / score epilogue of one 32 x 32 block whose rows start at r0 and
-// columns at c0 (mirror: also write the transposed words).
+// Epilogue of one 32 x 32 block whose rows start at r0 and columns at c0
+// (mirror: the tile is off the diagonal of a triangle, so the transposed
+// pairs are written too).  Row blocks of the transfer-distance batches are
+// 128 rows (tile_rep per 128-row block).
 template <int EPI>
-__device__ __forceinline__ void wide_block_epilogue(const MfmaArgs &a, const v16i &acc, u32 r0, u32 c0, u32 lane,
-                                                    bool mirror, int32_t thr)
+__device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &acc, u32 r0, u32 c0, u32 lane,
+                                               bool mirror, int32_t thr)
 {
     const u32 half = lane >> 5, col = lane & 31, j = c0 + col;
     if (EPI == E_MASK) {
@@ -278,6 +280,7 @@ __device__ __forceinline__ void wide_block_epilogue(const MfmaArgs &a, const v16
             const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
             const bool hit = acc[v] >= thr;
             const unsigned long long b = __ballot(hit && j < a.nB);
+            // low half: row 8*(v/4)+v%4, high half: that row + 4
             if ((lane == (u32)v || lane == (u32)(v + 32)) && c0 / 32 < a.mstride) {
                 const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
                 const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
@@ -290,7 +293,7 @@ __device__ __forceinline__ void wide_block_epilogue(const MfmaArgs &a, const v16
             colbits |= __shfl_xor(colbits, 32);
             if (half == 0 && j < a.nB && r0 / 32 < a.mstride) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
         }
-    } else {
+    } else if (EPI == E_SCORES) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
@@ -299,6 +302,43 @@ __device__ __forceinline__ void wide_block_epilogue(const MfmaArgs &a, const v16
                 if (mirror) a.S[(size_t)j * a.ldS + i] = acc[v];
             }
         }
+    } else if (EPI == E_LOCALMAX) {
+        // per row: max over j > i of (S, smallest j), reduced over the 32 lanes of a half
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+            unsigned long long key = 0;
+            if (j < a.nB && j > i) key = ((unsigned long long)(u32)acc[v] << 32) | (0xFFFFFFFFu - j);
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const unsigned long long q = __shfl_xor(key, o);
+                key = q > key ? q : key;
+            }
+            if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
+        }
+    } else if (EPI == E_TBE) {
+        const u32 pb = j < a.nB ? a.popB[j] : 0u;
+        const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
+        u32 cmin = 0xFFFFFFFFu;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+            const u32 out = i < a.nA ? (a.arow_map ? a.arow_map[i] : i) : 0xFFFFFFFFu;
+            const bool valid = out != 0xFFFFFFFFu && j < a.nB;
+            const u32 h = (valid ? a.popA[i] : 0u) + pb - 2u * (u32)acc[v];
+            const u32 hc = a.ntip - h;
+            u32 d = valid ? (h < hc ? h : hc) : 0xFFFFFFFFu;
+            cmin = d < cmin ? d : cmin;
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const u32 q = __shfl_xor(d, o);
+                d = q < d ? q : d;
+            }
+            if (col == 0 && out != 0xFFFFFFFFu) atomicMin(&a.amin[out], d);
+        }
+        const u32 q = __shfl_xor(cmin, 32);
+        cmin = q < cmin ? q : cmin;
+        if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
     }
 }
 
@@ -389,7 +429,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_wide(MfmaArgs a)
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y)
-            wide_block_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 128 + 32 * y, lane,
+            block_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 128 + 32 * y, lane,
                                      mirror, thr);
 }
 
@@ -482,8 +522,156 @@ __global__ __launch_bounds__(512, 1) void k_mfma_wide8(MfmaArgs a)
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y)
-            wide_block_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 64 + 32 * y, lane,
+            block_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 64 + 32 * y, lane,
                                      mirror, thr);
+}
+
+// ---------------------------------------------------------------------------
+// fp4 variant.  A 0/1 site becomes the fp4 (e2m1) value 1.0 = 0b0010 or 0.0,
+// and v_mfma_scale_f32_32x32x64_f8f6f4 with both operands in fp4 and unit
+// scales (exponent 127) adds 64 products of 1.0 x 1.0 per instruction to an
+// fp32 accumulator: every partial sum is an integer below 2^24, so the fp32
+// sums are exact and equal the AND counts.  The fp4 rate is twice the int8
+// rate, and an operand carries 64 sites in 16 bytes per lane instead of 32.
+// The expansion to nibbles runs once per launch into HBM (stage-major, 64-site
+// chunks of 32 bytes per row), so the tile loop only copies bytes to LDS.
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ u32 nib8(u32 b)  // bits 0..7 -> nibbles 0..7 of value 2 or 0
+{
+    u32 x = b & 0xFFu;
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x << 1;
+}
+
+// P[s][r] (128 sites, 16 bytes) -> E chunk 2s (sites 0..63) and 2s + 1 (sites
+// 64..127); chunk c of row r is the 32 bytes at E + (c * pad + r) * 32.
+__global__ void k_expand_fp4(const u32 *P, u32 pad, u32 stages, u32 *E)
+{
+    const u64 total = (u64)stages * pad;
+    for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < total; t += (u64)gridDim.x * blockDim.x) {
+        const u32 s = (u32)(t / pad), r = (u32)(t - (u64)s * pad);
+        const uint4 v = reinterpret_cast<const uint4 *>(P)[t];
+        uint4 *e0 = reinterpret_cast<uint4 *>(E) + ((size_t)(2 * s) * pad + r) * 2;
+        uint4 *e1 = reinterpret_cast<uint4 *>(E) + ((size_t)(2 * s + 1) * pad + r) * 2;
+        e0[0] = make_uint4(nib8(v.x), nib8(v.x >> 8), nib8(v.x >> 16), nib8(v.x >> 24));
+        e0[1] = make_uint4(nib8(v.y), nib8(v.y >> 8), nib8(v.y >> 16), nib8(v.y >> 24));
+        e1[0] = make_uint4(nib8(v.z), nib8(v.z >> 8), nib8(v.z >> 16), nib8(v.z >> 24));
+        e1[1] = make_uint4(nib8(v.w), nib8(v.w >> 8), nib8(v.w >> 16), nib8(v.w >> 24));
+    }
+}
+
+// 128 x 128 pair tile, 4 waves of 2 x 2 blocks, LDS stages of 128 sites (two
+// 64-site chunks, rows of 64 + 16 bytes), one stage of register prefetch.
+// PA / PB point at the expanded nibble arrays.
+template <int EPI>
+__global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4(MfmaArgs a)
+{
+    constexpr int KS = 2;               // chunks per LDS stage
+    constexpr int LR = KS * 32 + 16;    // LDS bytes per row
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LR];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][TILE * LR];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (a.triangle) {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0.0f;
+
+    const u32 nst = a.stages;  // 128-site stages
+    const bool opB = tid >= TILE;  // wave-uniform: waves 0-1 stage A rows, waves 2-3 B rows
+    const u32 row = tid & (TILE - 1);
+    const uint4 *E = reinterpret_cast<const uint4 *>(opB ? a.PB : a.PA);
+    const size_t pad = opB ? a.padB : a.padA;
+    const size_t rbase = (size_t)(opB ? J : I) * TILE + row;
+    unsigned char(*Ls)[TILE * LR] = opB ? Bs : As;
+    static_assert(KS == 2, "one stage = two 32-byte chunks per row");
+    struct Stage {
+        uint4 c0a, c0b, c1a, c1b;
+    };
+    auto ld = [&](u32 s) {
+        const uint4 *p0 = E + ((size_t)(s * KS) * pad + rbase) * 2;
+        const uint4 *p1 = E + ((size_t)(s * KS + 1) * pad + rbase) * 2;
+        return Stage{p0[0], p0[1], p1[0], p1[1]};
+    };
+    auto st = [&](unsigned char *dst, const Stage &g) {
+        uint4 *d = reinterpret_cast<uint4 *>(dst + row * LR);
+        d[0] = g.c0a;
+        d[1] = g.c0b;
+        d[2] = g.c1a;
+        d[3] = g.c1b;
+    };
+    // stage k is held in g0 for even k, g1 for odd k
+    Stage g0 = ld(0), g1{};
+    st(Ls[0], g0);
+    if (nst > 1) g1 = ld(1);
+    if (nst > 2) g0 = ld(2);
+    __syncthreads();
+    const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
+    for (u32 s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+#pragma unroll
+        for (int c = 0; c < KS; ++c) {
+            v8i fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const v4i ta = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + c * 32 + koff]);
+                const v4i tb = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LR + c * 32 + koff]);
+                fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
+                fb[x] = v8i{tb.x, tb.y, tb.z, tb.w, 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[x], fb[y], acc[x][y], 4, 4, 0, 127,
+                                                                                0, 127);
+        }
+        if (s + 1 < nst) {
+            if (s & 1) {
+                st(Ls[buf ^ 1], g0);
+                if (s + 3 < nst) g0 = ld(s + 3);
+            } else {
+                st(Ls[buf ^ 1], g1);
+                if (s + 3 < nst) g1 = ld(s + 3);
+            }
+        }
+        __syncthreads();
+    }
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            v16i ai;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
+            block_epilogue<EPI>(a, ai, I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror, thr);
+        }
 }
 
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
@@ -713,6 +901,38 @@ void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
         throw Error(CASPHY_EINTERNAL, "MFMA wide tiles: unsupported epilogue");
 }
 
+// Expands the packed operands to fp4 nibbles (stream-ordered scratch freed
+// after the launch) and runs the fp4 tiles.
+void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s)
+{
+    MfmaArgs a = in;
+    const u32 stages = a.stages;
+    DevBuf ea((size_t)stages * 2 * a.padA * 32), eb;
+    hipLaunchKernelGGL(k_expand_fp4, dim3(grid_for((u64)stages * a.padA)), dim3(256), 0, s, a.PA, a.padA, stages,
+                       ea.as<u32>());
+    const u32 *EB = ea.as<u32>();
+    if (a.PB != a.PA || a.padB != a.padA) {
+        eb.alloc((size_t)stages * 2 * a.padB * 32);
+        hipLaunchKernelGGL(k_expand_fp4, dim3(grid_for((u64)stages * a.padB)), dim3(256), 0, s, a.PB, a.padB, stages,
+                           eb.as<u32>());
+        EB = eb.as<u32>();
+    }
+    CASPHY_HIP(hipGetLastError());
+    a.PA = ea.as<u32>();
+    a.PB = EB;
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    if (epi == E_MASK)
+        hipLaunchKernelGGL(k_mfma_fp4<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_SCORES)
+        hipLaunchKernelGGL(k_mfma_fp4<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_LOCALMAX)
+        hipLaunchKernelGGL(k_mfma_fp4<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_TBE)
+        hipLaunchKernelGGL(k_mfma_fp4<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
+    else
+        throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+}
+
 void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 {
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
@@ -731,13 +951,14 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
-const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8"};
-constexpr int kNumVariants = 9;
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4"};
+constexpr int kNumVariants = 10;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
+constexpr int kFp4 = 9;
 
-bool variant_supports(int v, int epi) { return v < kWide || epi == E_MASK || epi == E_SCORES; }
+bool variant_supports(int v, int epi) { return v < kWide || v == kFp4 || epi == E_MASK || epi == E_SCORES; }
 
 int variant_from_env()
 {
@@ -752,7 +973,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
     if (!variant_supports(v, epi)) v = kDefaultVariant;
-    const u32 T = v == 6 ? RT : v >= kWide ? WT : TILE;
+    const u32 T = v == 6 ? RT : (v == kWide || v == kWide8) ? WT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntB = ceil_div(a.nB, T);
     a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1) / 2) : ntA * a.ntB;
@@ -767,6 +988,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case 6: launch_reg(epi, a, s); break;
     case kWide: launch_wide(epi, a, s); break;
     case kWide8: launch_wide8(epi, a, s); break;
+    case kFp4: launch_fp4(epi, a, s); break;
     default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());

@@ -16,18 +16,19 @@ def rand_bits(rng, n, W, p):
 
 
 # ------------------------------------------------------------------ scores
-@pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-wide", "mfma-wide8", "mfma-reg", "valu"])
+@pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-fp4", "mfma-wide", "mfma-wide8", "mfma-reg", "valu"])
 def bitpath(request, monkeypatch):
     """Runs a test on the autotuned int8 matrix-core path (every candidate
     variant runs on the first call of a shape class), on the fixed variants
-    (LDS-staged bytes with 128 x 128 or 256 x 256 workgroup tiles,
-    register-resident bits) and on the VALU popcount tiles."""
+    (LDS-staged bytes with 128 x 128 or 256 x 256 workgroup tiles, fp4
+    nibbles, register-resident bits) and on the VALU popcount tiles."""
     monkeypatch.setenv("CASPHY_MFMA", "0" if request.param == "valu" else "1")
     if request.param == "mfma-auto":
         monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
         monkeypatch.delenv("CASPHY_AUTOTUNE", raising=False)
     else:
-        variant = {"mfma-reg": "reg", "mfma-wide": "wide", "mfma-wide8": "wide8"}.get(request.param, "64x2")
+        variant = {"mfma-reg": "reg", "mfma-wide": "wide", "mfma-wide8": "wide8", "mfma-fp4": "fp4"}.get(request.param,
+                                                                                                  "64x2")
         monkeypatch.setenv("CASPHY_MFMA_VARIANT", variant)
     return request.param
 
