@@ -38,6 +38,7 @@ RADII = np.arange(1, 17, dtype=np.int32)
 AUTOSPLIT_BOUND = 9.5  # nNeighborBound default of the autoSplit loop, R/CasPhy_functions.R:1967
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
 MFMA_I8_PEAK_TOPS = 5000.0                # MI355X dense int8 matrix-core peak, 5.0 POP/s
+MFMA_FP4_PEAK_TOPS = 10100.0              # MI355X dense fp4 matrix-core peak, 10.1 PFLOP/s
 
 
 def autosplit_radius(counts: np.ndarray, n: int) -> int:
@@ -99,7 +100,8 @@ def calibration_copy(nbytes: int = 256 << 20) -> None:
 VARIANT_KERNEL = {"128x1": "k_mfma_tiles<0, 128, 1, 2>", "128x2": "k_mfma_tiles<0, 128, 2, 2>",
                   "128x4": "k_mfma_tiles<0, 128, 4, 2>", "64x2": "k_mfma_tiles<0, 64, 2, 4>",
                   "64x4": "k_mfma_tiles<0, 64, 4, 4>", "64x8": "k_mfma_tiles<0, 64, 8, 4>",
-                  "reg": "k_mfma_reg<0>", "wide": "k_mfma_wide<0>", "wide8": "k_mfma_wide8<0>"}
+                  "reg": "k_mfma_reg<0>", "wide": "k_mfma_wide<0>", "wide8": "k_mfma_wide8<0>",
+                  "fp4": "k_mfma_fp4<0>"}
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_mfma_tiles_mask_10000x4096_pmc.md")
 
 
@@ -424,6 +426,8 @@ def main():
     if roof["bound"] == "mfma":
         roof["variant"] = variant
         roof["kernel"] = VARIANT_KERNEL[variant]
+        if variant == "fp4":  # 0/1 sites as fp4 1.0/0.0: priced against the fp4 matrix-core peak
+            roof["peak"] = MFMA_FP4_PEAK_TOPS
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = measured_traffic(roof["kernel"])
     roof["kernel_ms"] = tile_s * 1000

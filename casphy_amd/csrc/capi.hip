@@ -209,7 +209,7 @@ void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
 // u64 offsets (offsets[n] = total) on the stream; nothing is read back.
 void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
                  const int32_t *thr_dev, RadiusOut &o, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
-                 const u32 *stage)
+                 const u32 *stage, const u32 *estage = nullptr)
 {
     const u32 nt = ceil_div(n, TILE);
     const u32 mstride = nt * 4;
@@ -224,6 +224,7 @@ void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &plan
     if (mfma) {
         MfmaArgs m;
         m.PA = m.PB = stage; m.padA = m.padB = stage_rows(n); m.stages = ldw / 4;
+        m.EA = m.EB = estage;
         m.nA = n; m.nB = n;
         m.thr = radius; m.thr_dev = thr_dev; m.mask = o.mask.as<u32>(); m.mstride = mstride;
         launch_mfma_tiles(E_MASK, m, s);
@@ -246,9 +247,9 @@ void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &plan
 
 void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
                     RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                    const u32 *stage = nullptr)
+                    const u32 *stage = nullptr, const u32 *estage = nullptr)
 {
-    radius_mask(metric, dmat, n, ldw, planes, radius, nullptr, o, s, ev0, ev1, stage);
+    radius_mask(metric, dmat, n, ldw, planes, radius, nullptr, o, s, ev0, ev1, stage, estage);
     const u32 mstride = ceil_div(n, TILE) * 4;
     d2h(&o.total, o.offsets.as<u64>() + n, 8, s);
     o.idx.reserve(std::max<u64>(o.total, 1) * 4);
@@ -296,11 +297,19 @@ void radius_lists(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &pla
     *nidx = o.total;
 }
 
+// Few queries over wide unit-weight bit rows take the scan form (one HBM pass);
+// CASPHY_SCAN=0 keeps the tiles.
+bool use_scan(int metric, const Planes &planes, u32 R, u32 ldw)
+{
+    const char *e = std::getenv("CASPHY_SCAN");
+    return metric == M_AND && planes.nplanes == 0 && R <= SCAN_Q && scan_fits(ldw) && !(e && e[0] == '0');
+}
+
 // tryRadius: query x cell scores (matrix-core tiles for unit-weight bits,
 // VALU tiles otherwise) followed by one counting workgroup per query.
 void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, const int32_t *radii,
                u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage = nullptr,
-               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const u32 *estage = nullptr)
 {
     for (u32 k = 0; k < R; ++k)
         CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
@@ -310,8 +319,7 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
     h2d(dq.p, q, (size_t)R * 4, s);
     h2d(dr.p, radii, (size_t)nradii * 4, s);
     const u32 ntA = ceil_div(R, TILE), ntB = ceil_div(n, TILE);
-    const char *scan_env = std::getenv("CASPHY_SCAN");
-    if (metric == M_AND && planes.nplanes == 0 && R <= SCAN_Q && scan_fits(ldw) && !(scan_env && scan_env[0] == '0')) {
+    if (use_scan(metric, planes, R, ldw)) {
         // few queries over wide rows: one HBM-bound pass over the cells
         DevBuf qr((size_t)R * ldw * 4);
         for (u32 k = 0; k < R; ++k)
@@ -332,6 +340,7 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
         launch_to_stage_major(dmat, R, ldw, ldw, sq.as<u32>(), s, dq.as<u32>());
         MfmaArgs m;
         m.PA = sq.as<u32>(); m.PB = stage; m.padA = stage_rows(R); m.padB = stage_rows(n); m.stages = ldw / 4;
+        m.EB = estage;
         m.nA = R; m.nB = n; m.triangle = 0;
         m.S = dS.as<int32_t>(); m.ldS = n;
         if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
@@ -652,7 +661,18 @@ struct casphy_cells {
     u32 n = 0, ldw = 0;
     DevBuf mat;
     Planes planes;
-    DevBuf stage;  // stage-major copy for the MFMA tiles (bit kind, unit weights)
+    DevBuf stage;   // stage-major copy for the MFMA tiles (bit kind, unit weights)
+    DevBuf estage;  // its fp4 expansion (for the fp4 tile variant)
+    bool mfma() const { return metric == M_AND && planes.nplanes == 0 && mfma_enabled(); }
+    // stage-major copy and its fp4 expansion, made on first use
+    void staged(hipStream_t s)
+    {
+        if (!stage.p) make_stage(mat.as<u32>(), n, ldw, stage, s);
+        if (!estage.p) {
+            estage.alloc(fp4_bytes(stage_rows(n), ldw / 4));
+            launch_expand_fp4(stage.as<u32>(), stage_rows(n), ldw / 4, estage.as<u32>(), s);
+        }
+    }
     RadiusOut out;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed[2] = {false, false};  // ev[0..1] / ev[2..3] recorded by some call
@@ -717,17 +737,11 @@ int casphy_cells_radius(casphy_cells *h, int32_t radius, uint64_t *nidx)
     return guarded([&] {
         require(h && nidx, "casphy_cells_radius: bad arguments");
         hipStream_t s = thread_stream();
-        if (h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled() && !h->stage.p)
-            make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
+        if (h->mfma()) h->staged(s);
         CASPHY_HIP(hipEventRecord(h->ev[2], s));
-        const u32 *stage = nullptr;
-        if (h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled()) {
-            if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
-            stage = h->stage.as<u32>();
-        }
         h->timed[0] = h->timed[1] = true;
         radius_compute(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radius, h->out, s, h->ev[0], h->ev[1],
-                       stage);
+                       h->mfma() ? h->stage.as<u32>() : nullptr, h->mfma() ? h->estage.as<u32>() : nullptr);
         CASPHY_HIP(hipEventRecord(h->ev[3], s));
         CASPHY_HIP(hipEventSynchronize(h->ev[3]));
         *nidx = h->out.total;
@@ -752,14 +766,11 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
         require(nradii >= 1 && nradii <= MAX_RADII, "casphy_cells_try_radii: 1..16 radii");
         if (R == 0) return;
         hipStream_t s = thread_stream();
-        const u32 *stage = nullptr;
-        if (h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled()) {
-            if (!h->stage.p) make_stage(h->mat.as<u32>(), h->n, h->ldw, h->stage, s);
-            stage = h->stage.as<u32>();
-        }
+        const bool tiles = h->mfma() && !use_scan(h->metric, h->planes, R, h->ldw);
+        if (tiles) h->staged(s);
         h->timed[0] = true;
-        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s, stage,
-                  h->ev[0], h->ev[1]);
+        try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s,
+                  tiles ? h->stage.as<u32>() : nullptr, h->ev[0], h->ev[1], tiles ? h->estage.as<u32>() : nullptr);
     });
 }
 
@@ -777,8 +788,8 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         int any;
         autosplit_limit(R, n, bound, smax, any);
         hipStream_t s = thread_stream();
-        const bool mfma = h->metric == M_AND && h->planes.nplanes == 0 && mfma_enabled();
-        if (mfma && !h->stage.p) make_stage(h->mat.as<u32>(), n, ldw, h->stage, s);
+        const bool mfma = h->mfma();
+        if (mfma) h->staged(s);
         CASPHY_HIP(hipEventRecord(h->ev[2], s));
 
         // one upload: radii (padded to 16) | query rows; results land after them
@@ -804,6 +815,7 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
             MfmaArgs m;
             m.PA = h->a_sq.as<u32>(); m.PB = h->stage.as<u32>(); m.padA = stage_rows(R); m.padB = stage_rows(n);
             m.stages = ldw / 4; m.nA = R; m.nB = n; m.triangle = 0;
+            m.EB = h->estage.as<u32>();
             m.S = dS; m.ldS = n;
             launch_mfma_tiles(E_SCORES, m, s);
         } else {
@@ -825,7 +837,7 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         RadiusOut &o = h->out;
         h->timed[0] = h->timed[1] = true;
         radius_mask(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
-                    mfma ? h->stage.as<u32>() : nullptr);
+                    mfma ? h->stage.as<u32>() : nullptr, mfma ? h->estage.as<u32>() : nullptr);
         const u32 mstride = ceil_div(n, TILE) * 4;
         const u64 cap = o.idx.bytes / 4;
         if (cap) launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, cap);

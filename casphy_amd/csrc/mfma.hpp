@@ -16,6 +16,8 @@ struct MfmaArgs {
     const u32 *PB = nullptr;  // and of the B rows (== PA for self scoring)
     u32 padA = 0, padB = 0;   // padded row counts (multiples of 128)
     u32 stages = 0;           // 128 sites each
+    const u32 *EA = nullptr;  // optional fp4 expansions of PA / PB (launch_expand_fp4); the
+    const u32 *EB = nullptr;  // fp4 variant expands what is not given
     u32 nA = 0, nB = 0;       // valid rows
     u32 ntB = 0;              // set by the launcher (tile grid depends on the variant)
     u32 ntiles = 0;           // set by the launcher
@@ -44,6 +46,11 @@ bool mfma_enabled();
 // Row popcounts of the first n rows of a stage-major matrix whose stages hold
 // `stride` rows.
 void launch_stage_popc(const u32 *P, u32 n, u32 stride, u32 ldw, u32 *pop, hipStream_t s);
+
+// fp4 expansion of a stage-major copy (rows padded to pad, `stages` 128-site
+// stages): every site becomes a nibble, 1.0 (0b0010) or 0; fp4_bytes() bytes.
+inline size_t fp4_bytes(u32 pad, u32 stages) { return (size_t)stages * 2 * pad * 32; }
+void launch_expand_fp4(const u32 *P, u32 pad, u32 stages, u32 *E, hipStream_t s);
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
 // JSON report of the autotuner's choices (shape class -> variant, timings).
 std::string mfma_tuning_report();
