@@ -674,6 +674,117 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4(MfmaArgs a)
         }
 }
 
+// fp4 tiles with the B operands read straight from the expanded array into
+// registers (each lane's 16-byte operand is contiguous there), so LDS carries
+// only the A rows: half the LDS traffic of k_mfma_fp4 for 1.5x its L2 reads.
+template <int EPI>
+__global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
+{
+    constexpr int LR = 2 * 32 + 16;  // LDS bytes per A row (one 128-site stage)
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LR];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (a.triangle) {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0.0f;
+
+    const u32 nst = a.stages;
+    // A staging: thread tid copies chunk (tid >> 7) of row (tid & 127) of each stage
+    const u32 arow_s = tid & (TILE - 1), ach = tid >> 7;
+    const uint4 *EA = reinterpret_cast<const uint4 *>(a.PA) + ((size_t)I * TILE + arow_s) * 2;
+    const size_t padA = a.padA, padB = a.padB;
+    auto lda = [&](u32 s, uint4 &p, uint4 &q) {
+        const uint4 *src = EA + (size_t)(2 * s + ach) * padA * 2;
+        p = src[0];
+        q = src[1];
+    };
+    // B operands: lane's column row of block y, 16 bytes at half (lane >> 5) of each chunk
+    const u32 koffB = lane >> 5;
+    const uint4 *EB0 = reinterpret_cast<const uint4 *>(a.PB) + ((size_t)J * TILE + wn * 64 + (lane & 31)) * 2 + koffB;
+    const uint4 *EB1 = EB0 + 32 * 2;
+    auto ldb = [&](u32 s, uint4 &b00, uint4 &b01, uint4 &b10, uint4 &b11) {  // b<chunk><y>
+        const size_t o0 = (size_t)(2 * s) * padB * 2, o1 = (size_t)(2 * s + 1) * padB * 2;
+        b00 = EB0[o0];
+        b01 = EB1[o0];
+        b10 = EB0[o1];
+        b11 = EB1[o1];
+    };
+    uint4 ap, aq, b00, b01, b10, b11;
+    lda(0, ap, aq);
+    {
+        uint4 *d = reinterpret_cast<uint4 *>(&As[0][arow_s * LR + ach * 32]);
+        d[0] = ap;
+        d[1] = aq;
+    }
+    ldb(0, b00, b01, b10, b11);
+    if (nst > 1) lda(1, ap, aq);
+    __syncthreads();
+    const u32 arow = wm * 64 + (lane & 31), koff = 16 * (lane >> 5);
+    for (u32 s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        // B operands of this stage; the next stage's are loaded behind them
+        const uint4 c00 = b00, c01 = b01, c10 = b10, c11 = b11;
+        if (s + 1 < nst) ldb(s + 1, b00, b01, b10, b11);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            v8i fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const v4i ta = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + c * 32 + koff]);
+                fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
+            }
+            const uint4 y0 = c ? c10 : c00, y1 = c ? c11 : c01;
+            fb[0] = v8i{(int)y0.x, (int)y0.y, (int)y0.z, (int)y0.w, 0, 0, 0, 0};
+            fb[1] = v8i{(int)y1.x, (int)y1.y, (int)y1.z, (int)y1.w, 0, 0, 0, 0};
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[x], fb[y], acc[x][y], 4, 4, 0, 127,
+                                                                                0, 127);
+        }
+        if (s + 1 < nst) {
+            uint4 *d = reinterpret_cast<uint4 *>(&As[buf ^ 1][arow_s * LR + ach * 32]);
+            d[0] = ap;
+            d[1] = aq;
+            if (s + 2 < nst) lda(s + 2, ap, aq);
+        }
+        __syncthreads();
+    }
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            v16i ai;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
+            block_epilogue<EPI>(a, ai, I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror, thr);
+        }
+}
+
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -909,7 +1020,7 @@ void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
 
 // Expands the packed operands to fp4 nibbles (stream-ordered scratch freed
 // after the launch) and runs the fp4 tiles.
-void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s)
+void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, bool bdirect)
 {
     MfmaArgs a = in;
     const u32 stages = a.stages;
@@ -932,16 +1043,28 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s)
     a.PA = EA;
     a.PB = EB;
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
-    if (epi == E_MASK)
+    if (bdirect) {
+        if (epi == E_MASK)
+            hipLaunchKernelGGL(k_mfma_fp4b<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_SCORES)
+            hipLaunchKernelGGL(k_mfma_fp4b<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_LOCALMAX)
+            hipLaunchKernelGGL(k_mfma_fp4b<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_TBE)
+            hipLaunchKernelGGL(k_mfma_fp4b<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
+        else
+            throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    } else if (epi == E_MASK) {
         hipLaunchKernelGGL(k_mfma_fp4<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
-    else if (epi == E_SCORES)
+    } else if (epi == E_SCORES) {
         hipLaunchKernelGGL(k_mfma_fp4<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
-    else if (epi == E_LOCALMAX)
+    } else if (epi == E_LOCALMAX) {
         hipLaunchKernelGGL(k_mfma_fp4<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
-    else if (epi == E_TBE)
+    } else if (epi == E_TBE) {
         hipLaunchKernelGGL(k_mfma_fp4<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
-    else
+    } else {
         throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    }
 }
 
 void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
@@ -962,14 +1085,15 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
 namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
-const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4"};
-constexpr int kNumVariants = 10;
+const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4",
+                                 "fp4b"};
+constexpr int kNumVariants = 11;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
-constexpr int kFp4 = 9;
+constexpr int kFp4 = 9, kFp4b = 10;
 
-bool variant_supports(int v, int epi) { return v < kWide || v == kFp4 || epi == E_MASK || epi == E_SCORES; }
+bool variant_supports(int v, int epi) { return v < kWide || v >= kFp4 || epi == E_MASK || epi == E_SCORES; }
 
 int variant_from_env()
 {
@@ -999,7 +1123,8 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case 6: launch_reg(epi, a, s); break;
     case kWide: launch_wide(epi, a, s); break;
     case kWide8: launch_wide8(epi, a, s); break;
-    case kFp4: launch_fp4(epi, a, s); break;
+    case kFp4: launch_fp4(epi, a, s, false); break;
+    case kFp4b: launch_fp4(epi, a, s, true); break;
     default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());
