@@ -623,22 +623,15 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4(MfmaArgs a)
         d[2] = g.c1a;
         d[3] = g.c1b;
     };
-    // stage k is held in g0 for even k, g1 for odd k
-    Stage g0 = ld(0), g1{};
-    st(Ls[0], g0);
-    if (nst > 1) g1 = ld(1);
-    if (nst > 2) g0 = ld(2);
-    __syncthreads();
     const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
-    for (u32 s = 0; s < nst; ++s) {
-        const int buf = s & 1;
+    auto compute = [&](const unsigned char *A, const unsigned char *B) {
 #pragma unroll
         for (int c = 0; c < KS; ++c) {
             v8i fa[2], fb[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
-                const v4i ta = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + c * 32 + koff]);
-                const v4i tb = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LR + c * 32 + koff]);
+                const v4i ta = *reinterpret_cast<const v4i *>(&A[(arow + 32 * x) * LR + c * 32 + koff]);
+                const v4i tb = *reinterpret_cast<const v4i *>(&B[(brow + 32 * x) * LR + c * 32 + koff]);
                 fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
                 fb[x] = v8i{tb.x, tb.y, tb.z, tb.w, 0, 0, 0, 0};
             }
@@ -649,15 +642,24 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4(MfmaArgs a)
                     acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[x], fb[y], acc[x][y], 4, 4, 0, 127,
                                                                                 0, 127);
         }
-        if (s + 1 < nst) {
-            if (s & 1) {
-                st(Ls[buf ^ 1], g0);
-                if (s + 3 < nst) g0 = ld(s + 3);
-            } else {
-                st(Ls[buf ^ 1], g1);
-                if (s + 3 < nst) g1 = ld(s + 3);
-            }
-        }
+    };
+    // stage k is held in g0 for even k, g1 for odd k; the loop runs two stages
+    // per trip (stages come in multiples of 8) so no register copies are needed,
+    // and loads are unconditional (past the end they re-read the last stage)
+    // so the compiler can count the loads in flight instead of draining them
+    const u32 last = nst - 1;
+    Stage g0 = ld(0), g1 = ld(1);
+    st(Ls[0], g0);
+    g0 = ld(min(2u, last));
+    __syncthreads();
+    for (u32 s = 0; s < nst; s += 2) {
+        compute(As[0], Bs[0]);
+        st(Ls[1], g1);
+        g1 = ld(min(s + 3, last));
+        __syncthreads();
+        compute(As[1], Bs[1]);
+        st(Ls[0], g0);  // past the end: a stage no one reads
+        g0 = ld(min(s + 4, last));
         __syncthreads();
     }
 
@@ -729,28 +731,20 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
         b10 = EB0[o1];
         b11 = EB1[o1];
     };
-    uint4 ap, aq, b00, b01, b10, b11;
-    lda(0, ap, aq);
-    {
-        uint4 *d = reinterpret_cast<uint4 *>(&As[0][arow_s * LR + ach * 32]);
-        d[0] = ap;
-        d[1] = aq;
-    }
-    ldb(0, b00, b01, b10, b11);
-    if (nst > 1) lda(1, ap, aq);
-    __syncthreads();
     const u32 arow = wm * 64 + (lane & 31), koff = 16 * (lane >> 5);
-    for (u32 s = 0; s < nst; ++s) {
-        const int buf = s & 1;
-        // B operands of this stage; the next stage's are loaded behind them
-        const uint4 c00 = b00, c01 = b01, c10 = b10, c11 = b11;
-        if (s + 1 < nst) ldb(s + 1, b00, b01, b10, b11);
+    auto stA = [&](unsigned char *dst, const uint4 &p, const uint4 &q) {
+        uint4 *d = reinterpret_cast<uint4 *>(&dst[arow_s * LR + ach * 32]);
+        d[0] = p;
+        d[1] = q;
+    };
+    auto compute = [&](const unsigned char *A, const uint4 &c00, const uint4 &c01, const uint4 &c10,
+                       const uint4 &c11) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             v8i fa[2], fb[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
-                const v4i ta = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + c * 32 + koff]);
+                const v4i ta = *reinterpret_cast<const v4i *>(&A[(arow + 32 * x) * LR + c * 32 + koff]);
                 fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
             }
             const uint4 y0 = c ? c10 : c00, y1 = c ? c11 : c01;
@@ -763,14 +757,150 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
                     acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[x], fb[y], acc[x][y], 4, 4, 0, 127,
                                                                                 0, 127);
         }
-        if (s + 1 < nst) {
-            uint4 *d = reinterpret_cast<uint4 *>(&As[buf ^ 1][arow_s * LR + ach * 32]);
-            d[0] = ap;
-            d[1] = aq;
-            if (s + 2 < nst) lda(s + 2, ap, aq);
-        }
+    };
+    // two stages per trip (stages come in multiples of 8): B operands of even
+    // stages in x**, odd stages in y**; A rows of the next stage in ap / aq
+    // loads are unconditional (past the end they re-read the last stage) so the
+    // compiler can count the loads in flight instead of draining them
+    const u32 last = nst - 1;
+    uint4 ap, aq, x00, x01, x10, x11, y00, y01, y10, y11;
+    lda(0, ap, aq);
+    stA(As[0], ap, aq);
+    ldb(0, x00, x01, x10, x11);
+    lda(1, ap, aq);
+    __syncthreads();
+    for (u32 s = 0; s < nst; s += 2) {
+        ldb(s + 1, y00, y01, y10, y11);
+        compute(As[0], x00, x01, x10, x11);
+        stA(As[1], ap, aq);
+        lda(min(s + 2, last), ap, aq);
+        __syncthreads();
+        ldb(min(s + 2, last), x00, x01, x10, x11);
+        compute(As[1], y00, y01, y10, y11);
+        stA(As[0], ap, aq);  // past the end: a stage no one reads
+        lda(min(s + 3, last), ap, aq);
         __syncthreads();
     }
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            v16i ai;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
+            block_epilogue<EPI>(a, ai, I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror, thr);
+        }
+}
+
+// fp4 tiles fed by LDS DMA: global_load_lds_dwordx4 moves each 16-byte
+// granule of a stage (A and B rows, 64 bytes per row per 128-site stage)
+// from the expanded array straight into LDS, with no VGPR staging and no
+// ds_write.  One DMA instruction fills 1 KB of LDS, lane i at byte 16 i, so
+// the lane -> granule map is chosen to give the swizzled row layout the
+// operand reads want: row r holds granules 0..3 (chunk c, half h at g = 2c + h)
+// at positions g ^ ((r >> 1) & 3), which keeps 8 consecutive rows of one
+// granule on distinct banks.  Two stage buffers (separate LDS objects, loop
+// unrolled by two): the DMA of stage s + 1 runs under the MFMAs of stage s.
+constexpr u32 FP4D_STAGE = 2 * TILE * 64;  // bytes of one stage buffer (A then B)
+
+__device__ __forceinline__ void fp4d_dma(unsigned char *buf, const unsigned char *EA, const unsigned char *EB,
+                                         size_t padA, size_t padB, u32 I, u32 J, u32 s, u32 wave, u32 lane)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const u32 q = (wave * 4 + k) * 64 + lane;  // granule index in the stage buffer
+        const bool opB = q >= TILE * 4;
+        const u32 q2 = q & (TILE * 4 - 1), row = q2 >> 2, g = (q2 & 3) ^ ((row >> 1) & 3);
+        const unsigned char *E = opB ? EB : EA;
+        const size_t pad = opB ? padB : padA;
+        const size_t rr = (size_t)(opB ? J : I) * TILE + row;
+        const unsigned char *src = E + ((size_t)(2 * s + (g >> 1)) * pad + rr) * 32 + (g & 1) * 16;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                         (__attribute__((address_space(3))) void *)(buf + (wave * 4 + k) * 1024),
+                                         16, 0, 0);
+    }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char S0[FP4D_STAGE];
+    __shared__ __attribute__((aligned(16))) unsigned char S1[FP4D_STAGE];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (a.triangle) {
+        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((u64)j * (j + 1) / 2 > t) --j;
+        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
+        J = j;
+        I = t - (u32)((u64)j * (j + 1) / 2);
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0.0f;
+
+    const u32 nst = a.stages, last = nst - 1;
+    const unsigned char *EA = reinterpret_cast<const unsigned char *>(a.PA);
+    const unsigned char *EB = reinterpret_cast<const unsigned char *>(a.PB);
+    const size_t padA = a.padA, padB = a.padB;
+    const u32 ra = wm * 64 + (lane & 31), rb = wn * 64 + (lane & 31), h = lane >> 5;
+    auto compute = [&](const unsigned char *buf) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            v8i fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const u32 r1 = ra + 32 * x, r2 = rb + 32 * x;
+                const u32 g = 2 * c + h;
+                const v4i ta = *reinterpret_cast<const v4i *>(buf + r1 * 64 + ((g ^ ((r1 >> 1) & 3)) * 16));
+                const v4i tb =
+                    *reinterpret_cast<const v4i *>(buf + TILE * 64 + r2 * 64 + ((g ^ ((r2 >> 1) & 3)) * 16));
+                fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
+                fb[x] = v8i{tb.x, tb.y, tb.z, tb.w, 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[x], fb[y], acc[x][y], 4, 4, 0, 127,
+                                                                                0, 127);
+        }
+    };
+    // stages come in multiples of 8; past the end the DMA re-reads the last
+    // stage into a buffer no one reads again
+    // sched_barrier keeps each DMA issue ahead of the MFMAs it overlaps
+    fp4d_dma(S0, EA, EB, padA, padB, I, J, 0, wave, lane);
+    for (u32 s = 0; s < nst; s += 2) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA of stage s has landed
+        __syncthreads();                     // ... and every wave's
+        fp4d_dma(S1, EA, EB, padA, padB, I, J, s + 1, wave, lane);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(S0);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        fp4d_dma(S0, EA, EB, padA, padB, I, J, min(s + 2, last), wave, lane);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(S1);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 
     const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const bool mirror = a.triangle && I != J;
@@ -1020,7 +1150,8 @@ void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
 
 // Expands the packed operands to fp4 nibbles (stream-ordered scratch freed
 // after the launch) and runs the fp4 tiles.
-void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, bool bdirect)
+// mode 0: LDS staging through registers, 1: B operands from global, 2: LDS DMA
+void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
 {
     MfmaArgs a = in;
     const u32 stages = a.stages;
@@ -1043,7 +1174,18 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, bool bdirect)
     a.PA = EA;
     a.PB = EB;
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
-    if (bdirect) {
+    if (mode == 2) {
+        if (epi == E_MASK)
+            hipLaunchKernelGGL(k_mfma_fp4d<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_SCORES)
+            hipLaunchKernelGGL(k_mfma_fp4d<E_SCORES>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_LOCALMAX)
+            hipLaunchKernelGGL(k_mfma_fp4d<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_TBE)
+            hipLaunchKernelGGL(k_mfma_fp4d<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
+        else
+            throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    } else if (mode == 1) {
         if (epi == E_MASK)
             hipLaunchKernelGGL(k_mfma_fp4b<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
         else if (epi == E_SCORES)
@@ -1086,12 +1228,12 @@ namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
 const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4",
-                                 "fp4b"};
-constexpr int kNumVariants = 11;
+                                 "fp4b", "fp4d"};
+constexpr int kNumVariants = 12;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
-constexpr int kFp4 = 9, kFp4b = 10;
+constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11;
 
 bool variant_supports(int v, int epi) { return v < kWide || v >= kFp4 || epi == E_MASK || epi == E_SCORES; }
 
@@ -1123,8 +1265,9 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case 6: launch_reg(epi, a, s); break;
     case kWide: launch_wide(epi, a, s); break;
     case kWide8: launch_wide8(epi, a, s); break;
-    case kFp4: launch_fp4(epi, a, s, false); break;
-    case kFp4b: launch_fp4(epi, a, s, true); break;
+    case kFp4: launch_fp4(epi, a, s, 0); break;
+    case kFp4b: launch_fp4(epi, a, s, 1); break;
+    case kFp4d: launch_fp4(epi, a, s, 2); break;
     default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());
