@@ -568,7 +568,7 @@ __global__ void k_expand_fp4(const u32 *P, u32 pad, u32 stages, u32 *E)
 // 64-site chunks, rows of 64 + 16 bytes), one stage of register prefetch.
 // PA / PB point at the expanded nibble arrays.
 template <int EPI>
-__global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4(MfmaArgs a)
+__global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
 {
     constexpr int KS = 2;               // chunks per LDS stage
     constexpr int LR = KS * 32 + 16;    // LDS bytes per row
