@@ -133,10 +133,12 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, 
     if (t == 1023) offsets[n] = part[1023];
 }
 
-// One wave per row.  Per chunk of 64 mask words: a wave prefix sum of the
-// word popcounts places every lane's ids in an LDS staging buffer (full words
-// as four 16-byte stores of consecutive ids), then the wave streams the chunk's
-// ids to HBM with consecutive lanes writing consecutive addresses.
+// One wave per row.  Per chunk of 64 mask words: a chunk of 64 full words is
+// the run of ids 32 w0 .. 32 w0 + 2047 and goes straight to HBM as 16-byte
+// stores (a few single ids align the run); any other chunk takes a wave
+// prefix sum of the word popcounts that places every lane's ids in an LDS
+// staging buffer, then the wave streams the chunk's ids to HBM with
+// consecutive lanes writing consecutive addresses.
 constexpr int CWAVES = 4;
 __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words,
                                                         const u64 *offsets, u32 *idx, u64 cap)
@@ -151,6 +153,27 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
     for (u32 w0 = 0; w0 < words; w0 += 64) {
         const u32 w = w0 + lane;
         u32 v = w < words ? m[w] : 0u;
+        if (w0 + 64 <= words && __all(v == 0xFFFFFFFFu)) {  // wave-uniform
+            const u32 id0 = w0 * 32;
+            const u32 head = (u32)((4 - (base & 3)) & 3);
+            if (lane < head && base + lane < cap) idx[base + lane] = id0 + lane;
+            const u64 b4 = base + head;  // multiple of 4: 16-byte aligned
+            const u32 nq = (2048 - head) >> 2;
+            uint4 *dst = reinterpret_cast<uint4 *>(idx + b4);
+            for (u32 q = lane; q < nq; q += 64) {
+                const u32 k = head + 4 * q;
+                if (b4 + 4 * q + 4 <= cap) {
+                    dst[q] = make_uint4(id0 + k, id0 + k + 1, id0 + k + 2, id0 + k + 3);
+                } else {
+                    for (u32 e = 0; e < 4; ++e)
+                        if (b4 + 4 * q + e < cap) idx[b4 + 4 * q + e] = id0 + k + e;
+                }
+            }
+            const u32 done = head + 4 * nq;
+            if (lane < 2048 - done && base + done + lane < cap) idx[base + done + lane] = id0 + done + lane;
+            base += 2048;
+            continue;
+        }
         const u32 c = (u32)__builtin_popcount(v);
         u32 inc = c;
         for (int o = 1; o < 64; o <<= 1) {
