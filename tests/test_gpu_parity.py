@@ -374,3 +374,22 @@ def test_try_radius_scan_form(hip, orc, monkeypatch, n, W):
         monkeypatch.setenv("CASPHY_SCAN", "0")
         assert np.array_equal(hip.try_radii(bmat, radii, q), got)
         monkeypatch.delenv("CASPHY_SCAN")
+
+
+def test_radius_full_rows(hip, orc, bitpath):
+    """Rows whose mask chunks are all ones (radius 0 and 1 on dense rows) take
+    the run-of-ids compaction path, also when a resident buffer has to grow."""
+    rng = np.random.default_rng(77)
+    mat = rand_bits(rng, 2100, 1, 0.6)
+    bmat = np.ascontiguousarray(mat.T)
+    for r in (0, 1, 9):
+        counts, idx = hip.radius_neighbor(bmat, r)
+        c2, i2 = orc.radius_bit(mat, r)
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2), r
+    cells = hip.Cells(bmat, "bit", wordmajor=True)
+    q = synth.query_rows(3, 2100, 64)
+    for radii in ([12, 16], [0, 1]):  # small lists first, then every pair
+        r, total = cells.autosplit(np.array(radii), q, 10.0)
+        counts, idx = cells.fetch()
+        c2, i2 = orc.radius_bit(mat, r)
+        assert r == radii[0] and np.array_equal(counts, c2) and np.array_equal(idx, i2)
