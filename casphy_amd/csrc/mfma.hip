@@ -228,7 +228,7 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
                     }
                     if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
                 }
-            } else if (EPI == E_TBE) {
+            } else if (EPI == E_TBE && r0 < a.nA) {  // a block past the last row has nothing to write
                 const u32 pb = j < a.nB ? a.popB[j] : 0u;
                 const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
                 u32 cmin = 0xFFFFFFFFu;
@@ -316,7 +316,7 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
             }
             if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
         }
-    } else if (EPI == E_TBE) {
+    } else if (EPI == E_TBE && r0 < a.nA) {  // a block past the last row has nothing to write
         const u32 pb = j < a.nB ? a.popB[j] : 0u;
         const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
         u32 cmin = 0xFFFFFFFFu;
@@ -1083,7 +1083,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
                     }
                     if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
                 }
-            } else if (EPI == E_TBE) {
+            } else if (EPI == E_TBE && r0 < a.nA) {  // a block past the last row has nothing to write
                 const u32 pb = j < a.nB ? a.popB[j] : 0u;
                 const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
                 u32 cmin = 0xFFFFFFFFu;
