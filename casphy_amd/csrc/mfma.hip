@@ -35,6 +35,48 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
+// (j, i) with i <= j of the t-th entry of the upper triangle in column order.
+__device__ __forceinline__ void tri_index(u32 t, u32 &i, u32 &j)
+{
+    u32 c = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((u64)c * (c + 1) / 2 > t) --c;
+    while ((u64)(c + 1) * (c + 2) / 2 <= t) ++c;
+    j = c;
+    i = t - (u32)((u64)c * (c + 1) / 2);
+}
+
+// Tile (I, J) of launch slot t.  With a.sup = S > 0 the tiles are numbered
+// by S x S super-tiles (upper triangle of super-tiles for self scoring), so
+// the workgroups one XCD holds at a time -- a contiguous run of slots -- share
+// about 2 S row blocks instead of sweeping whole columns; slots of a super-tile
+// that fall outside the triangle or the matrix hold no tile (false).
+__device__ __forceinline__ bool tile_of(const MfmaArgs &a, u32 t, u32 &I, u32 &J)
+{
+    const u32 S = a.sup;
+    if (S == 0) {
+        if (a.triangle) {
+            tri_index(t, I, J);
+        } else {
+            I = t / a.ntB;
+            J = t - I * a.ntB;
+        }
+        return true;
+    }
+    const u32 S2 = S * S, u = t / S2, l = t - u * S2;
+    u32 SI, SJ;
+    if (a.triangle) {
+        tri_index(u, SI, SJ);
+    } else {
+        const u32 nSB = (a.ntB + S - 1) / S;
+        SI = u / nSB;
+        SJ = u - SI * nSB;
+    }
+    I = SI * S + l / S;
+    J = SJ * S + l % S;
+    return a.triangle ? (I <= J && J < a.ntA) : (I < a.ntA && J < a.ntB);
+}
+
+
 constexpr int NTHR = 256;
 
 __global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 ldw, u32 kw, u32 nrows_pad, u32 *P)
@@ -113,16 +155,7 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -353,16 +386,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_wide(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -448,16 +472,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_wide8(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 2, wn = wave & 3;
 
@@ -579,16 +594,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -689,16 +695,7 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -834,16 +831,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -968,16 +956,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
     u32 I, J;
-    if (a.triangle) {
-        u32 j = (u32)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while ((u64)j * (j + 1) / 2 > t) --j;
-        while ((u64)(j + 1) * (j + 2) / 2 <= t) ++j;
-        J = j;
-        I = t - (u32)((u64)j * (j + 1) / 2);
-    } else {
-        I = t / a.ntB;
-        J = t - I * a.ntB;
-    }
+    if (!tile_of(a, t, I, J)) return;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
     const u32 half = lane >> 5, col = lane & 31;
@@ -1234,6 +1213,13 @@ constexpr int kDefaultVariant = 3;
 constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
 constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11;
+// super-tile edge for 128 x 128 tiles: 8 (8 x 8 tiles = 16 row blocks of 128);
+// CASPHY_MFMA_SUPER overrides, 0 keeps the plain column order
+u32 super_edge()
+{
+    const char *e = std::getenv("CASPHY_MFMA_SUPER");
+    return e ? (u32)std::atoi(e) : 8u;
+}
 
 bool variant_supports(int v, int epi) { return v < kWide || v >= kFp4 || epi == E_MASK || epi == E_SCORES; }
 
@@ -1252,9 +1238,18 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     if (!variant_supports(v, epi)) v = kDefaultVariant;
     const u32 T = v == 6 ? RT : (v == kWide || v == kWide8) ? WT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
+    a.ntA = ntA;
     a.ntB = ceil_div(a.nB, T);
-    a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1) / 2) : ntA * a.ntB;
-    if (a.ntiles == 0) return;
+    if (ntA == 0 || a.ntB == 0) return;
+    // 128 x 128 tiles run in 8 x 8 super-tiles (launch slots without a tile exit)
+    a.sup = T == TILE ? super_edge() : 0;
+    if (a.sup) {
+        const u64 nSA = ceil_div(ntA, a.sup), nSB = ceil_div(a.ntB, a.sup);
+        const u64 nsup = a.triangle ? nSA * (nSA + 1) / 2 : nSA * nSB;
+        a.ntiles = (u32)(nsup * a.sup * a.sup);
+    } else {
+        a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1) / 2) : ntA * a.ntB;
+    }
     switch (v) {
     case 0: launch_variant<128, 1, 2>(epi, a, s); break;
     case 1: launch_variant<128, 2, 2>(epi, a, s); break;

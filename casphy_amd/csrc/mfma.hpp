@@ -19,7 +19,8 @@ struct MfmaArgs {
     const u32 *EA = nullptr;  // optional fp4 expansions of PA / PB (launch_expand_fp4); the
     const u32 *EB = nullptr;  // fp4 variant expands what is not given
     u32 nA = 0, nB = 0;       // valid rows
-    u32 ntB = 0;              // set by the launcher (tile grid depends on the variant)
+    u32 ntA = 0, ntB = 0;     // set by the launcher (tile grid depends on the variant)
+    u32 sup = 0;              // set by the launcher: super-tile edge in tiles (0: plain order)
     u32 ntiles = 0;           // set by the launcher
     int triangle = 1;         // A == B, tiles I <= J
     int32_t thr = 0;          // E_MASK
