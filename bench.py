@@ -101,7 +101,8 @@ VARIANT_KERNEL = {"128x1": "k_mfma_tiles<0, 128, 1, 2>", "128x2": "k_mfma_tiles<
                   "128x4": "k_mfma_tiles<0, 128, 4, 2>", "64x2": "k_mfma_tiles<0, 64, 2, 4>",
                   "64x4": "k_mfma_tiles<0, 64, 4, 4>", "64x8": "k_mfma_tiles<0, 64, 8, 4>",
                   "reg": "k_mfma_reg<0>", "wide": "k_mfma_wide<0>", "wide8": "k_mfma_wide8<0>",
-                  "fp4": "k_mfma_fp4<0>", "fp4b": "k_mfma_fp4b<0>", "fp4d": "k_mfma_fp4d<0>"}
+                  "fp4": "k_mfma_fp4<0>", "fp4b": "k_mfma_fp4b<0>", "fp4d": "k_mfma_fp4d<0>",
+                  "fp4w": "k_mfma_fp4w<0>"}
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_mfma_tiles_mask_10000x4096_pmc.md")
 
 

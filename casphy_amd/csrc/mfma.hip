@@ -903,6 +903,113 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
         }
 }
 
+// fp4 with 256 x 256 pair tiles: 8 waves (2 x 4), each 128 x 64 pairs as
+// 4 x 2 MFMA blocks, stages moved by LDS DMA as in k_mfma_fp4d (same swizzled
+// 64-byte rows, A rows then B rows).  Per pair it reads half the bytes of the
+// 128 x 128 tiles, which is what bounds them (each CU streams its tiles' rows
+// from L2 at the rate its MFMAs consume them).
+constexpr u32 FP4W_STAGE = 2 * WT * 64;  // bytes of one stage buffer
+
+__device__ __forceinline__ void fp4w_dma(unsigned char *buf, const unsigned char *EA, const unsigned char *EB,
+                                         size_t padA, size_t padB, u32 I, u32 J, u32 s, u32 wave, u32 lane)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const u32 q = (wave * 4 + k) * 64 + lane;  // granule index in the stage buffer
+        const bool opB = q >= WT * 4;
+        const u32 q2 = q & (WT * 4 - 1), row = q2 >> 2, g = (q2 & 3) ^ ((row >> 1) & 3);
+        const unsigned char *E = opB ? EB : EA;
+        const size_t pad = opB ? padB : padA;
+        const size_t rr = (size_t)(opB ? J : I) * WT + row;
+        const unsigned char *src = E + ((size_t)(2 * s + (g >> 1)) * pad + rr) * 32 + (g & 1) * 16;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                         (__attribute__((address_space(3))) void *)(buf + (wave * 4 + k) * 1024),
+                                         16, 0, 0);
+    }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(512, 1) void k_mfma_fp4w(MfmaArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char S0[FP4W_STAGE];
+    __shared__ __attribute__((aligned(16))) unsigned char S1[FP4W_STAGE];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (!tile_of(a, t, I, J)) return;
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 2, wn = wave & 3;
+
+    v16f acc[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0.0f;
+
+    const u32 nst = a.stages, last = nst - 1;
+    const unsigned char *EA = reinterpret_cast<const unsigned char *>(a.PA);
+    const unsigned char *EB = reinterpret_cast<const unsigned char *>(a.PB);
+    const size_t padA = a.padA, padB = a.padB;
+    const u32 ra = wm * 128 + (lane & 31), rb = wn * 64 + (lane & 31), h = lane >> 5;
+    auto compute = [&](const unsigned char *buf) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const u32 g = 2 * c + h;
+            v8i fa[4], fb[2];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const u32 r1 = ra + 32 * x;
+                const v4i ta = *reinterpret_cast<const v4i *>(buf + r1 * 64 + ((g ^ ((r1 >> 1) & 3)) * 16));
+                fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const u32 r2 = rb + 32 * y;
+                const v4i tb = *reinterpret_cast<const v4i *>(buf + WT * 64 + r2 * 64 + ((g ^ ((r2 >> 1) & 3)) * 16));
+                fb[y] = v8i{tb.x, tb.y, tb.z, tb.w, 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[x], fb[y], acc[x][y], 4, 4, 0, 127,
+                                                                                0, 127);
+        }
+    };
+    fp4w_dma(S0, EA, EB, padA, padB, I, J, 0, wave, lane);
+    for (u32 s = 0; s < nst; s += 2) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA of stage s has landed
+        __syncthreads();                     // ... and every wave's
+        fp4w_dma(S1, EA, EB, padA, padB, I, J, s + 1, wave, lane);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(S0);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        fp4w_dma(S0, EA, EB, padA, padB, I, J, min(s + 2, last), wave, lane);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(S1);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            v16i ai;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
+            block_epilogue<EPI>(a, ai, I * WT + wm * 128 + 32 * x, J * WT + wn * 64 + 32 * y, lane, mirror, thr);
+        }
+}
+
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1129,7 +1236,8 @@ void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
 
 // Expands the packed operands to fp4 nibbles (stream-ordered scratch freed
 // after the launch) and runs the fp4 tiles.
-// mode 0: LDS staging through registers, 1: B operands from global, 2: LDS DMA
+// mode 0: LDS staging through registers, 1: B operands from global, 2: LDS DMA,
+// 3: LDS DMA with 256 x 256 tiles
 void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
 {
     MfmaArgs a = in;
@@ -1153,7 +1261,18 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
     a.PA = EA;
     a.PB = EB;
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
-    if (mode == 2) {
+    if (mode == 3) {
+        if (epi == E_MASK)
+            hipLaunchKernelGGL(k_mfma_fp4w<E_MASK>, dim3(grid), dim3(512), 0, s, a);
+        else if (epi == E_SCORES)
+            hipLaunchKernelGGL(k_mfma_fp4w<E_SCORES>, dim3(grid), dim3(512), 0, s, a);
+        else if (epi == E_LOCALMAX)
+            hipLaunchKernelGGL(k_mfma_fp4w<E_LOCALMAX>, dim3(grid), dim3(512), 0, s, a);
+        else if (epi == E_TBE)
+            hipLaunchKernelGGL(k_mfma_fp4w<E_TBE>, dim3(grid), dim3(512), 0, s, a);
+        else
+            throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    } else if (mode == 2) {
         if (epi == E_MASK)
             hipLaunchKernelGGL(k_mfma_fp4d<E_MASK>, dim3(grid), dim3(NTHR), 0, s, a);
         else if (epi == E_SCORES)
@@ -1207,12 +1326,12 @@ namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
 const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4",
-                                 "fp4b", "fp4d"};
-constexpr int kNumVariants = 12;
+                                 "fp4b", "fp4d", "fp4w"};
+constexpr int kNumVariants = 13;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11, 12};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
-constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11;
+constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11, kFp4w = 12;
 // super-tile edge for 128 x 128 tiles: 8 (8 x 8 tiles = 16 row blocks of 128);
 // CASPHY_MFMA_SUPER overrides, 0 keeps the plain column order
 u32 super_edge()
@@ -1236,7 +1355,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
     if (!variant_supports(v, epi)) v = kDefaultVariant;
-    const u32 T = v == 6 ? RT : (v == kWide || v == kWide8) ? WT : TILE;
+    const u32 T = v == 6 ? RT : (v == kWide || v == kWide8 || v == kFp4w) ? WT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntA = ntA;
     a.ntB = ceil_div(a.nB, T);
@@ -1263,6 +1382,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case kFp4: launch_fp4(epi, a, s, 0); break;
     case kFp4b: launch_fp4(epi, a, s, 1); break;
     case kFp4d: launch_fp4(epi, a, s, 2); break;
+    case kFp4w: launch_fp4(epi, a, s, 3); break;
     default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());
