@@ -655,6 +655,20 @@ int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, u
 }
 
 // ---------------------------------------------------------------- resident cells
+struct AutosplitKey {
+    u32 R = 0, nradii = 0;
+    u64 smax = 0;
+    int any = -1;
+    bool want = false;
+    u64 cap = 0;
+    const void *idx = nullptr, *mask = nullptr, *S = nullptr, *eq = nullptr, *in = nullptr, *pin = nullptr;
+    bool operator==(const AutosplitKey &o) const
+    {
+        return R == o.R && nradii == o.nradii && smax == o.smax && any == o.any && want == o.want && cap == o.cap &&
+               idx == o.idx && mask == o.mask && S == o.S && eq == o.eq && in == o.in && pin == o.pin;
+    }
+};
+
 struct casphy_cells {
     int kind = CASPHY_KIND_BIT;
     int metric = M_AND;
@@ -680,7 +694,16 @@ struct casphy_cells {
     // autoSplit round: uploads (radii | query rows), query stage copy, scores,
     // counts and the choice; pinned host staging for one upload and
     // the read-back of (choice, total)
-    DevBuf a_in, a_sq, a_S, a_counts, a_misc, a_pbuf;
+    DevBuf a_in, a_sq, a_eq, a_S, a_counts, a_misc, a_pbuf;
+    // captured autoSplit round and the shapes / buffers it was captured with
+    hipGraphExec_t ag_exec = nullptr;
+    AutosplitKey ag_key{};
+    bool ag_seen = false;
+    void drop_graph()
+    {
+        if (ag_exec) (void)hipGraphExecDestroy(ag_exec);
+        ag_exec = nullptr;
+    }
     void *pin = nullptr;
     size_t pin_bytes = 0;
     void pinned(size_t b)
@@ -728,6 +751,7 @@ void casphy_cells_destroy(casphy_cells *h)
     if (!h) return;
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    h->drop_graph();
     if (h->pin) (void)hipHostFree(h->pin);
     delete h;
 }
@@ -790,69 +814,118 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         hipStream_t s = thread_stream();
         const bool mfma = h->mfma();
         if (mfma) h->staged(s);
-        CASPHY_HIP(hipEventRecord(h->ev[2], s));
 
-        // one upload: radii (padded to 16) | query rows; results land after them
+        // every buffer the round touches is sized before anything is enqueued, so
+        // the enqueued work allocates nothing and can be captured into a graph
         const size_t up = (MAX_RADII + (size_t)R) * 4, res_off = (up + 15) / 16 * 16;
-        h->pinned(res_off + 16);
+        const size_t cnt_bytes = (size_t)R * nradii * 4;
+        h->pinned(res_off + 16 + cnt_bytes);
+        h->a_in.reserve(up);
+        h->a_S.reserve((size_t)R * n * 4);
+        h->a_counts.reserve(cnt_bytes);
+        h->a_misc.reserve(16);
+        if (mfma) {
+            h->a_sq.reserve((size_t)stage_rows(R) * ldw * 4);
+            h->a_eq.reserve(fp4_bytes(stage_rows(R), ldw / 4));
+        }
+        RadiusOut &o = h->out;
+        const u32 nt = ceil_div(n, TILE), mstride = nt * 4;
+        o.mask.reserve((size_t)nt * TILE * mstride * 4);
+        o.counts.reserve((size_t)n * 4);
+        o.offsets.reserve((size_t)(n + 1) * 8);
+        const u64 cap = o.idx.bytes / 4;
+
         int32_t *pr = static_cast<int32_t *>(h->pin);
         for (u32 k = 0; k < MAX_RADII; ++k) pr[k] = k < nradii ? radii[k] : 0x7FFFFFFF;
         std::memcpy(pr + MAX_RADII, q, (size_t)R * 4);
-        h->a_in.reserve(up);
-        CASPHY_HIP(hipMemcpyAsync(h->a_in.p, h->pin, up, hipMemcpyHostToDevice, s));
-        const int32_t *dr = h->a_in.as<int32_t>();
-        const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
-        h->a_S.reserve((size_t)R * n * 4);
-        h->a_counts.reserve((size_t)R * nradii * 4);
-        h->a_misc.reserve(16);
-        int32_t *choice = h->a_misc.as<int32_t>();
-
-        // tryRadius counts of the query rows, per-radius sums, device-side choice
-        int32_t *dS = h->a_S.as<int32_t>();
-        if (mfma) {
-            h->a_sq.reserve((size_t)stage_rows(R) * ldw * 4);
-            launch_to_stage_major(h->mat.as<u32>(), R, ldw, ldw, h->a_sq.as<u32>(), s, dq);
-            MfmaArgs m;
-            m.PA = h->a_sq.as<u32>(); m.PB = h->stage.as<u32>(); m.padA = stage_rows(R); m.padB = stage_rows(n);
-            m.stages = ldw / 4; m.nA = R; m.nB = n; m.triangle = 0;
-            m.EB = h->estage.as<u32>();
-            m.S = dS; m.ldS = n;
-            launch_mfma_tiles(E_SCORES, m, s);
-        } else {
-            TileArgs a;
-            a.A = h->mat.as<u32>(); a.Aidx = dq; a.nA = R; a.B = h->mat.as<u32>(); a.nB = n;
-            a.ldw = ldw; a.kw = ldw;
-            a.ntA = ceil_div(R, TILE); a.ntB = ceil_div(n, TILE);
-            a.ntiles = a.ntA * a.ntB;
-            a.S = dS; a.ldS = n;
-            apply_planes(h->planes, a, h->a_pbuf, s);
-            launch_tiles(h->metric, E_SCORES, a, s);
-        }
-        launch_count_radii(dS, R, n, n, dr, nradii, h->a_counts.as<u32>(), s);
-        launch_autosplit_choose(h->a_counts.as<u32>(), R, dr, nradii, smax, any, choice, s);
-
-        // radiusNeighbor lists at the chosen radius (nothing qualifies: threshold
-        // INT32_MAX, empty lists); ids beyond the current capacity are dropped and
-        // the compaction reruns once the total is known
-        RadiusOut &o = h->out;
-        h->timed[0] = h->timed[1] = true;
-        radius_mask(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
-                    mfma ? h->stage.as<u32>() : nullptr, mfma ? h->estage.as<u32>() : nullptr);
-        const u32 mstride = ceil_div(n, TILE) * 4;
-        const u64 cap = o.idx.bytes / 4;
-        if (cap) launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, cap);
-        CASPHY_HIP(hipEventRecord(h->ev[3], s));
         char *res = static_cast<char *>(h->pin) + res_off;
-        CASPHY_HIP(hipMemcpyAsync(res, choice, 8, hipMemcpyDeviceToHost, s));
-        CASPHY_HIP(hipMemcpyAsync(res + 8, o.offsets.as<u64>() + n, 8, hipMemcpyDeviceToHost, s));
-        if (counts) CASPHY_HIP(hipMemcpyAsync(counts, h->a_counts.p, (size_t)R * nradii * 4, hipMemcpyDeviceToHost, s));
+        const bool want = counts != nullptr;
+
+        auto enqueue = [&] {
+            CASPHY_HIP(hipEventRecord(h->ev[2], s));
+            CASPHY_HIP(hipMemcpyAsync(h->a_in.p, h->pin, up, hipMemcpyHostToDevice, s));
+            const int32_t *dr = h->a_in.as<int32_t>();
+            const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
+            int32_t *choice = h->a_misc.as<int32_t>();
+            // tryRadius counts of the query rows and the device-side choice
+            int32_t *dS = h->a_S.as<int32_t>();
+            if (mfma) {
+                launch_to_stage_major(h->mat.as<u32>(), R, ldw, ldw, h->a_sq.as<u32>(), s, dq);
+                launch_expand_fp4(h->a_sq.as<u32>(), stage_rows(R), ldw / 4, h->a_eq.as<u32>(), s);
+                MfmaArgs m;
+                m.PA = h->a_sq.as<u32>(); m.PB = h->stage.as<u32>(); m.padA = stage_rows(R); m.padB = stage_rows(n);
+                m.stages = ldw / 4; m.nA = R; m.nB = n; m.triangle = 0;
+                m.EA = h->a_eq.as<u32>(); m.EB = h->estage.as<u32>();
+                m.S = dS; m.ldS = n;
+                launch_mfma_tiles(E_SCORES, m, s);
+            } else {
+                TileArgs a;
+                a.A = h->mat.as<u32>(); a.Aidx = dq; a.nA = R; a.B = h->mat.as<u32>(); a.nB = n;
+                a.ldw = ldw; a.kw = ldw;
+                a.ntA = ceil_div(R, TILE); a.ntB = ceil_div(n, TILE);
+                a.ntiles = a.ntA * a.ntB;
+                a.S = dS; a.ldS = n;
+                apply_planes(h->planes, a, h->a_pbuf, s);
+                launch_tiles(h->metric, E_SCORES, a, s);
+            }
+            launch_count_radii(dS, R, n, n, dr, nradii, h->a_counts.as<u32>(), s);
+            launch_autosplit_choose(h->a_counts.as<u32>(), R, dr, nradii, smax, any, choice, s);
+            // radiusNeighbor lists at the chosen radius (nothing qualifies: threshold
+            // INT32_MAX, empty lists); ids beyond the current capacity are dropped
+            // and the compaction reruns once the total is known
+            radius_mask(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
+                        mfma ? h->stage.as<u32>() : nullptr, mfma ? h->estage.as<u32>() : nullptr);
+            if (cap) launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, cap);
+            CASPHY_HIP(hipEventRecord(h->ev[3], s));
+            CASPHY_HIP(hipMemcpyAsync(res, choice, 8, hipMemcpyDeviceToHost, s));
+            CASPHY_HIP(hipMemcpyAsync(res + 8, o.offsets.as<u64>() + n, 8, hipMemcpyDeviceToHost, s));
+            if (want) CASPHY_HIP(hipMemcpyAsync(res + 16, h->a_counts.p, cnt_bytes, hipMemcpyDeviceToHost, s));
+        };
+
+        // The round is a fixed chain of launches: the second call with the same
+        // shapes and buffers captures it into a graph (the first one also runs
+        // the autotuner), later calls replay the graph.  CASPHY_AUTOSPLIT_GRAPH=0
+        // keeps plain launches.
+        const AutosplitKey key{R, nradii, smax, any, want, cap, o.idx.p, o.mask.p, h->a_S.p, h->a_eq.p, h->a_in.p,
+                               h->pin};
+        const char *genv = std::getenv("CASPHY_AUTOSPLIT_GRAPH");
+        const bool graphs = !(genv && genv[0] == '0');
+        h->timed[0] = h->timed[1] = true;
+        if (graphs && h->ag_exec && h->ag_key == key) {
+            CASPHY_HIP(hipGraphLaunch(h->ag_exec, s));
+        } else {
+            h->drop_graph();
+            if (graphs && h->ag_seen && h->ag_key == key) {
+                hipGraph_t g = nullptr;
+                CASPHY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue();
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                CASPHY_HIP(hipStreamEndCapture(s, &g));
+                const hipError_t e = hipGraphInstantiate(&h->ag_exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                CASPHY_HIP(e);
+                CASPHY_HIP(hipGraphLaunch(h->ag_exec, s));
+            } else {
+                enqueue();
+            }
+            h->ag_key = key;
+            h->ag_seen = true;
+        }
         CASPHY_HIP(hipStreamSynchronize(s));
         int32_t ch[2];
         u64 total;
         std::memcpy(ch, res, 8);
         std::memcpy(&total, res + 8, 8);
+        if (want) std::memcpy(counts, res + 16, cnt_bytes);
         if (total > cap) {
-            o.idx.reserve(total * 4);
+            o.idx.reserve(total * 4);  // the captured round wrote into the old buffer: recapture next time
+            h->drop_graph();
+            h->ag_seen = false;
             launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, total);
             CASPHY_HIP(hipEventRecord(h->ev[3], s));
             CASPHY_HIP(hipStreamSynchronize(s));

@@ -1422,15 +1422,17 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
     const int forced = variant_from_env();
     if (forced >= 0) return launch_one(forced, epi, a, s);
     const char *at = std::getenv("CASPHY_AUTOTUNE");
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    CASPHY_HIP(hipStreamIsCapturing(s, &cap));
-    if ((at && at[0] == '0') || cap != hipStreamCaptureStatusNone) return launch_one(kDefaultVariant, epi, a, s);
+    if (at && at[0] == '0') return launch_one(kDefaultVariant, epi, a, s);
     const TuneKey key{epi, a.triangle, log2_ceil(a.nA), log2_ceil(a.nB), a.stages};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) return launch_one(it->second.variant, epi, a, s);
     }
+    // under stream capture nothing can be timed: the default variant
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    CASPHY_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return launch_one(kDefaultVariant, epi, a, s);
     TuneEntry e{};
     for (float &m : e.ms) m = -1.0f;
     hipEvent_t ev[2];

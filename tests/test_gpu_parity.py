@@ -176,6 +176,14 @@ def test_cells_autosplit_bit(hip, orc, bitpath):
         assert total == idx.size and np.array_equal(counts, c2) and np.array_equal(idx, i2)
         assert cells.autosplit(radii, q, bound) == (r, total)
     assert len(seen) >= 2
+    # repeated identical rounds: the second is captured into a graph, later ones replay it
+    want = autosplit_rule(ref, radii, 1.5)
+    c2, i2 = orc.radius_bit(mat, want)
+    for rep in range(4):
+        r, total, cnt = cells.autosplit(radii, q, 1.5, counts=True)
+        counts, idx = cells.fetch()
+        assert r == want and np.array_equal(cnt, ref), rep
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2), rep
 
 
 def test_cells_autosplit_dna(hip, orc):
