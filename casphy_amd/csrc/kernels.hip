@@ -2,6 +2,8 @@
 //
 // All of these are HBM-streaming kernels: coalesced 4/16-byte accesses, one
 // wave per row where a row is scanned, no atomics.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace casphy {
@@ -260,6 +262,26 @@ __global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u3
     out[1] = kk;
 }
 
+// Copies between device memory and mapped pinned host memory inside a kernel,
+// so a round trip needs no copy engine and no cross-engine wait.
+__global__ void k_copy_words(const u32 *src, u32 *dst, u32 n)
+{
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+__global__ void k_publish(const int32_t *choice, const u64 *total, const u32 *counts, u32 ncounts, u32 *out)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[0] = (u32)choice[0];
+        out[1] = (u32)choice[1];
+        const u64 t = *total;
+        out[2] = (u32)t;
+        out[3] = (u32)(t >> 32);
+    }
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < ncounts; i += gridDim.x * blockDim.x)
+        out[4 + i] = counts[i];
+}
+
 __global__ void k_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size)
 {
     const u32 lane = threadIdx.x & 63;
@@ -346,6 +368,20 @@ void launch_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii, u32
                              int32_t *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_autosplit_choose, dim3(1), dim3(1024), 0, s, counts, R, radii, nradii, smax, any, out);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_copy_words(const u32 *src, u32 *dst, u32 n, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy_words, dim3(std::min<u32>(ceil_div(n, 256), 64)), dim3(256), 0, s, src, dst, n);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_publish(const int32_t *choice, const u64 *total, const u32 *counts, u32 ncounts, u32 *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_publish, dim3(std::max<u32>(1, std::min<u32>(ceil_div(ncounts, 256), 64))), dim3(256), 0, s,
+                       choice, total, counts, ncounts, out);
     CASPHY_HIP(hipGetLastError());
 }
 

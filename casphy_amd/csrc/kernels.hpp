@@ -49,6 +49,12 @@ void launch_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q, co
                      u32 *counts, u32 R, int32_t *S, u32 *part, hipStream_t s, hipEvent_t ev0 = nullptr,
                      hipEvent_t ev1 = nullptr);
 
+// Word copies in a kernel (either side may be mapped pinned host memory).
+void launch_copy_words(const u32 *src, u32 *dst, u32 n, hipStream_t s);
+// out[0..1] = choice[0..1], out[2..3] = *total (low, high), out[4..] = counts.
+void launch_publish(const int32_t *choice, const u64 *total, const u32 *counts, u32 ncounts, u32 *out,
+                    hipStream_t s);
+
 // size[r] = min(p, ntip - p) with p = popcount of row r (u32 arithmetic).
 void launch_row_size(const u32 *mat, u32 n, u32 ldw, u32 ntip, u32 *size, hipStream_t s);
 
