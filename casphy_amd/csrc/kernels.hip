@@ -96,43 +96,86 @@ __global__ void k_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 l
     }
 }
 
-// One wave per row.
+// One wave per row; rows starting on a 16-byte boundary are read as 16-byte
+// words, four loads in flight per lane.
 __global__ void k_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts)
 {
     const u32 lane = threadIdx.x & 63;
     const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (row >= n) return;
     const u32 *m = mask + (size_t)row * mstride;
-    u32 c = 0;
-    for (u32 w = lane; w < words; w += 64) c += (u32)__builtin_popcount(m[w]);
+    u32 c = 0, tail = 0;
+    if ((reinterpret_cast<uintptr_t>(m) & 15) == 0) {
+        const uint4 *m4 = reinterpret_cast<const uint4 *>(m);
+        const u32 full = words / 4;
+        for (u32 w0 = 0; w0 < full; w0 += 256) {
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const u32 w = w0 + 64 * u + lane;
+                v[u] = w < full ? m4[w] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                c += (u32)(__builtin_popcount(v[u].x) + __builtin_popcount(v[u].y) + __builtin_popcount(v[u].z) +
+                           __builtin_popcount(v[u].w));
+        }
+        tail = full * 4;
+    }
+    for (u32 w = tail + lane; w < words; w += 64) c += (u32)__builtin_popcount(m[w]);
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
     if (lane == 0) counts[row] = c;
 }
 
-// Single workgroup of 1024 threads: each thread sums a contiguous chunk, the
-// chunk sums are scanned in LDS, then each thread writes its chunk's offsets.
+// Single workgroup of 1024 threads over blocks of 4,096 counts: thread t
+// loads counts 4t .. 4t+3 of the block, wave scans (shuffles) and a scan of the
+// 16 wave totals give each thread its exclusive offset, plus the running total
+// of the earlier blocks.
 __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, u64 *offsets)
 {
-    __shared__ u64 part[1024];
-    const u32 t = threadIdx.x;
-    const u32 chunk = (n + 1023) / 1024;
-    const u32 b = t * chunk, e = min(n, b + chunk);
-    u64 s = 0;
-    for (u32 i = b; i < e; ++i) s += counts[i];
-    part[t] = s;
-    __syncthreads();
-    for (u32 o = 1; o < 1024; o <<= 1) {
-        u64 v = t >= o ? part[t - o] : 0;
+    __shared__ u64 wsum[16];
+    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    u64 carry = 0;
+    for (u32 base = 0; base < n; base += 4096) {
+        const u32 i0 = base + 4 * t;
+        u32 v[4];
+        if (i0 + 3 < n && (reinterpret_cast<uintptr_t>(counts + i0) & 15) == 0) {
+            const uint4 x = *reinterpret_cast<const uint4 *>(counts + i0);
+            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = i0 + u < n ? counts[i0 + u] : 0u;
+        }
+        const u64 mine = (u64)v[0] + v[1] + v[2] + v[3];
+        u64 x = mine;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const u64 y = __shfl_up(x, o);
+            if (lane >= (u32)o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
         __syncthreads();
-        part[t] += v;
+        if (wv == 0) {
+            u64 w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const u64 y = __shfl_up(w, o);
+                if (lane >= (u32)o) w += y;
+            }
+            if (lane < 16) wsum[lane] = w;
+        }
         __syncthreads();
+        u64 run = carry + (wv ? wsum[wv - 1] : 0) + x - mine;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < n) {
+                offsets[i0 + u] = run;
+                run += v[u];
+            }
+        carry += wsum[15];
+        __syncthreads();  // wsum is rewritten by the next block
     }
-    u64 run = part[t] - s;  // exclusive prefix of this chunk
-    for (u32 i = b; i < e; ++i) {
-        offsets[i] = run;
-        run += counts[i];
-    }
-    if (t == 1023) offsets[n] = part[1023];
+    if (t == 0) offsets[n] = carry;
 }
 
 // One wave per row.  Per chunk of 64 mask words: a chunk of 64 full words is
@@ -205,35 +248,143 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
     }
 }
 
-// tryRadius counts from a query x cell score matrix: one workgroup per query,
-// per-thread counters for every radius, then a workgroup reduction.
+// tryRadius counts from a query x cell score matrix: one workgroup per query.
+//   * radii in 0 .. COUNT_LUT - 2 (the usual case): a table in LDS maps a score
+//     x to the one-hot byte field idx(x) - 1, idx(x) = #{k : radii[k] <= x}, so
+//     a score costs a clamp, one LDS read and four adds into four words of
+//     byte counters (flushed every 255 scores); radius k then counts the
+//     scores with idx >= #{j : radii[j] <= radii[k]};
+//   * other radii: one compare per radius and score.
+// Rows starting on a 16-byte boundary are read as 16-byte words, four loads in
+// flight per lane.  The per-thread counters are summed through LDS.
+constexpr u32 COUNT_LUT = 1024;  // table entries: scores -1 .. COUNT_LUT - 2
+
 __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii,
                                                      u32 nradii, u32 *counts)
 {
-    __shared__ u32 red[16][MAX_RADII_K];
+    __shared__ uint4 lds4[1088];  // the table, then 256 x 17 u32 for the reduction
+    __shared__ u32 tot[MAX_RADII_K];
     const u32 q = blockIdx.x, tid = threadIdx.x;
     if (q >= R) return;
     int32_t rr[MAX_RADII_K];
-    u32 c[MAX_RADII_K];
+    int32_t rmin = 0x7FFFFFFF, rmax = (int32_t)0x80000000;
 #pragma unroll
     for (int k = 0; k < MAX_RADII_K; ++k) {
         rr[k] = k < (int)nradii ? radii[k] : 0x7FFFFFFF;
-        c[k] = 0;
+        if (k < (int)nradii) {
+            rmin = min(rmin, rr[k]);
+            rmax = max(rmax, rr[k]);
+        }
     }
+    const bool table = rmin >= 0 && rmax <= (int32_t)COUNT_LUT - 2;
     const int32_t *row = S + (size_t)q * ldS;
-    for (u32 j = tid; j < n; j += 256) {
-        const int32_t v = row[j];
+    const bool vec = (reinterpret_cast<uintptr_t>(row) & 15) == 0;
+    const u32 n4 = vec ? n / 4 : 0;
+    u32 c[MAX_RADII_K];
 #pragma unroll
-        for (int k = 0; k < MAX_RADII_K; ++k) c[k] += v >= rr[k];
-    }
+    for (int k = 0; k < MAX_RADII_K; ++k) c[k] = 0;
+    if (table) {
+        // entry e holds score e - 1 (entry 0: every negative score, idx 0)
+        for (u32 e = tid; e < (u32)rmax + 2; e += 256) {
+            const int32_t x = (int32_t)e - 1;
+            u32 idx = 0;
 #pragma unroll
-    for (int k = 0; k < MAX_RADII_K; ++k) {
-        u32 x = c[k];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-        if ((tid & 63) == 0) red[tid >> 6][k] = x;
+            for (int k = 0; k < MAX_RADII_K; ++k) idx += rr[k] <= x;
+            u32 w[4] = {0, 0, 0, 0};
+            if (idx) {
+                const u32 f = idx - 1;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) w[j] = (f >> 2) == (u32)j ? 1u << (8 * (f & 3)) : 0u;
+            }
+            lds4[e] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        __syncthreads();
+        u32 h0 = 0, h1 = 0, h2 = 0, h3 = 0, pend = 0;
+        auto flush = [&] {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                c[b] += (h0 >> (8 * b)) & 0xFF;
+                c[4 + b] += (h1 >> (8 * b)) & 0xFF;
+                c[8 + b] += (h2 >> (8 * b)) & 0xFF;
+                c[12 + b] += (h3 >> (8 * b)) & 0xFF;
+            }
+            h0 = h1 = h2 = h3 = 0;
+            pend = 0;
+        };
+        auto add = [&](int32_t v) {
+            const uint4 t = lds4[min(max(v, -1), rmax) + 1];
+            h0 += t.x;
+            h1 += t.y;
+            h2 += t.z;
+            h3 += t.w;
+        };
+        const int4 *r4 = reinterpret_cast<const int4 *>(row);
+        for (u32 j0 = 0; j0 < n4; j0 += 1024) {
+            int4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const u32 j = j0 + 256 * u + tid;
+                v[u] = j < n4 ? r4[j] : make_int4(-1, -1, -1, -1);  // -1: table entry 0, no field
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                add(v[u].x);
+                add(v[u].y);
+                add(v[u].z);
+                add(v[u].w);
+            }
+            pend += 16;
+            if (pend > 255 - 16) flush();
+        }
+        for (u32 j = 4 * n4 + tid; j < n; j += 256) {
+            add(row[j]);
+            if (++pend == 255) flush();
+        }
+        flush();
+    } else {
+        auto cmp = [&](int32_t v) {
+#pragma unroll
+            for (int k = 0; k < MAX_RADII_K; ++k) c[k] += v >= rr[k];
+        };
+        const int4 *r4 = reinterpret_cast<const int4 *>(row);
+        for (u32 j = tid; j < n4; j += 256) {
+            const int4 v = r4[j];
+            cmp(v.x);
+            cmp(v.y);
+            cmp(v.z);
+            cmp(v.w);
+        }
+        for (u32 j = 4 * n4 + tid; j < n; j += 256) cmp(row[j]);
     }
+    // sum the 16 counters of the 256 threads: transpose through LDS (17-word
+    // rows), 16 threads per counter add 16 rows each, then a 16-lane reduction
     __syncthreads();
-    if (tid < nradii) counts[(size_t)tid * R + q] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    u32 *tr = reinterpret_cast<u32 *>(lds4);
+#pragma unroll
+    for (int k = 0; k < MAX_RADII_K; ++k) tr[tid * 17 + k] = c[k];
+    __syncthreads();
+    const u32 f = tid >> 4, part = tid & 15;
+    u32 x = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x += tr[(part * 16 + i) * 17 + f];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (part == 0) tot[f] = x;
+    __syncthreads();
+    if (tid < nradii) {
+        u32 v;
+        if (table) {
+            const int32_t mine = radii[tid];
+            u32 need = 0;  // idx of a score equal to this radius
+#pragma unroll
+            for (int k = 0; k < MAX_RADII_K; ++k) need += rr[k] <= mine;
+            v = 0;
+            for (u32 g = need - 1; g < MAX_RADII_K; ++g) v += tot[g];
+        } else {
+            v = tot[tid];
+        }
+        counts[(size_t)tid * R + q] = v;
+    }
 }
 
 // One workgroup of 16 waves: wave k sums the R counts of radius k, then the
@@ -244,8 +395,19 @@ __global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u3
     __shared__ u64 sums[MAX_RADII_K];
     const u32 lane = threadIdx.x & 63, k = threadIdx.x >> 6;
     u64 c = 0;
-    if (k < nradii)
-        for (u32 q = lane; q < R; q += 64) c += counts[(size_t)k * R + q];
+    if (k < nradii) {
+        const u32 *row = counts + (size_t)k * R;
+        for (u32 q0 = 0; q0 < R; q0 += 512) {  // eight loads in flight per lane
+            u32 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const u32 q = q0 + 64 * u + lane;
+                v[u] = q < R ? row[q] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c += v[u];
+        }
+    }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
     if (lane == 0) sums[k] = c;
     __syncthreads();

@@ -88,7 +88,7 @@ def test_score_indel_matrix(hip, orc):
 
 # ------------------------------------------------------------------ radius
 @pytest.mark.parametrize("n,W,p,r", [(1, 1, 0.3, 1), (33, 1, 0.2, 2), (250, 4, 0.05, 3), (700, 8, 0.03, 5),
-                                     (1000, 2, 0.1, 0), (257, 3, 0.1, 1000)])
+                                     (1000, 2, 0.1, 0), (257, 3, 0.1, 1000), (4999, 2, 0.05, 2)])
 def test_radius_bit(hip, orc, bitpath, n, W, p, r):
     rng = np.random.default_rng(n + W)
     mat = rand_bits(rng, n, W, p)
@@ -143,6 +143,20 @@ def test_try_radius(hip, orc, bitpath):
     for k, r in enumerate(radii):
         assert np.array_equal(got[k], orc.try_radius_bit(mat, r, q)), r
     assert np.array_equal(hip.try_radius(bmat, 3, q), orc.try_radius_bit(mat, 3, q))
+
+
+@pytest.mark.parametrize("n", [901, 1024])
+def test_try_radii_radius_sets(hip, orc, n):
+    # repeated radii, radius 0, the largest radius of the score table (1022),
+    # radii beyond it and negative radii (per-radius compares); 901 cells leave
+    # score rows off the 16-byte grid
+    mat = synth.pack_bits(np.random.default_rng(n).random((n, 3000)) < 0.35)  # pair scores ~367, self ~1050
+    bmat = np.ascontiguousarray(mat.T)
+    q = synth.query_rows(6, n, 70)
+    for radii in ([0, 5, 360, 360, 380, 400], [400, 1022, 1022], [5, 380, 1023], [-1, 2, 360], list(range(340, 420, 5))):
+        got = hip.try_radii(bmat, np.array(radii), q)
+        for k, r in enumerate(radii):
+            assert np.array_equal(got[k], orc.try_radius_bit(mat, r, q)), (radii, r)
 
 
 def autosplit_rule(counts, radii, bound):
