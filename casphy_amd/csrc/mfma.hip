@@ -13,8 +13,10 @@
 //     tile as 2 x 2 MFMA 32x32 blocks (64 int32 accumulators per lane);
 //   * fuses the same epilogues as the VALU tiles: the radius bitmask words are
 //     assembled with one 64-lane ballot per accumulator register (the two
-//     halves of the wave hold rows m and m+4 of one 32-column word), and the
-//     mirrored words from the per-lane column bits and one cross-half shuffle.
+//     halves of the wave hold rows m and m+4 of one 32-column word), gathered
+//     into 32 lanes with writelane and written by one store per block; the
+//     mirrored words come from the per-lane column bits and one cross-half
+//     shuffle.
 // Accumulator layout of a 32x32 MFMA block: lane l, register v holds row
 // 8*(v/4) + 4*(l/32) + v%4 and column l%32.  The A and B operands use the same
 // (lane, byte) -> k mapping, so the product does not depend on how the
@@ -297,6 +299,25 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
 // packed bits are prefetched into registers while the current one is used.
 constexpr u32 WT = 256;  // workgroup tile
 
+// Radius-mask compares of accumulator registers V..15 of a 32 x 32 block.
+// Register V holds row t = 8 (V / 4) + V % 4 in the low half of the wave and
+// row t + 4 in the high half, one column per lane: the ballot of the compare
+// is the mask word of both rows, which v_writelane_b32 (lane index an
+// immediate, no memory access) puts into lanes t and t + 4 of rowword.
+// colbits collects the lane's own hits (bit t) for the mirrored words.
+template <int V>
+__device__ __forceinline__ void mask_rows(const v16i &acc, int32_t thr, unsigned long long jmask, u32 &rowword,
+                                          u32 &colbits)
+{
+    constexpr int t = 8 * (V / 4) + (V % 4);
+    const bool hit = acc[V] >= thr;
+    const unsigned long long b = __ballot(hit) & jmask;
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(rowword) : "s"((u32)b), "i"(t));
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(rowword) : "s"((u32)(b >> 32)), "i"(t + 4));
+    colbits |= hit ? 1u << t : 0u;
+    if constexpr (V + 1 < 16) mask_rows<V + 1>(acc, thr, jmask, rowword, colbits);
+}
+
 // Epilogue of one 32 x 32 block whose rows start at r0 and columns at c0
 // (mirror: the tile is off the diagonal of a triangle, so the transposed
 // pairs are written too).  Row blocks of the transfer-distance batches are
@@ -307,22 +328,18 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
 {
     const u32 half = lane >> 5, col = lane & 31, j = c0 + col;
     if (EPI == E_MASK) {
-        u32 colbits = 0;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
-            const bool hit = acc[v] >= thr;
-            const unsigned long long b = __ballot(hit && j < a.nB);
-            // low half: row 8*(v/4)+v%4, high half: that row + 4
-            if ((lane == (u32)v || lane == (u32)(v + 32)) && c0 / 32 < a.mstride) {
-                const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
-                const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
-                const u32 i = r0 + m_out;
-                if (i < a.nA) a.mask[(size_t)i * a.mstride + c0 / 32] = word;
-            }
-            if (hit && r0 + m < a.nA) colbits |= 1u << m;
-        }
+        // the 32 row words gather in lanes 0..31 and leave in one store; the
+        // mirrored words (one per column) join each lane's 16 compares with the
+        // other half's 16
+        const unsigned long long jmask = __ballot(j < a.nB);
+        u32 rowword = 0, colbits = 0;
+        mask_rows<0>(acc, thr, jmask, rowword, colbits);
+        const u32 wc = c0 / 32;
+        if (lane < 32 && r0 + lane < a.nA && wc < a.mstride) a.mask[(size_t)(r0 + lane) * a.mstride + wc] = rowword;
         if (mirror) {
+            colbits <<= 4 * half;
+            const u32 rows = r0 < a.nA ? a.nA - r0 : 0u;  // rows past nA score the zero padding
+            colbits &= rows >= 32 ? 0xFFFFFFFFu : (1u << rows) - 1u;
             colbits |= __shfl_xor(colbits, 32);
             if (half == 0 && j < a.nB && r0 / 32 < a.mstride) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
         }
