@@ -344,12 +344,34 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
             if (half == 0 && j < a.nB && r0 / 32 < a.mstride) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
         }
     } else if (EPI == E_SCORES) {
+        if (r0 + 32 <= a.nA && c0 + 32 <= a.nB) {
+            // whole block in range: no per-element tests, row offsets fixed per register
+            const size_t ld = a.ldS;
+            int32_t *p = a.S + (size_t)(r0 + 4 * half) * ld + j;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-            if (i < a.nA && j < a.nB) {
-                a.S[(size_t)i * a.ldS + j] = acc[v];
-                if (mirror) a.S[(size_t)j * a.ldS + i] = acc[v];
+            for (int v = 0; v < 16; ++v) p[(size_t)(8 * (v / 4) + (v % 4)) * ld] = acc[v];
+            if (mirror) {
+                // registers 4g .. 4g+3 are 4 consecutive rows: one 16-byte store
+                // per group into row j when the rows of S are 16-byte aligned
+                int32_t *pm = a.S + (size_t)j * ld + r0 + 4 * half;
+                if (((ld & 3) | (reinterpret_cast<uintptr_t>(a.S) & 15)) == 0) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+                        *reinterpret_cast<v4i *>(pm + 8 * g) = v4i{acc[4 * g], acc[4 * g + 1], acc[4 * g + 2],
+                                                                    acc[4 * g + 3]};
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) pm[8 * (v / 4) + (v % 4)] = acc[v];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+                if (i < a.nA && j < a.nB) {
+                    a.S[(size_t)i * a.ldS + j] = acc[v];
+                    if (mirror) a.S[(size_t)j * a.ldS + i] = acc[v];
+                }
             }
         }
     } else if (EPI == E_LOCALMAX) {
