@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 N, NSITE, R = 10000, 4096, 1024
 RADII = np.arange(1, 17, dtype=np.int32)
+CHECK_REPLAYS = 50  # repeated device rounds checked against the first before timing
 AUTOSPLIT_BOUND = 9.5  # nNeighborBound default of the autoSplit loop, R/CasPhy_functions.R:1967
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
 MFMA_I8_PEAK_TOPS = 5000.0                # MI355X dense int8 matrix-core peak, 5.0 POP/s
@@ -194,28 +195,31 @@ def extras(cp, synth, cells, q) -> dict:
     parent, leaves = synth.yule_parent(ntip, rng)
     ref = synth.clade_bitsets(parent, leaves, ntip)
     moves = int(np.ceil(0.05 * (ntip - 3)))
-    boots = [synth.clade_bitsets(synth.spr(parent, leaves, moves, rng), leaves, ntip) for _ in range(2)]
-    cp.transfer_dis_batch(ref, boots, ntip)  # autotuner's first call of the shape class
+    allb, counts = replicate_block(synth, parent, leaves, ntip, moves, 4, rng)
+    boots = np.split(allb, np.cumsum(counts)[:-1])  # views of the one block
+    cp.transfer_dis_batch(ref, allb, ntip, counts)  # autotuner's first call of the shape class
     cp.transfer_dis(ref, boots[0], ntip)
     plain_s, pl = timed(lambda: [cp.transfer_dis(ref, b, ntip) for b in boots], 1)
-    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, boots, ntip), 1)
+    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, allb, ntip, counts), 1)
     for k in range(len(boots)):
         assert np.array_equal(bt[0][k], pl[k][0]) and np.array_equal(bt[2][k], pl[k][2])
-    pairs = ref.shape[0] * sum(b.shape[0] for b in boots)
+    pairs = ref.shape[0] * int(sum(counts))
     ex["tbe_65536_tips"] = {"replicates": len(boots), "plain_jobs_per_s": len(boots) / plain_s,
-                            "batched_jobs_per_s": len(boots) / batch_s, "bipartition_pairs_per_s": pairs / batch_s,
+                            "batched_jobs_per_s": len(boots) / batch_s, "speedup": plain_s / batch_s,
+                            "bipartition_pairs_per_s": pairs / batch_s,
                             "mfma_tops": 2 * ntip * pairs / batch_s / 1e12}
-    del boots
+    del boots, allb
     # small bootstrap trees, where one replicate does not fill the GPU: 4,096 tips x 64 replicates
     ntip = 4096
     parent, leaves = synth.yule_parent(ntip, rng)
     ref = synth.clade_bitsets(parent, leaves, ntip)
     moves = int(np.ceil(0.05 * (ntip - 3)))
-    boots = [synth.clade_bitsets(synth.spr(parent, leaves, moves, rng), leaves, ntip) for _ in range(64)]
-    cp.transfer_dis_batch(ref, boots, ntip)
+    allb, counts = replicate_block(synth, parent, leaves, ntip, moves, 64, rng)
+    boots = np.split(allb, np.cumsum(counts)[:-1])
+    cp.transfer_dis_batch(ref, allb, ntip, counts)
     cp.transfer_dis(ref, boots[0], ntip)
     plain_s, pl = timed(lambda: [cp.transfer_dis(ref, b, ntip) for b in boots], 1)
-    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, boots, ntip), 1)
+    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, allb, ntip, counts), 1)
     for k in range(len(boots)):
         assert np.array_equal(bt[0][k], pl[k][0]) and np.array_equal(bt[2][k], pl[k][2])
     ex["tbe_4096_tips_64_replicates"] = {"plain_jobs_per_s": len(boots) / plain_s,
@@ -227,6 +231,27 @@ def extras(cp, synth, cells, q) -> dict:
     ex["serving_mix"] = serving_mix(cp, synth)
     print("# serving done", file=sys.stderr, flush=True)
     return ex
+
+
+def mfma_dtype(roof: dict) -> str:
+    """Operand type of the pair tiles: fp4 (e2m1 1.0 / 0.0 sites, exact fp32
+    sums), int8 (0/1 bytes, int32 sums) or u32 words on the VALU."""
+    if roof["bound"] != "mfma":
+        return "u32"
+    return "fp4" if roof["variant"].startswith("fp4") else "int8"
+
+
+def replicate_block(synth, parent, leaves, ntip, moves, nrep, rng):
+    """Clade bitsets of nrep SPR-perturbed copies of a tree, back to back in
+    one array (the layout a caller holding all replicates passes in)."""
+    sets = [synth.clade_bitsets(synth.spr(parent, leaves, moves, rng), leaves, ntip) for _ in range(nrep)]
+    counts = [b.shape[0] for b in sets]
+    allb = np.empty((sum(counts), sets[0].shape[1]), np.uint32)
+    o = 0
+    for b in sets:
+        allb[o:o + b.shape[0]] = b
+        o += b.shape[0]
+    return allb, counts
 
 
 def scan_form(cp, synth, n: int = 100000, W: int = 2560, R: int = 4) -> dict:
@@ -356,8 +381,8 @@ def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 800
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
@@ -395,6 +420,9 @@ def main():
     assert r_dev == r_host and t_dev == t_host and np.array_equal(c_dev, counts)
     assert np.array_equal(c_d, c_host) and np.array_equal(i_d, i_host)
     del c_host, i_host, c_d, i_d
+    # the round, replayed from its graph, gives the same choice and list size every time
+    for _ in range(CHECK_REPLAYS):
+        assert step() == (r_dev, t_dev)
     for _ in range(args.warmup):
         step()
     cp._lib.check(cp._lib.load().casphy_synchronize())
@@ -437,7 +465,7 @@ def main():
         "metric": "cell-pair scores/s + tree-build & TBE jobs/s on 1 GPU, batched vs plain, bit-exact",
         "value": value, "unit": "cell-pair scores/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int8" if roof["bound"] == "mfma" else "u32", "data": "synthetic",
+        "vs_baseline": None, "dtype": mfma_dtype(roof), "data": "synthetic",
         "config": {"workload": "base-editor binary 10,000 cells x 4,096 sites (128 u32/cell): tryRadius "
                                "1,024 queries x radii 1..16 + radiusNeighbor lists at the autoSplit radius",
                    "cells": N, "sites": NSITE, "queries": R, "radius": r, "neighbour_ids": int(total)},

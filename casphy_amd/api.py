@@ -257,23 +257,33 @@ def transfer_dis(Y, X, ntip: int):
     return ymin, ysize, xmin, xsize
 
 
-def transfer_dis_batch(Y, Xs, ntip: int):
+def transfer_dis_batch(Y, Xs, ntip: int, counts=None):
     """Transfer distances of reference bitsets Y against several replicate
-    sets in one launch: (ymin (nrep, ny), ysize, [xmin_b], [xsize_b])."""
+    sets in one launch: (ymin (nrep, ny), ysize, [xmin_b], [xsize_b]).
+    Xs is a list of (rows_b, ncol) arrays, or, with counts = [rows_b], one
+    (sum rows_b, ncol) array holding the replicates back to back (no copy)."""
     Y = _u32(Y)
     ny, ncol = Y.shape
-    Xs = [_u32(x) for x in Xs]
-    offs = np.zeros(len(Xs) + 1, dtype=np.uint32)
-    offs[1:] = np.cumsum([x.shape[0] for x in Xs])
-    X = np.ascontiguousarray(np.concatenate(Xs, axis=0)) if Xs else np.zeros((0, ncol), np.uint32)
-    ymin = np.empty((len(Xs), ny), np.uint32)
+    if counts is None:
+        Xs = [_u32(x) for x in Xs]
+        rows = [x.shape[0] for x in Xs]
+        X = np.ascontiguousarray(np.concatenate(Xs, axis=0)) if Xs else np.zeros((0, ncol), np.uint32)
+    else:
+        rows = [int(c) for c in counts]
+        X = _u32(Xs)
+        if X.ndim != 2 or X.shape[0] != sum(rows) or (rows and X.shape[1] != ncol):
+            raise ValueError("transfer_dis_batch: Xs must be (sum(counts), ncol)")
+    offs = np.zeros(len(rows) + 1, dtype=np.uint32)
+    offs[1:] = np.cumsum(rows)
+    nrep = len(rows)
+    ymin = np.empty((nrep, ny), np.uint32)
     ysize = np.empty(ny, np.uint32)
     xmin = np.empty(max(int(offs[-1]), 1), np.uint32)
     xsize = np.empty(max(int(offs[-1]), 1), np.uint32)
-    check(load().casphy_transfer_batch(_p(Y), ny, _p(X), _p(offs), len(Xs), ncol, int(ntip), _p(ymin), _p(ysize),
+    check(load().casphy_transfer_batch(_p(Y), ny, _p(X), _p(offs), nrep, ncol, int(ntip), _p(ymin), _p(ysize),
                                        _p(xmin), _p(xsize)))
-    xm = [xmin[offs[b]:offs[b + 1]] for b in range(len(Xs))]
-    xs = [xsize[offs[b]:offs[b + 1]] for b in range(len(Xs))]
+    xm = [xmin[offs[b]:offs[b + 1]] for b in range(nrep)]
+    xs = [xsize[offs[b]:offs[b + 1]] for b in range(nrep)]
     return ymin, ysize, xm, xs
 
 

@@ -377,6 +377,10 @@ def test_transfer_batch(hip, orc, bitpath):
         y2, ys2, x2, xs2 = orc.transfer(ref, X, ntip)
         assert np.array_equal(ymin[b], y2) and np.array_equal(ysize, ys2)
         assert np.array_equal(xm[b], x2) and np.array_equal(xs[b], xs2)
+    # the same replicates back to back in one array
+    y3, ys3, xm3, xs3 = hip.transfer_dis_batch(ref, np.concatenate(boots), ntip, [b.shape[0] for b in boots])
+    assert np.array_equal(y3, ymin) and np.array_equal(ys3, ysize)
+    assert all(np.array_equal(a, b) for a, b in zip(xm3, xm)) and all(np.array_equal(a, b) for a, b in zip(xs3, xs))
 
 
 # ------------------------------------------------------------------ scan form
