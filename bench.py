@@ -115,8 +115,8 @@ def measured_traffic(kernel: str):
         return None
     for line in open(TRAFFIC_TABLE):
         cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0] == kernel:
-            return int(cells[5].replace(",", ""))
+        if len(cells) >= 6 and cells[0] == kernel:
+            return int(cells[-1].replace(",", ""))  # last column: HBM bytes per launch
     return None
 
 
