@@ -38,10 +38,12 @@ SIGNATURES = [
     ("casphy_join_indel_batch", C.c_int, [vp, vp, u32, u32, vp, u32, vp, u32, f32, vp]),
     ("casphy_transfer", C.c_int, [vp, u32, vp, u32, u32, u32, vp, vp, vp, vp]),
     ("casphy_transfer_batch", C.c_int, [vp, u32, vp, vp, u32, u32, u32, vp, vp, vp, vp]),
+    ("casphy_tip_merge", C.c_int, [vp, vp, u32, C.c_double, vp]),
     ("casphy_cells_create", C.c_int, [C.c_int, vp, u32, u32, C.c_int, vp, C.POINTER(vp)]),
     ("casphy_cells_destroy", None, [vp]),
     ("casphy_cells_radius", C.c_int, [vp, i32, C.POINTER(u64)]),
     ("casphy_cells_fetch", C.c_int, [vp, vp, vp]),
+    ("casphy_cells_tip_merge", C.c_int, [vp, C.c_double, vp]),
     ("casphy_cells_try_radii", C.c_int, [vp, vp, u32, vp, u32, vp]),
     ("casphy_cells_autosplit", C.c_int, [vp, vp, u32, vp, u32, C.c_double, C.POINTER(i32), C.POINTER(u64), vp]),
     ("casphy_cells_last_ms", C.c_int, [vp, C.POINTER(f32), C.POINTER(f32)]),

@@ -32,7 +32,7 @@ __all__ = [
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
-    "radiusWeightedDNAParallel", "mfma_tuning",
+    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge",
 ]
 
 
@@ -287,6 +287,16 @@ def transfer_dis_batch(Y, Xs, ntip: int, counts=None):
     return ymin, ysize, xm, xs
 
 
+def tip_merge(nneighbor, idx, p: float) -> np.ndarray:
+    """cu/tipMerge.cu on .nNeighbor / .idx contents: int32 label per tip (the
+    .member lines)."""
+    nnb, ids = _u32(nneighbor), _u32(idx)
+    n = nnb.size
+    member = np.empty(max(n, 1), np.int32)
+    check(load().casphy_tip_merge(_p(nnb), _p(ids) if ids.size else None, n, float(p), _p(member)))
+    return member[:n]
+
+
 # ------------------------------------------------------------------ Rcpp twins
 def greedyJoinEdgeBitReverse(mat, weight, verbose: bool = False, ancestralType: int = 0):
     """src/functions.cpp:3316-3461.  mat: (n, W) bytes of the W x n R matrix.
@@ -412,6 +422,13 @@ class Cells:
         idx = np.empty(max(self.total, 1), np.uint32)
         check(load().casphy_cells_fetch(self._h, _p(counts), _p(idx)))
         return counts, idx[: self.total]
+
+    def tip_merge(self, p: float) -> np.ndarray:
+        """tipMerge labels over the lists of the last radius / autosplit call,
+        which stay on the device."""
+        member = np.empty(max(self.n, 1), np.int32)
+        check(load().casphy_cells_tip_merge(self._h, float(p), _p(member)))
+        return member[: self.n]
 
     def try_radii(self, radii, q) -> np.ndarray:
         r = np.ascontiguousarray(radii, dtype=np.int32)

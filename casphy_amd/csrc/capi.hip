@@ -14,6 +14,7 @@
 #include "context.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
+#include "tipmerge.hpp"
 #include "tiles.hpp"
 
 namespace casphy {
@@ -614,6 +615,34 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, con
     });
 }
 
+int casphy_tip_merge(const uint32_t *nneighbor, const uint32_t *idx, uint32_t ntip, double p, int32_t *member)
+{
+    return guarded([&] {
+        require(ntip == 0 || (nneighbor && member), "casphy_tip_merge: bad arguments");
+        std::vector<u64> off((size_t)ntip + 1, 0);
+        for (u32 i = 0; i < ntip; ++i) {
+            CASPHY_REQUIRE(nneighbor[i] <= TIP_MERGE_MAX_LIST, CASPHY_EARG,
+                           "casphy_tip_merge: tip " + std::to_string(i) + " has more than 1,048,576 neighbours");
+            off[i + 1] = off[i] + nneighbor[i];
+        }
+        const u64 total = off[ntip];
+        require(total == 0 || idx, "casphy_tip_merge: bad arguments");
+        for (u32 i = 0; i < ntip; ++i)
+            for (u64 k = off[i]; k < off[i + 1]; ++k) {
+                CASPHY_REQUIRE(idx[k] < ntip, CASPHY_EARG, "casphy_tip_merge: neighbour id out of range in list " +
+                                                               std::to_string(i));
+                CASPHY_REQUIRE(k == off[i] || idx[k - 1] < idx[k], CASPHY_EARG,
+                               "casphy_tip_merge: list " + std::to_string(i) + " is not strictly ascending");
+            }
+        if (ntip == 0) return;
+        hipStream_t s = thread_stream();
+        DevBuf doff(((size_t)ntip + 1) * 8), dids(std::max<u64>(total, 1) * 4);
+        h2d(doff.p, off.data(), ((size_t)ntip + 1) * 8, s);
+        if (total) h2d(dids.p, idx, total * 4, s);
+        tip_merge_device(doff.as<u64>(), dids.as<u32>(), ntip, nullptr, 0, p, member, s);
+    });
+}
+
 int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, uint32_t *internal_order)
 {
     return guarded([&] {
@@ -781,6 +810,22 @@ int casphy_cells_fetch(casphy_cells *h, uint32_t *nneighbor, uint32_t *idx)
         hipStream_t s = thread_stream();
         d2h(nneighbor, h->out.counts.p, (size_t)h->n * 4, s);
         d2h(idx, h->out.idx.p, h->out.total * 4, s);
+    });
+}
+
+int casphy_cells_tip_merge(casphy_cells *h, double p, int32_t *member)
+{
+    return guarded([&] {
+        require(h && member && h->out.n == h->n, "casphy_cells_tip_merge: no radius result");
+        hipStream_t s = thread_stream();
+        const u32 n = h->n;
+        std::vector<u32> nnb(n);
+        d2h(nnb.data(), h->out.counts.p, (size_t)n * 4, s);
+        for (u32 i = 0; i < n; ++i)
+            CASPHY_REQUIRE(nnb[i] <= TIP_MERGE_MAX_LIST, CASPHY_EARG,
+                           "casphy_cells_tip_merge: tip " + std::to_string(i) + " has more than 1,048,576 neighbours");
+        tip_merge_device(h->out.offsets.as<u64>(), h->out.idx.as<u32>(), n, h->out.mask.as<u32>(),
+                         ceil_div(n, TILE) * 4, p, member, s);
     });
 }
 

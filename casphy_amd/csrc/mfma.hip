@@ -144,161 +144,6 @@ template <> struct Slice<64> {
     }
 };
 
-// KB sites per LDS stage, PF stages of global prefetch in registers, OCC
-// workgroups per CU the launch bounds ask for.
-template <int EPI, int KB, int PF, int OCC>
-__global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
-{
-    constexpr int LR = KB + 16;  // LDS bytes per row (the pad spreads rows over the banks)
-    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LR];
-    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][TILE * LR];
-
-    const u32 per = (a.ntiles + 7) / 8;
-    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
-    u32 I, J;
-    if (!tile_of(a, t, I, J)) return;
-    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const u32 wm = wave >> 1, wn = wave & 1;
-
-    v16i acc[2][2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
-
-    const u32 nst = a.stages * (128 / KB);  // a multiple of 8: rows are padded to 1,024 sites
-    const bool opB = tid >= TILE;  // wave-uniform: waves 0-1 stage A rows, waves 2-3 B rows
-    const u32 row = tid & (TILE - 1);
-    const u32 *P = opB ? a.PB : a.PA;
-    const size_t pad = opB ? a.padB : a.padA;
-    const u32 tile = opB ? J : I;
-    unsigned char(*Ls)[TILE * LR] = opB ? Bs : As;
-    Slice<KB> g[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u) g[u].load(P, pad, tile, u, row);
-    g[0].store(Ls[0], row);
-    if (PF < (int)nst) g[0].load(P, pad, tile, PF, row);
-    __syncthreads();
-    const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
-    for (u32 s0 = 0; s0 < nst; s0 += PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const u32 s = s0 + u;
-            const int buf = s & 1;
-#pragma unroll
-            for (int kk = 0; kk < KB; kk += 32) {
-                v4i fa[2], fb[2];
-#pragma unroll
-                for (int x = 0; x < 2; ++x) {
-                    fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + kk + koff]);
-                    fb[x] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LR + kk + koff]);
-                }
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-#pragma unroll
-                    for (int y = 0; y < 2; ++y)
-                        acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
-            }
-            const int nx = (u + 1) % PF;  // static after unrolling
-            if (s + 1 < nst) {
-                g[nx].store(Ls[buf ^ 1], row);
-                if (s + 1 + PF < nst) g[nx].load(P, pad, tile, s + 1 + PF, row);
-            }
-            __syncthreads();
-        }
-    }
-
-    const u32 half = lane >> 5, col = lane & 31;
-    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const u32 r0 = I * TILE + wm * 64 + 32 * x;  // block row base
-            const u32 c0 = J * TILE + wn * 64 + 32 * y;  // block column base
-            const u32 j = c0 + col;
-            if (EPI == E_MASK) {
-                u32 colbits = 0;
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
-                    const bool hit = acc[x][y][v] >= thr;
-                    const unsigned long long b = __ballot(hit && j < a.nB);
-                    // low half: row 8*(v/4)+v%4, high half: that row + 4
-                    if (lane == (u32)v || lane == (u32)(v + 32)) {
-                        const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
-                        const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
-                        const u32 i = r0 + m_out;
-                        if (i < a.nA) a.mask[(size_t)i * a.mstride + c0 / 32] = word;
-                    }
-                    if (hit && r0 + m < a.nA) colbits |= 1u << m;
-                }
-                if (a.triangle && I != J) {
-                    colbits |= __shfl_xor(colbits, 32);
-                    if (half == 0 && j < a.nB) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
-                }
-            } else if (EPI == E_SCORES) {
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    if (i < a.nA && j < a.nB) {
-                        a.S[(size_t)i * a.ldS + j] = acc[x][y][v];
-                        if (a.triangle && I != J) a.S[(size_t)j * a.ldS + i] = acc[x][y][v];
-                    }
-                }
-            } else if (EPI == E_LOCALMAX) {
-                // per row: max over j > i of (S, smallest j), reduced over the 32 lanes of a half
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    unsigned long long key = 0;
-                    if (j < a.nB && j > i) key = ((unsigned long long)(u32)acc[x][y][v] << 32) | (0xFFFFFFFFu - j);
-#pragma unroll
-                    for (int o = 1; o < 32; o <<= 1) {
-                        const unsigned long long q = __shfl_xor(key, o);
-                        key = q > key ? q : key;
-                    }
-                    if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
-                }
-            } else if (EPI == E_TBE && r0 < a.nA) {  // a block past the last row has nothing to write
-                const u32 pb = j < a.nB ? a.popB[j] : 0u;
-                const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
-                u32 cmin = 0xFFFFFFFFu;
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    const u32 out = i < a.nA ? (a.arow_map ? a.arow_map[i] : i) : 0xFFFFFFFFu;
-                    const bool valid = out != 0xFFFFFFFFu && j < a.nB;
-                    const u32 h = (valid ? a.popA[i] : 0u) + pb - 2u * (u32)acc[x][y][v];
-                    const u32 hc = a.ntip - h;
-                    u32 d = valid ? (h < hc ? h : hc) : 0xFFFFFFFFu;
-                    cmin = d < cmin ? d : cmin;
-#pragma unroll
-                    for (int o = 1; o < 32; o <<= 1) {
-                        const u32 q = __shfl_xor(d, o);
-                        d = q < d ? q : d;
-                    }
-                    if (col == 0 && out != 0xFFFFFFFFu) atomicMin(&a.amin[out], d);
-                }
-                const u32 q = __shfl_xor(cmin, 32);
-                cmin = q < cmin ? q : cmin;
-                if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
-            }
-        }
-}
-
-// ---------------------------------------------------------------------------
-// Wide LDS variant: a workgroup of 4 waves (2 x 2) owns a 256 x 256 pair tile
-// and each wave 128 x 128 pairs as 4 x 4 MFMA blocks, so every 16-byte operand
-// read from LDS feeds 4 MFMAs (twice the 2 x 2 blocking): per 64-site stage
-// the CU moves 32 KB of expanded bytes into LDS and 64 KB out of it for 128
-// MFMAs.  One workgroup per CU (256 accumulators per lane); the next stage's
-// packed bits are prefetched into registers while the current one is used.
-constexpr u32 WT = 256;  // workgroup tile
-
 // Radius-mask compares of accumulator registers V..15 of a 32 x 32 block.
 // Register V holds row t = 8 (V / 4) + V % 4 in the low half of the wave and
 // row t + 4 in the high half, one column per lane: the ballot of the compare
@@ -388,6 +233,21 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
             }
             if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
         }
+    } else if (EPI == E_PAIRS) {
+        // shared-neighbour counts of the linked pairs i < j, at their place in
+        // the upper part of row i's list; the 32 columns of a block are one
+        // word of each row of the neighbour bits
+        const u32 wc = c0 / 32, below = (1u << col) - 1u;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+            if (i < a.nA && j < a.nB && i < j) {
+                const size_t at = (size_t)i * a.mstride + wc;
+                const u32 word = a.mask[at];
+                if ((word >> col) & 1u)
+                    a.pout[a.prow[i] + a.pq[at] + (u32)__builtin_popcount(word & below)] = (u32)acc[v];
+            }
+        }
     } else if (EPI == E_TBE && r0 < a.nA) {  // a block past the last row has nothing to write
         const u32 pb = j < a.nB ? a.popB[j] : 0u;
         const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
@@ -413,6 +273,92 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
         if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
     }
 }
+
+// KB sites per LDS stage, PF stages of global prefetch in registers, OCC
+// workgroups per CU the launch bounds ask for.
+template <int EPI, int KB, int PF, int OCC>
+__global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
+{
+    constexpr int LR = KB + 16;  // LDS bytes per row (the pad spreads rows over the banks)
+    __shared__ __attribute__((aligned(16))) unsigned char As[2][TILE * LR];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs[2][TILE * LR];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles) return;
+    u32 I, J;
+    if (!tile_of(a, t, I, J)) return;
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1;
+
+    v16i acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0;
+
+    const u32 nst = a.stages * (128 / KB);  // a multiple of 8: rows are padded to 1,024 sites
+    const bool opB = tid >= TILE;  // wave-uniform: waves 0-1 stage A rows, waves 2-3 B rows
+    const u32 row = tid & (TILE - 1);
+    const u32 *P = opB ? a.PB : a.PA;
+    const size_t pad = opB ? a.padB : a.padA;
+    const u32 tile = opB ? J : I;
+    unsigned char(*Ls)[TILE * LR] = opB ? Bs : As;
+    Slice<KB> g[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) g[u].load(P, pad, tile, u, row);
+    g[0].store(Ls[0], row);
+    if (PF < (int)nst) g[0].load(P, pad, tile, PF, row);
+    __syncthreads();
+    const u32 arow = wm * 64 + (lane & 31), brow = wn * 64 + (lane & 31), koff = 16 * (lane >> 5);
+    for (u32 s0 = 0; s0 < nst; s0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const u32 s = s0 + u;
+            const int buf = s & 1;
+#pragma unroll
+            for (int kk = 0; kk < KB; kk += 32) {
+                v4i fa[2], fb[2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    fa[x] = *reinterpret_cast<const v4i *>(&As[buf][(arow + 32 * x) * LR + kk + koff]);
+                    fb[x] = *reinterpret_cast<const v4i *>(&Bs[buf][(brow + 32 * x) * LR + kk + koff]);
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+            }
+            const int nx = (u + 1) % PF;  // static after unrolling
+            if (s + 1 < nst) {
+                g[nx].store(Ls[buf ^ 1], row);
+                if (s + 1 + PF < nst) g[nx].load(P, pad, tile, s + 1 + PF, row);
+            }
+            __syncthreads();
+        }
+    }
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+            block_epilogue<EPI>(a, acc[x][y], I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror,
+                                thr);
+}
+
+// ---------------------------------------------------------------------------
+// Wide LDS variant: a workgroup of 4 waves (2 x 2) owns a 256 x 256 pair tile
+// and each wave 128 x 128 pairs as 4 x 4 MFMA blocks, so every 16-byte operand
+// read from LDS feeds 4 MFMAs (twice the 2 x 2 blocking): per 64-site stage
+// the CU moves 32 KB of expanded bytes into LDS and 64 KB out of it for 128
+// MFMAs.  One workgroup per CU (256 accumulators per lane); the next stage's
+// packed bits are prefetched into registers while the current one is used.
+constexpr u32 WT = 256;  // workgroup tile
 
 template <int EPI>
 __global__ __launch_bounds__(256, 1) void k_mfma_wide(MfmaArgs a)
@@ -1159,80 +1105,11 @@ __global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
         }
     }
 
+    const bool mirror = a.triangle && I != J;
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            const u32 r0 = arow0 + 32 * x;  // block row base
-            const u32 c0 = brow0 + 32 * y;  // block column base
-            const u32 j = c0 + col;
-            if (EPI == E_MASK) {
-                u32 colbits = 0;
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 m = 8 * (v / 4) + 4 * half + (v % 4);
-                    const bool hit = acc[x][y][v] >= thr;
-                    const unsigned long long b = __ballot(hit && j < a.nB);
-                    if ((lane == (u32)v || lane == (u32)(v + 32)) && c0 < a.mstride * 32) {
-                        const u32 m_out = 8 * (v / 4) + (v % 4) + (lane >= 32 ? 4 : 0);
-                        const u32 word = lane >= 32 ? (u32)(b >> 32) : (u32)b;
-                        const u32 i = r0 + m_out;
-                        if (i < a.nA) a.mask[(size_t)i * a.mstride + c0 / 32] = word;
-                    }
-                    if (hit && r0 + m < a.nA) colbits |= 1u << m;
-                }
-                if (a.triangle && I != J) {
-                    colbits |= __shfl_xor(colbits, 32);
-                    if (half == 0 && j < a.nB && r0 < a.mstride * 32)
-                        a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
-                }
-            } else if (EPI == E_SCORES) {
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    if (i < a.nA && j < a.nB) {
-                        a.S[(size_t)i * a.ldS + j] = acc[x][y][v];
-                        if (a.triangle && I != J) a.S[(size_t)j * a.ldS + i] = acc[x][y][v];
-                    }
-                }
-            } else if (EPI == E_LOCALMAX) {
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    unsigned long long key = 0;
-                    if (j < a.nB && j > i) key = ((unsigned long long)(u32)acc[x][y][v] << 32) | (0xFFFFFFFFu - j);
-#pragma unroll
-                    for (int o = 1; o < 32; o <<= 1) {
-                        const unsigned long long q = __shfl_xor(key, o);
-                        key = q > key ? q : key;
-                    }
-                    if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
-                }
-            } else if (EPI == E_TBE && r0 < a.nA) {  // a block past the last row has nothing to write
-                const u32 pb = j < a.nB ? a.popB[j] : 0u;
-                const u32 bbase = a.tile_rep ? a.tile_rep[r0 >> 7] * a.bstride : 0u;
-                u32 cmin = 0xFFFFFFFFu;
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
-                    const u32 out = i < a.nA ? (a.arow_map ? a.arow_map[i] : i) : 0xFFFFFFFFu;
-                    const bool valid = out != 0xFFFFFFFFu && j < a.nB;
-                    const u32 h = (valid ? a.popA[i] : 0u) + pb - 2u * (u32)acc[x][y][v];
-                    const u32 hc = a.ntip - h;
-                    u32 d = valid ? (h < hc ? h : hc) : 0xFFFFFFFFu;
-                    cmin = d < cmin ? d : cmin;
-#pragma unroll
-                    for (int o = 1; o < 32; o <<= 1) {
-                        const u32 q = __shfl_xor(d, o);
-                        d = q < d ? q : d;
-                    }
-                    if (col == 0 && out != 0xFFFFFFFFu) atomicMin(&a.amin[out], d);
-                }
-                const u32 q = __shfl_xor(cmin, 32);
-                cmin = q < cmin ? q : cmin;
-                if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
-            }
-        }
+        for (int y = 0; y < 4; ++y) block_epilogue<EPI>(a, acc[x][y], arow0 + 32 * x, brow0 + 32 * y, lane, mirror, thr);
 }
 
 template <int KB, int PF, int OCC>
@@ -1247,6 +1124,8 @@ void launch_variant(int epi, const MfmaArgs &a, hipStream_t s)
         hipLaunchKernelGGL((k_mfma_tiles<E_LOCALMAX, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
     else if (epi == E_TBE)
         hipLaunchKernelGGL((k_mfma_tiles<E_TBE, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_PAIRS)
+        hipLaunchKernelGGL((k_mfma_tiles<E_PAIRS, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
     else
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }
@@ -1309,6 +1188,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
             hipLaunchKernelGGL(k_mfma_fp4w<E_LOCALMAX>, dim3(grid), dim3(512), 0, s, a);
         else if (epi == E_TBE)
             hipLaunchKernelGGL(k_mfma_fp4w<E_TBE>, dim3(grid), dim3(512), 0, s, a);
+        else if (epi == E_PAIRS)
+            hipLaunchKernelGGL(k_mfma_fp4w<E_PAIRS>, dim3(grid), dim3(512), 0, s, a);
         else
             throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     } else if (mode == 2) {
@@ -1320,6 +1201,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
             hipLaunchKernelGGL(k_mfma_fp4d<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
         else if (epi == E_TBE)
             hipLaunchKernelGGL(k_mfma_fp4d<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_PAIRS)
+            hipLaunchKernelGGL(k_mfma_fp4d<E_PAIRS>, dim3(grid), dim3(NTHR), 0, s, a);
         else
             throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     } else if (mode == 1) {
@@ -1331,6 +1214,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
             hipLaunchKernelGGL(k_mfma_fp4b<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
         else if (epi == E_TBE)
             hipLaunchKernelGGL(k_mfma_fp4b<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_PAIRS)
+            hipLaunchKernelGGL(k_mfma_fp4b<E_PAIRS>, dim3(grid), dim3(NTHR), 0, s, a);
         else
             throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     } else if (epi == E_MASK) {
@@ -1341,6 +1226,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
         hipLaunchKernelGGL(k_mfma_fp4<E_LOCALMAX>, dim3(grid), dim3(NTHR), 0, s, a);
     } else if (epi == E_TBE) {
         hipLaunchKernelGGL(k_mfma_fp4<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
+    } else if (epi == E_PAIRS) {
+        hipLaunchKernelGGL(k_mfma_fp4<E_PAIRS>, dim3(grid), dim3(NTHR), 0, s, a);
     } else {
         throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     }
@@ -1357,6 +1244,8 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
         hipLaunchKernelGGL(k_mfma_reg<E_LOCALMAX>, dim3(grid), dim3(256), 0, s, a);
     else if (epi == E_TBE)
         hipLaunchKernelGGL(k_mfma_reg<E_TBE>, dim3(grid), dim3(256), 0, s, a);
+    else if (epi == E_PAIRS)
+        hipLaunchKernelGGL(k_mfma_reg<E_PAIRS>, dim3(grid), dim3(256), 0, s, a);
     else
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }

@@ -36,6 +36,13 @@ struct MfmaArgs {
     const u32 *arow_map = nullptr;  // E_TBE batches: padded A row -> output row (~0u = padding)
     const u32 *tile_rep = nullptr;  // E_TBE batches: 128-row block of A -> replicate (bmin row block)
     u32 bstride = 0;                // E_TBE batches: bmin entries per replicate
+    // E_PAIRS: for i < j with bit j set in row i of `mask` (stride mstride
+    // words), pout[prow[i] + pq[i * mstride + j / 32] + (set bits of that word
+    // below bit j)] = <a_i, b_j> (prow holds the row's base minus its bits <= i,
+    // modulo 2^64; pq the exclusive per-word prefix of set bits)
+    const u32 *pq = nullptr;
+    const u64 *prow = nullptr;
+    u32 *pout = nullptr;
 };
 
 // First kw words of cell-major rows of stride ldw -> stage-major P (rows padded

@@ -183,6 +183,21 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X,
                           const uint32_t *xoffsets, uint32_t nrep, uint32_t ncol, uint32_t ntip,
                           uint32_t *ymin, uint32_t *ysize, uint32_t *xmin, uint32_t *xsize);
 
+/* ------------------------------------------------------------------ tipMerge */
+/* Cluster labels of tips whose neighbour lists overlap: nneighbor[ntip] list
+ * lengths and idx[sum] the concatenated ascending lists (ids < ntip, no
+ * repeats), as radiusNeighbor writes them to .nNeighbor / .idx.  A pair
+ * (j, t > j) of linked tips joins when the number s of tips in both lists
+ * passes s > nneighbor[j] * p or s > nneighbor[t] * p (double precision);
+ * pairs are folded into categories in the reference's order (windows of
+ * 131,072 tips, then j, then list order) and member[ntip] receives each
+ * tip's category, tips in no category numbered after the categories in tip
+ * order -- the lines of the .member file.  A list longer than 1,048,576 ids
+ * is refused (CASPHY_EARG), as the reference refuses it.
+ * Replaces: cu/tipMerge.cu:143-461 (argv: prefix p; called by
+ * R/CasPhy_functions.R:316,2025). */
+int casphy_tip_merge(const uint32_t *nneighbor, const uint32_t *idx, uint32_t ntip, double p, int32_t *member);
+
 /* ------------------------------------------------------------------ resident */
 /* A packed cell matrix kept in HBM across calls (the serving / tuning form: one
  * upload, many radius and tryRadius passes).  kind: CASPHY_KIND_BIT (bit-packed
@@ -216,6 +231,12 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
  * loop (R/CasPhy_functions.R:2012-2025). */
 int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
                            uint32_t R, double bound, int32_t *radius, uint64_t *nidx, uint32_t *counts);
+/* casphy_tip_merge over the lists of the last casphy_cells_radius or
+ * casphy_cells_autosplit call, which stay on the device (the radius mask is
+ * the dense form's bit rows).  No counterpart in the reference, where the
+ * lists go through .idx / .nNeighbor files between radiusNeighbor and
+ * tipMerge (R/CasPhy_functions.R:2024-2026). */
+int casphy_cells_tip_merge(casphy_cells *h, double p, int32_t *member);
 /* Device time (ms, hipEvents on the handle's stream): tile_ms = the scoring
  * kernel of the last radius, try_radii or autosplit call (autosplit: the
  * radius-mask tiles); radius_ms = the whole last radius or autosplit call;

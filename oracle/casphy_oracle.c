@@ -23,6 +23,13 @@
  *                             factor relabel restated in Python.
  *   orc_transfer              brute force min(H, n-H) in numpy, identical trees
  *                             give 0, complement invariance.
+ *   orc_list_shared           numpy set intersection.
+ *   orc_tip_merge             p >= 1: no merges (labels 0..n-1); disjoint
+ *                             cliques: one label per clique in order of first
+ *                             tip, singletons after; the OR of the two sharing
+ *                             fractions and the tip-id-as-category filter on
+ *                             hand-built lists; window order across 131,072
+ *                             tips; labels never join tips not linked by lists.
  */
 #include "casphy_oracle.h"
 
@@ -603,4 +610,118 @@ void orc_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx
         for (uint32_t k = 0; k < ncol; ++k) p += (uint32_t)__builtin_popcount(X[(size_t)x * ncol + k]);
         xsize[x] = p > ntip - p ? ntip - p : p;
     }
+}
+
+/* ---- (f-1) tipMerge: clusters of tips sharing most of their neighbours ----- */
+
+/* |A & B| of two ascending id lists: the count the reference's warp kernel
+ * sharedIdxKernel (cu/tipMerge.cu:25-129) forms by intersecting 32-wide
+ * windows of both lists; for ascending lists without repeats that is the
+ * size of the set intersection, taken here by one merge pass. */
+uint32_t orc_list_shared(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb)
+{
+    uint32_t i = 0, j = 0, c = 0;
+    while (i < na && j < nb) {
+        if (a[i] < b[j]) {
+            ++i;
+        } else if (a[i] > b[j]) {
+            ++j;
+        } else {
+            ++c;
+            ++i;
+            ++j;
+        }
+    }
+    return c;
+}
+
+/* transitionRoot (cu/tipMerge.cu:130-141): follow category links to a root;
+ * -1 stays -1.  (The reference's host loop also calls it with a tip index in
+ * place of a category, cu/tipMerge.cu:331; that call is kept as it is.) */
+static int32_t tm_root(const int32_t *transition, int32_t x)
+{
+    if (x == -1) return -1;
+    while (transition[x] != x && transition[x] != -1) x = transition[x];
+    return x;
+}
+
+/* The host loop of cu/tipMerge.cu main (:143-461): tips are visited in
+ * windows of 131,072 (CACHEMAX, :8); for window [c0, c1) every tip j < c1
+ * takes, in list order, its neighbours t with max(j + 1, c0) <= t < c1 whose
+ * id, read as a category, does not lead to j's current root (:323-338, with
+ * the state before j's merges); each such pair is merged when the shared
+ * count exceeds p times either list length (:379-405), with the reference's
+ * category bookkeeping; unassigned tips are numbered after the categories
+ * (:443-451).  Lists longer than 1,048,576 ids (MAXNEIGHBOR, :7, :183-186)
+ * are refused (return 1).  offsets[ntip + 1] delimit the lists in idx. */
+int orc_tip_merge(const uint32_t *nnb, const uint64_t *offsets, const uint32_t *idx, uint32_t ntip, double p,
+                  int32_t *member_out)
+{
+    const uint32_t CHUNK = 1024u * 128u, MAXNB = 1024u * 1024u;
+    uint32_t maxnb = 0;
+    for (uint32_t i = 0; i < ntip; ++i) {
+        if (nnb[i] > MAXNB) return 1;
+        if (nnb[i] > maxnb) maxnb = nnb[i];
+    }
+    int32_t *member = (int32_t *)malloc(sizeof(int32_t) * (ntip ? ntip : 1));
+    int32_t *transition = (int32_t *)malloc(sizeof(int32_t) * (ntip ? ntip : 1));
+    uint32_t *cursor = (uint32_t *)malloc(sizeof(uint32_t) * (ntip ? ntip : 1));
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (maxnb ? maxnb : 1));
+    for (uint32_t i = 0; i < ntip; ++i) {
+        member[i] = -1;
+        transition[i] = -1;
+        const uint32_t *l = idx + offsets[i];
+        uint32_t k = 0;
+        while (k < nnb[i] && l[k] <= i) ++k;  /* searchOffset, :204-210 */
+        cursor[i] = k;
+    }
+    int32_t ncat = 0;
+    for (uint32_t c0 = 0; c0 < ntip; c0 += CHUNK) {
+        const uint32_t c1 = ntip - c0 < CHUNK ? ntip : c0 + CHUNK;
+        for (uint32_t j = 0; j < c1; ++j) {
+            const uint32_t *lj = idx + offsets[j];
+            int32_t r1 = member[j] == -1 ? -1 : tm_root(transition, member[j]);
+            uint32_t nc = 0;
+            for (uint32_t k = cursor[j]; k < nnb[j]; ++k) {
+                const uint32_t t = lj[k];
+                if (t >= c1) {
+                    cursor[j] = k;
+                    break;
+                }
+                if (t >= c0 && (r1 == -1 || r1 != tm_root(transition, (int32_t)t))) cand[nc++] = t;
+            }
+            for (uint32_t c = 0; c < nc; ++c) {
+                const uint32_t t = cand[c];
+                const uint32_t s = orc_list_shared(lj, nnb[j], idx + offsets[t], nnb[t]);
+                if ((double)s > nnb[j] * p || (double)s > nnb[t] * p) {
+                    const int32_t r2 = tm_root(transition, member[t]);
+                    if (r1 == -1) {
+                        if (r2 == -1) {
+                            transition[ncat] = ncat;
+                            member[j] = ncat;
+                            member[t] = ncat;
+                            r1 = ncat;
+                            ++ncat;
+                        } else {
+                            r1 = r2;
+                            member[j] = r2;
+                        }
+                    } else if (r2 == -1) {
+                        member[t] = r1;
+                    } else {
+                        transition[r1] = r2;
+                        member[j] = r2;
+                        r1 = r2;
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < ntip; ++i)
+        member_out[i] = member[i] == -1 ? ncat++ : tm_root(transition, member[i]);
+    free(member);
+    free(transition);
+    free(cursor);
+    free(cand);
+    return 0;
 }

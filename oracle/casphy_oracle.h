@@ -84,6 +84,15 @@ void orc_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx
                   uint32_t ncol, uint32_t ntip,
                   uint32_t *ymin, uint32_t *ysize, uint32_t *xmin, uint32_t *xsize);
 
+/* ---- (f-1) tipMerge ----------------------------------------------------- */
+/* Size of the intersection of two ascending id lists. */
+uint32_t orc_list_shared(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb);
+/* Cluster labels of cu/tipMerge.cu for ascending neighbour lists (nnb[i] ids
+ * of tip i at idx + offsets[i]) and sharing fraction p; 1 when a list is
+ * longer than the reference's 1,048,576-id limit. */
+int orc_tip_merge(const uint32_t *nnb, const uint64_t *offsets, const uint32_t *idx, uint32_t ntip, double p,
+                  int32_t *member_out);
+
 #ifdef __cplusplus
 }
 #endif

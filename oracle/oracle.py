@@ -63,6 +63,10 @@ def lib():
         L.orc_join_indel_lazy.argtypes = [i32p, u32, u32, f32p, u32p, u32, f32, u32p, i32p, u32p]
         L.orc_cladewise.argtypes = [u32p, u32, u32p, u32p]
         L.orc_transfer.argtypes = [u32p, u32, u32p, u32, u32, u32, u32p, u32p, u32p, u32p]
+        L.orc_list_shared.restype = u32
+        L.orc_list_shared.argtypes = [u32p, u32, u32p, u32]
+        L.orc_tip_merge.restype = i32
+        L.orc_tip_merge.argtypes = [u32p, C.POINTER(C.c_uint64), u32p, u32, C.c_double, i32p]
         _lib = L
     return _lib
 
@@ -209,3 +213,24 @@ def transfer(Y, X, ntip):
     xsize = np.zeros(nx, np.uint32)
     lib().orc_transfer(Y, ny, X, nx, ncol, int(ntip), ymin, ysize, xmin, xsize)
     return ymin, ysize, xmin, xsize
+
+
+# ---------------------------------------------------------------- tipMerge
+def list_shared(a, b) -> int:
+    a, b = _u32(a), _u32(b)
+    return int(lib().orc_list_shared(a, a.size, b, b.size))
+
+
+def tip_merge(nnb, idx, p: float) -> np.ndarray:
+    """Cluster labels (int32 per tip) of cu/tipMerge.cu for the concatenated
+    ascending neighbour lists idx with lengths nnb."""
+    nnb, idx = _u32(nnb), _u32(idx)
+    n = nnb.size
+    off = np.zeros(n + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(nnb, dtype=np.uint64)
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    rc = lib().orc_tip_merge(nnb, off.ctypes.data_as(C.POINTER(C.c_uint64)), idx if idx.size else _u32([0]), n,
+                             float(p), out)
+    if rc != 0:
+        raise RuntimeError(f"oracle tip_merge rc={rc}")
+    return out[:n]

@@ -66,3 +66,20 @@ def test_formats_roundtrip(tmp_path):
         f.truncate(20)
     with pytest.raises(ValueError):
         formats.read_bin(p)
+
+
+def test_tip_merge_lists_checked_before_gpu_work():
+    """tipMerge refuses malformed lists and over-long lists before any device work."""
+    from casphy_amd import _lib
+    L = _lib.load()
+    vp = C.c_void_p
+    member = np.zeros(3, np.int32)
+
+    def call(nnb, idx):
+        nnb = np.asarray(nnb, np.uint32)
+        idx = np.asarray(idx, np.uint32)
+        return L.casphy_tip_merge(nnb.ctypes.data_as(vp), idx.ctypes.data_as(vp), nnb.size, C.c_double(0.5),
+                                  member.ctypes.data_as(vp))
+    assert call([2, 1, 1], [1, 0, 1, 2]) == -1 and b"not strictly ascending" in L.casphy_last_error()
+    assert call([1, 1, 1], [0, 1, 3]) == -1 and b"out of range" in L.casphy_last_error()
+    assert call([1 << 20 | 1, 1, 1], [0]) == -1 and b"1,048,576" in L.casphy_last_error()

@@ -261,3 +261,124 @@ def test_transfer_identical_trees_and_complement(orc):
     comp = synth.pack_bits(~synth.unpack_bits(B, 40))
     ymin2, ysize2, _, _ = orc.transfer(B, comp, 40)
     assert np.all(ymin2 == 0) and np.array_equal(ysize2, ysize)
+
+
+# ------------------------------------------------------------------ tipMerge
+def _lists_arrays(lists):
+    nnb = np.array([len(l) for l in lists], dtype=np.uint32)
+    idx = np.concatenate([np.asarray(sorted(l), np.uint32) for l in lists]) if lists else np.zeros(0, np.uint32)
+    return nnb, idx
+
+
+def test_list_shared_vs_numpy(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        a = np.unique(rng.integers(0, 500, rng.integers(0, 120)))
+        b = np.unique(rng.integers(0, 500, rng.integers(0, 120)))
+        assert orc.list_shared(a, b) == np.intersect1d(a, b).size
+
+
+def _clique_lists(n, rng, max_size=4):
+    """Tips shuffled into disjoint cliques; each tip's list is its clique."""
+    perm = rng.permutation(n)
+    cliques, k = [], 0
+    while k < n:
+        s = int(rng.integers(1, max_size + 1))
+        cliques.append(sorted(int(x) for x in perm[k:k + s]))
+        k += s
+    lists = [None] * n
+    for c in cliques:
+        for x in c:
+            lists[x] = c
+    return cliques, lists
+
+
+def _clique_labels(n, cliques, window=131072):
+    """Closed form for disjoint cliques: a clique of two or more tips becomes a
+    category when its first pair is folded -- in the window of its second
+    smallest tip, at its smallest tip -- so categories are numbered by
+    (second smallest // window, smallest); single tips follow in tip order."""
+    multi = sorted((c for c in cliques if len(c) > 1), key=lambda c: (c[1] // window, c[0]))
+    out = np.full(n, -1, np.int64)
+    for lab, c in enumerate(multi):
+        out[c] = lab
+    nxt = len(multi)
+    for i in range(n):
+        if out[i] < 0:
+            out[i] = nxt
+            nxt += 1
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 97, 2000])
+def test_tip_merge_disjoint_cliques(orc, n):
+    rng = np.random.default_rng(n)
+    cliques, lists = _clique_lists(n, rng)
+    nnb, idx = _lists_arrays(lists)
+    for p in (0.0, 0.5, 0.99):
+        assert np.array_equal(orc.tip_merge(nnb, idx, p), _clique_labels(n, cliques))
+    # p = 1: s > |N| p never holds (s <= |N|), so no pair joins
+    assert np.array_equal(orc.tip_merge(nnb, idx, 1.0), np.arange(n))
+
+
+def test_tip_merge_window_order(orc):
+    """Past 131,072 tips the pairs are folded window by window: cliques whose
+    second tip lies in the second window are numbered after every clique of
+    the first window, whatever their smallest tip."""
+    n = 131072 + 3000
+    rng = np.random.default_rng(8)
+    cliques, lists = _clique_lists(n, rng, max_size=3)
+    nnb, idx = _lists_arrays(lists)
+    got = orc.tip_merge(nnb, idx, 0.5)
+    assert np.array_equal(got, _clique_labels(n, cliques))
+    assert not np.array_equal(got, _clique_labels(n, cliques, window=n))  # the window matters here
+
+
+def test_tip_merge_either_fraction(orc):
+    """s > |N(j)| p OR s > |N(t)| p, strict: tips 10 < 11 with |N| = 10 and 4."""
+    def run(l10, l11, p):
+        lists = [[k] for k in range(10)] + [l10, l11]
+        nnb, idx = _lists_arrays(lists)
+        return orc.tip_merge(nnb, idx, p)
+    big, small = list(range(8)) + [10, 11], [0, 8, 9, 11]      # s = |{0, 11}| = 2
+    joined = np.array(list(range(1, 11)) + [0, 0])
+    alone = np.arange(12)
+    # second operand (|N(t)| = 4): 2 > 4p iff p < 0.5
+    assert np.array_equal(run(big, small, 0.49), joined)
+    assert np.array_equal(run(big, small, 0.5), alone)
+    # first operand (|N(j)| = 4): s = |{0, 10, 11}| = 3 > 4p iff p < 0.75
+    small2, big2 = [0, 8, 10, 11], list(range(8)) + [10, 11]
+    assert np.array_equal(run(small2, big2, 0.74), joined)
+    assert np.array_equal(run(small2, big2, 0.75), alone)
+
+
+def test_tip_merge_labels_follow_links(orc):
+    """On radius lists: tips with one label are connected through the lists,
+    and lowering p never splits a cluster."""
+    mat = synth.binary_matrix(seed=31, n=400, nsite=256, p=0.02)
+    counts, idx = orc.radius_bit(mat, 3)
+    off = np.zeros(401, np.int64)
+    off[1:] = np.cumsum(counts)
+    # connected components of the list graph (union-find over all links)
+    par = np.arange(400)
+
+    def find(x):
+        while par[x] != x:
+            par[x] = par[par[x]]
+            x = par[x]
+        return x
+    for i in range(400):
+        for t in idx[off[i]:off[i + 1]]:
+            a, b = find(i), find(int(t))
+            if a != b:
+                par[a] = b
+    comp = np.array([find(i) for i in range(400)])
+    prev = None
+    for p in (0.9, 0.6, 0.3, 0.0):
+        lab = orc.tip_merge(counts, idx, p)
+        for v in np.unique(lab):
+            assert np.unique(comp[lab == v]).size == 1
+        if prev is not None:  # clusters at a higher p are inside one cluster at a lower p
+            for v in np.unique(prev):
+                assert np.unique(lab[prev == v]).size == 1
+        prev = lab

@@ -25,7 +25,8 @@ def have_tools():
 @pytest.mark.parametrize("tool,args", [("radiusNeighbor", ["PFX", 3]), ("radiusNeighborDNA", ["PFX", 3]),
                                        ("tryRadius", ["PFX", 3, 10]), ("tryRadiusDNA", ["PFX", 3, 10]),
                                        ("joinEdgeBitReverse", ["PFX", "PFX.w", 1]),
-                                       ("joinEdgeIndel", ["PFX", "PFX.w", 2, 0, 2])])
+                                       ("joinEdgeIndel", ["PFX", "PFX.w", 2, 0, 2]),
+                                       ("tipMerge", ["PFX", 0.8])])
 def test_missing_input_fails_loudly(tmp_path, tool, args):
     pfx = str(tmp_path / "none")
     r = run(tool, *[a.replace("PFX", pfx) if isinstance(a, str) else a for a in args])
@@ -142,3 +143,20 @@ def test_single_tip_join_matches_reference_output(tmp_path):
     assert list(raw) == [0, 2]
     raw = np.fromfile(pfx + ".nodeState2", "<u4")
     assert list(raw) == [0, 2, 5, 9]
+
+
+@pytest.mark.gpu
+def test_tip_merge_tool(tmp_path, orc):
+    """radiusNeighbor then tipMerge on files, as R/CasPhy_functions.R:2024-2026
+    runs them; .member holds one label per line."""
+    mat = synth.binary_matrix(seed=52, n=900, nsite=512, p=0.01)
+    pfx = str(tmp_path / "m")
+    formats.write_bin(pfx + ".bmat", mat.T)
+    formats.write_weights(pfx + ".w", np.ones(512))
+    assert run("radiusNeighbor", pfx, 2).returncode == 0
+    r = run("tipMerge", pfx, 0.8)
+    assert r.returncode == 0, r.stderr
+    assert "tipMerge:" in r.stdout
+    member = np.loadtxt(pfx + ".member", dtype=np.int64)
+    c2, i2 = orc.radius_bit(mat, 2)
+    assert np.array_equal(member, orc.tip_merge(c2, i2, 0.8))
