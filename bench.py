@@ -128,7 +128,7 @@ def timed(fn, reps=1):
     return (time.perf_counter() - t0) / reps, out
 
 
-def extras(cp, synth, cells, q) -> dict:
+def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
     """The other configurations, plain vs batched, each a device-resident or
     host-API pass as a user would run it; every batched result is compared
     with its plain counterpart."""
@@ -226,6 +226,8 @@ def extras(cp, synth, cells, q) -> dict:
                                          "batched_jobs_per_s": len(boots) / batch_s,
                                          "speedup": plain_s / batch_s}
     print("# TBE done", file=sys.stderr, flush=True)
+    ex["tipmerge_10000x4096"] = tip_merge_step(cp, cells, q, with_cpu)
+    print("# tipMerge done", file=sys.stderr, flush=True)
     ex["scan_100000x81920_bit"] = scan_form(cp, synth)
     print("# scan form done", file=sys.stderr, flush=True)
     ex["serving_mix"] = serving_mix(cp, synth)
@@ -239,6 +241,45 @@ def mfma_dtype(roof: dict) -> str:
     if roof["bound"] != "mfma":
         return "u32"
     return "fp4" if roof["variant"].startswith("fp4") else "int8"
+
+
+TIPMERGE_RADII = np.arange(20, 340, 20, dtype=np.int32)
+TIPMERGE_MEAN = 300   # autoSplit bound: mean neighbour count at most 300
+TIPMERGE_P = 0.8      # p_shared default, R/CasPhy_functions.R:313,333
+
+
+def tip_merge_step(cp, cells, q, with_cpu: bool) -> dict:
+    """The split step of the autoSplit pipeline on resident cells: a device
+    autoSplit round (radii 20..320, mean count bound 300) picks the radius and
+    keeps its lists on the device, then tipMerge clusters them
+    (R/CasPhy_functions.R:2024-2026 runs radiusNeighbor then tipMerge on files).
+    The CPU leg runs the oracle's restatement of cu/tipMerge.cu on the same
+    lists, single-threaded, and checks the labels."""
+    r, total = cells.autosplit(TIPMERGE_RADII, q, float(np.log10(TIPMERGE_MEAN)))
+    counts, idx = cells.fetch()
+    cells.tip_merge(TIPMERGE_P)  # first call of the pair-tile shape class: autotuner
+    dev_s, member = timed(lambda: cells.tip_merge(TIPMERGE_P), 5)
+    file_s, m2 = timed(lambda: cp.tip_merge(counts, idx, TIPMERGE_P), 2)
+    assert np.array_equal(member, m2)
+    forms = {}
+    for form in ("dense", "sparse"):
+        os.environ["CASPHY_TIPMERGE"] = form
+        cells.tip_merge(TIPMERGE_P)
+        forms[form + "_ms"] = timed(lambda: cells.tip_merge(TIPMERGE_P), 3)[0] * 1e3
+    os.environ.pop("CASPHY_TIPMERGE")
+    n = counts.size
+    out = {"radius": int(r), "neighbour_ids": int(total), "mean_list": float(counts.mean()),
+           "clusters": int(np.unique(member).size), "largest_cluster": int(np.bincount(member).max()),
+           "device_ms": dev_s * 1e3, "file_form_ms": file_s * 1e3, **forms,
+           "tips_per_s": n / dev_s}
+    if with_cpu:
+        from oracle import oracle
+        cpu_s, m3 = timed(lambda: oracle.tip_merge(counts, idx, TIPMERGE_P), 1)
+        assert np.array_equal(m3, member), "CPU/GPU tipMerge mismatch"
+        out["cpu_baseline"] = {"value": n / cpu_s, "unit": "tips/s", "cores": 1, "kind": "port",
+                               "sample": f"the full {n}-tip workload, {cpu_s:.2f} s"}
+        out["speedup_vs_cpu"] = cpu_s / dev_s
+    return out
 
 
 def replicate_block(synth, parent, leaves, ntip, moves, nrep, rng):
@@ -386,6 +427,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
+    ap.add_argument("--tipmerge", action="store_true", help="run only the tipMerge split step (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -404,6 +446,9 @@ def main():
     bmat = np.ascontiguousarray(mat.T)  # .bmat payload (word-major)
     cells = cp.Cells(bmat, "bit", wordmajor=True)
 
+    if args.tipmerge:
+        print(json.dumps(tip_merge_step(cp, cells, q, not args.no_cpu)))
+        return
     bound = AUTOSPLIT_BOUND - np.log10(N)
 
     def step():
@@ -477,7 +522,7 @@ def main():
         gc, _ = cells.fetch()
         out["cpu_baseline"] = cpu_baseline(mat, r, gc)
     if not args.no_extra:
-        out["extra"] = extras(cp, synth, cells, q)
+        out["extra"] = extras(cp, synth, cells, q, not args.no_cpu)
     print(json.dumps(out))
 
 

@@ -256,11 +256,13 @@ void tip_merge_device(const u64 *off, const u32 *ids, u32 n, const u32 *bits, u3
     if (n == 0) return;
     u64 total = 0;
     d2h(&total, off + n, 8, s);
-    // dense (matrix cores) when the lists fill more than 1/13 of the pairs:
-    // n^3 / 2 bit products on the matrix cores against sum over pairs of |N(t)|
-    // list reads in the sparse form (CASPHY_TIPMERGE=dense|sparse forces one)
+    // dense (matrix cores) when the lists fill more than 1/140 of the pairs:
+    // n^3 / 2 bit products on the matrix cores against the sum over pairs of
+    // |N(t)| list reads in the sparse form, both growing as (lists / n^2)^-2
+    // relative to each other; 1/140 is where the two forms took the same time
+    // on 10,000 cells (CASPHY_TIPMERGE=dense|sparse forces one)
     const char *env = std::getenv("CASPHY_TIPMERGE");
-    bool dense = mfma_enabled() && n <= DENSE_MAX && (double)total * 13.0 > (double)n * n;
+    bool dense = mfma_enabled() && n <= DENSE_MAX && (double)total * 140.0 > (double)n * n;
     if (env && !std::strcmp(env, "dense")) dense = mfma_enabled() && n <= DENSE_MAX;
     if (env && !std::strcmp(env, "sparse")) dense = false;
 
