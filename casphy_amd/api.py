@@ -32,7 +32,7 @@ __all__ = [
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
-    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge",
+    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep",
 ]
 
 
@@ -295,6 +295,23 @@ def tip_merge(nneighbor, idx, p: float) -> np.ndarray:
     member = np.empty(max(n, 1), np.int32)
     check(load().casphy_tip_merge(_p(nnb), _p(ids) if ids.size else None, n, float(p), _p(member)))
     return member[:n]
+
+
+def node_keep(edge, node_state, outmat, cutoff: int, indel: bool = False) -> np.ndarray:
+    """nodeKeepBit / nodeKeepIndel on .edge2 / .nodeState2 / .tboutmat
+    payloads: edge (nedge, 2) 0-based, node_state (nnode, width), outmat
+    (nout, width); the kept internal node ids in edge order (.retainNode)."""
+    e, st, om = _u32(edge), _u32(node_state), _u32(outmat)
+    nnode, width = st.shape
+    nout = om.shape[0] if om.ndim == 2 else 0
+    if nout and om.shape[1] != width:
+        raise CasphyError(-1, "outmat rows must have the node rows' width")
+    kept = np.empty(max(nnode, 1), np.uint32)
+    nk = C.c_uint32(0)
+    fn = load().casphy_node_keep_indel if indel else load().casphy_node_keep_bit
+    check(fn(_p(e) if e.size else None, e.shape[0] if e.ndim == 2 else 0, _p(st) if st.size else None, nnode, width,
+             _p(om) if om.size else None, nout, int(cutoff), _p(kept), C.byref(nk)))
+    return kept[:nk.value]
 
 
 # ------------------------------------------------------------------ Rcpp twins

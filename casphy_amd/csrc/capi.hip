@@ -14,6 +14,7 @@
 #include "context.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
+#include "nodekeep.hpp"
 #include "tipmerge.hpp"
 #include "tiles.hpp"
 
@@ -641,6 +642,49 @@ int casphy_tip_merge(const uint32_t *nneighbor, const uint32_t *idx, uint32_t nt
         if (total) h2d(dids.p, idx, total * 4, s);
         tip_merge_device(doff.as<u64>(), dids.as<u32>(), ntip, nullptr, 0, p, member, s);
     });
+}
+
+namespace {
+int node_keep_entry(bool indel, const char *name, const uint32_t *edge, uint32_t nedge, const uint32_t *state,
+                    uint32_t nnode, uint32_t width, const uint32_t *outmat, uint32_t nout, uint32_t cutoff,
+                    uint32_t *kept, uint32_t *nkept)
+{
+    return guarded([&] {
+        const std::string fn(name);
+        CASPHY_REQUIRE(nkept && (nedge == 0 || edge) && (nnode == 0 || kept), CASPHY_EARG, fn + ": bad arguments");
+        CASPHY_REQUIRE(((size_t)nnode * width == 0 || state) && ((size_t)nout * width == 0 || outmat), CASPHY_EARG,
+                       fn + ": bad arguments");
+        const u64 ntip = (u64)nnode + 1;
+        std::vector<unsigned char> seen(nnode, 0);
+        for (u32 i = 0; i < nedge; ++i) {
+            const u32 a = edge[2 * i], d = edge[2 * i + 1];
+            CASPHY_REQUIRE(a < ntip + nnode && d < ntip + nnode, CASPHY_EARG,
+                           fn + ": edge " + std::to_string(i) + " names a node beyond the tree");
+            if (d >= ntip) {
+                CASPHY_REQUIRE(!seen[d - ntip], CASPHY_EARG,
+                               fn + ": internal node " + std::to_string(d) + " is the child of two edges");
+                seen[d - ntip] = 1;
+            }
+        }
+        *nkept = node_keep(indel, edge, nedge, state, nnode, width, outmat, nout, cutoff, kept, thread_stream());
+    });
+}
+}  // namespace
+
+int casphy_node_keep_bit(const uint32_t *edge, uint32_t nedge, const uint32_t *node_state, uint32_t nnode,
+                         uint32_t nbin, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
+                         uint32_t *nkept)
+{
+    return node_keep_entry(false, "casphy_node_keep_bit", edge, nedge, node_state, nnode, nbin, outmat, nout, cutoff,
+                           kept, nkept);
+}
+
+int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t *node_state, uint32_t nnode,
+                           uint32_t ncol, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
+                           uint32_t *nkept)
+{
+    return node_keep_entry(true, "casphy_node_keep_indel", edge, nedge, node_state, nnode, ncol, outmat, nout,
+                           cutoff, kept, nkept);
 }
 
 int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, uint32_t *internal_order)

@@ -1,0 +1,94 @@
+// nodekeep.hip -- nodeKeep: which internal nodes of a joined tree to keep.
+//
+// cu/nodeKeepBit.cu and cu/nodeKeepIndel.cu walk the edges of a tree in file
+// order and, for each internal node they reach, upload its state row and count
+// per outgroup row the node's mutations the outgroup lacks (one launch and one
+// round trip per node); the node is kept when every outgroup lacks at least
+// `cutoff` of them, and the subtree below a kept node is skipped.  Here one
+// launch counts every (internal node, outgroup) pair -- one workgroup per node,
+// a wave per outgroup row, lanes over the row's words -- and leaves one flag
+// per node; the host then walks the edges with the reference's skip rule.
+#include <vector>
+
+#include "context.hpp"
+#include "nodekeep.hpp"
+
+namespace casphy {
+
+namespace {
+
+// retain[v] = 1 when every outgroup row lacks at least `cutoff` of node v's
+// mutations (bit rows: popcount(node & ~out); indel rows: sites with node
+// state > 1 and an outgroup state >= 1 that differs).
+template <bool INDEL>
+__global__ __launch_bounds__(256) void k_node_retain(const u32 *state, u32 width, const u32 *outmat, u32 nout,
+                                                     u32 cutoff, u32 *retain)
+{
+    __shared__ u32 fail;
+    const u32 v = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    const u32 *row = state + (size_t)v * width;
+    u32 miss = 0;
+    for (u32 j = wv; j < nout; j += 4) {
+        const u32 *o = outmat + (size_t)j * width;
+        u32 c = 0;
+        for (u32 w = lane; w < width; w += 64) {
+            const u32 a = row[w], b = o[w];
+            if (INDEL)
+                c += a > 1u && b != a && b >= 1u;
+            else
+                c += (u32)__builtin_popcount(a & ~b);
+        }
+        for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
+        miss |= c < cutoff;
+    }
+    if (lane == 0 && miss) atomicOr(&fail, 1u);
+    __syncthreads();
+    if (tid == 0) retain[v] = fail ? 0u : 1u;
+}
+
+}  // namespace
+
+// Kept internal node ids (file order of the edges) of nodeKeepBit (indel =
+// false) or nodeKeepIndel.  edge: nedge x 2 (0-based ids, tips 0..nnode);
+// state: nnode x width; outmat: nout x width.  Returns the count.
+u32 node_keep(bool indel, const u32 *edge, u32 nedge, const u32 *state, u32 nnode, u32 width, const u32 *outmat,
+              u32 nout, u32 cutoff, u32 *kept, hipStream_t s)
+{
+    const u32 ntip = nnode + 1;
+    std::vector<u32> retain(nnode, 1u);
+    if (nnode && nout && width) {
+        DevBuf dstate((size_t)nnode * width * 4), dout((size_t)nout * width * 4), dret((size_t)nnode * 4);
+        h2d(dstate.p, state, (size_t)nnode * width * 4, s);
+        h2d(dout.p, outmat, (size_t)nout * width * 4, s);
+        if (indel)
+            hipLaunchKernelGGL(k_node_retain<true>, dim3(nnode), dim3(256), 0, s, dstate.as<u32>(), width,
+                               dout.as<u32>(), nout, cutoff, dret.as<u32>());
+        else
+            hipLaunchKernelGGL(k_node_retain<false>, dim3(nnode), dim3(256), 0, s, dstate.as<u32>(), width,
+                               dout.as<u32>(), nout, cutoff, dret.as<u32>());
+        CASPHY_HIP(hipGetLastError());
+        d2h(retain.data(), dret.p, (size_t)nnode * 4, s);
+    } else if (nnode && nout && cutoff > 0) {
+        retain.assign(nnode, 0u);  // empty rows: no mutation to find
+    }
+    // cu/nodeKeepBit.cu:186-211: an edge to internal node d is skipped while
+    // the last evaluated node was kept and the edge leaves a node whose id is
+    // not below it; otherwise d is evaluated
+    u32 current = ntip - 1, flag = 0, nk = 0;
+    for (u32 i = 0; i < nedge; ++i) {
+        const u32 a = edge[2 * i], d = edge[2 * i + 1];
+        if (d + 1 > ntip) {
+            if (flag == 1 && current <= a) continue;
+            flag = retain[d - ntip];
+            if (flag == 1) {
+                current = d;
+                kept[nk++] = d;
+            }
+        }
+    }
+    return nk;
+}
+
+}  // namespace casphy

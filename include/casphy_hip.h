@@ -198,6 +198,27 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X,
  * R/CasPhy_functions.R:316,2025). */
 int casphy_tip_merge(const uint32_t *nneighbor, const uint32_t *idx, uint32_t ntip, double p, int32_t *member);
 
+/* ------------------------------------------------------------------ nodeKeep */
+/* Internal nodes of a joined tree to keep against outgroup rows: edge[nedge][2]
+ * (0-based, tips 0..nnode, internal nodes nnode+1.. as in .edge2), node_state
+ * nnode rows and outmat nout rows of `nbin` bit words (_bit) or `ncol` u32
+ * states shifted by +1 (_indel), as in .nodeState2 / .tboutmat.  Edges are
+ * walked in order; an edge to an internal node d is skipped while the last
+ * evaluated node was kept and the edge's ancestor id is not below it,
+ * otherwise d is kept when every outgroup row lacks at least `cutoff` of its
+ * mutations (bit: set bits of d clear in the row; indel: sites where d's
+ * state is above 1 and the row's is at least 1 and differs).  kept (room for
+ * nnode ids) receives the kept ids in edge order, *nkept their number -- the
+ * lines of the .retainNode file.
+ * Replaces: cu/nodeKeepBit.cu:33-51,186-211 and cu/nodeKeepIndel.cu:32-48,
+ * 133-158 (argv: prefix cutoff; called by R/CasPhy_functions.R:2277,2281). */
+int casphy_node_keep_bit(const uint32_t *edge, uint32_t nedge, const uint32_t *node_state, uint32_t nnode,
+                         uint32_t nbin, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
+                         uint32_t *nkept);
+int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t *node_state, uint32_t nnode,
+                           uint32_t ncol, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
+                           uint32_t *nkept);
+
 /* ------------------------------------------------------------------ resident */
 /* A packed cell matrix kept in HBM across calls (the serving / tuning form: one
  * upload, many radius and tryRadius passes).  kind: CASPHY_KIND_BIT (bit-packed

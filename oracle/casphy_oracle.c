@@ -25,11 +25,16 @@
  *                             give 0, complement invariance.
  *   orc_list_shared           numpy set intersection.
  *   orc_tip_merge             p >= 1: no merges (labels 0..n-1); disjoint
- *                             cliques: one label per clique in order of first
- *                             tip, singletons after; the OR of the two sharing
- *                             fractions and the tip-id-as-category filter on
- *                             hand-built lists; window order across 131,072
- *                             tips; labels never join tips not linked by lists.
+ *                             cliques: one label per clique numbered by (window
+ *                             of its second tip, its first tip), singletons
+ *                             after -- also past 131,072 tips; the OR of the
+ *                             two sharing fractions and the strict > on
+ *                             hand-built lists; labels never join tips that
+ *                             the lists do not link.
+ *   orc_node_keep             numpy counts of missing mutations per (node,
+ *                             outgroup); on DFS-ordered join trees the kept set
+ *                             is the retained nodes without a kept proper
+ *                             ancestor (a tree-walk form of the skip rule).
  */
 #include "casphy_oracle.h"
 
@@ -724,4 +729,54 @@ int orc_tip_merge(const uint32_t *nnb, const uint64_t *offsets, const uint32_t *
     free(cursor);
     free(cand);
     return 0;
+}
+
+/* ---- (f-3) nodeKeep: internal nodes kept against the outgroup rows ------------ */
+
+/* Mutations of a node's state row that an outgroup row lacks:
+ * bit form (cu/nodeKeepBit.cu:31-47): set bits of the node whose bit is clear
+ * in the outgroup row; indel form (cu/nodeKeepIndel.cu:32-48): sites where
+ * the node's state is above 1, the outgroup's is at least 1 and differs. */
+static uint32_t nk_found(int indel, const uint32_t *node, const uint32_t *out, uint32_t width)
+{
+    uint32_t c = 0;
+    for (uint32_t w = 0; w < width; ++w) {
+        if (indel)
+            c += node[w] > 1u && out[w] != node[w] && out[w] >= 1u;
+        else
+            c += (uint32_t)__builtin_popcount(node[w] & ~out[w]);
+    }
+    return c;
+}
+
+/* The host loop of nodeKeepBit / nodeKeepIndel (cu/nodeKeepBit.cu:185-214,
+ * cu/nodeKeepIndel.cu:133-158): edges in file order; an edge to an internal
+ * node d (d >= ntip = nnode + 1) is skipped while the last evaluation kept a
+ * node and the edge's ancestor id is not below that node's id; otherwise d is
+ * kept when every outgroup row lacks at least `cutoff` of its mutations (no
+ * outgroup rows: kept), and a kept d becomes the current node.  kept[] gets
+ * the kept ids in order; returns their number. */
+uint32_t orc_node_keep(int indel, const uint32_t *edge, uint32_t nedge, const uint32_t *state, uint32_t nnode,
+                       uint32_t width, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept)
+{
+    const uint32_t ntip = nnode + 1;
+    uint32_t current = ntip - 1, flag = 0, nk = 0;
+    for (uint32_t i = 0; i < nedge; ++i) {
+        const uint32_t a = edge[2 * i], d = edge[2 * i + 1];
+        if (d + 1 > ntip) {
+            if (flag == 1 && current <= a) continue;
+            const uint32_t *row = state + (size_t)(d - ntip) * width;
+            flag = 1;
+            for (uint32_t j = 0; j < nout; ++j)
+                if (nk_found(indel, row, outmat + (size_t)j * width, width) < cutoff) {
+                    flag = 0;
+                    break;
+                }
+            if (flag == 1) {
+                current = d;
+                kept[nk++] = d;
+            }
+        }
+    }
+    return nk;
 }

@@ -93,6 +93,14 @@ uint32_t orc_list_shared(const uint32_t *a, uint32_t na, const uint32_t *b, uint
 int orc_tip_merge(const uint32_t *nnb, const uint64_t *offsets, const uint32_t *idx, uint32_t ntip, double p,
                   int32_t *member_out);
 
+/* ---- (f-3) nodeKeep ------------------------------------------------------- */
+/* Kept internal nodes of nodeKeepBit (indel = 0: state / outgroup rows of
+ * `width` u32 words of bits) or nodeKeepIndel (indel = 1: `width` u32 states
+ * per row); edge[nedge][2] 0-based (tips 0..nnode, internal nodes after).
+ * Returns the number of kept ids written to kept (at most nnode). */
+uint32_t orc_node_keep(int indel, const uint32_t *edge, uint32_t nedge, const uint32_t *state, uint32_t nnode,
+                       uint32_t width, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept);
+
 #ifdef __cplusplus
 }
 #endif

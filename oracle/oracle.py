@@ -67,6 +67,8 @@ def lib():
         L.orc_list_shared.argtypes = [u32p, u32, u32p, u32]
         L.orc_tip_merge.restype = i32
         L.orc_tip_merge.argtypes = [u32p, C.POINTER(C.c_uint64), u32p, u32, C.c_double, i32p]
+        L.orc_node_keep.restype = u32
+        L.orc_node_keep.argtypes = [i32, u32p, u32, u32p, u32, u32, u32p, u32, u32, u32p]
         _lib = L
     return _lib
 
@@ -234,3 +236,16 @@ def tip_merge(nnb, idx, p: float) -> np.ndarray:
     if rc != 0:
         raise RuntimeError(f"oracle tip_merge rc={rc}")
     return out[:n]
+
+
+# ---------------------------------------------------------------- nodeKeep
+def node_keep(edge, state, outmat, cutoff: int, indel: bool = False) -> np.ndarray:
+    """Kept internal node ids of nodeKeepBit / nodeKeepIndel: edge (nedge, 2)
+    0-based, state (nnode, width), outmat (nout, width), all u32."""
+    edge, state, outmat = _u32(edge), _u32(state), _u32(outmat)
+    nnode, width = state.shape
+    nout = outmat.shape[0] if outmat.size else 0
+    kept = np.zeros(max(nnode, 1), np.uint32)
+    k = lib().orc_node_keep(int(indel), edge.reshape(-1) if edge.size else _u32([0]), edge.shape[0], state, nnode,
+                            width, outmat if outmat.size else _u32([0]), nout, int(cutoff), kept)
+    return kept[:k]

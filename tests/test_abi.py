@@ -83,3 +83,19 @@ def test_tip_merge_lists_checked_before_gpu_work():
     assert call([2, 1, 1], [1, 0, 1, 2]) == -1 and b"not strictly ascending" in L.casphy_last_error()
     assert call([1, 1, 1], [0, 1, 3]) == -1 and b"out of range" in L.casphy_last_error()
     assert call([1 << 20 | 1, 1, 1], [0]) == -1 and b"1,048,576" in L.casphy_last_error()
+
+
+def test_node_keep_edges_checked_before_gpu_work():
+    from casphy_amd import _lib
+    L = _lib.load()
+    vp = C.c_void_p
+    st = np.zeros((2, 1), np.uint32)
+    kept = np.zeros(2, np.uint32)
+    nk = C.c_uint32(0)
+
+    def call(edges):
+        e = np.asarray(edges, np.uint32)
+        return L.casphy_node_keep_bit(e.ctypes.data_as(vp), e.shape[0], st.ctypes.data_as(vp), 2, 1, None, 0, 1,
+                                      kept.ctypes.data_as(vp), C.byref(nk))
+    assert call([[3, 4], [4, 0], [4, 1], [3, 9]]) == -1 and b"beyond the tree" in L.casphy_last_error()
+    assert call([[3, 4], [3, 4], [4, 1], [3, 2]]) == -1 and b"child of two edges" in L.casphy_last_error()

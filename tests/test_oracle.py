@@ -382,3 +382,62 @@ def test_tip_merge_labels_follow_links(orc):
             for v in np.unique(prev):
                 assert np.unique(lab[prev == v]).size == 1
         prev = lab
+
+
+# ------------------------------------------------------------------ nodeKeep
+def _joined_tree_bit(orc, seed, n=200, nsite=256):
+    mat = synth.binary_matrix(seed=seed, n=n, nsite=nsite, p=0.02)
+    edge, ns, _ = orc.join_bit(mat)
+    eo, order = orc.cladewise(edge, n)
+    return mat, eo, ns[order]
+
+
+def _keep_by_tree_walk(eo, retain, nnode):
+    """Kept set of the skip rule on a tree whose internal ids are in preorder:
+    a node is kept when it passes and no proper ancestor is kept."""
+    ntip = nnode + 1
+    parent = {int(d): int(a) for a, d in eo}
+    kept = []
+    under_kept = {}
+    for v in sorted(d for d in parent if d >= ntip):
+        p = parent[v]
+        blocked = under_kept.get(p, False) or (p in kept)
+        under_kept[v] = blocked
+        if not blocked and retain[v - ntip]:
+            kept.append(v)
+    return kept
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_node_keep_bit_tree_walk(orc, seed):
+    mat, eo, ns2 = _joined_tree_bit(orc, 60 + seed)
+    nnode = ns2.shape[0]
+    assert all(d > a for a, d in eo if d > nnode)  # internal ids in preorder
+    rng = np.random.default_rng(seed)
+    bits = synth.unpack_bits(ns2).astype(bool)
+    for nout in (0, 1, 40):
+        out = mat[rng.choice(mat.shape[0], nout, replace=False)] if nout else np.zeros((0, ns2.shape[1]), np.uint32)
+        ob = synth.unpack_bits(out).astype(bool) if nout else np.zeros((0, bits.shape[1]), bool)
+        missing = (bits[:, None, :] & ~ob[None, :, :]).sum(2) if nout else np.zeros((nnode, 0), int)
+        for cutoff in (0, 1, 2, 5):
+            retain = np.all(missing >= cutoff, axis=1)
+            want = _keep_by_tree_walk(eo, retain, nnode)
+            assert orc.node_keep(eo, ns2, out, cutoff).tolist() == want, (nout, cutoff)
+
+
+def test_node_keep_indel_tree_walk(orc):
+    rng = np.random.default_rng(9)
+    mat = synth.indel_matrix(seed=5, depth=7, nsite=40)
+    n, ncol = mat.shape
+    w = np.ones((mat.max(), ncol), dtype=np.float32)
+    edge, ns, _ = orc.join_indel(mat, w)
+    eo, order = orc.cladewise(edge, n)
+    ns2 = (ns[order] + 1).astype(np.uint32)          # .nodeState2 holds states + 1
+    nnode = ns2.shape[0]
+    out = (mat[rng.choice(n, 30, replace=False)] + 1).astype(np.uint32)   # .tboutmat: outmat + 1
+    a = ns2[:, None, :].astype(np.int64)
+    b = out[None, :, :].astype(np.int64)
+    missing = ((a > 1) & (b >= 1) & (b != a)).sum(2)
+    for cutoff in (1, 2, 4):
+        retain = np.all(missing >= cutoff, axis=1)
+        assert orc.node_keep(eo, ns2, out, cutoff, indel=True).tolist() == _keep_by_tree_walk(eo, retain, nnode)

@@ -26,7 +26,8 @@ def have_tools():
                                        ("tryRadius", ["PFX", 3, 10]), ("tryRadiusDNA", ["PFX", 3, 10]),
                                        ("joinEdgeBitReverse", ["PFX", "PFX.w", 1]),
                                        ("joinEdgeIndel", ["PFX", "PFX.w", 2, 0, 2]),
-                                       ("tipMerge", ["PFX", 0.8])])
+                                       ("tipMerge", ["PFX", 0.8]), ("nodeKeepBit", ["PFX", 1]),
+                                       ("nodeKeepIndel", ["PFX", 1])])
 def test_missing_input_fails_loudly(tmp_path, tool, args):
     pfx = str(tmp_path / "none")
     r = run(tool, *[a.replace("PFX", pfx) if isinstance(a, str) else a for a in args])
@@ -160,3 +161,23 @@ def test_tip_merge_tool(tmp_path, orc):
     member = np.loadtxt(pfx + ".member", dtype=np.int64)
     c2, i2 = orc.radius_bit(mat, 2)
     assert np.array_equal(member, orc.tip_merge(c2, i2, 0.8))
+
+
+@pytest.mark.gpu
+def test_node_keep_tool(tmp_path, orc):
+    """joinEdgeBitReverse then nodeKeepBit on the same prefix, as
+    R/CasPhy_functions.R:2204,2275-2286 runs them; .retainNode holds the kept
+    node ids one per line."""
+    mat = synth.binary_matrix(seed=53, n=400, nsite=256, p=0.02)
+    pfx = str(tmp_path / "k")
+    formats.write_bin(pfx + ".tbmat", mat)
+    formats.write_weights(pfx + ".w", np.ones(256))
+    assert run("joinEdgeBitReverse", pfx, pfx + ".w", 1).returncode == 0
+    out = mat[np.random.default_rng(1).choice(400, 150, replace=False)]
+    formats.write_bin(pfx + ".tboutmat", out)
+    r = run("nodeKeepBit", pfx, 2)
+    assert r.returncode == 0, r.stderr
+    eo = formats.read_bin(pfx + ".edge2")
+    ns2 = formats.read_bin(pfx + ".nodeState2")
+    kept = np.loadtxt(pfx + ".retainNode", dtype=np.int64, ndmin=1) if open(pfx + ".retainNode").read() else []
+    assert np.array_equal(np.asarray(kept), orc.node_keep(eo, ns2, out, 2))
