@@ -158,6 +158,7 @@ def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
     cp.join_bit(mat)  # first call of the shape class runs the tile autotuner
     join_s, _ = timed(lambda: cp.join_bit(mat), 1)
     ex["binary_join_10000x4096"] = {"plain_jobs_per_s": 1 / join_s, "ms_per_job": join_s * 1e3}
+    ex["nodekeep_10000x4096"] = node_keep_step(cp, mat, with_cpu)
     print("# binary join done", file=sys.stderr, flush=True)
 
     # DNA 100,000 x 1,000: tuning in chunks of 16 radii from radius_initial 100, step 10, then the lists
@@ -280,6 +281,32 @@ def tip_merge_step(cp, cells, q, with_cpu: bool) -> dict:
                                "sample": f"the full {n}-tip workload, {cpu_s:.2f} s"}
         out["speedup_vs_cpu"] = cpu_s / dev_s
     return out
+
+
+NODEKEEP_OUT = 1000    # outgroup rows (cells of the matrix)
+NODEKEEP_CUTOFF = 1
+
+
+def node_keep_step(cp, mat, with_cpu: bool) -> dict:
+    """nodeKeepBit after the 10,000-cell join (R/CasPhy_functions.R:2204,
+    2275-2286): the joined tree's .edge2 / .nodeState2 and 1,000 outgroup
+    rows; the CPU leg runs the oracle's restatement on the same input."""
+    eo, ns2 = cp.join_edge_bit_reverse(mat)
+    out = mat[np.random.default_rng(5).choice(mat.shape[0], NODEKEEP_OUT, replace=False)]
+    cp.node_keep(eo, ns2, out, NODEKEEP_CUTOFF)
+    dev_s, kept = timed(lambda: cp.node_keep(eo, ns2, out, NODEKEEP_CUTOFF), 3)
+    nnode = ns2.shape[0]
+    res = {"internal_nodes": int(nnode), "outgroups": NODEKEEP_OUT, "cutoff": NODEKEEP_CUTOFF,
+           "kept": int(kept.size), "ms": dev_s * 1e3,
+           "node_outgroup_pairs_per_s": nnode * NODEKEEP_OUT / dev_s}
+    if with_cpu:
+        from oracle import oracle
+        cpu_s, k2 = timed(lambda: oracle.node_keep(eo, ns2, out, NODEKEEP_CUTOFF), 1)
+        assert np.array_equal(k2, kept), "CPU/GPU nodeKeep mismatch"
+        res["cpu_baseline"] = {"value": 1 / cpu_s, "unit": "trees/s", "cores": 1, "kind": "port",
+                               "sample": f"the full tree, {cpu_s * 1e3:.1f} ms"}
+        res["speedup_vs_cpu"] = cpu_s / dev_s
+    return res
 
 
 def replicate_block(synth, parent, leaves, ntip, moves, nrep, rng):
@@ -427,7 +454,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
-    ap.add_argument("--tipmerge", action="store_true", help="run only the tipMerge split step (profiling)")
+    ap.add_argument("--only", choices=["tipmerge", "nodekeep"],
+                    help="run only the tipMerge split step or the nodeKeep step (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -446,8 +474,11 @@ def main():
     bmat = np.ascontiguousarray(mat.T)  # .bmat payload (word-major)
     cells = cp.Cells(bmat, "bit", wordmajor=True)
 
-    if args.tipmerge:
+    if args.only == "tipmerge":
         print(json.dumps(tip_merge_step(cp, cells, q, not args.no_cpu)))
+        return
+    if args.only == "nodekeep":
+        print(json.dumps(node_keep_step(cp, mat, not args.no_cpu)))
         return
     bound = AUTOSPLIT_BOUND - np.log10(N)
 
