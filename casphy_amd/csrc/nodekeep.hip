@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void k_node_retain(const u32 *state, u32 width
         bool miss = false;
         if (j < nout) {
             u32 c = 0;
+#pragma unroll 8
             for (u32 w = 0; w < width; ++w) {
                 const u32 a = row[w], b = outT[(size_t)w * nout + j];
                 if (INDEL)
