@@ -32,7 +32,7 @@ __all__ = [
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
-    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep",
+    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep", "clade_bitsets",
 ]
 
 
@@ -295,6 +295,21 @@ def tip_merge(nneighbor, idx, p: float) -> np.ndarray:
     member = np.empty(max(n, 1), np.int32)
     check(load().casphy_tip_merge(_p(nnb), _p(ids) if ids.size else None, n, float(p), _p(member)))
     return member[:n]
+
+
+def clade_bitsets(edge, nnode: int, ntip: int):
+    """nodeDesBig on R's 1-based edge matrix (nedge, 2) in cladewise order:
+    (rows (nnode-1, ceil(ntip/32)) u32 in nodeDesBig's order, node sizes as it
+    returns them)."""
+    e = np.ascontiguousarray(edge, dtype=np.int32)
+    anc, des = np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1])
+    nbin = ((ntip - 1) >> 5) + 1
+    rows = np.empty((max(nnode - 1, 1), nbin), np.uint32)
+    sizes = np.empty(max(nnode - 1, 1), np.int32)
+    nr = C.c_uint32(0)
+    check(load().casphy_clade_bitsets(_p(anc), _p(des), e.shape[0], int(nnode), int(ntip), _p(rows), C.byref(nr),
+                                      _p(sizes)))
+    return rows[: nr.value], sizes[: max(nnode - 1, 0)]
 
 
 def node_keep(edge, node_state, outmat, cutoff: int, indel: bool = False) -> np.ndarray:

@@ -14,6 +14,7 @@
 #include "context.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
+#include "clades.hpp"
 #include "nodekeep.hpp"
 #include "tipmerge.hpp"
 #include "tiles.hpp"
@@ -685,6 +686,24 @@ int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t 
 {
     return node_keep_entry(true, "casphy_node_keep_indel", edge, nedge, node_state, nnode, ncol, outmat, nout,
                            cutoff, kept, nkept);
+}
+
+int casphy_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge, uint32_t nnode, uint32_t ntip,
+                         uint32_t *rows, uint32_t *nrows, int32_t *node_size)
+{
+    return guarded([&] {
+        require(nrows && (nedge == 0 || (anc && des)) && (nnode < 2 || (rows && node_size)),
+                "casphy_clade_bitsets: bad arguments");
+        const CladePlan p = plan_clades(anc, des, nedge, nnode, ntip);
+        const u32 nr = (u32)p.lo.size();
+        *nrows = nr;
+        if (!p.size.empty()) std::memcpy(node_size, p.size.data(), p.size.size() * 4);
+        if (!nr) return;
+        hipStream_t s = thread_stream();
+        DevBuf drows((size_t)nr * p.nbin * 4);
+        build_clade_rows(p, drows.as<u32>(), s);
+        d2h(rows, drows.p, (size_t)nr * p.nbin * 4, s);
+    });
 }
 
 int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, uint32_t *internal_order)

@@ -253,3 +253,34 @@ def clade_bitsets(parent: np.ndarray, leaves: np.ndarray, ntip: int) -> np.ndarr
     internal = np.nonzero(internal_mask)[0]
     internal = internal[internal != 0]
     return np.ascontiguousarray(bits[internal])
+
+
+def ape_edges(parent: np.ndarray, leaves: np.ndarray):
+    """R's phylo$edge for a rooted tree given as a parent array (root 0) with
+    tip t at node leaves[t]: 1-based, tips 1..ntip, internal nodes numbered
+    from ntip + 1 in preorder, edges in cladewise order (depth first, children
+    in increasing node id).  Returns (edge (nedge, 2) int32, nnode, ntip)."""
+    nn = parent.size
+    ntip = leaves.size
+    kids = [[] for _ in range(nn)]
+    for v in range(1, nn):
+        kids[int(parent[v])].append(v)
+    label = np.zeros(nn, dtype=np.int64)
+    label[leaves] = np.arange(1, ntip + 1)
+    nxt = ntip + 1
+    label[0] = nxt
+    nxt += 1
+    edges = []
+    stack = [(0, 0)]
+    while stack:
+        v, k = stack.pop()
+        if k < len(kids[v]):
+            stack.append((v, k + 1))
+            c = kids[v][k]
+            if kids[c]:
+                label[c] = nxt
+                nxt += 1
+            edges.append((label[v], label[c]))
+            if kids[c]:
+                stack.append((c, 0))
+    return np.array(edges, dtype=np.int32).reshape(-1, 2), int(nxt - ntip - 1), ntip

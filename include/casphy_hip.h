@@ -198,6 +198,22 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X,
  * R/CasPhy_functions.R:316,2025). */
 int casphy_tip_merge(const uint32_t *nneighbor, const uint32_t *idx, uint32_t ntip, double p, int32_t *member);
 
+/* ------------------------------------------------------------ clade bitsets */
+/* Clade bitsets of a rooted tree given as the two columns of R's 1-based
+ * edge matrix (ape phylo$edge: tips 1..ntip, internal nodes ntip+1..
+ * ntip+nnode, one root, each node's edge listed before its children's edges
+ * as in cladewise order).  rows receives one row of ceil(ntip / 32) u32 words
+ * per internal node except the root (bit t-1 set for each tip t below it), in
+ * the order nodeDesBig writes them (internal children by a backward scan of
+ * the edges), *nrows their number (nnode - 1 for a tree), and node_size the
+ * vector nodeDesBig returns (node_size[k] = tips below internal node
+ * nnode - 1 - k, 0-based among the internal nodes, k < nnode - 1).  Edge lists
+ * in other orders are refused (CASPHY_EARG); there is no row cache limit.
+ * Replaces: src/functions.cpp:1277-1366 (nodeDesBig, called by
+ * R/bootstrap_functions.R:3). */
+int casphy_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge, uint32_t nnode, uint32_t ntip,
+                         uint32_t *rows, uint32_t *nrows, int32_t *node_size);
+
 /* ------------------------------------------------------------------ nodeKeep */
 /* Internal nodes of a joined tree to keep against outgroup rows: edge[nedge][2]
  * (0-based, tips 0..nnode, internal nodes nnode+1.. as in .edge2), node_state

@@ -31,6 +31,10 @@
  *                             two sharing fractions and the strict > on
  *                             hand-built lists; labels never join tips that
  *                             the lists do not link.
+ *   orc_clade_bitsets         on cladewise trees every row is the clade of
+ *                             its node (numpy walk from each tip to the root),
+ *                             sizes are the row popcounts, rows come in the
+ *                             backward-scan order of the internal children.
  *   orc_node_keep             numpy counts of missing mutations per (node,
  *                             outgroup); on DFS-ordered join trees the kept set
  *                             is the retained nodes without a kept proper
@@ -779,4 +783,42 @@ uint32_t orc_node_keep(int indel, const uint32_t *edge, uint32_t nedge, const ui
         }
     }
     return nk;
+}
+
+/* ---- (f-2) bipartition producer: clade bitsets of a tree ------------------- */
+
+/* nodeDesBig (src/functions.cpp:1277-1366) with an unbounded row cache: edges
+ * (1-based ids, tips 1..ntip, internal nodes ntip+1..ntip+nnode) are taken
+ * from the last to the first; a tip child sets its bit in the ancestor's row
+ * and adds 1 to its size, an internal child ORs its row into the ancestor's,
+ * adds its size and is written out as the next row (so rows come in the order
+ * the backward scan completes them; the root is never written).  size_out
+ * receives the reference's return vector: size_out[k] = size of internal node
+ * nnode - 1 - k (0-based internal index), k < nnode - 1.  Returns the number of
+ * rows written. */
+uint32_t orc_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge, uint32_t nnode, uint32_t ntip,
+                           uint32_t *rows_out, int32_t *size_out)
+{
+    const uint32_t nbin = ((ntip - 1) >> 5) + 1;
+    uint32_t *rows = (uint32_t *)calloc((size_t)(nnode ? nnode : 1) * nbin, sizeof(uint32_t));
+    int32_t *size = (int32_t *)calloc(nnode ? nnode : 1, sizeof(int32_t));
+    uint32_t nrow = 0;
+    for (uint32_t e = nedge; e-- > 0;) {
+        const int32_t a = anc[e] - 1, d = des[e] - 1;
+        uint32_t *ra = rows + (size_t)(a - (int32_t)ntip) * nbin;
+        if (d < (int32_t)ntip) {
+            size[a - (int32_t)ntip] += 1;
+            ra[d >> 5] |= 1u << (d & 31);
+        } else {
+            uint32_t *rd = rows + (size_t)(d - (int32_t)ntip) * nbin;
+            size[a - (int32_t)ntip] += size[d - (int32_t)ntip];
+            for (uint32_t w = 0; w < nbin; ++w) ra[w] |= rd[w];
+            memcpy(rows_out + (size_t)nrow * nbin, rd, (size_t)nbin * 4);
+            ++nrow;
+        }
+    }
+    for (uint32_t i = 1; i < nnode; ++i) size_out[nnode - 1 - i] = size[i];
+    free(rows);
+    free(size);
+    return nrow;
 }

@@ -101,6 +101,13 @@ int orc_tip_merge(const uint32_t *nnb, const uint64_t *offsets, const uint32_t *
 uint32_t orc_node_keep(int indel, const uint32_t *edge, uint32_t nedge, const uint32_t *state, uint32_t nnode,
                        uint32_t width, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept);
 
+/* ---- (f-2) bipartition producer ------------------------------------------- */
+/* Clade bitsets of nodeDesBig: anc/des the columns of R's 1-based edge matrix;
+ * rows_out holds (nnode - 1) x ceil(ntip / 32) words, size_out nnode - 1
+ * entries (the reference's return vector).  Returns the rows written. */
+uint32_t orc_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge, uint32_t nnode, uint32_t ntip,
+                           uint32_t *rows_out, int32_t *size_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,8 @@ def lib():
         L.orc_tip_merge.restype = i32
         L.orc_tip_merge.argtypes = [u32p, C.POINTER(C.c_uint64), u32p, u32, C.c_double, i32p]
         L.orc_node_keep.restype = u32
+        L.orc_clade_bitsets.restype = u32
+        L.orc_clade_bitsets.argtypes = [i32p, i32p, u32, u32, u32, u32p, i32p]
         L.orc_node_keep.argtypes = [i32, u32p, u32, u32p, u32, u32, u32p, u32, u32, u32p]
         _lib = L
     return _lib
@@ -249,3 +251,16 @@ def node_keep(edge, state, outmat, cutoff: int, indel: bool = False) -> np.ndarr
     k = lib().orc_node_keep(int(indel), edge.reshape(-1) if edge.size else _u32([0]), edge.shape[0], state, nnode,
                             width, outmat if outmat.size else _u32([0]), nout, int(cutoff), kept)
     return kept[:k]
+
+
+# ---------------------------------------------------------------- clade bitsets
+def clade_bitsets(edge, nnode: int, ntip: int):
+    """(rows (nnode-1, ceil(ntip/32)) u32, sizes (nnode-1,) int32) of
+    nodeDesBig for R's 1-based edge matrix (nedge, 2)."""
+    e = np.ascontiguousarray(edge, dtype=np.int32)
+    anc, des = np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1])
+    nbin = ((ntip - 1) >> 5) + 1
+    rows = np.zeros((max(nnode - 1, 1), nbin), np.uint32)
+    sizes = np.zeros(max(nnode - 1, 1), np.int32)
+    k = lib().orc_clade_bitsets(anc, des, e.shape[0], nnode, ntip, rows, sizes)
+    return rows[:k], sizes[: max(nnode - 1, 0)]

@@ -99,3 +99,16 @@ def test_node_keep_edges_checked_before_gpu_work():
                                       kept.ctypes.data_as(vp), C.byref(nk))
     assert call([[3, 4], [4, 0], [4, 1], [3, 9]]) == -1 and b"beyond the tree" in L.casphy_last_error()
     assert call([[3, 4], [3, 4], [4, 1], [3, 2]]) == -1 and b"child of two edges" in L.casphy_last_error()
+
+
+def test_clade_bitsets_edge_order_checked():
+    """Edge lists where a node's edge comes after its children's are refused
+    (nodeDesBig would write partial rows for them)."""
+    from casphy_amd import api
+    good = np.array([[4, 5], [5, 1], [5, 2], [4, 3]], np.int32)      # 3 tips, root 4
+    bad = good[[1, 2, 0, 3]]
+    with pytest.raises(api.CasphyError) as e:
+        api.clade_bitsets(bad, 2, 3)
+    assert e.value.code == -1 and "leaves a node before the edge into it" in str(e.value)
+    with pytest.raises(api.CasphyError):
+        api.clade_bitsets(np.array([[4, 5], [5, 1], [5, 2], [5, 3], [4, 5]], np.int32), 2, 3)

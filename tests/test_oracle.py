@@ -441,3 +441,56 @@ def test_node_keep_indel_tree_walk(orc):
     for cutoff in (1, 2, 4):
         retain = np.all(missing >= cutoff, axis=1)
         assert orc.node_keep(eo, ns2, out, cutoff, indel=True).tolist() == _keep_by_tree_walk(eo, retain, nnode)
+
+
+# ------------------------------------------------------------------ clade bitsets
+def _clades_by_walk(edge, nnode, ntip):
+    """Clade of every internal node by walking from each tip to the root."""
+    par = {int(d): int(a) for a, d in edge}
+    nbin = ((ntip - 1) >> 5) + 1
+    rows = np.zeros((nnode + ntip + 1, nbin), np.uint32)
+    for t in range(1, ntip + 1):
+        v = par.get(t)
+        while v is not None:
+            rows[v, (t - 1) >> 5] |= np.uint32(1) << np.uint32((t - 1) & 31)
+            v = par.get(v)
+    return rows
+
+
+def _random_multifurcating(rng, ntip):
+    """Random rooted tree with internal nodes of 2..4 children (parent array)."""
+    nodes = list(range(ntip))
+    parent = {}
+    nxt = ntip
+    while len(nodes) > 1:
+        k = min(len(nodes), int(rng.integers(2, 5)))
+        pick = rng.choice(len(nodes), k, replace=False)
+        for i in pick:
+            parent[nodes[i]] = nxt
+        nodes = [x for j, x in enumerate(nodes) if j not in set(pick.tolist())] + [nxt]
+        nxt += 1
+    root = nodes[0]
+    # renumber so the root is 0 and tips follow their order
+    order = [root] + [v for v in range(nxt) if v != root]
+    idx = {v: i for i, v in enumerate(order)}
+    par = np.full(nxt, -1, np.int64)
+    for c, p in parent.items():
+        par[idx[c]] = idx[p]
+    leaves = np.array([idx[t] for t in range(ntip)], np.int64)
+    return par, leaves
+
+
+@pytest.mark.parametrize("ntip,multi", [(2, False), (3, False), (37, False), (500, False), (64, True), (333, True)])
+def test_clade_bitsets_walk(orc, ntip, multi):
+    rng = np.random.default_rng(ntip)
+    parent, leaves = _random_multifurcating(rng, ntip) if multi else synth.yule_parent(ntip, rng)
+    edge, nnode, nt = synth.ape_edges(parent, leaves)
+    rows, sizes = orc.clade_bitsets(edge, nnode, nt)
+    walk = _clades_by_walk(edge, nnode, nt)
+    internal_children = [int(d) for a, d in edge[::-1] if d > nt]   # backward scan order
+    assert rows.shape[0] == nnode - 1 == len(internal_children)
+    for k, v in enumerate(internal_children):
+        assert np.array_equal(rows[k], walk[v]), k
+    pop = synth.unpack_bits(walk).sum(1)
+    # the reference's return vector: internal node nnode-1-k (0-based), k < nnode-1
+    assert [int(x) for x in sizes] == [int(pop[nt + 1 + (nnode - 1 - k)]) for k in range(nnode - 1)]
