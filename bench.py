@@ -210,6 +210,13 @@ def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
                             "bipartition_pairs_per_s": pairs / batch_s,
                             "mfma_tops": 2 * ntip * pairs / batch_s / 1e12}
     del boots, allb
+    # the same job from trees: R edge matrices in, clade bitsets built on the device
+    trng = np.random.default_rng(14)
+    ref_e, ref_n, _ = synth.ape_edges(parent, leaves)
+    reps = [synth.ape_edges(synth.spr(parent, leaves, moves, trng), leaves)[:2] for _ in range(4)]
+    cp.transfer_trees(ref_e, ref_n, [e for e, _ in reps], [n for _, n in reps], ntip)
+    tree_s, tt = timed(lambda: cp.transfer_trees(ref_e, ref_n, [e for e, _ in reps], [n for _, n in reps], ntip), 1)
+    ex["tbe_65536_tips"]["from_trees_jobs_per_s"] = len(reps) / tree_s
     # small bootstrap trees, where one replicate does not fill the GPU: 4,096 tips x 64 replicates
     ntip = 4096
     parent, leaves = synth.yule_parent(ntip, rng)

@@ -41,6 +41,7 @@ SIGNATURES = [
     ("casphy_tip_merge", C.c_int, [vp, vp, u32, C.c_double, vp]),
     ("casphy_node_keep_bit", C.c_int, [vp, u32, vp, u32, u32, vp, u32, u32, vp, C.POINTER(u32)]),
     ("casphy_clade_bitsets", C.c_int, [vp, vp, u32, u32, u32, vp, C.POINTER(u32), vp]),
+    ("casphy_transfer_trees", C.c_int, [vp, vp, u32, u32, vp, vp, vp, vp, u32, u32, vp, vp, vp, vp]),
     ("casphy_node_keep_indel", C.c_int, [vp, u32, vp, u32, u32, vp, u32, u32, vp, C.POINTER(u32)]),
     ("casphy_cells_create", C.c_int, [C.c_int, vp, u32, u32, C.c_int, vp, C.POINTER(vp)]),
     ("casphy_cells_destroy", None, [vp]),

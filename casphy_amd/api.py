@@ -32,7 +32,7 @@ __all__ = [
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
-    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep", "clade_bitsets",
+    "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep", "clade_bitsets", "transfer_trees",
 ]
 
 
@@ -310,6 +310,33 @@ def clade_bitsets(edge, nnode: int, ntip: int):
     check(load().casphy_clade_bitsets(_p(anc), _p(des), e.shape[0], int(nnode), int(ntip), _p(rows), C.byref(nr),
                                       _p(sizes)))
     return rows[: nr.value], sizes[: max(nnode - 1, 0)]
+
+
+def transfer_trees(ref_edge, ref_nnode: int, rep_edges, rep_nnodes, ntip: int):
+    """Transfer distances of a reference tree against replicate trees, all as
+    R 1-based edge matrices (cladewise): (ymin (nrep, ref_nnode-1), ysize,
+    [xmin_b], [xsize_b]) over the trees' clade rows in nodeDesBig's order."""
+    ye = np.ascontiguousarray(ref_edge, dtype=np.int32)
+    ya, yd = np.ascontiguousarray(ye[:, 0]), np.ascontiguousarray(ye[:, 1])
+    reps = [np.ascontiguousarray(e, dtype=np.int32) for e in rep_edges]
+    nrep = len(reps)
+    eoff = np.zeros(nrep + 1, np.uint32)
+    eoff[1:] = np.cumsum([e.shape[0] for e in reps])
+    allx = np.concatenate(reps) if nrep else np.zeros((0, 2), np.int32)
+    xa, xd = np.ascontiguousarray(allx[:, 0]), np.ascontiguousarray(allx[:, 1])
+    xn = np.ascontiguousarray(rep_nnodes, dtype=np.uint32)
+    ny = int(ref_nnode) - 1
+    rows = [int(n) - 1 for n in rep_nnodes]
+    roff = np.zeros(nrep + 1, np.int64)
+    roff[1:] = np.cumsum(rows)
+    ymin = np.empty((nrep, ny), np.uint32)
+    ysize = np.empty(ny, np.uint32)
+    xmin = np.empty(max(int(roff[-1]), 1), np.uint32)
+    xsize = np.empty(max(int(roff[-1]), 1), np.uint32)
+    check(load().casphy_transfer_trees(_p(ya), _p(yd), ye.shape[0], int(ref_nnode), _p(xa) if xa.size else None,
+                                       _p(xd) if xd.size else None, _p(eoff), _p(xn), nrep, int(ntip), _p(ymin),
+                                       _p(ysize), _p(xmin), _p(xsize)))
+    return ymin, ysize, [xmin[roff[b]:roff[b + 1]] for b in range(nrep)], [xsize[roff[b]:roff[b + 1]] for b in range(nrep)]
 
 
 def node_keep(edge, node_state, outmat, cutoff: int, indel: bool = False) -> np.ndarray:

@@ -556,6 +556,70 @@ int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t 
     });
 }
 
+namespace {
+// Batched transfer distance on device bit rows: Y rows in my, the replicates'
+// rows back to back in mx (replicate b: rows xoffsets[b] .. xoffsets[b+1]),
+// minima starting from init = 32 * ncol; results to host buffers.
+void transfer_batch_rows(const BitMat &my, const BitMat *mx, const uint32_t *xoffsets, uint32_t nrep, uint32_t ntip,
+                         uint32_t init, uint32_t *ymin, uint32_t *ysize, uint32_t *xmin, uint32_t *xsize,
+                         hipStream_t s)
+{
+    const u32 ny = my.n, nx = nrep ? xoffsets[nrep] : 0;
+    // every replicate starts on a fresh 128-row tile so that a tile belongs to one replicate
+    std::vector<u32> rowidx, rowmap, rep_of_tile;
+    for (u32 b = 0; b < nrep; ++b) {
+        const u32 cnt = xoffsets[b + 1] - xoffsets[b];
+        const u32 padded = ceil_div(cnt, TILE) * TILE;
+        for (u32 r = 0; r < padded; ++r) {
+            rowidx.push_back(r < cnt ? xoffsets[b] + r : 0xFFFFFFFFu);
+            rowmap.push_back(r < cnt ? xoffsets[b] + r : 0xFFFFFFFFu);
+        }
+        for (u32 t = 0; t < padded / TILE; ++t) rep_of_tile.push_back(b);
+    }
+    const u32 npad = (u32)rowidx.size();
+    DevBuf dymin((size_t)nrep * ny * 4 + 4), dys((size_t)ny * 4), dxmin((size_t)std::max(nx, 1u) * 4),
+        dxs((size_t)std::max(nx, 1u) * 4), py((size_t)ny * 4), sy;
+    launch_fill_u32(dymin.as<u32>(), (u64)nrep * ny, init, s);
+    launch_fill_u32(dxmin.as<u32>(), nx, init, s);
+    launch_row_size(my.d.as<u32>(), ny, my.ldw, ntip, dys.as<u32>(), s);
+    if (nx) {
+        launch_row_size(mx->d.as<u32>(), nx, mx->ldw, ntip, dxs.as<u32>(), s);
+        DevBuf didx((size_t)npad * 4), dmap((size_t)npad * 4), drep(rep_of_tile.size() * 4),
+            sx((size_t)stage_rows(npad) * mx->ldw * 4), px((size_t)npad * 4);
+        h2d(didx.p, rowidx.data(), (size_t)npad * 4, s);
+        h2d(dmap.p, rowmap.data(), (size_t)npad * 4, s);
+        h2d(drep.p, rep_of_tile.data(), rep_of_tile.size() * 4, s);
+        launch_to_stage_major(mx->d.as<u32>(), npad, mx->ldw, mx->ldw, sx.as<u32>(), s, didx.as<u32>());
+        make_stage(my.d.as<u32>(), ny, my.ldw, sy, s);
+        launch_stage_popc(sx.as<u32>(), npad, stage_rows(npad), mx->ldw, px.as<u32>(), s);  // padding rows: 0
+        launch_row_popc(my.d.as<u32>(), ny, my.ldw, my.ldw, py.as<u32>(), s);
+        MfmaArgs q;
+        q.PA = sx.as<u32>(); q.PB = sy.as<u32>();
+        q.padA = stage_rows(npad); q.padB = stage_rows(ny);
+        q.stages = mx->ldw / 4; q.nA = npad; q.nB = ny;
+        q.triangle = 0;
+        q.ntip = ntip; q.popA = px.as<u32>(); q.popB = py.as<u32>();
+        q.amin = dxmin.as<u32>(); q.bmin = dymin.as<u32>();
+        q.arow_map = dmap.as<u32>(); q.tile_rep = drep.as<u32>(); q.bstride = ny;
+        launch_mfma_tiles(E_TBE, q, s);
+        d2h(xmin, dxmin.p, (size_t)nx * 4, s);
+        d2h(xsize, dxs.p, (size_t)nx * 4, s);
+    }
+    d2h(ymin, dymin.p, (size_t)nrep * ny * 4, s);
+    d2h(ysize, dys.p, (size_t)ny * 4, s);
+}
+
+// Clade rows of a tree in a BitMat (rows of ldw words, zero padding).
+BitMat clade_mat(const CladePlan &p, hipStream_t s)
+{
+    BitMat m;
+    m.n = (u32)p.lo.size(); m.W = p.nbin; m.ldw = round_up(std::max<u32>(p.nbin, 1), KC_BIT);
+    m.d.alloc(std::max<size_t>((size_t)m.n * m.ldw * 4, 4));
+    build_clade_rows(p, m.d.as<u32>(), m.ldw, s);
+    return m;
+}
+}  // namespace
+
 int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, const uint32_t *xoffsets, uint32_t nrep,
                           uint32_t ncol, uint32_t ntip, uint32_t *ymin, uint32_t *ysize, uint32_t *xmin,
                           uint32_t *xsize)
@@ -569,51 +633,48 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, con
         const u32 nx = nrep ? xoffsets[nrep] : 0;
         require(nx == 0 || (xmin && xsize), "casphy_transfer_batch: bad arguments");
         hipStream_t s = thread_stream();
-        const u32 init = 32u * ncol;
-        // every replicate starts on a fresh 128-row tile so that a tile belongs to one replicate
-        std::vector<u32> rowidx, rowmap, rep_of_tile;
-        for (u32 b = 0; b < nrep; ++b) {
-            const u32 cnt = xoffsets[b + 1] - xoffsets[b];
-            const u32 padded = ceil_div(cnt, TILE) * TILE;
-            for (u32 r = 0; r < padded; ++r) {
-                rowidx.push_back(r < cnt ? xoffsets[b] + r : 0xFFFFFFFFu);
-                rowmap.push_back(r < cnt ? xoffsets[b] + r : 0xFFFFFFFFu);
-            }
-            for (u32 t = 0; t < padded / TILE; ++t) rep_of_tile.push_back(b);
-        }
-        const u32 npad = (u32)rowidx.size();
         BitMat my = upload_bit(Y, ncol, ny, false, s);
-        DevBuf dymin((size_t)nrep * ny * 4 + 4), dys((size_t)ny * 4), dxmin((size_t)std::max(nx, 1u) * 4),
-            dxs((size_t)std::max(nx, 1u) * 4), py((size_t)ny * 4), sy;
-        launch_fill_u32(dymin.as<u32>(), (u64)nrep * ny, init, s);
-        launch_fill_u32(dxmin.as<u32>(), nx, init, s);
-        launch_row_size(my.d.as<u32>(), ny, my.ldw, ntip, dys.as<u32>(), s);
         if (nx) {
             BitMat mx = upload_bit(X, ncol, nx, false, s);
-            launch_row_size(mx.d.as<u32>(), nx, mx.ldw, ntip, dxs.as<u32>(), s);
-            DevBuf didx((size_t)npad * 4), dmap((size_t)npad * 4), drep(rep_of_tile.size() * 4),
-                sx((size_t)stage_rows(npad) * mx.ldw * 4), px((size_t)npad * 4);
-            h2d(didx.p, rowidx.data(), (size_t)npad * 4, s);
-            h2d(dmap.p, rowmap.data(), (size_t)npad * 4, s);
-            h2d(drep.p, rep_of_tile.data(), rep_of_tile.size() * 4, s);
-            launch_to_stage_major(mx.d.as<u32>(), npad, mx.ldw, mx.ldw, sx.as<u32>(), s, didx.as<u32>());
-            make_stage(my.d.as<u32>(), ny, my.ldw, sy, s);
-            launch_stage_popc(sx.as<u32>(), npad, stage_rows(npad), mx.ldw, px.as<u32>(), s);  // padding rows: 0
-            launch_row_popc(my.d.as<u32>(), ny, my.ldw, my.ldw, py.as<u32>(), s);
-            MfmaArgs q;
-            q.PA = sx.as<u32>(); q.PB = sy.as<u32>();
-            q.padA = stage_rows(npad); q.padB = stage_rows(ny);
-            q.stages = mx.ldw / 4; q.nA = npad; q.nB = ny;
-            q.triangle = 0;
-            q.ntip = ntip; q.popA = px.as<u32>(); q.popB = py.as<u32>();
-            q.amin = dxmin.as<u32>(); q.bmin = dymin.as<u32>();
-            q.arow_map = dmap.as<u32>(); q.tile_rep = drep.as<u32>(); q.bstride = ny;
-            launch_mfma_tiles(E_TBE, q, s);
-            d2h(xmin, dxmin.p, (size_t)nx * 4, s);
-            d2h(xsize, dxs.p, (size_t)nx * 4, s);
+            transfer_batch_rows(my, &mx, xoffsets, nrep, ntip, 32u * ncol, ymin, ysize, xmin, xsize, s);
+        } else {
+            transfer_batch_rows(my, nullptr, xoffsets, nrep, ntip, 32u * ncol, ymin, ysize, xmin, xsize, s);
         }
-        d2h(ymin, dymin.p, (size_t)nrep * ny * 4, s);
-        d2h(ysize, dys.p, (size_t)ny * 4, s);
+    });
+}
+
+int casphy_transfer_trees(const int32_t *y_anc, const int32_t *y_des, uint32_t y_nedge, uint32_t y_nnode,
+                          const int32_t *x_anc, const int32_t *x_des, const uint32_t *x_edge_offsets,
+                          const uint32_t *x_nnode, uint32_t nrep, uint32_t ntip, uint32_t *ymin, uint32_t *ysize,
+                          uint32_t *xmin, uint32_t *xsize)
+{
+    return guarded([&] {
+        require(y_anc && y_des && ymin && ysize && (nrep == 0 || (x_anc && x_des && x_edge_offsets && x_nnode)),
+                "casphy_transfer_trees: bad arguments");
+        require(y_nnode >= 2, "casphy_transfer_trees: the reference tree needs a non-root internal node");
+        hipStream_t s = thread_stream();
+        const CladePlan py = plan_clades(y_anc, y_des, y_nedge, y_nnode, ntip);
+        std::vector<CladePlan> px(nrep);
+        std::vector<u32> xoff(nrep + 1, 0);
+        for (u32 b = 0; b < nrep; ++b) {
+            require(x_edge_offsets[b + 1] >= x_edge_offsets[b], "casphy_transfer_trees: edge offsets must ascend");
+            px[b] = plan_clades(x_anc + x_edge_offsets[b], x_des + x_edge_offsets[b],
+                                x_edge_offsets[b + 1] - x_edge_offsets[b], x_nnode[b], ntip);
+            xoff[b + 1] = xoff[b] + (u32)px[b].lo.size();
+        }
+        require(xoff[nrep] == 0 || (xmin && xsize), "casphy_transfer_trees: bad arguments");
+        BitMat my = clade_mat(py, s);
+        require(my.n > 0, "casphy_transfer_trees: the reference tree has no clade rows");
+        if (xoff[nrep]) {
+            BitMat mx;
+            mx.n = xoff[nrep]; mx.W = py.nbin; mx.ldw = my.ldw;
+            mx.d.alloc((size_t)mx.n * mx.ldw * 4);
+            for (u32 b = 0; b < nrep; ++b)
+                build_clade_rows(px[b], mx.d.as<u32>() + (size_t)xoff[b] * mx.ldw, mx.ldw, s);
+            transfer_batch_rows(my, &mx, xoff.data(), nrep, ntip, 32u * py.nbin, ymin, ysize, xmin, xsize, s);
+        } else {
+            transfer_batch_rows(my, nullptr, xoff.data(), nrep, ntip, 32u * py.nbin, ymin, ysize, xmin, xsize, s);
+        }
     });
 }
 
@@ -701,7 +762,7 @@ int casphy_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge,
         if (!nr) return;
         hipStream_t s = thread_stream();
         DevBuf drows((size_t)nr * p.nbin * 4);
-        build_clade_rows(p, drows.as<u32>(), s);
+        build_clade_rows(p, drows.as<u32>(), p.nbin, s);
         d2h(rows, drows.p, (size_t)nr * p.nbin * 4, s);
     });
 }

@@ -21,15 +21,16 @@ namespace casphy {
 
 namespace {
 
-// Row r: the bits of tips order[lo[r] .. lo[r] + cnt[r]) (nbin words; LDS
-// row of nbin words when `lds`, else atomics on the pre-zeroed global row).
+// Row r (at rows + r * stride): the bits of tips order[lo[r] .. lo[r] +
+// cnt[r]) in nbin words, built in an LDS row when LDS, else by atomics on the
+// pre-zeroed global row.
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_clade_rows(const u32 *order, const u32 *lo, const u32 *cnt, u32 nbin,
-                                                    u32 *rows)
+                                                    u32 stride, u32 *rows)
 {
     extern __shared__ u32 row[];
     const u32 r = blockIdx.x, tid = threadIdx.x;
-    u32 *dst = rows + (size_t)r * nbin;
+    u32 *dst = rows + (size_t)r * stride;
     if (LDS) {
         for (u32 w = tid; w < nbin; w += 256) row[w] = 0u;
         __syncthreads();
@@ -128,22 +129,23 @@ CladePlan plan_clades(const int32_t *anc, const int32_t *des, u32 nedge, u32 nno
     return p;
 }
 
-void build_clade_rows(const CladePlan &p, u32 *rows, hipStream_t s)
+void build_clade_rows(const CladePlan &p, u32 *rows, u32 stride, hipStream_t s)
 {
     const u32 nrow = (u32)p.lo.size();
     if (!nrow) return;
+    CASPHY_REQUIRE(stride >= p.nbin, CASPHY_EINTERNAL, "clade rows: stride below the row width");
     DevBuf dord(std::max<size_t>(p.order.size(), 1) * 4), dlo((size_t)nrow * 4), dcnt((size_t)nrow * 4);
     if (!p.order.empty()) h2d(dord.p, p.order.data(), p.order.size() * 4, s);
     h2d(dlo.p, p.lo.data(), (size_t)nrow * 4, s);
     h2d(dcnt.p, p.cnt.data(), (size_t)nrow * 4, s);
-    if (p.nbin <= LDS_ROW_MAX) {
+    const bool lds = p.nbin <= LDS_ROW_MAX;
+    if (!lds || stride > p.nbin) launch_fill_u32(rows, (u64)nrow * stride, 0u, s);  // atomics / padding words
+    if (lds)
         hipLaunchKernelGGL(k_clade_rows<true>, dim3(nrow), dim3(256), (size_t)p.nbin * 4, s, dord.as<u32>(),
-                           dlo.as<u32>(), dcnt.as<u32>(), p.nbin, rows);
-    } else {
-        launch_fill_u32(rows, (u64)nrow * p.nbin, 0u, s);
+                           dlo.as<u32>(), dcnt.as<u32>(), p.nbin, stride, rows);
+    else
         hipLaunchKernelGGL(k_clade_rows<false>, dim3(nrow), dim3(256), 0, s, dord.as<u32>(), dlo.as<u32>(),
-                           dcnt.as<u32>(), p.nbin, rows);
-    }
+                           dcnt.as<u32>(), p.nbin, stride, rows);
     CASPHY_HIP(hipGetLastError());
 }
 

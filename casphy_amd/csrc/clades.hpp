@@ -19,7 +19,8 @@ struct CladePlan {
 // anc/des: R's 1-based edge matrix columns (tips 1..ntip, internal nodes
 // ntip+1..ntip+nnode, one root, every node's edge before its children's).
 CladePlan plan_clades(const int32_t *anc, const int32_t *des, u32 nedge, u32 nnode, u32 ntip);
-// rows: device buffer of plan.lo.size() x plan.nbin words.
-void build_clade_rows(const CladePlan &p, u32 *rows, hipStream_t s);
+// rows: device buffer of plan.lo.size() rows of `stride` >= plan.nbin words
+// (words past nbin are zeroed).
+void build_clade_rows(const CladePlan &p, u32 *rows, u32 stride, hipStream_t s);
 
 }  // namespace casphy

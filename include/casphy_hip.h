@@ -235,6 +235,21 @@ int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t 
                            uint32_t ncol, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
                            uint32_t *nkept);
 
+/* Transfer distances from trees: the reference tree and nrep replicate trees
+ * as R edge-matrix columns (cladewise, as casphy_clade_bitsets; replicate b's
+ * edges at x_edge_offsets[b] .. x_edge_offsets[b+1], x_nnode[b] internal
+ * nodes, all trees on the same ntip tips).  The clade bitsets of every tree
+ * are built on the device (rows in nodeDesBig's order) and scored as
+ * casphy_transfer_batch with Y = the reference's rows and X = the replicates'
+ * rows: ymin[b * ny + y], ysize[ny], xmin / xsize per replicate row, where
+ * ny = y_nnode - 1.  No counterpart in the reference, where R writes the
+ * bitsets of each tree with nodeDesBig (R/bootstrap_functions.R:3) before
+ * TransferDisStream reads them back (cu/TransferDisStream.cu:119-160). */
+int casphy_transfer_trees(const int32_t *y_anc, const int32_t *y_des, uint32_t y_nedge, uint32_t y_nnode,
+                          const int32_t *x_anc, const int32_t *x_des, const uint32_t *x_edge_offsets,
+                          const uint32_t *x_nnode, uint32_t nrep, uint32_t ntip, uint32_t *ymin, uint32_t *ysize,
+                          uint32_t *xmin, uint32_t *xsize);
+
 /* ------------------------------------------------------------------ resident */
 /* A packed cell matrix kept in HBM across calls (the serving / tuning form: one
  * upload, many radius and tryRadius passes).  kind: CASPHY_KIND_BIT (bit-packed

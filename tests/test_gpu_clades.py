@@ -43,3 +43,25 @@ def test_clade_bitsets_caterpillar_and_star(hip, orc):
     rows, sizes = hip.clade_bitsets(edge, nnode, nt)
     r2, s2 = orc.clade_bitsets(edge, nnode, nt)
     assert rows.shape == (2, 18750) and np.array_equal(rows, r2) and np.array_equal(sizes, s2)
+
+
+def test_transfer_trees(hip, orc):
+    """Trees in, transfer distances out: equal to the batched transfer over the
+    trees' clade bitsets and to the oracle's transfer on the oracle's bitsets."""
+    rng = np.random.default_rng(44)
+    ntip = 3000
+    parent, leaves = synth.yule_parent(ntip, rng)
+    ref, rn, _ = synth.ape_edges(parent, leaves)
+    reps = []
+    for k in (5, 60, 300):
+        e, n, _ = synth.ape_edges(synth.spr(parent, leaves, k, rng), leaves)
+        reps.append((e, n))
+    ymin, ysize, xm, xs = hip.transfer_trees(ref, rn, [e for e, _ in reps], [n for _, n in reps], ntip)
+    Y, _ = orc.clade_bitsets(ref, rn, ntip)
+    Xs = [orc.clade_bitsets(e, n, ntip)[0] for e, n in reps]
+    yb, ysb, xmb, xsb = hip.transfer_dis_batch(Y, Xs, ntip)
+    assert np.array_equal(ymin, yb) and np.array_equal(ysize, ysb)
+    for b, X in enumerate(Xs):
+        assert np.array_equal(xm[b], xmb[b]) and np.array_equal(xs[b], xsb[b])
+        y2, ys2, x2, xs2 = orc.transfer(Y, X, ntip)
+        assert np.array_equal(ymin[b], y2) and np.array_equal(xm[b], x2) and np.array_equal(xs[b], xs2)
