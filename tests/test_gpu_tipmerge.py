@@ -15,11 +15,17 @@ def form(request, monkeypatch):
     return request.param
 
 
-@pytest.mark.parametrize("n,r", [(1, 1), (2, 1), (5, 1), (300, 2), (1500, 4), (2500, 1), (3001, 3)])
-def test_tip_merge_radius_lists(hip, orc, form, n, r):
+ALL_P = (0.0, 0.3, 0.5, 0.8, 1.0)
+
+
+# list lengths: nearly all tips (1,200 at radius 1), about 500 of 1,500 at
+# radius 12, about 370 of 3,001 at radius 24
+@pytest.mark.parametrize("n,r,ps", [(1, 1, ALL_P), (2, 1, ALL_P), (5, 1, ALL_P), (300, 2, ALL_P),
+                                    (1200, 1, (0.3, 0.8)), (1500, 12, (0.0, 0.5, 0.8)), (3001, 24, (0.8,))])
+def test_tip_merge_radius_lists(hip, orc, form, n, r, ps):
     mat = synth.binary_matrix(seed=n, n=n, nsite=512, p=0.01)
     counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), r)
-    for p in (0.0, 0.3, 0.5, 0.8, 1.0):
+    for p in ps:
         assert np.array_equal(hip.tip_merge(counts, idx, p), orc.tip_merge(counts, idx, p)), (n, r, p)
 
 
@@ -55,7 +61,7 @@ def test_cells_tip_merge(hip, orc):
     """On resident cells the lists and the radius mask stay on the device."""
     mat = synth.binary_matrix(seed=41, n=2000, nsite=1024, p=0.005)
     cells = hip.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
-    for r in (1, 3, 6):
+    for r in (15, 30, 60):
         cells.radius(r)
         counts, idx = cells.fetch()
         for p in (0.2, 0.8):
