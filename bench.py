@@ -189,6 +189,9 @@ def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
                                    "cell_pair_scores_per_s": npairs / rad_s, "input_gen_s": gen}
     dcells.close()
     print("# DNA done", file=sys.stderr, flush=True)
+    ex["dna_vote_root_10000_tips"] = vote_root_step(cp, synth, dna, with_cpu)
+    ex["dna_fasta_encode_100000x1000"] = fasta_step(cp, dna, with_cpu)
+    print("# DNA format edge done", file=sys.stderr, flush=True)
 
     # transfer distance: 65,536-tip Yule reference vs SPR-perturbed bootstrap trees
     rng = np.random.default_rng(4)
@@ -288,6 +291,52 @@ def tip_merge_step(cp, cells, q, with_cpu: bool) -> dict:
                                "sample": f"the full {n}-tip workload, {cpu_s:.2f} s"}
         out["speedup_vs_cpu"] = cpu_s / dev_s
     return out
+
+
+VOTE_TIPS = 10000      # voteRoot2: tree on 10,000 sequences of the DNA alignment
+VOTE_SAMPLES = 16384   # outgroup samples, R's floor (R/DNA_functions.R:108)
+
+
+def vote_root_step(cp, synth, dna, with_cpu: bool) -> dict:
+    """voteRoot2 on the 100,000 x 1,000 DNA alignment: a Yule tree on its
+    first 10,000 sequences with Fitch states, 16,384 further sequences as
+    outgroup samples (R/DNA_functions.R:107-133); a host-API pass (upload,
+    features, matrix-core counts with the argmax epilogue, votes back)."""
+    st, edge, mat, idx = synth.vote_inputs(3, VOTE_TIPS, VOTE_SAMPLES, 1000, mat=dna)
+    cp.vote_root(st, edge, mat, idx)  # first call of the shape class: autotuner
+    dev_s, (root, nsh) = timed(lambda: cp.vote_root(st, edge, mat, idx), 3)
+    nnode, nbin = st.shape
+    pairs = nnode * VOTE_SAMPLES
+    res = {"nodes": int(nnode), "samples": VOTE_SAMPLES, "positions": 4 * int(nbin), "ms": dev_s * 1e3,
+           "node_sample_pairs_per_s": pairs / dev_s, "position_pairs_per_s": pairs * 4 * nbin / dev_s,
+           "distinct_roots": int(np.unique(root).size)}
+    if with_cpu:
+        from oracle import oracle
+        k = 1024
+        cpu_s, (r2, n2) = timed(lambda: oracle.vote_root(st, edge, mat, idx[:k]), 1)
+        assert np.array_equal(r2, root[:k]) and np.array_equal(n2, nsh[:k]), "CPU/GPU voteRoot2 mismatch"
+        res["cpu_baseline"] = {"value": nnode * k / cpu_s, "unit": "node-sample pairs/s", "cores": 1,
+                               "kind": "port", "sample": f"{k} of the {VOTE_SAMPLES} samples, {cpu_s:.2f} s"}
+    return res
+
+
+def fasta_step(cp, dna, with_cpu: bool) -> dict:
+    """fasta2bin's column filter and encoding of the 100,000 x 1,000 DNA
+    alignment as characters (host-API pass: characters up, codes back)."""
+    b = dna.view(np.uint8).reshape(dna.shape[0], -1)[:, :1000]
+    chars = np.frombuffer(b"?ATTCCCCGGGGGGGG-", np.uint8)
+    seqs = np.ascontiguousarray(chars[b])
+    cp.fasta_encode(seqs[:64])
+    dev_s, (var, valid, out) = timed(lambda: cp.fasta_encode(seqs), 3)
+    res = {"sequences": int(seqs.shape[0]), "columns": int(seqs.shape[1]), "kept": int(var.sum()),
+           "ms": dev_s * 1e3, "characters_per_s": seqs.size / dev_s}
+    if with_cpu:
+        from oracle import oracle
+        cpu_s, w = timed(lambda: oracle.fasta_encode(seqs), 1)
+        assert all(np.array_equal(x, y) for x, y in zip((var, valid, out), w)), "CPU/GPU fasta2bin mismatch"
+        res["cpu_baseline"] = {"value": seqs.size / cpu_s, "unit": "characters/s", "cores": 1, "kind": "port",
+                               "sample": f"the full alignment, {cpu_s:.2f} s"}
+    return res
 
 
 NODEKEEP_OUT = 1000    # outgroup rows (cells of the matrix)
@@ -461,8 +510,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
-    ap.add_argument("--only", choices=["tipmerge", "nodekeep"],
-                    help="run only the tipMerge split step or the nodeKeep step (profiling)")
+    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta"],
+                    help="run only the tipMerge split step, the nodeKeep step, the voteRoot2 pass or the "
+                         "fasta2bin encoding (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -486,6 +536,12 @@ def main():
         return
     if args.only == "nodekeep":
         print(json.dumps(node_keep_step(cp, mat, not args.no_cpu)))
+        return
+    if args.only == "voteroot":
+        print(json.dumps(vote_root_step(cp, synth, synth.dna_matrix(seed=3, n=100000, ncol=1000), not args.no_cpu)))
+        return
+    if args.only == "fasta":
+        print(json.dumps(fasta_step(cp, synth.dna_matrix(seed=3, n=100000, ncol=1000), not args.no_cpu)))
         return
     bound = AUTOSPLIT_BOUND - np.log10(N)
 

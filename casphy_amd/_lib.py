@@ -43,6 +43,9 @@ SIGNATURES = [
     ("casphy_clade_bitsets", C.c_int, [vp, vp, u32, u32, u32, vp, C.POINTER(u32), vp]),
     ("casphy_transfer_trees", C.c_int, [vp, vp, u32, u32, vp, vp, vp, vp, u32, u32, vp, vp, vp, vp]),
     ("casphy_node_keep_indel", C.c_int, [vp, u32, vp, u32, u32, vp, u32, u32, vp, C.POINTER(u32)]),
+    ("casphy_vote_root", C.c_int, [vp, u32, u32, vp, u32, vp, u64, vp, u32, vp, vp]),
+    ("casphy_find_root", C.c_int, [vp, u32, u32, vp, u32, vp, u64, vp, u32, vp, vp]),
+    ("casphy_fasta_encode", C.c_int, [vp, u32, u64, vp, vp, vp, C.POINTER(u64)]),
     ("casphy_cells_create", C.c_int, [C.c_int, vp, u32, u32, C.c_int, vp, C.POINTER(vp)]),
     ("casphy_cells_destroy", None, [vp]),
     ("casphy_cells_radius", C.c_int, [vp, i32, C.POINTER(u64)]),

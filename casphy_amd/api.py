@@ -21,6 +21,7 @@ Errors raise CasphyError (the status code and message of the C ABI).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -33,6 +34,7 @@ __all__ = [
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
     "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep", "clade_bitsets", "transfer_trees",
+    "vote_root", "find_root", "findRootAVX2Big", "fasta_encode", "fasta2bin",
 ]
 
 
@@ -354,6 +356,79 @@ def node_keep(edge, node_state, outmat, cutoff: int, indel: bool = False) -> np.
     check(fn(_p(e) if e.size else None, e.shape[0] if e.ndim == 2 else 0, _p(st) if st.size else None, nnode, width,
              _p(om) if om.size else None, nout, int(cutoff), _p(kept), C.byref(nk)))
     return kept[:nk.value]
+
+
+# ------------------------------------------------------------------ DNA format edge
+def _root_vote(fn, node_state, edge, mat, idx, dtype):
+    st = _u32(node_state)
+    if st.ndim != 2:
+        raise CasphyError(-1, "node_state must be (nnode, nbin)")
+    nnode, nbin = st.shape
+    e = _u32(edge).reshape(-1, 2) if np.size(edge) else np.zeros((0, 2), np.uint32)
+    m = _u32(mat).reshape(-1, nbin) if nbin else np.zeros((0, 0), np.uint32)
+    nmat = m.shape[0] if nbin else int(np.asarray(mat).shape[0])
+    ix = _u32(idx).reshape(-1)
+    ns = ix.size
+    root, nsh = np.empty(max(ns, 1), dtype), np.empty(max(ns, 1), dtype)
+    check(fn(_p(st) if st.size else None, nnode, nbin, _p(e) if e.size else None, e.shape[0],
+             _p(m) if m.size else None, nmat, _p(ix) if ns else None, ns, _p(root), _p(nsh)))
+    return root[:ns], nsh[:ns]
+
+
+def vote_root(node_state, edge, mat, idx):
+    """voteRoot2 (cu/voteRoot2.cu): node_state (nnode, nbin) u32 byte codes,
+    edge (nedge, 2) 1-based (parent, child), samples mat[idx]; per sample the
+    0-based node of the most shared positions and that count (.rootVote)."""
+    return _root_vote(load().casphy_vote_root, node_state, edge, mat, idx, np.uint32)
+
+
+def find_root(node_state, edge, mat, idx):
+    """findRoot / findRootAVX2Big (src/functions.cpp:4170,4317) on the same
+    inputs: per sample the 1-based node (first edge child in edge order among
+    the largest counts) and the count, int32."""
+    return _root_vote(load().casphy_find_root, node_state, edge, mat, idx, np.int32)
+
+
+def findRootAVX2Big(edge, nodeState, mat, outIdx):
+    """R-shaped twin: edge (nedge, 2) 1-based, nodeState (nPositions, nNodes)
+    and mat (nPositions, nmat) column-major as R holds them, outIdx 0-based;
+    returns the (nSamples, 2) int32 matrix [rootFound, nShared]."""
+    st = np.ascontiguousarray((np.asarray(nodeState).astype(np.int64) & 0xFFFFFFFF).T).astype(np.uint32)
+    m = np.ascontiguousarray((np.asarray(mat).astype(np.int64) & 0xFFFFFFFF).T).astype(np.uint32)
+    root, nsh = find_root(st, np.asarray(edge, dtype=np.int64).astype(np.uint32), m, np.asarray(outIdx))
+    return np.stack([root, nsh], axis=1)
+
+
+def fasta_encode(seqs):
+    """fasta2bin's encoding of equal-length sequences, an (nseq, len) uint8
+    array or a list of bytes: (is_variable bool (len,), valid bases (nseq,)
+    u32, codes (nseq, width) u32 -- the .bin payload)."""
+    if not isinstance(seqs, np.ndarray):
+        seqs = [bytes(x) for x in seqs]
+        if any(len(x) != len(seqs[0]) for x in seqs):
+            raise CasphyError(-1, "All sequences must have the same length for variant filtering")
+        seqs = np.frombuffer(b"".join(seqs), np.uint8).reshape(len(seqs), -1) if seqs else np.zeros((0, 0), np.uint8)
+    a = np.ascontiguousarray(seqs, dtype=np.uint8)
+    nseq, length = a.shape
+    isv = np.empty(max(length, 1), np.uint8)
+    valid = np.empty(max(nseq, 1), np.uint32)
+    out = np.empty(max(nseq * ((length + 3) // 4), 1), np.uint32)
+    width = C.c_uint64(0)
+    check(load().casphy_fasta_encode(_p(a) if a.size else None, nseq, length, _p(isv), _p(valid), _p(out),
+                                     C.byref(width)))
+    w = width.value
+    return isv[:length].astype(bool), valid[:nseq], out[: nseq * w].reshape(nseq, w)
+
+
+def fasta2bin(fasta_path: str, prefix: str) -> None:
+    """The fasta2bin executable (cu/fasta2bin.cu:256-363) as a call: runs
+    casphy_amd/bin/fasta2bin, which writes prefix.bin, prefix_names.txt and
+    prefix_positions.txt."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bin", "fasta2bin")
+    r = subprocess.run([exe, fasta_path, prefix], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise CasphyError(r.returncode, (r.stderr or r.stdout).strip())
 
 
 # ------------------------------------------------------------------ Rcpp twins

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -15,6 +16,8 @@
 #include "kernels.hpp"
 #include "mfma.hpp"
 #include "clades.hpp"
+#include "fasta.hpp"
+#include "voteroot.hpp"
 #include "nodekeep.hpp"
 #include "tipmerge.hpp"
 #include "tiles.hpp"
@@ -747,6 +750,81 @@ int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t 
 {
     return node_keep_entry(true, "casphy_node_keep_indel", edge, nedge, node_state, nnode, ncol, outmat, nout,
                            cutoff, kept, nkept);
+}
+
+namespace {
+// Checks a root-vote call and returns the sample rows mat[idx[s]], gathered.
+std::vector<u32> root_vote_samples(const char *fn, const uint32_t *state, uint32_t nnode, uint32_t nbin,
+                                   const uint32_t *edge, uint32_t nedge, const uint32_t *mat, uint64_t nmat,
+                                   const uint32_t *idx, uint32_t nsample, const void *root, const void *nshared)
+{
+    const std::string f(fn);
+    CASPHY_REQUIRE(nnode >= 1, CASPHY_EARG, f + ": the tree has no node");
+    CASPHY_REQUIRE(((size_t)nnode * nbin == 0 || state) && (nedge == 0 || edge), CASPHY_EARG, f + ": bad arguments");
+    CASPHY_REQUIRE(nsample == 0 || (idx && root && nshared && (nbin == 0 || mat)), CASPHY_EARG,
+                   f + ": bad arguments");
+    std::vector<u32> rows((size_t)nsample * nbin);
+    for (u32 k = 0; k < nsample; ++k) {
+        CASPHY_REQUIRE(idx[k] < nmat, CASPHY_EARG,
+                       f + ": sample " + std::to_string(k) + " names row " + std::to_string(idx[k]) + " of " +
+                           std::to_string(nmat));
+        if (nbin) std::memcpy(&rows[(size_t)k * nbin], mat + (size_t)idx[k] * nbin, (size_t)nbin * 4);
+    }
+    return rows;
+}
+}  // namespace
+
+int casphy_vote_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                     const uint32_t *mat, uint64_t nmat, const uint32_t *idx, uint32_t nsample, uint32_t *root,
+                     uint32_t *nshared)
+{
+    return guarded([&] {
+        const std::vector<u32> rows =
+            root_vote_samples("casphy_vote_root", state, nnode, nbin, edge, nedge, mat, nmat, idx, nsample, root,
+                              nshared);
+        const RootVote plan = plan_vote_voteroot2(edge, nedge, nnode);
+        std::vector<u64> keys(nsample);
+        vote_root_keys(plan, state, nnode, nbin, rows.data(), nsample, keys.data(), thread_stream());
+        for (u32 k = 0; k < nsample; ++k) {  // every node is a candidate: a key is never 0
+            root[k] = plan.candidates[0xFFFFFFFFu - (u32)keys[k]];
+            nshared[k] = (u32)(keys[k] >> 32);
+        }
+    });
+}
+
+int casphy_find_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                     const uint32_t *mat, uint64_t nmat, const uint32_t *idx, uint32_t nsample, int32_t *root,
+                     int32_t *nshared)
+{
+    return guarded([&] {
+        const std::vector<u32> rows =
+            root_vote_samples("casphy_find_root", state, nnode, nbin, edge, nedge, mat, nmat, idx, nsample, root,
+                              nshared);
+        const RootVote plan = plan_vote_findroot(edge, nedge, nnode);
+        std::vector<u64> keys(nsample);
+        vote_root_keys(plan, state, nnode, nbin, rows.data(), nsample, keys.data(), thread_stream());
+        for (u32 k = 0; k < nsample; ++k) {
+            if (!keys[k]) {  // no edge: the reference's starting values
+                root[k] = 0;
+                nshared[k] = -1;
+            } else {
+                root[k] = (int32_t)plan.candidates[0xFFFFFFFFu - (u32)keys[k]] + 1;
+                nshared[k] = (int32_t)(keys[k] >> 32);
+            }
+        }
+    });
+}
+
+int casphy_fasta_encode(const uint8_t *seqs, uint32_t nseq, uint64_t len, uint8_t *is_variable, uint32_t *valid,
+                        uint32_t *out, uint64_t *width)
+{
+    return guarded([&] {
+        require(width && (nseq == 0 || valid) && (len == 0 || is_variable) &&
+                    ((u64)nseq * len == 0 || (seqs && out)),
+                "casphy_fasta_encode: bad arguments");
+        if (!nseq && len) std::memset(is_variable, 0, len);
+        *width = fasta_encode(seqs, nseq, len, is_variable, valid, out, thread_stream());
+    });
 }
 
 int casphy_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge, uint32_t nnode, uint32_t ntip,

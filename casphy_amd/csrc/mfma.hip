@@ -233,6 +233,19 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
             }
             if (col == 0 && i < a.nA && key) atomicMax((unsigned long long *)&a.keys[i], key);
         }
+    } else if (EPI == E_ARGMAX) {
+        // per column: max over the A rows i of (S, smallest i); a lane holds 16
+        // rows of its column, the two halves meet in one exchange
+        unsigned long long key = 0;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const u32 i = r0 + 8 * (v / 4) + 4 * half + (v % 4);
+            const unsigned long long k = ((unsigned long long)(u32)acc[v] << 32) | (0xFFFFFFFFu - i);
+            key = (i < a.nA && k > key) ? k : key;
+        }
+        const unsigned long long q = __shfl_xor(key, 32);
+        key = q > key ? q : key;
+        if (half == 0 && j < a.nB && key) atomicMax((unsigned long long *)&a.keys[j], key);
     } else if (EPI == E_PAIRS) {
         // shared-neighbour counts of the linked pairs i < j, at their place in
         // the upper part of row i's list; the 32 columns of a block are one
@@ -1126,6 +1139,8 @@ void launch_variant(int epi, const MfmaArgs &a, hipStream_t s)
         hipLaunchKernelGGL((k_mfma_tiles<E_TBE, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
     else if (epi == E_PAIRS)
         hipLaunchKernelGGL((k_mfma_tiles<E_PAIRS, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
+    else if (epi == E_ARGMAX)
+        hipLaunchKernelGGL((k_mfma_tiles<E_ARGMAX, KB, PF, OCC>), dim3(grid), dim3(NTHR), 0, s, a);
     else
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }
@@ -1190,6 +1205,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
             hipLaunchKernelGGL(k_mfma_fp4w<E_TBE>, dim3(grid), dim3(512), 0, s, a);
         else if (epi == E_PAIRS)
             hipLaunchKernelGGL(k_mfma_fp4w<E_PAIRS>, dim3(grid), dim3(512), 0, s, a);
+        else if (epi == E_ARGMAX)
+            hipLaunchKernelGGL(k_mfma_fp4w<E_ARGMAX>, dim3(grid), dim3(512), 0, s, a);
         else
             throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     } else if (mode == 2) {
@@ -1203,6 +1220,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
             hipLaunchKernelGGL(k_mfma_fp4d<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
         else if (epi == E_PAIRS)
             hipLaunchKernelGGL(k_mfma_fp4d<E_PAIRS>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_ARGMAX)
+            hipLaunchKernelGGL(k_mfma_fp4d<E_ARGMAX>, dim3(grid), dim3(NTHR), 0, s, a);
         else
             throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     } else if (mode == 1) {
@@ -1216,6 +1235,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
             hipLaunchKernelGGL(k_mfma_fp4b<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
         else if (epi == E_PAIRS)
             hipLaunchKernelGGL(k_mfma_fp4b<E_PAIRS>, dim3(grid), dim3(NTHR), 0, s, a);
+        else if (epi == E_ARGMAX)
+            hipLaunchKernelGGL(k_mfma_fp4b<E_ARGMAX>, dim3(grid), dim3(NTHR), 0, s, a);
         else
             throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     } else if (epi == E_MASK) {
@@ -1228,6 +1249,8 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
         hipLaunchKernelGGL(k_mfma_fp4<E_TBE>, dim3(grid), dim3(NTHR), 0, s, a);
     } else if (epi == E_PAIRS) {
         hipLaunchKernelGGL(k_mfma_fp4<E_PAIRS>, dim3(grid), dim3(NTHR), 0, s, a);
+    } else if (epi == E_ARGMAX) {
+        hipLaunchKernelGGL(k_mfma_fp4<E_ARGMAX>, dim3(grid), dim3(NTHR), 0, s, a);
     } else {
         throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
     }
@@ -1246,6 +1269,8 @@ void launch_reg(int epi, const MfmaArgs &a, hipStream_t s)
         hipLaunchKernelGGL(k_mfma_reg<E_TBE>, dim3(grid), dim3(256), 0, s, a);
     else if (epi == E_PAIRS)
         hipLaunchKernelGGL(k_mfma_reg<E_PAIRS>, dim3(grid), dim3(256), 0, s, a);
+    else if (epi == E_ARGMAX)
+        hipLaunchKernelGGL(k_mfma_reg<E_ARGMAX>, dim3(grid), dim3(256), 0, s, a);
     else
         throw Error(CASPHY_EINTERNAL, "MFMA tiles: unsupported epilogue");
 }

@@ -29,7 +29,7 @@ struct MfmaArgs {
     u32 mstride = 0;
     int32_t *S = nullptr;     // E_SCORES
     u64 ldS = 0;
-    u64 *keys = nullptr;      // E_LOCALMAX
+    u64 *keys = nullptr;      // E_LOCALMAX (per A row), E_ARGMAX (per B row)
     u32 ntip = 0;             // E_TBE: H = popA + popB - 2 <a, b>
     const u32 *popA = nullptr, *popB = nullptr;
     u32 *amin = nullptr, *bmin = nullptr;

@@ -284,3 +284,39 @@ def ape_edges(parent: np.ndarray, leaves: np.ndarray):
             if kids[c]:
                 stack.append((c, 0))
     return np.array(edges, dtype=np.int32).reshape(-1, 2), int(nxt - ntip - 1), ntip
+
+
+# --------------------------------------------------------------- root votes
+def vote_inputs(seed: int, ntip: int, nsample: int, ncol: int, mat: np.ndarray | None = None):
+    """voteRoot2 inputs: a Yule tree on the first ntip rows of a JC69 alignment
+    (fasta2bin byte codes; `mat` or dna_matrix(seed, ntip + nsample, ncol)),
+    internal states by a Fitch pass over the byte codes (per position the
+    children's common bases, else their union), and the next nsample rows as
+    outgroup samples.  Returns (node_state (ntip + nnode, nbin) u32, edge
+    (nedge, 2) u32 1-based, mat, idx (nsample,) u32)."""
+    if mat is None:
+        mat = dna_matrix(seed=seed, n=ntip + nsample, ncol=ncol)
+    rng = np.random.default_rng(seed + 1)
+    parent, leaves = yule_parent(ntip, rng)
+    edge, nnode, ntip = ape_edges(parent, leaves)
+    nbin = mat.shape[1]
+    st = np.zeros((ntip + nnode, nbin), np.uint32)
+    st[:ntip] = mat[:ntip]
+    inter = np.zeros((ntip + nnode, nbin * 4), np.uint8)
+    union = np.zeros((ntip + nnode, nbin * 4), np.uint8)
+    seen = np.zeros(ntip + nnode, bool)
+    for a, d in edge[::-1]:  # children before parents
+        a, d = int(a) - 1, int(d) - 1
+        if d >= ntip:  # internal child complete: its Fitch set
+            i, u = inter[d], union[d]
+            st[d] = np.where((i & 0x0F) != 0, i, u).view(np.uint32)
+        b = st[d].view(np.uint8)
+        if seen[a]:
+            inter[a] &= b
+            union[a] |= b
+        else:
+            inter[a], union[a], seen[a] = b, b, True
+    r = ntip  # the root
+    st[r] = np.where((inter[r] & 0x0F) != 0, inter[r], union[r]).view(np.uint32)
+    idx = np.arange(ntip, ntip + nsample, dtype=np.uint32)
+    return st, edge.astype(np.uint32), mat, idx

@@ -235,6 +235,43 @@ int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t 
                            uint32_t ncol, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
                            uint32_t *nkept);
 
+/* ------------------------------------------------------------- DNA format edge */
+/* Root votes of outgroup samples: state nnode rows of nbin words of fasta2bin
+ * byte codes (4 per word; tips and internal nodes, as .nodeState),
+ * edge[nedge][2] 1-based (parent, child) as .edge, samples the rows idx[s]
+ * (< nmat) of mat (nmat rows of nbin words, the .bin matrix).  A node's edge
+ * state is the OR of its and its parent's rows (zero for a node that is no
+ * edge's child); a sample's count against it is the number of positions whose
+ * AND has a base bit (low nibble).  root[s] / nshared[s] receive the node of
+ * the largest count and the count: vote_root with voteRoot2's reduction order
+ * (0-based node; among the largest counts the first by (bit-reversed block of
+ * 8 mod 32, block / 32, bit-reversed node mod 8), node 0 when every count is
+ * 0), find_root with
+ * findRoot's (the first edge child in edge order; 1-based node, 0 and -1 with
+ * no edge).
+ * Replaces: cu/voteRoot2.cu:34-169,171-323 (argv: matfile prefix; called by
+ * R/DNA_functions.R:119-133) and findRoot / findRootAVX2Big
+ * src/functions.cpp:4170-4212,4317-4410 (called by R/DNA_functions.R:117). */
+int casphy_vote_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                     const uint32_t *mat, uint64_t nmat, const uint32_t *idx, uint32_t nsample, uint32_t *root,
+                     uint32_t *nshared);
+int casphy_find_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                     const uint32_t *mat, uint64_t nmat, const uint32_t *idx, uint32_t nsample, int32_t *root,
+                     int32_t *nshared);
+
+/* fasta2bin's encoding of an alignment: seqs nseq rows of len characters
+ * (row-major, the sequences of a FASTA file as read).  is_variable[len]
+ * receives 1 for the kept columns (some base -- A, T/U, C, G, counted with
+ * the IUPAC ambiguity codes' bases -- in some but not all sequences),
+ * valid[nseq] each sequence's characters other than '-', '.', 'N', 'n', out
+ * (room for nseq * ceil(len / 4) words) the kept columns as byte codes, 4 per
+ * word, *width the words per row (ceil(kept / 4)): the payload of the .bin
+ * file, whose header is [nseq, width].
+ * Replaces: cu/fasta2bin.cu:12-118,189-210 (main :256-363 reads the FASTA
+ * file and writes the .bin, _names.txt and _positions.txt files). */
+int casphy_fasta_encode(const uint8_t *seqs, uint32_t nseq, uint64_t len, uint8_t *is_variable, uint32_t *valid,
+                        uint32_t *out, uint64_t *width);
+
 /* Transfer distances from trees: the reference tree and nrep replicate trees
  * as R edge-matrix columns (cladewise, as casphy_clade_bitsets; replicate b's
  * edges at x_edge_offsets[b] .. x_edge_offsets[b+1], x_nnode[b] internal

@@ -822,3 +822,182 @@ uint32_t orc_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedg
     free(size);
     return nrow;
 }
+
+/* ---- (f-4) voteRoot2: root votes of outgroup samples ------------------------- */
+
+/* Positions (bytes) of a word pair whose AND has a nucleotide bit (low nibble). */
+static uint32_t vr_shared(uint32_t a, uint32_t b)
+{
+    const uint32_t v = a & b;
+    uint32_t c = 0;
+    for (int k = 0; k < 4; ++k) c += ((v >> (8 * k)) & 0x0Fu) != 0;
+    return c;
+}
+
+/* cu/voteRoot2.cu:33-49,212-307: edge state of node d = state(parent) |
+ * state(d) per word for every edge (1-based ids; nodes that are no edge's
+ * child keep a zero state); per sample row, the count of shared positions
+ * with every node's edge state, and the node of the largest count chosen as
+ * the reference's reductions choose it: nodes in blocks of 8 reduced by the
+ * tree (4, 2, 1) in which the lower lane keeps ties (findRootKernel :104-113,
+ * node ids past nnode count 0), then lane L of 32 keeps the first strictly
+ * larger block value over blocks L, L + 32, ... starting from (0, node 0), and
+ * the 32 lanes reduce by the tree (16, 8, 4, 2, 1) in which the lower lane
+ * keeps ties (rowMaxKernel :116-141).  root[s] = 0-based node, nshared[s] its
+ * count. */
+void orc_vote_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                   const uint32_t *samples, uint32_t nsample, uint32_t *root, uint32_t *nshared)
+{
+    uint32_t *es = (uint32_t *)calloc((size_t)(nnode ? nnode : 1) * (nbin ? nbin : 1), sizeof(uint32_t));
+    for (uint32_t e = 0; e < nedge; ++e) {
+        const uint32_t a = edge[2 * e] - 1, d = edge[2 * e + 1] - 1;
+        for (uint32_t w = 0; w < nbin; ++w) es[(size_t)d * nbin + w] = state[(size_t)a * nbin + w] | state[(size_t)d * nbin + w];
+    }
+    const uint32_t nblk = (nnode + 7) / 8;
+    uint32_t *bval = (uint32_t *)malloc(sizeof(uint32_t) * (nblk ? nblk : 1));
+    uint32_t *bidx = (uint32_t *)malloc(sizeof(uint32_t) * (nblk ? nblk : 1));
+    for (uint32_t s = 0; s < nsample; ++s) {
+        const uint32_t *row = samples + (size_t)s * nbin;
+        for (uint32_t b = 0; b < nblk; ++b) {
+            uint32_t v[8], id[8];
+            for (uint32_t x = 0; x < 8; ++x) {
+                const uint32_t node = 8 * b + x;
+                uint32_t c = 0;
+                if (node < nnode)
+                    for (uint32_t w = 0; w < nbin; ++w) c += vr_shared(es[(size_t)node * nbin + w], row[w]);
+                v[x] = c;
+                id[x] = node;
+            }
+            for (uint32_t h = 4; h > 0; h >>= 1)
+                for (uint32_t x = 0; x < h; ++x)
+                    if (v[x] < v[x + h]) {
+                        v[x] = v[x + h];
+                        id[x] = id[x + h];
+                    }
+            bval[b] = v[0];
+            bidx[b] = id[0];
+        }
+        uint32_t lv[32], li[32];
+        for (uint32_t L = 0; L < 32; ++L) {
+            lv[L] = 0;
+            li[L] = 0;
+            for (uint32_t i = L; i < nblk; i += 32)
+                if (lv[L] < bval[i]) {
+                    lv[L] = bval[i];
+                    li[L] = bidx[i];
+                }
+        }
+        for (uint32_t off = 16; off > 0; off >>= 1)
+            for (uint32_t L = 0; L + off < 32; ++L)
+                if (lv[L] < lv[L + off]) {
+                    lv[L] = lv[L + off];
+                    li[L] = li[L + off];
+                }
+        root[s] = li[0];
+        nshared[s] = lv[0];
+    }
+    free(es);
+    free(bval);
+    free(bidx);
+}
+
+/* src/functions.cpp:4317-4410 (findRootAVX2Big; findRoot :4170-4212 is the
+ * same): the same edge states and counts, candidates the edge children in
+ * edge order, the first strictly larger count kept starting from -1; root[s]
+ * = child id + 1 (1-based; 0 with no edge), nshared[s] its count (-1 with no
+ * edge). */
+void orc_find_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                   const uint32_t *samples, uint32_t nsample, int32_t *root, int32_t *nshared)
+{
+    uint32_t *es = (uint32_t *)calloc((size_t)(nnode ? nnode : 1) * (nbin ? nbin : 1), sizeof(uint32_t));
+    for (uint32_t e = 0; e < nedge; ++e) {
+        const uint32_t a = edge[2 * e] - 1, d = edge[2 * e + 1] - 1;
+        for (uint32_t w = 0; w < nbin; ++w) es[(size_t)d * nbin + w] = state[(size_t)a * nbin + w] | state[(size_t)d * nbin + w];
+    }
+    for (uint32_t s = 0; s < nsample; ++s) {
+        const uint32_t *row = samples + (size_t)s * nbin;
+        int64_t best = -1, node = -1;
+        for (uint32_t e = 0; e < nedge; ++e) {
+            const uint32_t d = edge[2 * e + 1] - 1;
+            int64_t c = 0;
+            for (uint32_t w = 0; w < nbin; ++w) c += vr_shared(es[(size_t)d * nbin + w], row[w]);
+            if (c > best) {
+                best = c;
+                node = d;
+            }
+        }
+        root[s] = (int32_t)(node + 1);
+        nshared[s] = (int32_t)best;
+    }
+    free(es);
+}
+
+/* ---- (f-4) fasta2bin: variable columns, valid bases, byte codes -------------- */
+
+/* cu/fasta2bin.cu:12-32 (encode_char). */
+static uint32_t fa_code(unsigned char c)
+{
+    switch (c) {
+    case 'A': case 'a': return 0x01;
+    case 'T': case 't': case 'U': case 'u': return 0x02;
+    case 'C': case 'c': return 0x04;
+    case 'G': case 'g': return 0x08;
+    case 'W': case 'w': return 0x03;
+    case 'M': case 'm': return 0x05;
+    case 'R': case 'r': return 0x09;
+    case 'Y': case 'y': return 0x06;
+    case 'K': case 'k': return 0x0A;
+    case 'S': case 's': return 0x0C;
+    case 'H': case 'h': return 0x07;
+    case 'D': case 'd': return 0x0B;
+    case 'V': case 'v': return 0x0D;
+    case 'B': case 'b': return 0x0E;
+    case 'N': case 'n': return 0x0F;
+    default: return 0x10;
+    }
+}
+
+/* seqs: nseq rows of len characters (row-major).  cu/fasta2bin.cu:35-45
+ * (count_valid_bases: characters other than '-', '.', 'N', 'n'), :47-93
+ * (find_variable_positions, restated with its early exit: the per-base counts
+ * grow one sequence at a time and the column is variable as soon as some base
+ * count is above 0 and below the number of sequences seen), :96-118 and
+ * :189-210 (kept columns in order, 4 byte codes per word, byte j at bits 8j,
+ * zero padding).  out: nseq x ceil(nvar / 4) words; returns that width. */
+uint64_t orc_fasta_encode(const unsigned char *seqs, uint32_t nseq, uint64_t len, unsigned char *is_variable,
+                          uint32_t *valid, uint32_t *out)
+{
+    for (uint32_t s = 0; s < nseq; ++s) {
+        uint32_t v = 0;
+        for (uint64_t p = 0; p < len; ++p) {
+            const unsigned char c = seqs[(size_t)s * len + p];
+            v += !(c == '-' || c == '.' || c == 'N' || c == 'n');
+        }
+        valid[s] = v;
+    }
+    uint64_t nvar = 0;
+    for (uint64_t p = 0; p < len; ++p) {
+        uint32_t cnt[4] = {0, 0, 0, 0}, seen = 0;
+        int var = 0;
+        for (uint32_t s = 0; s < nseq && !var; ++s) {
+            const uint32_t code = fa_code(seqs[(size_t)s * len + p]);
+            for (int b = 0; b < 4; ++b) cnt[b] += (code >> b) & 1u;
+            ++seen;
+            for (int b = 0; b < 4; ++b)
+                if (cnt[b] > 0 && cnt[b] < seen) var = 1;
+        }
+        is_variable[p] = (unsigned char)var;
+        nvar += (uint64_t)var;
+    }
+    const uint64_t width = (nvar + 3) / 4;
+    for (uint32_t s = 0; s < nseq; ++s) {
+        uint64_t k = 0;
+        for (uint64_t w = 0; w < width; ++w) out[(size_t)s * width + w] = 0;
+        for (uint64_t p = 0; p < len; ++p) {
+            if (!is_variable[p]) continue;
+            out[(size_t)s * width + k / 4] |= fa_code(seqs[(size_t)s * len + p]) << (8 * (k % 4));
+            ++k;
+        }
+    }
+    return width;
+}

@@ -108,6 +108,23 @@ uint32_t orc_node_keep(int indel, const uint32_t *edge, uint32_t nedge, const ui
 uint32_t orc_clade_bitsets(const int32_t *anc, const int32_t *des, uint32_t nedge, uint32_t nnode, uint32_t ntip,
                            uint32_t *rows_out, int32_t *size_out);
 
+/* ---- (f-4) DNA format edge ------------------------------------------------ */
+/* voteRoot2: state nnode x nbin words (4 byte codes each), edge nedge x 2
+ * 1-based (parent, child), samples nsample x nbin.  root = 0-based node of the
+ * most shared positions under the reference's reduction order, nshared the
+ * count. */
+void orc_vote_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                   const uint32_t *samples, uint32_t nsample, uint32_t *root, uint32_t *nshared);
+/* findRoot / findRootAVX2Big: first edge child of the largest count in edge
+ * order; root 1-based (0 and nshared -1 with no edge). */
+void orc_find_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
+                   const uint32_t *samples, uint32_t nsample, int32_t *root, int32_t *nshared);
+/* fasta2bin on nseq equal-length rows of len characters: is_variable[len],
+ * valid[nseq] (bases other than '-', '.', 'N', 'n'), out nseq x width words
+ * of the variable columns' byte codes; returns width = ceil(nvar / 4). */
+uint64_t orc_fasta_encode(const unsigned char *seqs, uint32_t nseq, uint64_t len, unsigned char *is_variable,
+                          uint32_t *valid, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,11 @@ def lib():
         L.orc_clade_bitsets.restype = u32
         L.orc_clade_bitsets.argtypes = [i32p, i32p, u32, u32, u32, u32p, i32p]
         L.orc_node_keep.argtypes = [i32, u32p, u32, u32p, u32, u32, u32p, u32, u32, u32p]
+        L.orc_vote_root.argtypes = [u32p, u32, u32, u32p, u32, u32p, u32, u32p, u32p]
+        L.orc_find_root.argtypes = [u32p, u32, u32, u32p, u32, u32p, u32, i32p, i32p]
+        u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        L.orc_fasta_encode.restype = u64
+        L.orc_fasta_encode.argtypes = [u8p, u32, u64, u8p, u32p, u32p]
         _lib = L
     return _lib
 
@@ -264,3 +269,84 @@ def clade_bitsets(edge, nnode: int, ntip: int):
     sizes = np.zeros(max(nnode - 1, 1), np.int32)
     k = lib().orc_clade_bitsets(anc, des, e.shape[0], nnode, ntip, rows, sizes)
     return rows[:k], sizes[: max(nnode - 1, 0)]
+
+
+# ---------------------------------------------------------------- DNA format edge
+def _root_inputs(state, edge, mat, idx):
+    state = _u32(state)
+    nnode, nbin = state.shape
+    edge = _u32(edge).reshape(-1, 2)
+    mat = _u32(mat).reshape(-1, nbin)
+    samples = _u32(mat[np.asarray(idx, dtype=np.int64)]) if len(idx) else np.zeros((0, nbin), np.uint32)
+    return state, nnode, nbin, edge, samples
+
+
+def vote_root(state, edge, mat, idx):
+    """voteRoot2: (root 0-based, nshared) per sample row mat[idx[s]]; state
+    (nnode, nbin) u32 byte codes, edge (nedge, 2) 1-based."""
+    state, nnode, nbin, edge, samples = _root_inputs(state, edge, mat, idx)
+    ns = samples.shape[0]
+    root, nsh = np.zeros(max(ns, 1), np.uint32), np.zeros(max(ns, 1), np.uint32)
+    lib().orc_vote_root(state, nnode, nbin, edge.reshape(-1) if edge.size else _u32([0]), edge.shape[0],
+                        samples if ns else _u32([0]), ns, root, nsh)
+    return root[:ns], nsh[:ns]
+
+
+def find_root(state, edge, mat, idx):
+    """findRoot / findRootAVX2Big: (root 1-based, nshared) int32 per sample."""
+    state, nnode, nbin, edge, samples = _root_inputs(state, edge, mat, idx)
+    ns = samples.shape[0]
+    root, nsh = np.zeros(max(ns, 1), np.int32), np.zeros(max(ns, 1), np.int32)
+    lib().orc_find_root(state, nnode, nbin, edge.reshape(-1) if edge.size else _u32([0]), edge.shape[0],
+                        samples if ns else _u32([0]), ns, root, nsh)
+    return root[:ns], nsh[:ns]
+
+
+def fasta_encode(seqs):
+    """fasta2bin on an (nseq, len) uint8 array of equal-length sequences:
+    (is_variable bool (len,), valid (nseq,) u32, codes (nseq, width) u32)."""
+    seqs = np.ascontiguousarray(seqs, dtype=np.uint8)
+    nseq, length = seqs.shape
+    isv = np.zeros(max(length, 1), np.uint8)
+    valid = np.zeros(max(nseq, 1), np.uint32)
+    out = np.zeros(max(nseq * ((length + 3) // 4), 1), np.uint32)
+    width = lib().orc_fasta_encode(seqs if seqs.size else np.zeros(1, np.uint8), nseq, length, isv, valid, out)
+    return isv[:length].astype(bool), valid[:nseq], out[: nseq * width].reshape(nseq, width)
+
+
+def read_fasta(text: bytes):
+    """cu/fasta2bin.cu:121-187 (read_fasta): (names, sequences) of a FASTA
+    text, line by line: empty lines skipped; a '>' line ends the running
+    sequence (kept only when non-empty) and names the next one (the text after
+    '>' from its first non-blank character, cut at the first blank when that
+    blank comes after it); other lines join the running sequence with their
+    whitespace removed.  Raises ValueError as the reference throws."""
+    names, seqs = [], []
+    name, cur = b"", b""
+    lines = text.split(b"\n")
+    if lines and lines[-1] == b"":
+        lines.pop()  # getline: a final newline ends the last line
+    ws = b" \t\n\v\f\r"
+    for line in lines:
+        if not line:
+            continue
+        if line[:1] == b">":
+            if cur:
+                seqs.append(cur)
+                names.append(name)
+                cur = b""
+            name = line[1:]
+            first = next((i for i, c in enumerate(name) if c not in b" \t"), None)
+            blank = next((i for i, c in enumerate(name) if c in b" \t"), None)
+            if first is not None:
+                name = name[first:] if blank is None or blank < first else name[first:blank]
+        else:
+            cur += bytes(c for c in line if c not in ws)
+    if cur:
+        seqs.append(cur)
+        names.append(name)
+    if not seqs:
+        raise ValueError("No valid sequences found in file")
+    if any(len(x) != len(seqs[0]) for x in seqs):
+        raise ValueError("All sequences must have the same length for variant filtering")
+    return names, seqs

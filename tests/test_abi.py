@@ -112,3 +112,31 @@ def test_clade_bitsets_edge_order_checked():
     assert e.value.code == -1 and "leaves a node before the edge into it" in str(e.value)
     with pytest.raises(api.CasphyError):
         api.clade_bitsets(np.array([[4, 5], [5, 1], [5, 2], [5, 3], [4, 5]], np.int32), 2, 3)
+
+
+def test_root_vote_arguments_checked_before_gpu_work():
+    from casphy_amd import api
+    st = np.zeros((4, 2), np.uint32)
+    mat = np.zeros((3, 2), np.uint32)
+    good = np.array([[1, 2], [1, 3], [3, 4]], np.uint32)
+    for fn in (api.vote_root, api.find_root):
+        with pytest.raises(api.CasphyError) as e:
+            fn(st, good, mat, [0, 3])
+        assert "names row 3 of 3" in str(e.value)
+        with pytest.raises(api.CasphyError) as e:
+            fn(st, np.array([[1, 2], [1, 5]], np.uint32), mat, [0])
+        assert "outside 1..4" in str(e.value)
+        with pytest.raises(api.CasphyError) as e:
+            fn(st, np.array([[1, 2], [3, 2]], np.uint32), mat, [0])
+        assert "child of two edges" in str(e.value)
+        with pytest.raises(api.CasphyError) as e:
+            fn(np.zeros((0, 2), np.uint32), np.zeros((0, 2), np.uint32), mat, [0])
+        assert "no node" in str(e.value)
+
+
+def test_fasta_encode_arguments_checked():
+    from casphy_amd import _lib
+    L = _lib.load()
+    seqs = np.frombuffer(b"ACGT", np.uint8)
+    rc = L.casphy_fasta_encode(seqs.ctypes.data_as(C.c_void_p), 1, 4, None, None, None, None)
+    assert rc == -1 and b"bad arguments" in L.casphy_last_error()

@@ -27,7 +27,7 @@ def have_tools():
                                        ("joinEdgeBitReverse", ["PFX", "PFX.w", 1]),
                                        ("joinEdgeIndel", ["PFX", "PFX.w", 2, 0, 2]),
                                        ("tipMerge", ["PFX", 0.8]), ("nodeKeepBit", ["PFX", 1]),
-                                       ("nodeKeepIndel", ["PFX", 1])])
+                                       ("nodeKeepIndel", ["PFX", 1]), ("voteRoot2", ["PFX.bin", "PFX"])])
 def test_missing_input_fails_loudly(tmp_path, tool, args):
     pfx = str(tmp_path / "none")
     r = run(tool, *[a.replace("PFX", pfx) if isinstance(a, str) else a for a in args])
@@ -181,3 +181,20 @@ def test_node_keep_tool(tmp_path, orc):
     ns2 = formats.read_bin(pfx + ".nodeState2")
     kept = np.loadtxt(pfx + ".retainNode", dtype=np.int64, ndmin=1) if open(pfx + ".retainNode").read() else []
     assert np.array_equal(np.asarray(kept), orc.node_keep(eo, ns2, out, 2))
+
+
+@pytest.mark.skipif(not have_tools(), reason="executables not built")
+def test_fasta2bin_input_errors(tmp_path):
+    """fasta2bin's own messages and exit code 1 (cu/fasta2bin.cu:143-144,178-186,
+    258-263,356-359); the text is read before any device work."""
+    r = run("fasta2bin", tmp_path / "x.fa")
+    assert r.returncode == 1 and "Usage:" in r.stdout
+    r = run("fasta2bin", tmp_path / "none.fa", tmp_path / "o")
+    assert r.returncode == 1 and "Error: Cannot open file" in r.stderr
+    (tmp_path / "e.fa").write_bytes(b">a\n>b\n\n")
+    r = run("fasta2bin", tmp_path / "e.fa", tmp_path / "o")
+    assert r.returncode == 1 and "No valid sequences found in file" in r.stderr
+    (tmp_path / "u.fa").write_bytes(b">a\nACGT\n>b\nACG\n")
+    r = run("fasta2bin", tmp_path / "u.fa", tmp_path / "o")
+    assert r.returncode == 1 and "same length" in r.stderr
+    assert not os.path.exists(str(tmp_path / "o.bin"))
