@@ -39,6 +39,17 @@
  *                             outgroup); on DFS-ordered join trees the kept set
  *                             is the retained nodes without a kept proper
  *                             ancestor (a tree-walk form of the skip rule).
+ *   orc_vote_root             numpy counts over unpacked bytes; the winner in
+ *                             closed form (first by bit-reversed block mod 32,
+ *                             block / 32, bit-reversed node mod 8 among the
+ *                             largest counts) against the literal simulation
+ *                             of the reductions here; hand-worked ties.
+ *   orc_find_root             numpy counts, first maximum in edge order.
+ *   orc_fasta_encode          numpy final-count form of the column rule (OR
+ *                             of the IUPAC base sets has a base the AND
+ *                             lacks), codes from the IUPAC base sets, on
+ *                             random alphabets and the reference's example
+ *                             alignment; a hand-worked case.
  */
 #include "casphy_oracle.h"
 
@@ -834,17 +845,16 @@ static uint32_t vr_shared(uint32_t a, uint32_t b)
     return c;
 }
 
-/* cu/voteRoot2.cu:33-49,212-307: edge state of node d = state(parent) |
- * state(d) per word for every edge (1-based ids; nodes that are no edge's
- * child keep a zero state); per sample row, the count of shared positions
- * with every node's edge state, and the node of the largest count chosen as
- * the reference's reductions choose it: nodes in blocks of 8 reduced by the
- * tree (4, 2, 1) in which the lower lane keeps ties (findRootKernel :104-113,
- * node ids past nnode count 0), then lane L of 32 keeps the first strictly
- * larger block value over blocks L, L + 32, ... starting from (0, node 0), and
- * the 32 lanes reduce by the tree (16, 8, 4, 2, 1) in which the lower lane
- * keeps ties (rowMaxKernel :116-141).  root[s] = 0-based node, nshared[s] its
- * count. */
+/* cu/voteRoot2.cu:34-50 (inferEdgeStateKernel, launched :229-232 on a zeroed
+ * buffer): edge state of node d = state(parent) | state(d) per word for every
+ * edge (1-based ids; nodes that are no edge's child keep a zero state);
+ * :76-138 (findRootKernel): per sample row, the count of shared positions with
+ * every node's edge state, nodes in blocks of 8 reduced by the tree (4, 2, 1)
+ * in which the lower slot keeps ties (:125-133; node ids past nnode count 0);
+ * :140-169 (rowMaxKernel, 32 threads): lane L keeps the first strictly larger
+ * block value over blocks L, L + 32, ... starting from (0, node 0), and the 32
+ * lanes reduce by the tree (16, 8, 4, 2, 1) in which the lower lane keeps
+ * ties.  root[s] = 0-based node, nshared[s] its count (:309-322). */
 void orc_vote_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
                    const uint32_t *samples, uint32_t nsample, uint32_t *root, uint32_t *nshared)
 {
