@@ -796,8 +796,10 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
 // ds_write.  One DMA instruction fills 1 KB of LDS, lane i at byte 16 i, so
 // the lane -> granule map is chosen to give the swizzled row layout the
 // operand reads want: row r holds granules 0..3 (chunk c, half h at g = 2c + h)
-// at positions g ^ ((r >> 1) & 3), which keeps 8 consecutive rows of one
-// granule on distinct banks.  Two stage buffers (separate LDS objects, loop
+// at positions g ^ ((r >> 2) & 3), which keeps 16 consecutive rows of one
+// granule on distinct banks (a 16-byte read serves 16 lanes per cycle from
+// 64 banks: rows r, r + 4, r + 8, r + 12 share a bank window of 64-byte rows
+// and take its 4 positions).  Two stage buffers (separate LDS objects, loop
 // unrolled by two): the DMA of stage s + 1 runs under the MFMAs of stage s.
 constexpr u32 FP4D_STAGE = 2 * TILE * 64;  // bytes of one stage buffer (A then B)
 
@@ -808,7 +810,7 @@ __device__ __forceinline__ void fp4d_dma(unsigned char *buf, const unsigned char
     for (int k = 0; k < 4; ++k) {
         const u32 q = (wave * 4 + k) * 64 + lane;  // granule index in the stage buffer
         const bool opB = q >= TILE * 4;
-        const u32 q2 = q & (TILE * 4 - 1), row = q2 >> 2, g = (q2 & 3) ^ ((row >> 1) & 3);
+        const u32 q2 = q & (TILE * 4 - 1), row = q2 >> 2, g = (q2 & 3) ^ ((row >> 2) & 3);
         const unsigned char *E = opB ? EB : EA;
         const size_t pad = opB ? padB : padA;
         const size_t rr = (size_t)(opB ? J : I) * TILE + row;
@@ -854,9 +856,9 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
             for (int x = 0; x < 2; ++x) {
                 const u32 r1 = ra + 32 * x, r2 = rb + 32 * x;
                 const u32 g = 2 * c + h;
-                const v4i ta = *reinterpret_cast<const v4i *>(buf + r1 * 64 + ((g ^ ((r1 >> 1) & 3)) * 16));
+                const v4i ta = *reinterpret_cast<const v4i *>(buf + r1 * 64 + ((g ^ ((r1 >> 2) & 3)) * 16));
                 const v4i tb =
-                    *reinterpret_cast<const v4i *>(buf + TILE * 64 + r2 * 64 + ((g ^ ((r2 >> 1) & 3)) * 16));
+                    *reinterpret_cast<const v4i *>(buf + TILE * 64 + r2 * 64 + ((g ^ ((r2 >> 2) & 3)) * 16));
                 fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
                 fb[x] = v8i{tb.x, tb.y, tb.z, tb.w, 0, 0, 0, 0};
             }
@@ -915,7 +917,7 @@ __device__ __forceinline__ void fp4w_dma(unsigned char *buf, const unsigned char
     for (int k = 0; k < 4; ++k) {
         const u32 q = (wave * 4 + k) * 64 + lane;  // granule index in the stage buffer
         const bool opB = q >= WT * 4;
-        const u32 q2 = q & (WT * 4 - 1), row = q2 >> 2, g = (q2 & 3) ^ ((row >> 1) & 3);
+        const u32 q2 = q & (WT * 4 - 1), row = q2 >> 2, g = (q2 & 3) ^ ((row >> 2) & 3);
         const unsigned char *E = opB ? EB : EA;
         const size_t pad = opB ? padB : padA;
         const size_t rr = (size_t)(opB ? J : I) * WT + row;
@@ -961,13 +963,13 @@ __global__ __launch_bounds__(512, 1) void k_mfma_fp4w(MfmaArgs a)
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
                 const u32 r1 = ra + 32 * x;
-                const v4i ta = *reinterpret_cast<const v4i *>(buf + r1 * 64 + ((g ^ ((r1 >> 1) & 3)) * 16));
+                const v4i ta = *reinterpret_cast<const v4i *>(buf + r1 * 64 + ((g ^ ((r1 >> 2) & 3)) * 16));
                 fa[x] = v8i{ta.x, ta.y, ta.z, ta.w, 0, 0, 0, 0};
             }
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
                 const u32 r2 = rb + 32 * y;
-                const v4i tb = *reinterpret_cast<const v4i *>(buf + WT * 64 + r2 * 64 + ((g ^ ((r2 >> 1) & 3)) * 16));
+                const v4i tb = *reinterpret_cast<const v4i *>(buf + WT * 64 + r2 * 64 + ((g ^ ((r2 >> 2) & 3)) * 16));
                 fb[y] = v8i{tb.x, tb.y, tb.z, tb.w, 0, 0, 0, 0};
             }
 #pragma unroll
