@@ -559,6 +559,8 @@ int casphy_transfer(const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t 
     });
 }
 
+}  // extern "C"
+
 namespace {
 // Batched transfer distance on device bit rows: Y rows in my, the replicates'
 // rows back to back in mx (replicate b: rows xoffsets[b] .. xoffsets[b+1]),
@@ -622,6 +624,8 @@ BitMat clade_mat(const CladePlan &p, hipStream_t s)
     return m;
 }
 }  // namespace
+
+extern "C" {
 
 int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, const uint32_t *xoffsets, uint32_t nrep,
                           uint32_t ncol, uint32_t ntip, uint32_t *ymin, uint32_t *ysize, uint32_t *xmin,
@@ -709,6 +713,8 @@ int casphy_tip_merge(const uint32_t *nneighbor, const uint32_t *idx, uint32_t nt
     });
 }
 
+}  // extern "C"
+
 namespace {
 int node_keep_entry(bool indel, const char *name, const uint32_t *edge, uint32_t nedge, const uint32_t *state,
                     uint32_t nnode, uint32_t width, const uint32_t *outmat, uint32_t nout, uint32_t cutoff,
@@ -736,6 +742,8 @@ int node_keep_entry(bool indel, const char *name, const uint32_t *edge, uint32_t
 }
 }  // namespace
 
+extern "C" {
+
 int casphy_node_keep_bit(const uint32_t *edge, uint32_t nedge, const uint32_t *node_state, uint32_t nnode,
                          uint32_t nbin, const uint32_t *outmat, uint32_t nout, uint32_t cutoff, uint32_t *kept,
                          uint32_t *nkept)
@@ -751,6 +759,8 @@ int casphy_node_keep_indel(const uint32_t *edge, uint32_t nedge, const uint32_t 
     return node_keep_entry(true, "casphy_node_keep_indel", edge, nedge, node_state, nnode, ncol, outmat, nout,
                            cutoff, kept, nkept);
 }
+
+}  // extern "C"
 
 namespace {
 // Checks a root-vote call and returns the sample rows mat[idx[s]], gathered.
@@ -773,6 +783,8 @@ std::vector<u32> root_vote_samples(const char *fn, const uint32_t *state, uint32
     return rows;
 }
 }  // namespace
+
+extern "C" {
 
 int casphy_vote_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const uint32_t *edge, uint32_t nedge,
                      const uint32_t *mat, uint64_t nmat, const uint32_t *idx, uint32_t nsample, uint32_t *root,
