@@ -220,6 +220,19 @@ def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
     cp.transfer_trees(ref_e, ref_n, [e for e, _ in reps], [n for _, n in reps], ntip)
     tree_s, tt = timed(lambda: cp.transfer_trees(ref_e, ref_n, [e for e, _ in reps], [n for _, n in reps], ntip), 1)
     ex["tbe_65536_tips"]["from_trees_jobs_per_s"] = len(reps) / tree_s
+    # ... and from Newick text: read (tips matched by label), clade bitsets,
+    # transfer minima and TBE support in one pass per job set
+    nrng = np.random.default_rng(15)
+    ref_text = synth.newick(parent, leaves, rng=nrng)
+    rep_text = "\n".join(synth.newick(synth.spr(parent, leaves, moves, nrng), leaves, rng=nrng) for _ in range(4))
+
+    def from_newick():
+        rt = cp.read_newick(ref_text)[0]
+        return cp.transfer_support(rt, cp.read_newick(rep_text, tip_labels=rt[3]))
+    from_newick()
+    nwk_s, (sup, _, _) = timed(from_newick, 1)
+    ex["tbe_65536_tips"]["from_newick_jobs_per_s"] = 4 / nwk_s
+    ex["tbe_65536_tips"]["from_newick_mean_support"] = float(np.nanmean(sup))
     # small bootstrap trees, where one replicate does not fill the GPU: 4,096 tips x 64 replicates
     ntip = 4096
     parent, leaves = synth.yule_parent(ntip, rng)

@@ -46,6 +46,9 @@ SIGNATURES = [
     ("casphy_vote_root", C.c_int, [vp, u32, u32, vp, u32, vp, u64, vp, u32, vp, vp]),
     ("casphy_find_root", C.c_int, [vp, u32, u32, vp, u32, vp, u64, vp, u32, vp, vp]),
     ("casphy_fasta_encode", C.c_int, [vp, u32, u64, vp, vp, vp, C.POINTER(u64)]),
+    ("casphy_newick_read", C.c_int, [vp, u64, vp, u64, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32),
+                                     C.POINTER(vp), C.POINTER(u32), C.POINTER(vp), C.POINTER(u64)]),
+    ("casphy_tbe_support", C.c_int, [vp, u32, u32, vp, u32, vp]),
     ("casphy_cells_create", C.c_int, [C.c_int, vp, u32, u32, C.c_int, vp, C.POINTER(vp)]),
     ("casphy_cells_destroy", None, [vp]),
     ("casphy_cells_radius", C.c_int, [vp, i32, C.POINTER(u64)]),

@@ -34,7 +34,8 @@ __all__ = [
     "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
     "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep", "clade_bitsets", "transfer_trees",
-    "vote_root", "find_root", "findRootAVX2Big", "fasta_encode", "fasta2bin",
+    "vote_root", "find_root", "findRootAVX2Big", "fasta_encode", "fasta2bin", "read_newick", "tbe_support",
+    "transfer_support",
 ]
 
 
@@ -356,6 +357,64 @@ def node_keep(edge, node_state, outmat, cutoff: int, indel: bool = False) -> np.
     check(fn(_p(e) if e.size else None, e.shape[0] if e.ndim == 2 else 0, _p(st) if st.size else None, nnode, width,
              _p(om) if om.size else None, nout, int(cutoff), _p(kept), C.byref(nk)))
     return kept[:nk.value]
+
+
+# ------------------------------------------------------------------ trees from Newick
+def read_newick(text, tip_labels=None):
+    """Every tree of a Newick text (str or bytes) as ape's read.tree numbers
+    it: a list of (edge (nedge, 2) int32 1-based cladewise, nnode, ntip, tip
+    labels).  With tip_labels, tips are numbered by the position of their
+    label there (so that trees share one numbering)."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    order = None
+    if tip_labels is not None:
+        order = b"".join((t.encode() if isinstance(t, str) else bytes(t)) + b"\0" for t in tip_labels)
+    buf = C.create_string_buffer(data, len(data) + 1)
+    base = C.cast(buf, C.c_void_p).value
+    obuf = C.create_string_buffer(order, len(order)) if order else None
+    L = load()
+    trees, pos = [], 0
+    while pos < len(data):
+        consumed, ntip, nnode, nedge, nlab = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        ep, lp = C.c_void_p(), C.c_void_p()
+        check(L.casphy_newick_read(base + pos, len(data) - pos, C.cast(obuf, C.c_void_p) if obuf else None,
+                                   len(order) if order else 0, C.byref(consumed), C.byref(ntip), C.byref(nnode),
+                                   C.byref(ep), C.byref(nedge), C.byref(lp), C.byref(nlab)))
+        pos += consumed.value
+        if ntip.value == 0:
+            break
+        try:
+            ne = nedge.value
+            cols = np.ctypeslib.as_array(C.cast(ep, C.POINTER(C.c_int32)), (2 * ne,)).copy() if ne else \
+                np.zeros(0, np.int32)
+            labels = C.string_at(lp.value, nlab.value).split(b"\0")[:-1]
+        finally:
+            L.casphy_free(ep)
+            L.casphy_free(lp)
+        edge = np.ascontiguousarray(cols.reshape(2, ne).T)
+        trees.append((edge, nnode.value, ntip.value, [x.decode("utf-8", "surrogateescape") for x in labels]))
+    return trees
+
+
+def tbe_support(ymin, ysize, ntip: int) -> np.ndarray:
+    """Transfer bootstrap expectation of the reference clades from the
+    transfer minima (nrep, ny) and clade sizes (ny,) of transfer_trees."""
+    ym = _u32(ymin)
+    ys = _u32(ysize)
+    nrep, ny = (ym.shape if ym.ndim == 2 else (0, ys.size))
+    out = np.empty(max(ny, 1), np.float64)
+    check(load().casphy_tbe_support(_p(ym) if ym.size else None, nrep, ny, _p(ys) if ys.size else None, int(ntip),
+                                    _p(out)))
+    return out[:ny]
+
+
+def transfer_support(ref_tree, rep_trees):
+    """TBE of the reference tree's clades (nodeDesBig's row order) over the
+    replicate trees, all as read_newick returns them with one tip numbering:
+    (support, clade sizes, transfer minima (nrep, ny))."""
+    edge, nnode, ntip, _ = ref_tree
+    ymin, ysize, _, _ = transfer_trees(edge, nnode, [t[0] for t in rep_trees], [t[1] for t in rep_trees], ntip)
+    return tbe_support(ymin, ysize, ntip), ysize, ymin
 
 
 # ------------------------------------------------------------------ DNA format edge

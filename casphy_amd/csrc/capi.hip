@@ -17,6 +17,7 @@
 #include "mfma.hpp"
 #include "clades.hpp"
 #include "fasta.hpp"
+#include "newick.hpp"
 #include "voteroot.hpp"
 #include "nodekeep.hpp"
 #include "tipmerge.hpp"
@@ -836,6 +837,65 @@ int casphy_fasta_encode(const uint8_t *seqs, uint32_t nseq, uint64_t len, uint8_
                 "casphy_fasta_encode: bad arguments");
         if (!nseq && len) std::memset(is_variable, 0, len);
         *width = fasta_encode(seqs, nseq, len, is_variable, valid, out, thread_stream());
+    });
+}
+
+int casphy_newick_read(const char *text, uint64_t len, const char *tip_order, uint64_t tip_order_bytes,
+                       uint64_t *consumed, uint32_t *ntip, uint32_t *nnode, int32_t **edge, uint32_t *nedge,
+                       char **labels, uint64_t *labels_bytes)
+{
+    return guarded([&] {
+        require((text || !len) && consumed && ntip && nnode && edge && nedge && labels && labels_bytes &&
+                    (tip_order || !tip_order_bytes),
+                "casphy_newick_read: bad arguments");
+        *edge = nullptr;
+        *labels = nullptr;
+        std::vector<std::string> order;
+        if (tip_order) {
+            require(tip_order_bytes > 0 && tip_order[tip_order_bytes - 1] == '\0',
+                    "casphy_newick_read: the tip order must end with a NUL");
+            for (uint64_t b = 0; b < tip_order_bytes;) {
+                const std::string l(tip_order + b);
+                b += l.size() + 1;
+                order.push_back(l);
+            }
+        }
+        u64 pos = 0;
+        NewickTree t = read_newick(text, len, pos, tip_order ? &order : nullptr);
+        *consumed = pos;
+        *ntip = t.ntip;
+        *nnode = t.nnode;
+        *nedge = (u32)t.anc.size();
+        *labels_bytes = 0;
+        if (!t.ntip) return;
+        int32_t *e = static_cast<int32_t *>(std::malloc(std::max<size_t>(t.anc.size() * 8, 8)));
+        require(e != nullptr, "casphy_newick_read: out of host memory");
+        std::copy(t.anc.begin(), t.anc.end(), e);  // R's edge matrix: the parent column, then the child column
+        std::copy(t.des.begin(), t.des.end(), e + t.anc.size());
+        size_t bytes = 0;
+        for (auto &l : t.labels) bytes += l.size() + 1;
+        char *lb = static_cast<char *>(std::malloc(std::max<size_t>(bytes, 1)));
+        if (!lb) {
+            std::free(e);
+            require(false, "casphy_newick_read: out of host memory");
+        }
+        size_t at = 0;
+        for (auto &l : t.labels) {
+            std::memcpy(lb + at, l.c_str(), l.size() + 1);
+            at += l.size() + 1;
+        }
+        *edge = e;
+        *labels = lb;
+        *labels_bytes = bytes;
+    });
+}
+
+int casphy_tbe_support(const uint32_t *ymin, uint32_t nrep, uint32_t ny, const uint32_t *ysize, uint32_t ntip,
+                       double *support)
+{
+    return guarded([&] {
+        require((ny == 0 || (ysize && support)) && ((u64)nrep * ny == 0 || ymin), "casphy_tbe_support: bad arguments");
+        tbe_support(ymin, nrep, ny, ysize, ntip, support);
     });
 }
 

@@ -320,3 +320,40 @@ def vote_inputs(seed: int, ntip: int, nsample: int, ncol: int, mat: np.ndarray |
     st[r] = np.where((inter[r] & 0x0F) != 0, inter[r], union[r]).view(np.uint32)
     idx = np.arange(ntip, ntip + nsample, dtype=np.uint32)
     return st, edge.astype(np.uint32), mat, idx
+
+
+def newick(parent: np.ndarray, leaves: np.ndarray, labels=None, rng: np.random.Generator | None = None) -> str:
+    """Newick text of a rooted tree given as a parent array (root 0) with tip
+    t at node leaves[t] (label labels[t], default "t<t>"), children in
+    increasing node id (the order ape_edges lists them); with rng, random
+    branch lengths and internal node labels that a reader drops."""
+    nn = parent.size
+    kids = [[] for _ in range(nn)]
+    for v in range(1, nn):
+        kids[int(parent[v])].append(v)
+    name = {}
+    for t, v in enumerate(leaves):
+        name[int(v)] = labels[t] if labels is not None else f"t{t}"
+    out = []
+    stack = [(0, 0)]
+    while stack:
+        v, k = stack.pop()
+        if not kids[v]:
+            out.append(name[v])
+        elif k == 0:
+            out.append("(")
+            stack.append((v, 1))
+            stack.append((kids[v][0], 0))
+            continue
+        elif k < len(kids[v]):
+            out.append(",")
+            stack.append((v, k + 1))
+            stack.append((kids[v][k], 0))
+            continue
+        else:
+            out.append(")")
+            if rng is not None:
+                out.append(f"n{v}")
+        if rng is not None and v != 0:
+            out.append(f":{rng.random():.5f}")
+    return "".join(out) + ";"

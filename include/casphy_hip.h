@@ -272,6 +272,32 @@ int casphy_find_root(const uint32_t *state, uint32_t nnode, uint32_t nbin, const
 int casphy_fasta_encode(const uint8_t *seqs, uint32_t nseq, uint64_t len, uint8_t *is_variable, uint32_t *valid,
                         uint32_t *out, uint64_t *width);
 
+/* One Newick tree of `text` (leading blanks and [comments] skipped, up to and
+ * including its ';'; *consumed = the bytes read, so a caller loops over a
+ * file of trees; *ntip = 0 when only blanks remain).  Numbered as ape's
+ * read.tree numbers it: tips 1..ntip in the order their labels appear, or,
+ * with tip_order (NUL-separated labels, tip_order_bytes in all), by the
+ * position of their label there; internal nodes ntip+1.. in the order of
+ * their '(' (the root first); edges in that preorder (cladewise).  *edge
+ * (library-allocated) holds R's edge matrix column-major: the nedge parent
+ * ids, then the nedge child ids; *labels (library-allocated) the tip labels
+ * NUL-separated in tip order.  Branch lengths and internal node labels are
+ * dropped.  No counterpart in the reference, where R reads the trees with
+ * ape::read.tree before nodeDesBig (R/bootstrap_functions.R:1-3). */
+int casphy_newick_read(const char *text, uint64_t len, const char *tip_order, uint64_t tip_order_bytes,
+                       uint64_t *consumed, uint32_t *ntip, uint32_t *nnode, int32_t **edge, uint32_t *nedge,
+                       char **labels, uint64_t *labels_bytes);
+
+/* Transfer bootstrap expectation (Lemoine et al., Nature 556:452, 2018) of
+ * the reference tree's clade rows from casphy_transfer_trees' outputs: with p
+ * = min(ysize[y], ntip - ysize[y]) and the transfer index of each replicate
+ * min(ymin[b * ny + y], p - 1), support[y] = 1 - (sum over the nrep
+ * replicates of the indices) / ((p - 1) * nrep), NaN when p < 2.  No
+ * counterpart in the reference, which stops at the transfer minima
+ * (cu/TransferDisStream.cu:286-344). */
+int casphy_tbe_support(const uint32_t *ymin, uint32_t nrep, uint32_t ny, const uint32_t *ysize, uint32_t ntip,
+                       double *support);
+
 /* Transfer distances from trees: the reference tree and nrep replicate trees
  * as R edge-matrix columns (cladewise, as casphy_clade_bitsets; replicate b's
  * edges at x_edge_offsets[b] .. x_edge_offsets[b+1], x_nnode[b] internal

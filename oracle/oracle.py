@@ -350,3 +350,104 @@ def read_fasta(text: bytes):
     if any(len(x) != len(seqs[0]) for x in seqs):
         raise ValueError("All sequences must have the same length for variant filtering")
     return names, seqs
+
+
+# ---------------------------------------------------------------- trees from Newick
+def read_newick(text, tip_labels=None):
+    """Newick trees numbered as ape's read.tree (ape >= 5.0, a dependency of
+    CasPhy, DESCRIPTION:10-12) numbers them: tips 1..n by order of
+    appearance (or by position in tip_labels), internal nodes n+1.. by the
+    order of their '(' (root first), edges in that preorder.  Pure Python:
+    the text is cut into tokens, then an explicit stack of open nodes builds
+    the edge list.  Returns [(edge (nedge, 2) int32, nnode, ntip, labels)]."""
+    import re
+    if isinstance(text, bytes):
+        text = text.decode("utf-8", "surrogateescape")
+    tok = re.compile(r"\s+|\[[^\]]*\]|'(?:[^']|'')*'|[(),;]|:[^(),;:\[\s]*|[^(),;:\[\s']+")
+    trees, toks = [], []
+    for m in tok.finditer(text):
+        t = m.group(0)
+        if t[0].isspace() or t[0] == "[":
+            continue
+        toks.append(t)
+        if t == ";":
+            trees.append(toks)
+            toks = []
+    if any(t not in ";" for t in toks):
+        raise ValueError("newick: missing ';'")
+    out = []
+    for toks in trees:
+        edges, tips, nint, stack, i = [], [], 0, [], 0
+        expect = True
+
+        def label(i):
+            if i < len(toks) and toks[i] not in "(),;" and not toks[i].startswith(":"):
+                t = toks[i]
+                return (t[1:-1].replace("''", "'") if t.startswith("'") else t), i + 1
+            return "", i
+
+        def length(i):
+            return i + 1 if i < len(toks) and toks[i].startswith(":") else i
+
+        while True:
+            t = toks[i]
+            if expect:
+                if t == "(":
+                    if stack:
+                        edges.append((stack[-1], ("n", nint)))
+                    stack.append(nint)
+                    nint += 1
+                    i += 1
+                    continue
+                lab, i = label(i)
+                tips.append(lab)
+                if stack:
+                    edges.append((stack[-1], ("t", len(tips) - 1)))
+                i = length(i)
+                expect = False
+                if not stack:
+                    assert toks[i] == ";"
+                    break
+            elif t == ",":
+                i += 1
+                expect = True
+            elif t == ")":
+                stack.pop()
+                _, i = label(i + 1)
+                i = length(i)
+            elif t == ";":
+                if stack:
+                    raise ValueError("newick: unbalanced '('")
+                break
+            else:
+                raise ValueError("newick: unexpected " + t)
+        n = len(tips)
+        if tip_labels is not None:
+            pos = {lab: k for k, lab in enumerate(tip_labels)}
+            num = [pos[lab] + 1 for lab in tips]
+            labels = list(tip_labels)
+        else:
+            num = list(range(1, n + 1))
+            labels = tips
+        e = np.array([(n + 1 + a, num[c[1]] if c[0] == "t" else n + 1 + c[1]) for a, c in edges],
+                     np.int32).reshape(-1, 2)
+        out.append((e, nint, n, labels))
+    return out
+
+
+def tbe_support(ymin, ysize, ntip):
+    """Transfer bootstrap expectation (Lemoine et al., Nature 556:452, 2018):
+    a reference branch with p tips on its smaller side has, against each
+    replicate, transfer index min(ymin, p - 1) (a replicate's tip branch is
+    p - 1 away), and TBE = 1 - sum of the indices / ((p - 1) * nrep); NaN when
+    p < 2."""
+    ym = np.asarray(ymin, dtype=np.int64).reshape(-1, len(ysize))
+    out = np.empty(len(ysize))
+    for y, sz in enumerate(ysize):
+        p = min(int(sz), ntip - int(sz))
+        if p < 2 or ym.shape[0] == 0:
+            out[y] = np.nan
+            continue
+        s = sum(min(int(v), p - 1) for v in ym[:, y])
+        out[y] = 1.0 - float(s) / (float(p - 1) * float(ym.shape[0]))
+    return out

@@ -27,7 +27,8 @@ def have_tools():
                                        ("joinEdgeBitReverse", ["PFX", "PFX.w", 1]),
                                        ("joinEdgeIndel", ["PFX", "PFX.w", 2, 0, 2]),
                                        ("tipMerge", ["PFX", 0.8]), ("nodeKeepBit", ["PFX", 1]),
-                                       ("nodeKeepIndel", ["PFX", 1]), ("voteRoot2", ["PFX.bin", "PFX"])])
+                                       ("nodeKeepIndel", ["PFX", 1]), ("voteRoot2", ["PFX.bin", "PFX"]),
+                                       ("tbeSupport", ["PFX.nwk", "PFX.trees", "PFX"])])
 def test_missing_input_fails_loudly(tmp_path, tool, args):
     pfx = str(tmp_path / "none")
     r = run(tool, *[a.replace("PFX", pfx) if isinstance(a, str) else a for a in args])
