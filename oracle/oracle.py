@@ -8,6 +8,11 @@ Array conventions follow the reference's on-disk layouts: binary and DNA
 matrices are cell-major (n, W) uint32, indel matrices (n, ncol) int32 with
 -1 missing / 0 unmutated / k>0 allele, weights float32 (the executables convert
 the .w doubles to float before summing, cu/radiusNeighbor.cu:240-243).
+
+Three restatements are pure Python here rather than C: read_fasta (the FASTA
+line rules of cu/fasta2bin.cu:121-187), read_newick (ape's read.tree
+numbering) and tbe_support (the TBE formula).  Their pins are in
+tests/test_oracle_dna_edge.py and tests/test_oracle_trees.py.
 """
 from __future__ import annotations
 
