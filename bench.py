@@ -193,6 +193,22 @@ def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
     ex["dna_fasta_encode_100000x1000"] = fasta_step(cp, dna, with_cpu)
     print("# DNA format edge done", file=sys.stderr, flush=True)
 
+    ex.update(tbe_step(cp, synth))
+    print("# TBE done", file=sys.stderr, flush=True)
+    ex["tipmerge_10000x4096"] = tip_merge_step(cp, cells, q, with_cpu)
+    print("# tipMerge done", file=sys.stderr, flush=True)
+    ex["scan_100000x81920_bit"] = scan_form(cp, synth)
+    print("# scan form done", file=sys.stderr, flush=True)
+    ex["serving_mix"] = serving_mix(cp, synth)
+    print("# serving done", file=sys.stderr, flush=True)
+    return ex
+
+
+def tbe_step(cp, synth) -> dict:
+    """Transfer distance: a 65,536-tip Yule reference against 4 SPR-perturbed
+    bootstrap trees (plain per replicate vs batched, from edge matrices, from
+    Newick text with TBE support), and 4,096-tip trees x 64 replicates."""
+    ex = {}
     # transfer distance: 65,536-tip Yule reference vs SPR-perturbed bootstrap trees
     rng = np.random.default_rng(4)
     ntip = 65536
@@ -249,13 +265,6 @@ def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
     ex["tbe_4096_tips_64_replicates"] = {"plain_jobs_per_s": len(boots) / plain_s,
                                          "batched_jobs_per_s": len(boots) / batch_s,
                                          "speedup": plain_s / batch_s}
-    print("# TBE done", file=sys.stderr, flush=True)
-    ex["tipmerge_10000x4096"] = tip_merge_step(cp, cells, q, with_cpu)
-    print("# tipMerge done", file=sys.stderr, flush=True)
-    ex["scan_100000x81920_bit"] = scan_form(cp, synth)
-    print("# scan form done", file=sys.stderr, flush=True)
-    ex["serving_mix"] = serving_mix(cp, synth)
-    print("# serving done", file=sys.stderr, flush=True)
     return ex
 
 
@@ -523,9 +532,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
-    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta"],
-                    help="run only the tipMerge split step, the nodeKeep step, the voteRoot2 pass or the "
-                         "fasta2bin encoding (profiling)")
+    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe"],
+                    help="run only the tipMerge split step, the nodeKeep step, the voteRoot2 pass, the "
+                         "fasta2bin encoding or the transfer-distance jobs (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -552,6 +561,9 @@ def main():
         return
     if args.only == "voteroot":
         print(json.dumps(vote_root_step(cp, synth, synth.dna_matrix(seed=3, n=100000, ncol=1000), not args.no_cpu)))
+        return
+    if args.only == "tbe":
+        print(json.dumps(tbe_step(cp, synth)))
         return
     if args.only == "fasta":
         print(json.dumps(fasta_step(cp, synth.dna_matrix(seed=3, n=100000, ncol=1000), not args.no_cpu)))
