@@ -10,6 +10,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "context.hpp"
@@ -58,9 +60,83 @@ hipStream_t thread_stream()
     return g_stream;
 }
 
+namespace {
+// Copies of 64 MiB and more between pageable host memory and the device go
+// through two pinned 32 MiB buffers: the copy engine fills (or drains) one
+// while host threads copy the other out (or in).  The buffers are allocated
+// on the first such copy and kept until the process ends; one copy at a time
+// uses them.
+constexpr size_t STAGED_MIN = 64ull << 20, STAGE_CHUNK = 32ull << 20;
+constexpr unsigned STAGE_THREADS = 4;
+std::mutex g_stage_mu;
+void *g_stage[2] = {nullptr, nullptr};
+
+void parallel_copy(void *dst, const void *src, size_t bytes)
+{
+    const size_t part = (bytes / STAGE_THREADS + 4095) / 4096 * 4096;
+    std::thread t[STAGE_THREADS - 1];
+    for (unsigned k = 1; k < STAGE_THREADS; ++k) {
+        const size_t off = std::min(bytes, k * part), len = std::min(bytes - off, part);
+        t[k - 1] = std::thread([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len); });
+    }
+    std::memcpy(dst, src, std::min(bytes, part));
+    for (auto &x : t) x.join();
+}
+
+void staged_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    for (auto &b : g_stage)
+        if (!b) CASPHY_HIP(hipHostMalloc(&b, STAGE_CHUNK, hipHostMallocPortable));
+    hipEvent_t ev[2];
+    CASPHY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    const hipError_t e1 = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    if (e1 != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        CASPHY_HIP(e1);
+    }
+    const size_t nchunk = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    auto len = [&](size_t k) { return std::min(STAGE_CHUNK, bytes - k * STAGE_CHUNK); };
+    hipError_t err = hipSuccess;
+    auto run = [&](hipError_t e) {
+        if (err == hipSuccess) err = e;
+        return err == hipSuccess;
+    };
+    if (kind == hipMemcpyDeviceToHost) {
+        // chunk k into buffer k % 2; chunk k + 1 is in flight while chunk k is copied out
+        auto issue = [&](size_t k) {
+            return run(hipMemcpyAsync(g_stage[k & 1], static_cast<const char *>(src) + k * STAGE_CHUNK, len(k),
+                                      hipMemcpyDeviceToHost, s)) &&
+                   run(hipEventRecord(ev[k & 1], s));
+        };
+        issue(0);
+        for (size_t k = 0; k < nchunk && err == hipSuccess; ++k) {
+            if (k + 1 < nchunk && !issue(k + 1)) break;
+            if (!run(hipEventSynchronize(ev[k & 1]))) break;
+            parallel_copy(static_cast<char *>(dst) + k * STAGE_CHUNK, g_stage[k & 1], len(k));
+        }
+    } else {
+        // buffer k % 2 is refilled once the copy of chunk k - 2 out of it is done
+        for (size_t k = 0; k < nchunk && err == hipSuccess; ++k) {
+            if (k >= 2 && !run(hipEventSynchronize(ev[k & 1]))) break;
+            parallel_copy(g_stage[k & 1], static_cast<const char *>(src) + k * STAGE_CHUNK, len(k));
+            if (!run(hipMemcpyAsync(static_cast<char *>(dst) + k * STAGE_CHUNK, g_stage[k & 1], len(k),
+                                    hipMemcpyHostToDevice, s)))
+                break;
+            run(hipEventRecord(ev[k & 1], s));
+        }
+    }
+    run(hipStreamSynchronize(s));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    CASPHY_HIP(err);
+}
+}  // namespace
+
 void h2d(void *dst, const void *src, size_t bytes, hipStream_t s)
 {
     if (!bytes) return;
+    if (bytes >= STAGED_MIN) return staged_copy(dst, src, bytes, hipMemcpyHostToDevice, s);
     CASPHY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     CASPHY_HIP(hipStreamSynchronize(s));
 }
@@ -71,6 +147,7 @@ void d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
         CASPHY_HIP(hipStreamSynchronize(s));
         return;
     }
+    if (bytes >= STAGED_MIN) return staged_copy(dst, src, bytes, hipMemcpyDeviceToHost, s);
     CASPHY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     CASPHY_HIP(hipStreamSynchronize(s));
 }

@@ -120,6 +120,22 @@ def test_fasta_encode(hip, orc, nseq, length):
             assert np.array_equal(g, w), alphabet
 
 
+def test_fasta_encode_staged_copies(hip, orc):
+    """67,109 x 1,001 characters: the upload (67.2 MB) and the read-back of
+    the codes (67.1 MB) are above 64 MiB and go through the pinned staging
+    buffers in 32 MiB chunks with a partial last chunk; identical to the
+    oracle's encoding."""
+    rng = np.random.default_rng(91)
+    a = np.frombuffer(b"ACGT-N", np.uint8)
+    seqs = a[rng.integers(0, a.size, (67109, 1001))]
+    assert seqs.nbytes >= 64 << 20
+    got = hip.fasta_encode(seqs)
+    assert got[2].nbytes >= 64 << 20
+    want = orc.fasta_encode(seqs)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
 def test_fasta_encode_full_size(hip):
     """100,000 x 1,000 characters of the JC69 alignment: each kept column's
     codes decode back to the characters, dropped columns are constant, valid
