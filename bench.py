@@ -1,30 +1,36 @@
 """Benchmark of the CasPhy hot path on one MI355X.
 
-Workload (the base-editor configuration): 10,000 cells x 4,096 sites packed
-into 128 u32 words per cell, simulated on a random division tree with
-per-generation point mutation p = 0.005 and 5% missing (seed 2).  One step is
-one plain pass of the radius hot path over the resident matrix:
+Headline workload (the DNA configuration): 100,000 sequences x 1,000 columns
+(ACGT with 2% gaps, evolved down a Yule tree under Jukes-Cantor, seed 3) in
+fasta2bin byte codes, resident in HBM.  One step is the autoSplit radius
+search of R/CasPhy_functions.R:1964-2025 as R runs it for DNA
+(radius_initial 100, step 10, nReplicate 1,024, nNeighborBound 9.5) over the
+resident cells:
 
-  1. tryRadius for 1,024 seeded query rows at radii 1..16 (one launch);
-  2. the radius that autoSplit's rule takes from those counts
-     (R/CasPhy_functions.R:2004-2021 with radius_initial = 1, step = 1:
-     the first r whose log10(mean count) <= 9.5 - log10(n)), chosen on the
-     device;
-  3. radiusNeighbor lists at that radius for all 10,000 cells (upper-triangle
-     pair tiles -> neighbour bitmask -> counts -> scan -> on-device compaction).
-All three run as one device round (casphy_cells_autosplit) with one host
-synchronisation; before timing, its result is checked against the host-driven
-loop (tryRadius counts back to the host, host rule, radiusNeighbor).
+  1. tryRadiusDNA for 1,024 seeded query rows at the radii 100, 110, ...,
+     16 radii per device round, until a radius r passes the rule
+     log10(mean count) <= 9.5 - log10(n), chosen on the device;
+  2. in the round where one passes, radiusNeighborDNA lists at r for all
+     100,000 sequences (upper-triangle pair tiles -> neighbour bitmask ->
+     counts -> scan -> on-device compaction; 2.6e9 ids, more than 2^31).
+Each round is one device pass with one host synchronisation
+(casphy_cells_autosplit); before timing, the result is checked against the
+host-driven loop (tryRadius counts back to the host, host rule,
+radiusNeighbor) and sampled rows against the CPU oracle.
 
-value = cell-pair scores per second: (n(n+1)/2 + 1024 n) scores per step.
-The matrix is resident in HBM before timing; per step only the radii and the
-query list (4 KB) go up and the chosen radius and list length (16 B) come back.
+value = cell-pair scores per second: (rounds x 1,024 x n + n(n+1)/2) scores
+per step.  The sequences are resident before timing; per round only the radii
+and the query list (4 KB) go up and the choice and list length (16 B) come
+back.  The other configurations are reported under "extra".
 """
 from __future__ import annotations
 
 import argparse
+import functools
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -33,17 +39,104 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N, NSITE, R = 10000, 4096, 1024
-RADII = np.arange(1, 17, dtype=np.int32)
-CHECK_REPLAYS = 50  # repeated device rounds checked against the first before timing
+# the DNA configuration (headline)
+N3, NCOL3, R = 100000, 1000, 1024
+DNA_RADIUS_INITIAL, DNA_STEP = 100, 10   # autoSplit defaults, R/CasPhy_functions.R:1964-1966
 AUTOSPLIT_BOUND = 9.5  # nNeighborBound default of the autoSplit loop, R/CasPhy_functions.R:1967
+# the base-editor configuration
+N, NSITE = 10000, 4096
+RADII = np.arange(1, 17, dtype=np.int32)
+CHECK_REPLAYS = 50  # repeated device rounds checked against the first before timing (base-editor round)
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
 MFMA_I8_PEAK_TOPS = 5000.0                # MI355X dense int8 matrix-core peak, 5.0 POP/s
 MFMA_FP4_PEAK_TOPS = 10100.0              # MI355X dense fp4 matrix-core peak, 10.1 PFLOP/s
+HBM_PEAK_GBS = 8000.0
+CPU_THREADS_MAX = 16                      # the GPU box's CPU share
+
+
+@functools.lru_cache(maxsize=None)
+def host_info() -> dict:
+    """Core count and CPU model of the host the CPU baselines run on."""
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"nproc": aff, "cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_threads() -> int:
+    return max(1, min(CPU_THREADS_MAX, host_info()["nproc"]))
+
+
+def cpu_rows_timed(fn, nrows: int, chunk: int, threads: int, budget_s: float, check=None):
+    """Runs fn(i0, i1) over consecutive row chunks on `threads` host threads
+    (ctypes calls release the GIL) until `budget_s` has passed; check(i0,
+    result) compares each chunk with the GPU.  Returns (rows done, seconds)."""
+    from concurrent.futures import ThreadPoolExecutor
+    rows_done, nxt = 0, 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < budget_s and nxt < nrows:
+            jobs = []
+            for _ in range(threads):
+                if nxt >= nrows:
+                    break
+                i1 = min(nxt + chunk, nrows)
+                jobs.append((nxt, i1, ex.submit(fn, nxt, i1)))
+                nxt = i1
+            for i0, i1, f in jobs:
+                res = f.result()
+                if check is not None:
+                    check(i0, i1, res)
+                rows_done += i1 - i0
+    return rows_done, time.perf_counter() - t0
+
+
+def cpu_radius_baseline(kind: str, mat: np.ndarray, radius: int, counts: np.ndarray, idx: np.ndarray,
+                        offsets: np.ndarray, budget_s: float = 8.0, stride: int = 1) -> dict:
+    """The reference's CPU radius path (radiusWeightedParallel /
+    radiusWeightedDNAParallel restated in the oracle: per query row, one pass
+    over every target) on bounded samples of query rows, on 1 host thread and
+    on all of this process's host threads; each chunk's counts and ids are
+    compared with the GPU lists.  Rows are taken every `stride` rows."""
+    from oracle import oracle
+    n = mat.shape[0]
+    rows = np.arange(0, n, stride)
+
+    def run(i0, i1):
+        out = []
+        for k in range(i0, i1):
+            r = int(rows[k])
+            out.append((r, oracle.radius_pass(kind, mat, radius, (r, r + 1))))
+        return out
+
+    def check(i0, i1, res):
+        for r, (c, ids) in res:
+            assert c[0] == counts[r], "CPU/GPU count mismatch"
+            assert np.array_equal(ids, idx[offsets[r]:offsets[r + 1]]), "CPU/GPU list mismatch"
+
+    out = {}
+    for threads in (1, cpu_threads()):
+        done, el = cpu_rows_timed(run, rows.size, 2, threads, budget_s, check)
+        out[threads] = (done * n / el, done, el)
+    t1, tall = 1, cpu_threads()
+    return {"value": out[tall][0], "unit": "cell-pair scores/s", "cores": tall, "kind": "port",
+            "value_1_thread": out[t1][0], "host": host_info(),
+            "sample": f"{out[tall][1]} query rows x {n} targets at radius {radius} on {tall} threads "
+                      f"({out[tall][2]:.1f} s), {out[t1][1]} rows on 1 thread ({out[t1][2]:.1f} s); "
+                      f"every row's count and ids equal the GPU lists"}
 
 
 def autosplit_radius(counts: np.ndarray, n: int) -> int:
-    """Host form of the autoSplit stopping rule over the tried radii."""
+    """Host form of the autoSplit stopping rule over the tried radii 1..16."""
     bound = AUTOSPLIT_BOUND - np.log10(n)
     for k, r in enumerate(RADII):
         if np.log10(max(counts[k].mean(), 1e-30)) <= bound:
@@ -51,35 +144,33 @@ def autosplit_radius(counts: np.ndarray, n: int) -> int:
     return -1
 
 
-def cpu_baseline(mat: np.ndarray, radius: int, gpu_counts: np.ndarray, budget_s: float = 12.0) -> dict:
-    """The reference's CPU radius path (radiusWeightedParallel restated in the
-    oracle: per query, float sum over its set sites) on a bounded sample of
-    query rows, one row chunk per host thread."""
-    from concurrent.futures import ThreadPoolExecutor
+def dna_search(cells, q, n: int, bound: float, start: int = DNA_RADIUS_INITIAL, step: int = DNA_STEP,
+               limit: int = NCOL3):
+    """R's autoSplit radius search on the device: rounds of 16 radii
+    (start, start + step, ...) until one passes; returns (radius, ids, rounds)."""
+    rounds = 0
+    while True:
+        radii = np.arange(start, start + 16 * step, step, dtype=np.int32)
+        r, total = cells.autosplit(radii, q, bound)
+        rounds += 1
+        if r >= 0 or radii[-1] >= limit:
+            return r, total, rounds
+        start += 16 * step
 
-    from oracle import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
-    chunk = 8
-    rows_done = 0
-    t0 = time.perf_counter()
-    nxt = 0
-    with ThreadPoolExecutor(threads) as ex:
-        while time.perf_counter() - t0 < budget_s and nxt < mat.shape[0]:
-            jobs = []
-            for _ in range(threads):
-                if nxt >= mat.shape[0]:
-                    break
-                jobs.append((nxt, ex.submit(oracle.radius_bit, mat, radius, None, (nxt, min(nxt + chunk, mat.shape[0])))))
-                nxt += chunk
-            for start, f in jobs:
-                c, _ = f.result()
-                assert np.array_equal(c, gpu_counts[start:start + c.size]), "CPU/GPU count mismatch"
-                rows_done += c.size
-    el = time.perf_counter() - t0
-    return {"value": rows_done * mat.shape[0] / el, "unit": "cell-pair scores/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{rows_done} query rows x {mat.shape[0]} cells at radius {radius} "
-                      f"(radiusWeightedParallel per-query loop, {threads} threads, {el:.1f} s)"}
+
+def dna_search_host(cells, q, n: int, bound: float, start: int = DNA_RADIUS_INITIAL, step: int = DNA_STEP,
+                    limit: int = NCOL3):
+    """The same search as R runs it with the executables: one tryRadius pass
+    per radius, the rule on the host, then the radiusNeighbor lists."""
+    r = start
+    tries = 0
+    while r <= limit:
+        cnt = cells.try_radii([r], q)[0]
+        tries += 1
+        if np.log10(max(cnt.mean(), 1e-30)) <= bound:
+            return r, cells.radius(r), tries
+        r += step
+    return -1, 0, tries
 
 
 def calibration_copy(nbytes: int = 256 << 20) -> None:
@@ -105,15 +196,16 @@ VARIANT_KERNEL = {"128x1": "k_mfma_tiles<0, 128, 1, 2>", "128x2": "k_mfma_tiles<
                   "fp4": "k_mfma_fp4<0>", "fp4b": "k_mfma_fp4b<0>", "fp4d": "k_mfma_fp4d<0>",
                   "fp4w": "k_mfma_fp4w<0>"}
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_mfma_tiles_mask_10000x4096_pmc.md")
+TRAFFIC_TABLE_DNA = os.path.join(ROOT, "profiles", "k_mfma_mask_dna_100000x1000_pmc.md")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
+def measured_traffic(kernel: str, table: str = TRAFFIC_TABLE):
+    """HBM bytes per launch of `kernel` on a workload, from the committed
     summary of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled
     by what the counter reads for a 256 MiB copy); None when not measured."""
-    if not os.path.exists(TRAFFIC_TABLE):
+    if not os.path.exists(table):
         return None
-    for line in open(TRAFFIC_TABLE):
+    for line in open(table):
         cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
         if len(cells) >= 6 and cells[0] == kernel:
             return int(cells[-1].replace(",", ""))  # last column: HBM bytes per launch
@@ -128,75 +220,181 @@ def timed(fn, reps=1):
     return (time.perf_counter() - t0) / reps, out
 
 
-def extras(cp, synth, cells, q, with_cpu: bool = True) -> dict:
-    """The other configurations, plain vs batched, each a device-resident or
-    host-API pass as a user would run it; every batched result is compared
-    with its plain counterpart."""
-    ex = {}
+def mask_roofline(cp, tile_ms: float, pairs: int, ops_per_pair: int, shape: tuple, traffic_table: str,
+                  valu: bool = False, valu_ops_per_pair: int = 0) -> dict:
+    """Roofline of a radius-mask launch: achieved ops over its hipEvent
+    duration against the peak of the form the tile autotuner chose for the
+    shape class (epilogue 0, triangle, log2 rows, sites)."""
+    tile_s = tile_ms / 1000
+    if valu:
+        roof = {"bound": "valu", "achieved": valu_ops_per_pair * pairs / tile_s / 1e12, "peak": VALU_PEAK_TOPS,
+                "unit": "Tlane-op/s", "kernel": "k_tiles<DNA2, MASK>", "variant": "valu"}
+    else:
+        tuning = cp.mfma_tuning()
+        chosen = [e["chosen"] for e in tuning if e.get("epilogue") == 0 and e.get("triangle") == 1 and
+                  (e.get("log2_rows_a"), e.get("sites")) == shape]
+        forced = os.environ.get("CASPHY_MFMA_VARIANT")
+        variant = forced if forced in VARIANT_KERNEL else (chosen[0] if chosen else "64x2")
+        fp4 = variant.startswith("fp4")
+        roof = {"bound": "mfma", "achieved": ops_per_pair * pairs / tile_s / 1e12,
+                "peak": MFMA_FP4_PEAK_TOPS if fp4 else MFMA_I8_PEAK_TOPS, "unit": "TOP/s",
+                "kernel": VARIANT_KERNEL[variant], "variant": variant}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = measured_traffic(roof["kernel"], traffic_table)
+    roof["kernel_ms"] = tile_ms
+    roof["algorithmic"] = (f"{valu_ops_per_pair if valu else ops_per_pair} ops per cell pair x {pairs} pairs "
+                           f"per launch")
+    return roof
+
+
+def base_editor_round(cp, synth, with_cpu: bool, steps: int = 100, warmup: int = 10) -> dict:
+    """The base-editor configuration (10,000 cells x 4,096 sites, seed 2): one
+    device autoSplit round (tryRadius of 1,024 queries at radii 1..16, the
+    radius choice, radiusNeighbor lists) per step on the resident .bmat, after
+    checking it against the host-driven loop and 50 graph replays; the mask
+    tiles' roofline from their hipEvents; the CPU path on 1 and all threads."""
+    mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)
+    q = synth.query_rows(2, N, R)
+    cells = cp.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
+    bound = AUTOSPLIT_BOUND - np.log10(N)
+    counts = cells.try_radii(RADII, q)
+    r_host = autosplit_radius(counts, N)
+    t_host = cells.radius(r_host)
+    c_host, i_host = cells.fetch()
+    r_dev, t_dev, c_dev = cells.autosplit(RADII, q, bound, counts=True)
+    c_d, i_d = cells.fetch()
+    assert r_dev == r_host and t_dev == t_host and np.array_equal(c_dev, counts)
+    assert np.array_equal(c_d, c_host) and np.array_equal(i_d, i_host)
+    for _ in range(CHECK_REPLAYS):
+        assert cells.autosplit(RADII, q, bound) == (r_dev, t_dev)
+    for _ in range(warmup):
+        cells.autosplit(RADII, q, bound)
+    tile_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cells.autosplit(RADII, q, bound)
+        tile_ms.append(cells.last_ms()[0])
+    el = time.perf_counter() - t0
+    pairs = N * (N + 1) // 2 + R * N
+    out = {"cells": N, "sites": NSITE, "queries": R, "radius": r_dev, "neighbour_ids": int(t_dev),
+           "steps": steps, "ms_per_round": 1e3 * el / steps, "cell_pair_scores_per_s": pairs * steps / el,
+           "roofline": mask_roofline(cp, float(np.mean(tile_ms)), N * (N + 1) // 2, 2 * NSITE, (14, NSITE),
+                                     TRAFFIC_TABLE)}
+    if with_cpu:
+        off = np.zeros(N + 1, np.int64)
+        off[1:] = np.cumsum(c_d, dtype=np.int64)
+        out["cpu_baseline"] = cpu_radius_baseline("bit", mat, r_dev, c_d, i_d, off, budget_s=6.0)
+    del c_host, i_host, c_d, i_d
     # tryRadius: one launch per radius (the autoSplit loop) vs all 16 radii at once
     plain_s, per = timed(lambda: [cells.try_radii([r], q) for r in RADII], 3)
     batch_s, allr = timed(lambda: cells.try_radii(RADII, q), 3)
     assert all(np.array_equal(per[k][0], allr[k]) for k in range(len(RADII)))
-    ex["tryradius_16_radii"] = {"plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
-                                "speedup": plain_s / batch_s}
-    print("# tryRadius done", file=sys.stderr, flush=True)
+    out["tryradius_16_radii"] = {"plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3, "speedup": plain_s / batch_s}
+    # binary tree build of the 10,000-cell matrix
+    cp.join_bit(mat)  # first call of the shape class runs the tile autotuner
+    join_s, _ = timed(lambda: cp.join_bit(mat), 1)
+    out["binary_join"] = {"plain_jobs_per_s": 1 / join_s, "ms_per_job": join_s * 1e3}
+    out["nodekeep"] = node_keep_step(cp, mat, with_cpu)
+    out["tipmerge"] = tip_merge_step(cp, cells, q, with_cpu)
+    cells.close()
+    return out
 
-    # indel tree builds, 1,024 cells x 200 sites: one join vs bootstrap replicates in one launch
+
+def indel_config(cp, synth, with_cpu: bool) -> dict:
+    """The indel configuration (1,024 cells x 200 sites, seed 1): the full
+    shared-mutation score matrix, one tree build, and 256 bootstrap
+    replicates (site resampling) in one persistent launch; the CPU path
+    (greedyJoinEdge2 and countApoFloat restated in the oracle) on 1 thread
+    and on all threads (replicates across threads, as R's fork cluster)."""
     base = synth.indel_matrix(seed=1)
+    n, ncol = base.shape
+    cp.score_indel_matrix(base)
+    sc_s, S = timed(lambda: cp.score_indel_matrix(base), 5)
     rng = np.random.default_rng(11)
-    reps = [base[:, rng.integers(0, base.shape[1], base.shape[1])] for _ in range(256)]
+    reps = [base[:, rng.integers(0, ncol, ncol)] for _ in range(256)]
     cp.join_indel(reps[0])
     plain_s, e_plain = timed(lambda: cp.join_indel(reps[0]), 3)
     batch_s, e_batch = timed(lambda: cp.join_indel_batch(reps), 1)
     assert np.array_equal(e_batch[0], e_plain[0])
-    ex["indel_join_1024x200"] = {"plain_jobs_per_s": 1 / plain_s, "batched_jobs_per_s": len(reps) / batch_s,
-                                 "batched_jobs": len(reps), "speedup": plain_s * len(reps) / batch_s}
-    print("# indel joins done", file=sys.stderr, flush=True)
+    out = {"cells": n, "sites": ncol,
+           "score_matrix": {"ms": sc_s * 1e3, "cell_pair_scores_per_s": n * n / sc_s},
+           "plain_jobs_per_s": 1 / plain_s, "batched_jobs_per_s": len(reps) / batch_s, "batched_jobs": len(reps),
+           "speedup": plain_s * len(reps) / batch_s}
+    if with_cpu:
+        from concurrent.futures import ThreadPoolExecutor
 
-    # binary tree build of the 10,000-cell matrix
-    mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)
-    cp.join_bit(mat)  # first call of the shape class runs the tile autotuner
-    join_s, _ = timed(lambda: cp.join_bit(mat), 1)
-    ex["binary_join_10000x4096"] = {"plain_jobs_per_s": 1 / join_s, "ms_per_job": join_s * 1e3}
-    ex["nodekeep_10000x4096"] = node_keep_step(cp, mat, with_cpu)
-    print("# binary join done", file=sys.stderr, flush=True)
+        from oracle import oracle
 
-    # DNA 100,000 x 1,000: tuning in chunks of 16 radii from radius_initial 100, step 10, then the lists
-    t = time.perf_counter()
-    dna = synth.dna_matrix(seed=3, n=100000, ncol=1000)
-    gen = time.perf_counter() - t
-    dcells = cp.Cells(dna, "dna")
-    dq = synth.query_rows(3, dna.shape[0], R)
-    bound = 9.5 - np.log10(dna.shape[0])
-    start, radius, tries = 100, None, 0
-    t0 = time.perf_counter()
-    while radius is None and start <= 1000:
-        radii = np.arange(start, start + 160, 10, dtype=np.int32)
-        cnt = dcells.try_radii(radii, dq)
-        tries += 1
-        for k, r in enumerate(radii):
-            if np.log10(max(cnt[k].mean(), 1e-30)) <= bound:
-                radius = int(r)
-                break
-        start += 160
-    tune_s = time.perf_counter() - t0
-    rad_s, total = timed(lambda: dcells.radius(radius), 1)
-    rad_s, total = timed(lambda: dcells.radius(radius), 2)
-    tile_ms, _ = dcells.last_ms()
-    npairs = dna.shape[0] * (dna.shape[0] + 1) // 2
-    ex["dna_radius_100000x1000"] = {"radius": radius, "tuning_launches": tries, "tuning_ms": tune_s * 1e3,
-                                   "list_pass_ms": rad_s * 1e3, "pair_tile_ms": tile_ms, "neighbour_ids": int(total),
-                                   "cell_pair_scores_per_s": npairs / rad_s, "input_gen_s": gen}
-    dcells.close()
-    print("# DNA done", file=sys.stderr, flush=True)
+        def weights(m):
+            return np.ones((max(1, int(m.max())), m.shape[1]), np.float32)
+        t = time.perf_counter()
+        S2 = oracle.score_indel_matrix(base, weights(base))
+        sc_cpu = time.perf_counter() - t
+        assert np.array_equal(S2.astype(np.int32), S), "CPU/GPU indel score mismatch"
+        k1 = 0
+        t = time.perf_counter()
+        while time.perf_counter() - t < 4.0 and k1 < len(reps):
+            e2, _, _ = oracle.join_indel(reps[k1], weights(reps[k1]))
+            assert np.array_equal(e2, e_batch[k1]), "CPU/GPU indel join mismatch"
+            k1 += 1
+        one = k1 / (time.perf_counter() - t)
+        th = cpu_threads()
+        kk = min(len(reps), 4 * th)
+        t = time.perf_counter()
+        with ThreadPoolExecutor(th) as ex:
+            res = list(ex.map(lambda m: oracle.join_indel(m, weights(m))[0], reps[:kk]))
+        allc = kk / (time.perf_counter() - t)
+        assert all(np.array_equal(res[k], e_batch[k]) for k in range(kk)), "CPU/GPU indel join mismatch"
+        out["cpu_baseline"] = {"value": allc, "unit": "tree-build jobs/s", "cores": th, "kind": "port",
+                               "value_1_thread": one, "host": host_info(),
+                               "score_matrix_ms_1_thread": sc_cpu * 1e3,
+                               "sample": f"{kk} bootstrap joins on {th} threads, {k1} on 1 thread "
+                                         f"(greedyJoinEdge2 restated), edges equal the GPU's"}
+    return out
+
+
+def dna_plain_vs_batched(cp, dna, dq, bound: float, batched: dict) -> dict:
+    """The DNA configuration as the executables would run it: VALU tiles on
+    the 2-bit + mask form, one tryRadius launch per radius with the rule on
+    the host, then the radiusNeighbor lists (plain), against the headline's
+    autotuned device rounds (batched); the same radius and list length."""
+    os.environ["CASPHY_DNA_FORM"] = "valu"
+    try:
+        cells = cp.Cells(dna, "dna")
+        dna_search_host(cells, dq, dna.shape[0], bound)
+        el, (r, total, tries) = timed(lambda: dna_search_host(cells, dq, dna.shape[0], bound), 1)
+        tile_ms, _ = cells.last_ms()
+        cells.close()
+    finally:
+        os.environ.pop("CASPHY_DNA_FORM", None)
+    assert r == batched["radius"] and total == batched["neighbour_ids"], "plain/batched DNA search mismatch"
+    n = dna.shape[0]
+    pairs = tries * R * n + n * (n + 1) // 2
+    return {"plain": {"form": "VALU 2-bit + mask tiles, one tryRadius launch per radius", "tryradius_launches": tries,
+                      "ms_per_search": el * 1e3, "mask_tiles_ms": tile_ms, "cell_pair_scores_per_s": pairs / el},
+            "batched_autotuned": {"form": f"device rounds of 16 radii, {batched['form']}",
+                                  "ms_per_search": batched["ms_per_step"],
+                                  "mask_tiles_ms": batched["mask_tiles_ms"]},
+            "speedup": el * 1e3 / batched["ms_per_step"]}
+
+
+def extras(cp, synth, dna, dq, headline: dict, with_cpu: bool = True) -> dict:
+    """The other configurations, plain vs batched, each a device-resident or
+    host-API pass as a user would run it; every batched result is compared
+    with its plain counterpart."""
+    ex = {}
+    bound = AUTOSPLIT_BOUND - np.log10(dna.shape[0])
+    ex["dna_100000x1000_plain_vs_batched"] = dna_plain_vs_batched(cp, dna, dq, bound, headline)
+    print("# DNA plain search done", file=sys.stderr, flush=True)
     ex["dna_vote_root_10000_tips"] = vote_root_step(cp, synth, dna, with_cpu)
     ex["dna_fasta_encode_100000x1000"] = fasta_step(cp, dna, with_cpu)
     print("# DNA format edge done", file=sys.stderr, flush=True)
-
+    ex["base_editor_10000x4096"] = base_editor_round(cp, synth, with_cpu)
+    print("# base-editor round done", file=sys.stderr, flush=True)
+    ex["indel_1024x200"] = indel_config(cp, synth, with_cpu)
+    print("# indel configuration done", file=sys.stderr, flush=True)
     ex.update(tbe_step(cp, synth))
     print("# TBE done", file=sys.stderr, flush=True)
-    ex["tipmerge_10000x4096"] = tip_merge_step(cp, cells, q, with_cpu)
-    print("# tipMerge done", file=sys.stderr, flush=True)
     ex["scan_100000x81920_bit"] = scan_form(cp, synth)
     print("# scan form done", file=sys.stderr, flush=True)
     ex["serving_mix"] = serving_mix(cp, synth)
@@ -266,14 +464,6 @@ def tbe_step(cp, synth) -> dict:
                                          "batched_jobs_per_s": len(boots) / batch_s,
                                          "speedup": plain_s / batch_s}
     return ex
-
-
-def mfma_dtype(roof: dict) -> str:
-    """Operand type of the pair tiles: fp4 (e2m1 1.0 / 0.0 sites, exact fp32
-    sums), int8 (0/1 bytes, int32 sums) or u32 words on the VALU."""
-    if roof["bound"] != "mfma":
-        return "u32"
-    return "fp4" if roof["variant"].startswith("fp4") else "int8"
 
 
 TIPMERGE_RADII = np.arange(20, 340, 20, dtype=np.int32)
@@ -527,14 +717,13 @@ def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 800
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
-    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe"],
-                    help="run only the tipMerge split step, the nodeKeep step, the voteRoot2 pass, the "
-                         "fasta2bin encoding or the transfer-distance jobs (profiling)")
+    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe", "base", "indel"],
+                    help="run only one of the other configurations (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -545,103 +734,99 @@ def main():
 
     if args.calibrate:
         calibration_copy()
-
-    t = time.perf_counter()
-    mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)
-    q = synth.query_rows(2, N, R)
-    print(f"# input generated in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    bmat = np.ascontiguousarray(mat.T)  # .bmat payload (word-major)
-    cells = cp.Cells(bmat, "bit", wordmajor=True)
-
-    if args.only == "tipmerge":
-        print(json.dumps(tip_merge_step(cp, cells, q, not args.no_cpu)))
+    with_cpu = not args.no_cpu
+    if args.only in ("tipmerge", "nodekeep"):
+        mat = synth.binary_matrix(seed=2, n=N, nsite=NSITE, p=0.005, missing=0.05)
+        if args.only == "nodekeep":
+            print(json.dumps(node_keep_step(cp, mat, with_cpu)))
+        else:
+            cells = cp.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
+            print(json.dumps(tip_merge_step(cp, cells, synth.query_rows(2, N, R), with_cpu)))
         return
-    if args.only == "nodekeep":
-        print(json.dumps(node_keep_step(cp, mat, not args.no_cpu)))
-        return
-    if args.only == "voteroot":
-        print(json.dumps(vote_root_step(cp, synth, synth.dna_matrix(seed=3, n=100000, ncol=1000), not args.no_cpu)))
+    if args.only in ("voteroot", "fasta"):
+        dna = synth.dna_matrix(seed=3, n=N3, ncol=NCOL3)
+        step = vote_root_step(cp, synth, dna, with_cpu) if args.only == "voteroot" else fasta_step(cp, dna, with_cpu)
+        print(json.dumps(step))
         return
     if args.only == "tbe":
         print(json.dumps(tbe_step(cp, synth)))
         return
-    if args.only == "fasta":
-        print(json.dumps(fasta_step(cp, synth.dna_matrix(seed=3, n=100000, ncol=1000), not args.no_cpu)))
+    if args.only == "base":
+        print(json.dumps(base_editor_round(cp, synth, with_cpu, steps=args.steps, warmup=args.warmup)))
         return
-    bound = AUTOSPLIT_BOUND - np.log10(N)
+    if args.only == "indel":
+        print(json.dumps(indel_config(cp, synth, with_cpu)))
+        return
 
-    def step():
-        # tryRadius at 16 radii, radius choice and radius lists: one device round
-        return cells.autosplit(RADII, q, bound)
+    t = time.perf_counter()
+    dna = synth.dna_matrix(seed=3, n=N3, ncol=NCOL3)
+    q = synth.query_rows(3, N3, R)
+    print(f"# input generated in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    cells = cp.Cells(dna, "dna")  # upload, one-hot / 2-bit forms, form tuner
+    form = [e for e in cp.mfma_tuning() if "dna_form" in e]
+    form = form[-1] if form else {"dna_form": os.environ.get("CASPHY_DNA_FORM", "mfma")}
+    bound = AUTOSPLIT_BOUND - np.log10(N3)
 
-    # the device round equals the host-driven loop (tryRadius, host rule, radiusNeighbor)
-    counts = cells.try_radii(RADII, q)
-    r_host = autosplit_radius(counts, N)
-    t_host = cells.radius(r_host)
+    # the device rounds equal the host-driven loop (tryRadius per radius, host
+    # rule, radiusNeighbor); the first search also runs the tile autotuner
+    r_host, t_host, tries = dna_search_host(cells, q, N3, bound)
     c_host, i_host = cells.fetch()
-    r_dev, t_dev, c_dev = cells.autosplit(RADII, q, bound, counts=True)
-    c_d, i_d = cells.fetch()
-    assert r_dev == r_host and t_dev == t_host and np.array_equal(c_dev, counts)
-    assert np.array_equal(c_d, c_host) and np.array_equal(i_d, i_host)
-    del c_host, i_host, c_d, i_d
-    # the round, replayed from its graph, gives the same choice and list size every time
-    for _ in range(CHECK_REPLAYS):
-        assert step() == (r_dev, t_dev)
+    r_dev, t_dev, rounds = dna_search(cells, q, N3, bound)
+    c_dev, i_dev = cells.fetch()
+    assert r_dev == r_host and t_dev == t_host, "device rounds / host loop mismatch"
+    assert np.array_equal(c_dev, c_host) and np.array_equal(i_dev, i_host), "device rounds / host loop lists differ"
+    del c_host, i_host
+    print(f"# checked: radius {r_dev}, {t_dev} ids, {rounds} rounds", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        step()
+        assert dna_search(cells, q, N3, bound)[:2] == (r_dev, t_dev)
     cp._lib.check(cp._lib.load().casphy_synchronize())
-    tile_ms = []
+    tile_ms, round_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r, total = step()
-        tile_ms.append(cells.last_ms()[0])
+        r, total, nr = dna_search(cells, q, N3, bound)
+        tm, rm = cells.last_ms()  # the last round: the one that builds the lists
+        tile_ms.append(tm)
+        round_ms.append(rm)
     cp._lib.check(cp._lib.load().casphy_synchronize())
     el = time.perf_counter() - t0
+    assert (r, total, nr) == (r_dev, t_dev, rounds)
     ms = 1000 * el / args.steps
-    pairs = N * (N + 1) // 2 + R * N
+    pairs_tile = N3 * (N3 + 1) // 2
+    pairs = rounds * R * N3 + pairs_tile
     value = pairs * args.steps / el
-
-    # roofline of the dominant kernel (pair tiles with the radius-mask epilogue)
-    tile_s = float(np.mean(tile_ms)) / 1000
-    pairs_tile = N * (N + 1) // 2
-    if os.environ.get("CASPHY_MFMA", "1") != "0":
-        # int8 matrix cores: one MAC (2 ops) per site per pair
-        roof = {"bound": "mfma", "achieved": 2 * NSITE * pairs_tile / tile_s / 1e12, "peak": MFMA_I8_PEAK_TOPS,
-                "unit": "TOP/s", "kernel": "k_mfma_tiles<MASK>"}
-    else:
-        # VALU: AND + popcount-accumulate per pair-word
-        roof = {"bound": "valu", "achieved": 2 * (NSITE // 32) * pairs_tile / tile_s / 1e12,
-                "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s", "kernel": "k_tiles<AND, MASK>"}
-    tuning = cp.mfma_tuning() if roof["bound"] == "mfma" else []
-    chosen = [e["chosen"] for e in tuning if e["epilogue"] == 0 and e["triangle"] == 1]
-    forced = os.environ.get("CASPHY_MFMA_VARIANT")
-    variant = forced if forced in VARIANT_KERNEL else (chosen[0] if chosen else "64x2")
-    if roof["bound"] == "mfma":
-        roof["variant"] = variant
-        roof["kernel"] = VARIANT_KERNEL[variant]
-        if variant.startswith("fp4"):  # 0/1 sites as fp4 1.0/0.0: priced against the fp4 matrix-core peak
-            roof["peak"] = MFMA_FP4_PEAK_TOPS
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic(roof["kernel"])
-    roof["kernel_ms"] = tile_s * 1000
-    roof["algorithmic"] = f"{2 * NSITE} ops per cell pair x {pairs_tile} pairs per launch"
+    valu = form.get("dna_form") == "valu"
+    # algorithmic work of the mask tiles: 4 one-hot sites (A, C, G, T) per
+    # position, one multiply-add per site = 8,000 ops per pair (matrix cores);
+    # 32 positions per 3 words x 6 lane-ops = 192 per pair (VALU 2-bit form)
+    roof = mask_roofline(cp, float(np.mean(tile_ms)), pairs_tile, 2 * 4 * NCOL3, (17, 4096), TRAFFIC_TABLE_DNA,
+                         valu=valu, valu_ops_per_pair=6 * ((NCOL3 + 31) // 32))
+    headline = {"radius": r_dev, "neighbour_ids": int(t_dev), "ms_per_step": ms,
+                "mask_tiles_ms": float(np.mean(tile_ms)), "form": "VALU 2-bit + mask tiles" if valu else
+                f"one-hot sites on the {roof.get('variant')} matrix-core tiles"}
     out = {
         "metric": "cell-pair scores/s + tree-build & TBE jobs/s on 1 GPU, batched vs plain, bit-exact",
         "value": value, "unit": "cell-pair scores/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": mfma_dtype(roof), "data": "synthetic",
-        "config": {"workload": "base-editor binary 10,000 cells x 4,096 sites (128 u32/cell): tryRadius "
-                               "1,024 queries x radii 1..16 + radiusNeighbor lists at the autoSplit radius",
-                   "cells": N, "sites": NSITE, "queries": R, "radius": r, "neighbour_ids": int(total)},
+        "vs_baseline": None, "dtype": "u32" if valu else ("fp4" if roof["variant"].startswith("fp4") else "int8"),
+        "data": "synthetic (Yule tree, Jukes-Cantor, 2% gaps, seed 3)",
+        "config": {"workload": "DNA MSA 100,000 sequences x 1,000 columns: autoSplit radius search as R runs it "
+                               "(tryRadiusDNA 1,024 queries at radii 100, 110, ..., 16 per device round) + "
+                               "radiusNeighborDNA lists at the chosen radius",
+                   "cells": N3, "columns": NCOL3, "queries": R, "radius": r_dev, "device_rounds": rounds,
+                   "neighbour_ids": int(t_dev), "dna_form": form,
+                   "last_round_ms": float(np.mean(round_ms))},
         "roofline": roof,
-        "mfma_tuning": tuning,
+        "mfma_tuning": cp.mfma_tuning(),
     }
-    if not args.no_cpu:
-        cells.radius(r)
-        gc, _ = cells.fetch()
-        out["cpu_baseline"] = cpu_baseline(mat, r, gc)
+    if with_cpu:
+        off = np.zeros(N3 + 1, np.int64)
+        off[1:] = np.cumsum(c_dev, dtype=np.int64)
+        out["cpu_baseline"] = cpu_radius_baseline("dna", dna, r_dev, c_dev, i_dev, off, budget_s=8.0, stride=97)
+    del c_dev, i_dev
+    cells.close()
+    print("# headline done", file=sys.stderr, flush=True)
     if not args.no_extra:
-        out["extra"] = extras(cp, synth, cells, q, not args.no_cpu)
+        out["extra"] = extras(cp, synth, dna, q, headline, with_cpu)
     print(json.dumps(out))
 
 

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -240,14 +241,44 @@ BitMat upload_bit(const u32 *host, u32 W, u32 n, bool wordmajor, hipStream_t s)
     return m;
 }
 
-// DNA in the metric chosen for its content (2-bit + mask or nibbles).
+// DNA in the form chosen for its content.  With A/C/G/T and gaps only, a
+// position's code has at most one nucleotide bit, so the low nibbles of the
+// byte codes are one-hot bits of (position, base): two sequences share a base
+// at a position exactly when their one-hot bits there share a set bit, and
+// S(i, j) is the AND-popcount of the nibble rows -- the matrix-core tiles
+// score it as bit cells of 4 sites per position (M_AND, unit weights).  The
+// VALU form of the same data is 2-bit + mask (M_DNA2, 12 bytes per 32
+// positions).  IUPAC ambiguity codes (several bits in a nibble) make the
+// share test an OR over bits, which is no dot product: those run as nibbles
+// on the VALU tiles (M_NIB).
+// CASPHY_DNA_FORM=mfma|valu forces the form of ACGT data; the default is the
+// matrix cores when they are enabled (resident cells time both, see
+// tune_dna_form).
 struct DnaMat {
     DevBuf d;
     u32 n = 0, ldw = 0;
     int metric = M_DNA2;
+    DevBuf d2;          // the 2-bit + mask copy kept for the form tuner (M_AND only)
+    u32 ldw2 = 0;
 };
 
-DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s)
+int dna_form_env()  // 1 = matrix cores, 0 = VALU, -1 = not forced
+{
+    const char *e = std::getenv("CASPHY_DNA_FORM");
+    if (e && !std::strcmp(e, "mfma")) return 1;
+    if (e && !std::strcmp(e, "valu")) return 0;
+    return -1;
+}
+
+void pack_dna2(const u32 *raw, u32 n, u32 nwRaw, DevBuf &d, u32 &ldw, hipStream_t s)
+{
+    ldw = round_up(std::max<u32>(3 * ceil_div(nwRaw, 8), 1), KC_DNA2);
+    d.alloc((size_t)n * ldw * 4);
+    launch_dna_pack2(raw, n, nwRaw, d.as<u32>(), ldw, s);
+}
+
+// keep_valu: also keep the 2-bit + mask copy when the one-hot form is chosen
+DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s, bool keep_valu = false)
 {
     DnaMat m;
     m.n = n;
@@ -257,17 +288,19 @@ DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s)
     launch_dna_detect(raw.as<u32>(), (u64)n * nwRaw, flag.as<u32>(), s);
     u32 ambiguous = 0;
     d2h(&ambiguous, flag.p, 4, s);
-    if (!ambiguous) {
-        const u32 groups = ceil_div(nwRaw, 8);
-        m.metric = M_DNA2;
-        m.ldw = round_up(std::max<u32>(3 * groups, 1), KC_DNA2);
-        m.d.alloc((size_t)n * m.ldw * 4);
-        launch_dna_pack2(raw.as<u32>(), n, nwRaw, m.d.as<u32>(), m.ldw, s);
-    } else {
-        m.metric = M_NIB;
+    const int forced = dna_form_env();
+    const bool onehot = !ambiguous && mfma_enabled() && forced != 0;
+    if (onehot || ambiguous) {
+        m.metric = ambiguous ? M_NIB : M_AND;
         m.ldw = round_up(std::max<u32>(ceil_div(nwRaw, 2), 1), KC_BIT);
+        CASPHY_REQUIRE(!onehot || 32ull * m.ldw < (1u << 24), CASPHY_EARG,
+                       "DNA rows of more than 4,194,304 positions exceed the exact fp32 sums of the matrix cores");
         m.d.alloc((size_t)n * m.ldw * 4);
         launch_dna_pack_nib(raw.as<u32>(), n, nwRaw, m.d.as<u32>(), m.ldw, s);
+        if (onehot && keep_valu && forced < 0) pack_dna2(raw.as<u32>(), n, nwRaw, m.d2, m.ldw2, s);
+    } else {
+        m.metric = M_DNA2;
+        pack_dna2(raw.as<u32>(), n, nwRaw, m.d, m.ldw, s);
     }
     CASPHY_HIP(hipStreamSynchronize(s));
     return m;
@@ -289,11 +322,11 @@ void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
     launch_to_stage_major(dmat, n, ldw, ldw, P.as<u32>(), s);
 }
 
-// Radius mask (threshold `radius`, or *thr_dev when given), per-row counts and
-// u64 offsets (offsets[n] = total) on the stream; nothing is read back.
-void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
-                 const int32_t *thr_dev, RadiusOut &o, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
-                 const u32 *stage, const u32 *estage = nullptr)
+// Radius-mask tiles (threshold `radius`, or *thr_dev when given) between the
+// events ev0 / ev1 (may be null); the mask is sized for n rows.
+void radius_mask_tiles(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
+                       const int32_t *thr_dev, RadiusOut &o, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                       const u32 *stage, const u32 *estage = nullptr)
 {
     const u32 nt = ceil_div(n, TILE);
     const u32 mstride = nt * 4;
@@ -323,10 +356,26 @@ void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &plan
         launch_tiles(metric, E_MASK, a, s);
     }
     if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
+}
+
+// Per-row counts and u64 offsets (offsets[n] = total) of the mask of n rows;
+// thr_dev: the mask launch's device threshold (INT32_MAX: it scored nothing).
+void radius_mask_counts(u32 n, const int32_t *thr_dev, RadiusOut &o, hipStream_t s)
+{
+    const u32 mstride = ceil_div(n, TILE) * 4;
     o.counts.reserve((size_t)n * 4);
     o.offsets.reserve((size_t)(n + 1) * 8);
-    launch_row_popc(o.mask.as<u32>(), n, mstride, mstride, o.counts.as<u32>(), s);
+    launch_row_popc(o.mask.as<u32>(), n, mstride, mstride, o.counts.as<u32>(), s, thr_dev);
     launch_scan_counts(o.counts.as<u32>(), n, o.offsets.as<u64>(), s);
+}
+
+// Radius mask, per-row counts and offsets on the stream; nothing is read back.
+void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
+                 const int32_t *thr_dev, RadiusOut &o, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+                 const u32 *stage, const u32 *estage = nullptr)
+{
+    radius_mask_tiles(metric, dmat, n, ldw, planes, radius, thr_dev, o, s, ev0, ev1, stage, estage);
+    radius_mask_counts(n, thr_dev, o, s);
 }
 
 void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
@@ -1076,13 +1125,17 @@ struct casphy_cells {
     // the read-back of (choice, total)
     DevBuf a_in, a_sq, a_eq, a_S, a_counts, a_misc, a_pbuf;
     // captured autoSplit round and the shapes / buffers it was captured with
-    hipGraphExec_t ag_exec = nullptr;
+    // (the round's two small-kernel phases; the mask launch between them is a
+    // plain launch bracketed by the handle's events)
+    hipGraphExec_t ag_exec[2] = {nullptr, nullptr};
     AutosplitKey ag_key{};
     bool ag_seen = false;
     void drop_graph()
     {
-        if (ag_exec) (void)hipGraphExecDestroy(ag_exec);
-        ag_exec = nullptr;
+        for (auto &g : ag_exec) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     }
     void *pin = nullptr;      // pinned host staging, mapped into the device's address space
     void *pin_dev = nullptr;  // the device's address of it
@@ -1098,6 +1151,62 @@ struct casphy_cells {
         pin_bytes = b;
     }
 };
+
+}  // extern "C"
+
+namespace {
+
+// Form tuner of resident ACGT cells: times the radius-mask tiles of the
+// one-hot matrix-core form (after the tile autotuner has picked its variant
+// for the shape) and of the 2-bit + mask VALU form over the first
+// min(n, 8,192) cells, keeps the faster form and records both times in the
+// tuning report.
+void tune_dna_form(casphy_cells &h, DnaMat &m, hipStream_t s)
+{
+    const u32 k = std::min<u32>(h.n, 8192);
+    RadiusOut o;
+    DevBuf stage, estage;
+    make_stage(h.mat.as<u32>(), k, h.ldw, stage, s);
+    estage.alloc(fp4_bytes(stage_rows(k), h.ldw / 4));
+    launch_expand_fp4(stage.as<u32>(), stage_rows(k), h.ldw / 4, estage.as<u32>(), s);
+    hipEvent_t ev[2];
+    CASPHY_HIP(hipEventCreate(&ev[0]));
+    CASPHY_HIP(hipEventCreate(&ev[1]));
+    float ms[2] = {0, 0};
+    try {
+        for (int form = 0; form < 2; ++form) {
+            for (int rep = 0; rep < 2; ++rep) {  // the first run is untimed (variant tuning, code load)
+                if (form == 0)
+                    radius_mask(M_AND, h.mat.as<u32>(), k, h.ldw, Planes{}, 1, nullptr, o, s, ev[0], ev[1],
+                                stage.as<u32>(), estage.as<u32>());
+                else
+                    radius_mask(M_DNA2, m.d2.as<u32>(), k, m.ldw2, Planes{}, 1, nullptr, o, s, ev[0], ev[1], nullptr);
+            }
+            CASPHY_HIP(hipEventSynchronize(ev[1]));
+            CASPHY_HIP(hipEventElapsedTime(&ms[form], ev[0], ev[1]));
+        }
+    } catch (...) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    const bool valu = ms[1] < ms[0];
+    add_tuning_note("{\"dna_form\":\"" + std::string(valu ? "valu" : "mfma") + "\",\"cells\":" + std::to_string(k) +
+                    ",\"positions\":" + std::to_string(8 * h.ldw) + ",\"ms\":{\"mfma\":" + std::to_string(ms[0]) +
+                    ",\"valu\":" + std::to_string(ms[1]) + "}}");
+    if (valu) {
+        h.mat = std::move(m.d2);
+        h.ldw = m.ldw2;
+        h.metric = M_DNA2;
+    }
+    m.d2.release();
+}
+
+}  // namespace
+
+extern "C" {
 
 int casphy_cells_create(int kind, const uint32_t *mat, uint32_t n, uint32_t W, int wordmajor,
                         const double *weight, casphy_cells **out)
@@ -1118,10 +1227,11 @@ int casphy_cells_create(int kind, const uint32_t *mat, uint32_t n, uint32_t W, i
             h->planes = make_planes(weight, 32 * W, h->ldw);
         } else {
             require(!weight, "casphy_cells_create: DNA takes unit weights only");
-            DnaMat m = upload_dna(mat, n, W, s);
+            DnaMat m = upload_dna(mat, n, W, s, true);
             h->ldw = m.ldw;
             h->mat = std::move(m.d);
             h->metric = m.metric;
+            if (m.d2.p) tune_dna_form(*h, m, s);
         }
         for (auto &e : h->ev) CASPHY_HIP(hipEventCreate(&e));
         *out = h.release();
@@ -1175,8 +1285,10 @@ int casphy_cells_tip_merge(casphy_cells *h, double p, int32_t *member)
         for (u32 i = 0; i < n; ++i)
             CASPHY_REQUIRE(nnb[i] <= TIP_MERGE_MAX_LIST, CASPHY_EARG,
                            "casphy_cells_tip_merge: tip " + std::to_string(i) + " has more than 1,048,576 neighbours");
-        tip_merge_device(h->out.offsets.as<u64>(), h->out.idx.as<u32>(), n, h->out.mask.as<u32>(),
-                         ceil_div(n, TILE) * 4, p, member, s);
+        // an autoSplit round in which no radius qualified leaves empty lists and
+        // an unwritten mask: the dense form then builds its rows from the lists
+        tip_merge_device(h->out.offsets.as<u64>(), h->out.idx.as<u32>(), n,
+                         h->out.total ? h->out.mask.as<u32>() : nullptr, ceil_div(n, TILE) * 4, p, member, s);
     });
 }
 
@@ -1241,14 +1353,13 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
 
         // the inputs come in and the results go out through kernels reading and
         // writing the mapped pinned buffer (no copy engine in the round)
-        auto enqueue = [&] {
-            CASPHY_HIP(hipEventRecord(h->ev[2], s));
+        const int32_t *dr = h->a_in.as<int32_t>();
+        const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
+        int32_t *choice = h->a_misc.as<int32_t>();
+        int32_t *dS = h->a_S.as<int32_t>();
+        // phase 0: tryRadius counts of the query rows and the device-side choice
+        auto phase0 = [&] {
             launch_copy_words(static_cast<const u32 *>(h->pin_dev), h->a_in.as<u32>(), (u32)(up / 4), s);
-            const int32_t *dr = h->a_in.as<int32_t>();
-            const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
-            int32_t *choice = h->a_misc.as<int32_t>();
-            // tryRadius counts of the query rows and the device-side choice
-            int32_t *dS = h->a_S.as<int32_t>();
             if (mfma) {
                 launch_to_stage_major(h->mat.as<u32>(), R, ldw, ldw, h->a_sq.as<u32>(), s, dq);
                 launch_expand_fp4(h->a_sq.as<u32>(), stage_rows(R), ldw / 4, h->a_eq.as<u32>(), s);
@@ -1270,51 +1381,63 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
             }
             launch_count_radii(dS, R, n, n, dr, nradii, h->a_counts.as<u32>(), s);
             launch_autosplit_choose(h->a_counts.as<u32>(), R, dr, nradii, smax, any, choice, s);
-            // radiusNeighbor lists at the chosen radius (nothing qualifies: threshold
-            // INT32_MAX, empty lists); ids beyond the current capacity are dropped
-            // and the compaction reruns once the total is known
-            radius_mask(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
-                        mfma ? h->stage.as<u32>() : nullptr, mfma ? h->estage.as<u32>() : nullptr);
+        };
+        // phase 1: counts, offsets and the lists at the chosen radius (nothing
+        // qualifies: threshold INT32_MAX, the tiles and the list kernels skip);
+        // ids beyond the current capacity are dropped and the compaction reruns
+        // once the total is known
+        auto phase1 = [&] {
+            radius_mask_counts(n, choice, o, s);
             if (cap) launch_compact(o.mask.as<u32>(), n, mstride, mstride, o.offsets.as<u64>(), o.idx.as<u32>(), s, cap);
-            CASPHY_HIP(hipEventRecord(h->ev[3], s));
             launch_publish(choice, o.offsets.as<u64>() + n, h->a_counts.as<u32>(), want ? R * nradii : 0,
                            reinterpret_cast<u32 *>(static_cast<char *>(h->pin_dev) + res_off), s);
         };
+        // the radius-mask tiles read their threshold from `choice`: a plain
+        // launch between the two phases, timed by the handle's events
+        auto mask_tiles = [&] {
+            radius_mask_tiles(h->metric, h->mat.as<u32>(), n, ldw, h->planes, 0, choice, o, s, h->ev[0], h->ev[1],
+                              mfma ? h->stage.as<u32>() : nullptr, mfma ? h->estage.as<u32>() : nullptr);
+        };
 
-        // The round is a fixed chain of launches: the second call with the same
-        // shapes and buffers captures it into a graph (the first one also runs
-        // the autotuner), later calls replay the graph.  CASPHY_AUTOSPLIT_GRAPH=0
-        // keeps plain launches.
+        // The two phases are fixed chains of small launches: the second call with
+        // the same shapes and buffers captures each into a graph (the first one
+        // also runs the autotuner), later calls replay them.
+        // CASPHY_AUTOSPLIT_GRAPH=0 keeps plain launches.
         const AutosplitKey key{R, nradii, smax, any, want, cap, o.idx.p, o.mask.p, h->a_S.p, h->a_eq.p, h->a_in.p,
                                h->pin_dev};
         const char *genv = std::getenv("CASPHY_AUTOSPLIT_GRAPH");
         const bool graphs = !(genv && genv[0] == '0');
         h->timed[0] = h->timed[1] = true;
-        if (graphs && h->ag_exec && h->ag_key == key) {
-            CASPHY_HIP(hipGraphLaunch(h->ag_exec, s));
-        } else {
+        auto capture = [&](int k, const std::function<void()> &body) {
+            hipGraph_t g = nullptr;
+            CASPHY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            CASPHY_HIP(hipStreamEndCapture(s, &g));
+            const hipError_t e = hipGraphInstantiate(&h->ag_exec[k], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            CASPHY_HIP(e);
+        };
+        const bool replay = graphs && h->ag_exec[0] && h->ag_exec[1] && h->ag_key == key;
+        if (!replay) {
             h->drop_graph();
             if (graphs && h->ag_seen && h->ag_key == key) {
-                hipGraph_t g = nullptr;
-                CASPHY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue();
-                } catch (...) {
-                    (void)hipStreamEndCapture(s, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                CASPHY_HIP(hipStreamEndCapture(s, &g));
-                const hipError_t e = hipGraphInstantiate(&h->ag_exec, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                CASPHY_HIP(e);
-                CASPHY_HIP(hipGraphLaunch(h->ag_exec, s));
-            } else {
-                enqueue();
+                capture(0, phase0);
+                capture(1, phase1);
             }
-            h->ag_key = key;
-            h->ag_seen = true;
         }
+        CASPHY_HIP(hipEventRecord(h->ev[2], s));
+        if (h->ag_exec[0]) CASPHY_HIP(hipGraphLaunch(h->ag_exec[0], s)); else phase0();
+        mask_tiles();
+        if (h->ag_exec[1]) CASPHY_HIP(hipGraphLaunch(h->ag_exec[1], s)); else phase1();
+        CASPHY_HIP(hipEventRecord(h->ev[3], s));
+        h->ag_key = key;
+        h->ag_seen = true;
         CASPHY_HIP(hipStreamSynchronize(s));
         int32_t ch[2];
         u64 total;

@@ -98,11 +98,17 @@ __global__ void k_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 l
 
 // One wave per row; rows starting on a 16-byte boundary are read as 16-byte
 // words, four loads in flight per lane.
-__global__ void k_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts)
+// A device threshold of INT32_MAX (skip) means the mask launch scored nothing:
+// every count is 0 and the mask is not read.
+__global__ void k_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, const int32_t *skip)
 {
     const u32 lane = threadIdx.x & 63;
     const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (row >= n) return;
+    if (skip && *skip == 0x7FFFFFFF) {
+        if (lane == 0) counts[row] = 0;
+        return;
+    }
     const u32 *m = mask + (size_t)row * mstride;
     u32 c = 0, tail = 0;
     if ((reinterpret_cast<uintptr_t>(m) & 15) == 0) {
@@ -182,22 +188,27 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, 
 // the run of ids 32 w0 .. 32 w0 + 2047 and goes straight to HBM as 16-byte
 // stores (a few single ids align the run); any other chunk takes a wave
 // prefix sum of the word popcounts that places every lane's ids in an LDS
-// staging buffer, then the wave streams the chunk's ids to HBM with
-// consecutive lanes writing consecutive addresses.
+// staging buffer at the position congruent, modulo 4, to their place in the
+// list, so the wave streams the chunk's ids to HBM as aligned 16-byte stores
+// (single ids at the two ends).  The next chunk's words are loaded while the
+// current one is placed.  Rows with an empty list are not read.
 constexpr int CWAVES = 4;
 __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words,
                                                         const u64 *offsets, u32 *idx, u64 cap)
 {
-    __shared__ __attribute__((aligned(16))) u32 stage[CWAVES][64 * 32];
+    __shared__ __attribute__((aligned(16))) u32 stage[CWAVES][64 * 32 + 4];
     const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u32 row = blockIdx.x * CWAVES + wv;
     if (row >= n) return;
     u32 *buf = stage[wv];
     const u32 *m = mask + (size_t)row * mstride;
     u64 base = offsets[row];
+    if (offsets[row + 1] == base) return;  // empty list: the mask row is not read
+    u32 vnext = lane < words ? m[lane] : 0u;
     for (u32 w0 = 0; w0 < words; w0 += 64) {
         const u32 w = w0 + lane;
-        u32 v = w < words ? m[w] : 0u;
+        u32 v = vnext;
+        vnext = w + 64 < words ? m[w + 64] : 0u;
         if (w0 + 64 <= words && __all(v == 0xFFFFFFFFu)) {  // wave-uniform
             const u32 id0 = w0 * 32;
             const u32 head = (u32)((4 - (base & 3)) & 3);
@@ -226,7 +237,9 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
             if (lane >= (u32)o) inc += y;
         }
         const u32 total = __shfl(inc, 63);
-        u32 p = inc - c;
+        if (total == 0) continue;  // wave-uniform
+        const u32 sh = (u32)(base & 3);  // list position base + k sits at buf[sh + k]
+        u32 p = sh + inc - c;
         if (v == 0xFFFFFFFFu && (p & 3) == 0) {
             const u32 id0 = w * 32;
             uint4 *d = reinterpret_cast<uint4 *>(buf + p);
@@ -240,8 +253,22 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        for (u32 k = lane; k < total; k += 64)
-            if (base + k < cap) idx[base + k] = buf[k];
+        // 16-byte groups g of the list positions base - sh + 4g .. + 3 (aligned in
+        // HBM and in LDS); the first and last group may hold positions of other
+        // chunks, which are left alone
+        const u64 a0 = base - sh;
+        const u32 ng = (sh + total + 3) >> 2;
+        for (u32 g = lane; g < ng; g += 64) {
+            const u32 lo = 4 * g;
+            if (lo >= sh && lo + 4 <= sh + total && a0 + lo + 4 <= cap) {
+                *reinterpret_cast<uint4 *>(idx + a0 + lo) = *reinterpret_cast<const uint4 *>(buf + lo);
+            } else {
+                for (u32 e = 0; e < 4; ++e) {
+                    const u32 k = lo + e;
+                    if (k >= sh && k < sh + total && a0 + k < cap) idx[a0 + k] = buf[k];
+                }
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         base += total;
@@ -498,9 +525,10 @@ void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hi
     CASPHY_HIP(hipGetLastError());
 }
 
-void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s)
+void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s,
+                     const int32_t *skip)
 {
-    hipLaunchKernelGGL(k_row_popc, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, counts);
+    hipLaunchKernelGGL(k_row_popc, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, counts, skip);
     CASPHY_HIP(hipGetLastError());
 }
 

@@ -19,7 +19,10 @@ void launch_dna_pack2(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hipSt
 void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hipStream_t s);
 
 // Radius mask -> CSR lists.
-void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s);
+// skip (device, may be null): when it holds INT32_MAX every count is 0 and the
+// mask is not read (a radius-mask launch that scored nothing).
+void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s,
+                     const int32_t *skip = nullptr);
 // Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).
 void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s);
 // Ids at positions >= cap are not written (cap = capacity of idx).

@@ -29,6 +29,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <vector>
 
 namespace casphy {
 
@@ -78,6 +79,15 @@ __device__ __forceinline__ bool tile_of(const MfmaArgs &a, u32 t, u32 &I, u32 &J
     return a.triangle ? (I <= J && J < a.ntA) : (I < a.ntA && J < a.ntB);
 }
 
+
+// A radius-mask launch whose device threshold is INT32_MAX (an autoSplit round
+// in which no radius qualified) scores nothing: no pair can reach it, and the
+// counts and compaction that follow skip the rows without reading the mask.
+template <int EPI>
+__device__ __forceinline__ bool mask_skipped(const MfmaArgs &a)
+{
+    return EPI == E_MASK && a.thr_dev && *a.thr_dev == 0x7FFFFFFF;
+}
 
 constexpr int NTHR = 256;
 
@@ -298,7 +308,7 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -382,7 +392,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_wide(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -468,7 +478,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_wide8(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -590,7 +600,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -691,7 +701,7 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -829,7 +839,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -936,7 +946,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_fp4w(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1061,7 +1071,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_reg(MfmaArgs a)
 {
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
     u32 I, J;
     if (!tile_of(a, t, I, J)) return;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1388,6 +1398,15 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     CASPHY_HIP(hipStreamIsCapturing(s, &cap));
     if (cap != hipStreamCaptureStatusNone) return launch_one(kDefaultVariant, epi, a, s);
+    // a radius mask whose threshold lives in device memory may be a launch that
+    // scores nothing (threshold INT32_MAX): the candidates are timed with a
+    // host threshold instead, and the real launch follows with the choice
+    MfmaArgs t = a;
+    const bool relaunch = epi == E_MASK && a.thr_dev;
+    if (relaunch) {
+        t.thr_dev = nullptr;
+        t.thr = 1;
+    }
     TuneEntry e{};
     for (float &m : e.ms) m = -1.0f;
     hipEvent_t ev[2];
@@ -1397,9 +1416,9 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
     e.variant = kDefaultVariant;
     for (int v : kCandidates) {
         if (!variant_supports(v, epi)) continue;
-        launch_one(v, epi, a, s);  // untimed: first-use costs (code load, scratch pool growth)
+        launch_one(v, epi, t, s);  // untimed: first-use costs (code load, scratch pool growth)
         CASPHY_HIP(hipEventRecord(ev[0], s));
-        launch_one(v, epi, a, s);
+        launch_one(v, epi, t, s);
         CASPHY_HIP(hipEventRecord(ev[1], s));
         CASPHY_HIP(hipEventSynchronize(ev[1]));
         float ms = 0;
@@ -1409,14 +1428,29 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
+    if (relaunch) launch_one(e.variant, epi, a, s);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune.emplace(key, e);
+}
+
+namespace {
+std::vector<std::string> g_notes;  // guarded by g_tune_mu
+}
+
+void add_tuning_note(const std::string &json_object)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_notes.push_back(json_object);
 }
 
 std::string mfma_tuning_report()
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     std::string out = "[";
+    for (auto &note : g_notes) {
+        if (out.size() > 1) out += ",";
+        out += note;
+    }
     for (auto &kv : g_tune) {
         if (out.size() > 1) out += ",";
         out += "{\"epilogue\":" + std::to_string(kv.first.epi) + ",\"triangle\":" + std::to_string(kv.first.tri) +

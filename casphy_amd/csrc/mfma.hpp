@@ -60,7 +60,9 @@ void launch_stage_popc(const u32 *P, u32 n, u32 stride, u32 ldw, u32 *pop, hipSt
 inline size_t fp4_bytes(u32 pad, u32 stages) { return (size_t)stages * 2 * pad * 32; }
 void launch_expand_fp4(const u32 *P, u32 pad, u32 stages, u32 *E, hipStream_t s);
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
-// JSON report of the autotuner's choices (shape class -> variant, timings).
+// JSON report of the autotuner's choices (shape class -> variant, timings),
+// followed by the notes other tuners added (one JSON object each).
 std::string mfma_tuning_report();
+void add_tuning_note(const std::string &json_object);
 
 }  // namespace casphy

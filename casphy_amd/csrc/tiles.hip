@@ -71,6 +71,9 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (t >= a.ntiles) return;
+    // a device threshold of INT32_MAX (autoSplit round with no qualifying
+    // radius): nothing to score; the row counts skip the mask as well
+    if (EPI == E_MASK && a.thr_dev && *a.thr_dev == 0x7FFFFFFF) return;
     u32 I, J;
     tile_coords(a, t, I, J);
 

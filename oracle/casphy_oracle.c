@@ -97,6 +97,18 @@ float orc_score_indel(const int32_t *a, const int32_t *b, uint32_t ncol, const f
     return s;
 }
 
+/* All pairs of countApoFloat as greedyJoinEdge2 fills cacheShared
+ * (src/functions.cpp:1906-1915), the diagonal included: S[i*n+j]. */
+void orc_score_indel_matrix(const int32_t *mat, uint32_t n, uint32_t ncol, const float *w, float *S)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = i; j < n; ++j) {
+            const float v = orc_score_indel(mat + (size_t)i * ncol, mat + (size_t)j * ncol, ncol, w);
+            S[(size_t)i * n + j] = v;
+            S[(size_t)j * n + i] = v;
+        }
+}
+
 /* ---------------------------------------------------------------------------
  * (a-3) DNA similarity.  cu/radiusNeighborDNA.cu:27-53 packs the low nibble of
  * every byte code (cu/fasta2bin.cu:12-32) and :55-87 counts the non-zero nibbles

@@ -30,6 +30,7 @@ void orc_score_bit_matrix(const uint32_t *mat, uint32_t n, uint32_t W, const flo
 /* a, b: ncol int32 states (-1 missing, 0 unmutated, k>0 allele);
  * w: nState x ncol, state-major ((k-1)*ncol + s).                           */
 float orc_score_indel(const int32_t *a, const int32_t *b, uint32_t ncol, const float *w);
+void orc_score_indel_matrix(const int32_t *mat, uint32_t n, uint32_t ncol, const float *w, float *S);
 
 /* ---- (a-3) DNA similarity over fasta2bin byte codes ------------------------ */
 /* a, b: nwRaw u32 words, 4 byte-codes each.  Count of positions whose codes

@@ -58,6 +58,7 @@ def lib():
                                           C.POINTER(C.c_uint64)]
         L.orc_try_radius_bit.argtypes = [u32p, u32, u32, f32p, f32, u32p, u32, u32p]
         L.orc_try_radius_dna.argtypes = [u32p, u32, u32, u32, u32p, u32, u32p]
+        L.orc_score_indel_matrix.argtypes = [i32p, u32, u32, f32p, f32p]
         L.orc_join_bit.restype = i32
         L.orc_join_bit.argtypes = [u32p, u32, u32, f32p, i32, u32p, u32p, u32p]
         L.orc_join_bit_lazy.restype = i32
@@ -119,6 +120,15 @@ def score_indel(a, b, w) -> float:
     return float(lib().orc_score_indel(a, b, a.size, _f32(w)))
 
 
+def score_indel_matrix(mat, w) -> np.ndarray:
+    """All pairs of score_indel, the diagonal included (float32)."""
+    mat = np.ascontiguousarray(mat, dtype=np.int32)
+    n, ncol = mat.shape
+    S = np.empty((n, n), dtype=np.float32)
+    lib().orc_score_indel_matrix(mat, n, ncol, _f32(w), S)
+    return S
+
+
 def score_dna(a, b) -> int:
     a, b = _u32(a), _u32(b)
     return int(lib().orc_score_dna(a, b, a.size))
@@ -148,6 +158,30 @@ def _radius_rows(kind, mat, radius, w, rows=None):
                                    idx.ctypes.data_as(C.c_void_p), total, C.byref(pos))
     assert rc == 0
     return counts, idx[:total]
+
+
+def radius_pass(kind, mat, radius, rows, w=None):
+    """One pass of the reference's per-query radius loop over query rows
+    rows = (i0, i1): (counts, idx) with the ids written as they are found
+    (radiusWeightedParallel / radiusWeightedDNAParallel, src/functions.cpp:
+    1825-1857 / 4021-4036, visit every target once per query).  Used to time
+    the CPU path; radius_bit / radius_dna size their output first."""
+    mat = _u32(mat)
+    n, W = mat.shape
+    i0, i1 = rows
+    counts = np.zeros(i1 - i0, dtype=np.uint32)
+    idx = np.empty((i1 - i0) * n, dtype=np.uint32)
+    pos = C.c_uint64(0)
+    L = lib()
+    if kind == "bit":
+        w = unit_weights_bit(W) if w is None else _f32(w)
+        rc = L.orc_radius_bit_rows(mat, n, W, w, float(radius), i0, i1, counts, idx.ctypes.data_as(C.c_void_p),
+                                   idx.size, C.byref(pos))
+    else:
+        rc = L.orc_radius_dna_rows(mat, n, W, int(radius), i0, i1, counts, idx.ctypes.data_as(C.c_void_p),
+                                   idx.size, C.byref(pos))
+    assert rc == 0
+    return counts, idx[:pos.value]
 
 
 def radius_bit(mat, radius, w=None, rows=None):

@@ -50,7 +50,7 @@ def test_autotuner_report(hip, monkeypatch):
     S1 = hip.score_bit_matrix(mat)     # first call of this shape class: tuning
     S2 = hip.score_bit_matrix(mat)     # cached choice
     assert np.array_equal(S1, S2)
-    rep = [e for e in hip.mfma_tuning() if e["epilogue"] == 1 and e["log2_rows_a"] == 10]  # score epilogue
+    rep = [e for e in hip.mfma_tuning() if e.get("epilogue") == 1 and e["log2_rows_a"] == 10]  # score epilogue
     assert rep and all(e["chosen"] in e["ms"] for e in rep)
     assert all(t > 0 for e in rep for t in e["ms"].values())
 
@@ -69,6 +69,16 @@ def test_weights_outside_contract_rejected(hip):
         with pytest.raises(hip.CasphyError) as e:
             hip.score_bit_matrix(mat, w)
         assert e.value.code == -4
+
+
+def test_score_indel_matrix_config1(hip, orc):
+    """The indel configuration's full 1,024 x 1,024 score matrix (200 sites,
+    alleles from 100 ids) against the oracle's countApoFloat over all pairs."""
+    mat = synth.indel_matrix(seed=1)
+    w = np.ones((mat.max(), mat.shape[1]), dtype=np.float32)
+    S = hip.score_indel_matrix(mat)
+    assert S.shape == (1024, 1024)
+    assert np.array_equal(S, orc.score_indel_matrix(mat, w).astype(np.int32))
 
 
 def test_score_indel_matrix(hip, orc):
@@ -115,8 +125,21 @@ def test_radius_bit_synthetic_config(hip, orc, bitpath):
         assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
-@pytest.mark.parametrize("n,ncol", [(2, 3), (100, 90), (600, 1000)])
-def test_radius_dna_2bit(hip, orc, n, ncol):
+@pytest.fixture(params=["auto", "mfma", "valu"])
+def dnaform(request, monkeypatch):
+    """ACGT data on the one-hot matrix-core form, on the 2-bit + mask VALU
+    form, or on the form the resident cells' tuner picks."""
+    monkeypatch.setenv("CASPHY_MFMA", "1")
+    monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
+    if request.param == "auto":
+        monkeypatch.delenv("CASPHY_DNA_FORM", raising=False)
+    else:
+        monkeypatch.setenv("CASPHY_DNA_FORM", request.param)
+    return request.param
+
+
+@pytest.mark.parametrize("n,ncol", [(2, 3), (100, 90), (600, 1000), (257, 1201)])
+def test_radius_dna_2bit(hip, orc, dnaform, n, ncol):
     mat = synth.dna_matrix(seed=n, n=n, ncol=ncol)
     for r in (1, ncol // 2, int(ncol * 0.9)):
         counts, idx = hip.radius_neighbor_dna(mat, r)
@@ -200,7 +223,7 @@ def test_cells_autosplit_bit(hip, orc, bitpath):
         assert np.array_equal(counts, c2) and np.array_equal(idx, i2), rep
 
 
-def test_cells_autosplit_dna(hip, orc):
+def test_cells_autosplit_dna(hip, orc, dnaform):
     mat = synth.dna_matrix(seed=11, n=400, ncol=300)
     cells = hip.Cells(mat, "dna")
     q = synth.query_rows(12, 400, 130)
@@ -233,7 +256,42 @@ def test_cells_autosplit_limit_exact(hip):
             assert r == autosplit_rule(cnt, radii, float(bound)), (k, bound)
 
 
-def test_try_radius_dna(hip, orc):
+@pytest.mark.parametrize("kind", ["bit", "dna"])
+def test_cells_autosplit_none_qualifies(hip, orc, kind):
+    """A round in which no radius qualifies skips the mask tiles, counts and
+    compaction on the device: empty lists, singleton tipMerge labels, and the
+    next qualifying round (replayed from its graph) is exact again."""
+    if kind == "bit":
+        mat = synth.binary_matrix(seed=31, n=900, nsite=1024, p=0.01)
+        cells = hip.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
+        radii = np.arange(1, 17)
+        ref_lists = lambda r: orc.radius_bit(mat, r)  # noqa: E731
+        ref_try = lambda r, q: orc.try_radius_bit(mat, r, q)  # noqa: E731
+    else:
+        mat = synth.dna_matrix(seed=31, n=900, ncol=500)
+        cells = hip.Cells(mat, "dna")
+        radii = np.arange(200, 360, 10)
+        ref_lists = lambda r: orc.radius_dna(mat, r)  # noqa: E731
+        ref_try = lambda r, q: orc.try_radius_dna(mat, r, q)  # noqa: E731
+    q = synth.query_rows(32, mat.shape[0], 200)
+    ref = np.stack([ref_try(int(r), q) for r in radii])
+    good = float(np.log10(ref[len(radii) // 2].mean()))  # some radius qualifies
+    want = autosplit_rule(ref, radii, good)
+    assert 0 <= want <= radii[len(radii) // 2]
+    c2, i2 = ref_lists(want)
+    for rep in range(3):
+        r, total = cells.autosplit(radii, q, good)
+        counts, idx = cells.fetch()
+        assert r == want and np.array_equal(counts, c2) and np.array_equal(idx, i2), rep
+        r, total = cells.autosplit(radii, q, -40.0)
+        assert r == -1 and total == 0, rep
+        counts, idx = cells.fetch()
+        assert not counts.any() and idx.size == 0, rep
+        member = cells.tip_merge(0.8)
+        assert np.array_equal(member, np.arange(mat.shape[0])), rep
+
+
+def test_try_radius_dna(hip, orc, dnaform):
     mat = synth.dna_matrix(seed=11, n=400, ncol=300)
     q = synth.query_rows(12, 400, 130)
     radii = [100, 200, 250, 280, 299]

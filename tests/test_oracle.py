@@ -57,6 +57,17 @@ def test_score_indel_onehot(orc):
     assert orc.score_indel(mat[3], mat[7], wi) == refw[3, 7]
 
 
+def test_score_indel_matrix_closed_form(orc):
+    """The all-pairs form (diagonal included) equals the one-hot closed form,
+    unit and integer weights."""
+    mat = synth.indel_matrix(seed=6, depth=6, nsite=40, p=0.08, nallele=9)
+    w = np.ones((mat.max(), mat.shape[1]), dtype=np.float32)
+    assert np.array_equal(orc.score_indel_matrix(mat, w), brute.score_matrix_indel(mat))
+    wi = np.random.default_rng(3).integers(0, 6, w.shape)
+    assert np.array_equal(orc.score_indel_matrix(mat, wi.astype(np.float32)),
+                          brute.score_matrix_indel(mat, wi.astype(np.int64)))
+
+
 def test_score_dna_iupac_text(orc):
     rng = np.random.default_rng(3)
     alphabet = "ATCGUWMRYKSHDVBN-.X"
