@@ -393,7 +393,7 @@ def extras(cp, synth, dna, dq, headline: dict, with_cpu: bool = True) -> dict:
     print("# base-editor round done", file=sys.stderr, flush=True)
     ex["indel_1024x200"] = indel_config(cp, synth, with_cpu)
     print("# indel configuration done", file=sys.stderr, flush=True)
-    ex.update(tbe_step(cp, synth))
+    ex.update(tbe_step(cp, synth, with_cpu))
     print("# TBE done", file=sys.stderr, flush=True)
     ex["scan_100000x81920_bit"] = scan_form(cp, synth)
     print("# scan form done", file=sys.stderr, flush=True)
@@ -402,40 +402,80 @@ def extras(cp, synth, dna, dq, headline: dict, with_cpu: bool = True) -> dict:
     return ex
 
 
-def tbe_step(cp, synth) -> dict:
-    """Transfer distance: a 65,536-tip Yule reference against 4 SPR-perturbed
-    bootstrap trees (plain per replicate vs batched, from edge matrices, from
-    Newick text with TBE support), and 4,096-tip trees x 64 replicates."""
+TBE_TIPS, TBE_REPS = 65536, 1000
+
+
+def tbe_step(cp, synth, with_cpu: bool = True, nrep: int = TBE_REPS) -> dict:
+    """The batched transfer configuration: a 65,536-tip Yule reference tree
+    (seed 4) against 1,000 bootstrap trees, each ceil(5% of the internal
+    edges) random SPR moves away from it, as R edge matrices.  Batched: every
+    replicate through casphy_transfer_trees in one call (replicates streamed
+    through the device in chunks, tree walks, no bitsets).  Plain: one
+    replicate per call on the bitset pair tiles (the per-replicate
+    TransferDisStream form), on a few replicates, whose minima must equal the
+    batched ones.  CPU: the oracle's brute force over bitset pairs on sampled
+    reference rows, 1 and all threads."""
     ex = {}
-    # transfer distance: 65,536-tip Yule reference vs SPR-perturbed bootstrap trees
     rng = np.random.default_rng(4)
-    ntip = 65536
+    ntip = TBE_TIPS
     parent, leaves = synth.yule_parent(ntip, rng)
-    ref = synth.clade_bitsets(parent, leaves, ntip)
-    moves = int(np.ceil(0.05 * (ntip - 3)))
-    allb, counts = replicate_block(synth, parent, leaves, ntip, moves, 4, rng)
-    boots = np.split(allb, np.cumsum(counts)[:-1])  # views of the one block
-    cp.transfer_dis_batch(ref, allb, ntip, counts)  # autotuner's first call of the shape class
-    cp.transfer_dis(ref, boots[0], ntip)
-    plain_s, pl = timed(lambda: [cp.transfer_dis(ref, b, ntip) for b in boots], 1)
-    batch_s, bt = timed(lambda: cp.transfer_dis_batch(ref, allb, ntip, counts), 1)
-    for k in range(len(boots)):
-        assert np.array_equal(bt[0][k], pl[k][0]) and np.array_equal(bt[2][k], pl[k][2])
-    pairs = ref.shape[0] * int(sum(counts))
-    ex["tbe_65536_tips"] = {"replicates": len(boots), "plain_jobs_per_s": len(boots) / plain_s,
-                            "batched_jobs_per_s": len(boots) / batch_s, "speedup": plain_s / batch_s,
-                            "bipartition_pairs_per_s": pairs / batch_s,
-                            "mfma_tops": 2 * ntip * pairs / batch_s / 1e12}
-    del boots, allb
-    # the same job from trees: R edge matrices in, clade bitsets built on the device
-    trng = np.random.default_rng(14)
     ref_e, ref_n, _ = synth.ape_edges(parent, leaves)
-    reps = [synth.ape_edges(synth.spr(parent, leaves, moves, trng), leaves)[:2] for _ in range(4)]
-    cp.transfer_trees(ref_e, ref_n, [e for e, _ in reps], [n for _, n in reps], ntip)
-    tree_s, tt = timed(lambda: cp.transfer_trees(ref_e, ref_n, [e for e, _ in reps], [n for _, n in reps], ntip), 1)
-    ex["tbe_65536_tips"]["from_trees_jobs_per_s"] = len(reps) / tree_s
-    # ... and from Newick text: read (tips matched by label), clade bitsets,
-    # transfer minima and TBE support in one pass per job set
+    moves = int(np.ceil(0.05 * (ntip - 3)))
+    t = time.perf_counter()
+    reps = synth.spr_edges_batch(parent, leaves, moves, nrep, 4, threads=cpu_threads())
+    gen_s = time.perf_counter() - t
+    edges, nnodes = [e for e, _ in reps], [n for _, n in reps]
+    cp.transfer_trees(ref_e, ref_n, edges[:2], nnodes[:2], ntip)  # first use: code load, pools
+    batch_s, (ymin, ysize, xmin, xsize) = timed(lambda: cp.transfer_trees(ref_e, ref_n, edges, nnodes, ntip), 1)
+    print(f"# TBE batched: {nrep} replicates in {batch_s:.2f} s", file=sys.stderr, flush=True)
+    ny = ymin.shape[1]
+    pairs = sum(ny * (n - 1) for n in nnodes)
+    # plain: one replicate per call on the bitset tiles
+    nplain = 3
+    os.environ["CASPHY_TBE"] = "bitsets"
+    try:
+        cp.transfer_trees(ref_e, ref_n, edges[:1], nnodes[:1], ntip)
+        plain_s, pl = timed(lambda: [cp.transfer_trees(ref_e, ref_n, [edges[b]], [nnodes[b]], ntip)
+                                     for b in range(nplain)], 1)
+    finally:
+        os.environ.pop("CASPHY_TBE", None)
+    for b in range(nplain):
+        assert np.array_equal(pl[b][0][0], ymin[b]) and np.array_equal(pl[b][2][0], xmin[b]), "plain/batched TBE"
+    out = {"tips": ntip, "replicates": nrep, "spr_moves": moves, "clade_rows": ny,
+           "batched_jobs_per_s": nrep / batch_s, "batched_s": batch_s,
+           "plain_jobs_per_s": nplain / plain_s, "plain_form": "bitset pair tiles (fp4 matrix cores), one call per "
+                                                                "replicate",
+           "speedup": (plain_s / nplain) / (batch_s / nrep), "clade_pairs_per_s": pairs / batch_s,
+           "replicates_generated_s": gen_s,
+           "mean_ymin_zero": float((ymin == 0).mean())}
+    if with_cpu:
+        from oracle import oracle
+        Y, _ = oracle.clade_bitsets(ref_e, ref_n, ntip)
+        X, _ = oracle.clade_bitsets(edges[0], nnodes[0], ntip)
+        rows = np.random.default_rng(41).permutation(ny)
+
+        def run(i0, i1):
+            sel = rows[i0:i1]
+            return sel, oracle.transfer(Y[sel], X, ntip)
+
+        def check(i0, i1, res):
+            sel, (y2, ys2, _, _) = res
+            assert np.array_equal(y2, ymin[0][sel]) and np.array_equal(ys2, ysize[sel]), "CPU/GPU TBE mismatch"
+        res = {}
+        for th in (1, cpu_threads()):
+            done, el = cpu_rows_timed(run, ny, 2, th, 5.0, check)
+            res[th] = (done * X.shape[0] / el, done, el)
+        th = cpu_threads()
+        out["cpu_baseline"] = {"value": res[th][0] / (ny * X.shape[0]), "unit": "TBE jobs/s", "cores": th,
+                               "kind": "port", "value_1_thread": res[1][0] / (ny * X.shape[0]),
+                               "clade_pairs_per_s": res[th][0], "host": host_info(),
+                               "sample": f"{res[th][1]} reference rows x {X.shape[0]} replicate rows on {th} threads "
+                                         f"({res[th][2]:.1f} s), {res[1][1]} rows on 1 thread ({res[1][2]:.1f} s), "
+                                         f"brute force over 2,048-word bitsets; every row equals the GPU's"}
+    ex["tbe_65536_tips_1000_replicates"] = out
+    del reps, edges, ymin, xmin
+    # from Newick text: read (tips matched by label), trees in, transfer minima
+    # and TBE support in one pass per job set
     nrng = np.random.default_rng(15)
     ref_text = synth.newick(parent, leaves, rng=nrng)
     rep_text = "\n".join(synth.newick(synth.spr(parent, leaves, moves, nrng), leaves, rng=nrng) for _ in range(4))
@@ -445,9 +485,10 @@ def tbe_step(cp, synth) -> dict:
         return cp.transfer_support(rt, cp.read_newick(rep_text, tip_labels=rt[3]))
     from_newick()
     nwk_s, (sup, _, _) = timed(from_newick, 1)
-    ex["tbe_65536_tips"]["from_newick_jobs_per_s"] = 4 / nwk_s
-    ex["tbe_65536_tips"]["from_newick_mean_support"] = float(np.nanmean(sup))
-    # small bootstrap trees, where one replicate does not fill the GPU: 4,096 tips x 64 replicates
+    ex["tbe_65536_tips_1000_replicates"]["from_newick_jobs_per_s"] = 4 / nwk_s
+    ex["tbe_65536_tips_1000_replicates"]["from_newick_mean_support"] = float(np.nanmean(sup))
+    # small bootstrap trees: 4,096 tips x 64 replicates, bitsets in (plain per
+    # replicate vs one batched launch) and trees in
     ntip = 4096
     parent, leaves = synth.yule_parent(ntip, rng)
     ref = synth.clade_bitsets(parent, leaves, ntip)
@@ -749,7 +790,7 @@ def main():
         print(json.dumps(step))
         return
     if args.only == "tbe":
-        print(json.dumps(tbe_step(cp, synth)))
+        print(json.dumps(tbe_step(cp, synth, with_cpu)))
         return
     if args.only == "base":
         print(json.dumps(base_editor_round(cp, synth, with_cpu, steps=args.steps, warmup=args.warmup)))

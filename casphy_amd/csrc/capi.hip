@@ -24,6 +24,7 @@
 #include "voteroot.hpp"
 #include "nodekeep.hpp"
 #include "tipmerge.hpp"
+#include "tbe.hpp"
 #include "tiles.hpp"
 
 namespace casphy {
@@ -787,6 +788,19 @@ int casphy_transfer_trees(const int32_t *y_anc, const int32_t *y_des, uint32_t y
                 "casphy_transfer_trees: bad arguments");
         require(y_nnode >= 2, "casphy_transfer_trees: the reference tree needs a non-root internal node");
         hipStream_t s = thread_stream();
+        // default: the tree walks (no bitsets); CASPHY_TBE=bitsets builds the
+        // clade bitsets on the device and scores them on the pair tiles
+        const char *form = std::getenv("CASPHY_TBE");
+        if (!(form && !std::strcmp(form, "bitsets"))) {
+            if (nrep) {
+                u64 nx = 0;
+                for (u32 b = 0; b < nrep; ++b) nx += x_nnode[b] > 0 ? x_nnode[b] - 1 : 0;
+                require(nx == 0 || (xmin && xsize), "casphy_transfer_trees: bad arguments");
+            }
+            transfer_trees_dp(y_anc, y_des, y_nedge, y_nnode, x_anc, x_des, x_edge_offsets, x_nnode, nrep, ntip,
+                              ymin, ysize, xmin, xsize, s);
+            return;
+        }
         const CladePlan py = plan_clades(y_anc, y_des, y_nedge, y_nnode, ntip);
         std::vector<CladePlan> px(nrep);
         std::vector<u32> xoff(nrep + 1, 0);

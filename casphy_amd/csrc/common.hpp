@@ -1,8 +1,5 @@
-// common.hpp -- shared definitions of the CasPhy MI355X library.
-//
-// Error convention of the C ABI: every entry point returns 0 on success or a
-// negative CASPHY_E* code, and casphy_last_error() returns a thread-local
-// message.  No C++ exception crosses the ABI.
+// common.hpp -- shared definitions of the CasPhy MI355X library (HIP side;
+// the integer types, Error and CASPHY_REQUIRE are in host.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,17 +10,9 @@
 #include <stdexcept>
 #include <string>
 
-#include "../../include/casphy_hip.h"
+#include "host.hpp"
 
 namespace casphy {
-
-using u32 = uint32_t;
-using u64 = uint64_t;
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
 
 void set_last_error(const std::string &msg);
 
@@ -33,11 +22,6 @@ void set_last_error(const std::string &msg);
         if (e_ != hipSuccess)                                                               \
             throw ::casphy::Error(CASPHY_EHIP, std::string(#call) + ": " + hipGetErrorString(e_) + \
                                                    " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
-    } while (0)
-
-#define CASPHY_REQUIRE(cond, code, msg)                                                     \
-    do {                                                                                    \
-        if (!(cond)) throw ::casphy::Error((code), (msg));                                  \
     } while (0)
 
 // Runs `body`, mapping exceptions to status codes and the thread-local message.
@@ -58,8 +42,6 @@ int guarded(F &&body)
         return CASPHY_EINTERNAL;
     }
 }
-
-inline u32 ceil_div(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 
 // Per-thread stream on the thread's selected device (defined in capi.hip).
 hipStream_t thread_stream();

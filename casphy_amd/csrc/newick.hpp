@@ -4,7 +4,7 @@
 #include <string>
 #include <vector>
 
-#include "common.hpp"
+#include "host.hpp"
 
 namespace casphy {
 

@@ -13,6 +13,7 @@
 #include "context.hpp"
 #include "kernels.hpp"
 #include "nodekeep.hpp"
+#include "plan.hpp"
 
 namespace casphy {
 
@@ -85,21 +86,7 @@ u32 node_keep(bool indel, const u32 *edge, u32 nedge, const u32 *state, u32 nnod
     } else if (nnode && nout && cutoff > 0) {
         retain.assign(nnode, 0u);  // empty rows: no mutation to find
     }
-    // cu/nodeKeepBit.cu:186-211: an edge to internal node d is skipped while
-    // the last evaluated node was kept and the edge leaves a node whose id is
-    // not below it; otherwise d is evaluated
-    u32 current = ntip - 1, flag = 0, nk = 0;
-    for (u32 i = 0; i < nedge; ++i) {
-        const u32 a = edge[2 * i], d = edge[2 * i + 1];
-        if (d + 1 > ntip) {
-            if (flag == 1 && current <= a) continue;
-            flag = retain[d - ntip];
-            if (flag == 1) {
-                current = d;
-                kept[nk++] = d;
-            }
-        }
-    }
+    const u32 nk = node_keep_walk(edge, nedge, retain.data(), ntip, kept);
     return nk;
 }
 

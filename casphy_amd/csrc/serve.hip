@@ -102,7 +102,10 @@ struct casphy_server {
     bool batching = true;  // CASPHY_SERVE_BATCH=0: one job per launch (queue-only form)
 
     void worker_main();
-    void run_group(std::vector<Job *> &group);
+    // Runs a group's launch and fills each job's `out`; returns the status of
+    // every job (0 or a CASPHY_E* code) and the error message.  Job status,
+    // error and completion time are published by worker_main under `m`.
+    std::vector<int> run_group(std::vector<Job *> &group, std::string &err);
 };
 
 void casphy_server::worker_main()
@@ -142,14 +145,30 @@ void casphy_server::worker_main()
             ++launches;
             if (group.size() > 1) batched_jobs += group.size();
         }
-        run_group(group);
+        std::string err;
+        std::vector<int> rc;
+        try {
+            rc = run_group(group, err);
+        } catch (const std::exception &e) {  // host allocation of the outputs
+            rc.assign(group.size(), CASPHY_ENOMEM);
+            err = e.what();
+        }
         {
+            // a waiter may free a job as soon as it sees its status: everything
+            // the worker writes into a job is written here, before the notify,
+            // and nothing after
             std::lock_guard<std::mutex> lk(m);
             in_use -= bytes;
             const auto now = clock_t_::now();
-            for (Job *j : group) {
+            for (size_t k = 0; k < group.size(); ++k) {
+                Job *j = group[k];
                 j->t_done = now;
-                if (j->status == 0) lat_ms.push_back(ms_between(j->t_submit, now));
+                if (rc[k] == 0) {
+                    lat_ms.push_back(ms_between(j->t_submit, now));
+                } else {
+                    j->err = err;
+                }
+                j->status = rc[k];
             }
             t_last = now;
         }
@@ -158,55 +177,55 @@ void casphy_server::worker_main()
     }
 }
 
-void casphy_server::run_group(std::vector<Job *> &group)
+std::vector<int> casphy_server::run_group(std::vector<Job *> &group, std::string &err)
 {
-    auto fail = [&](int rc) {
-        const std::string msg = casphy_last_error();
-        for (Job *j : group) { j->status = rc; j->err = msg; }
+    std::vector<int> rc(group.size(), 0);
+    auto fail = [&](int code) {
+        err = casphy_last_error();
+        std::fill(rc.begin(), rc.end(), code);
+        return rc;
     };
     Job *h = group.front();
     if (h->kind == CASPHY_JOB_TRANSFER) {
         std::vector<u32> &o = h->out;
         o.assign((size_t)2 * h->ny + 2 * h->nx, 0);
         const u32 *Y = h->mat.data(), *X = h->mat.data() + (size_t)h->ny * h->ncol;
-        int rc = casphy_transfer(Y, h->ny, X, h->nx, h->ncol, h->ntip, o.data(), o.data() + h->ny,
-                                 o.data() + 2 * h->ny, o.data() + 2 * h->ny + h->nx);
-        if (rc) fail(rc); else h->status = 0;
-        return;
+        const int r = casphy_transfer(Y, h->ny, X, h->nx, h->ncol, h->ntip, o.data(), o.data() + h->ny,
+                                      o.data() + 2 * h->ny, o.data() + 2 * h->ny + h->nx);
+        return r ? fail(r) : rc;
     }
     for (Job *j : group) j->out.assign((size_t)(2 * j->n - 2) * 2, 0);
     if (group.size() == 1) {
-        int rc = h->kind == CASPHY_KIND_BIT
-                     ? casphy_join_bit(h->mat.data(), h->n, h->width, nullptr, h->anc, h->out.data(), nullptr, nullptr)
-                     : casphy_join_indel(h->imat.data(), h->n, h->width, nullptr, 0, nullptr, 0, 2.0f, h->out.data(),
-                                         nullptr, nullptr);
-        if (rc) fail(rc); else h->status = 0;
-        return;
+        const int r = h->kind == CASPHY_KIND_BIT
+                          ? casphy_join_bit(h->mat.data(), h->n, h->width, nullptr, h->anc, h->out.data(), nullptr,
+                                            nullptr)
+                          : casphy_join_indel(h->imat.data(), h->n, h->width, nullptr, 0, nullptr, 0, 2.0f,
+                                              h->out.data(), nullptr, nullptr);
+        return r ? fail(r) : rc;
     }
     std::vector<u32> offs(group.size() + 1, 0);
     for (size_t k = 0; k < group.size(); ++k) offs[k + 1] = offs[k] + group[k]->n;
     std::vector<u32> edges((size_t)(2 * (u64)offs.back() - 2 * group.size()) * 2);
-    int rc;
+    int r;
     if (h->kind == CASPHY_KIND_BIT) {
         std::vector<u32> all((size_t)offs.back() * h->width);
         for (size_t k = 0; k < group.size(); ++k)
             std::copy(group[k]->mat.begin(), group[k]->mat.end(), all.begin() + (size_t)offs[k] * h->width);
-        rc = casphy_join_bit_batch(all.data(), offs.data(), (u32)group.size(), h->width, nullptr, h->anc,
-                                   edges.data());
+        r = casphy_join_bit_batch(all.data(), offs.data(), (u32)group.size(), h->width, nullptr, h->anc, edges.data());
     } else {
         std::vector<int32_t> all((size_t)offs.back() * h->width);
         for (size_t k = 0; k < group.size(); ++k)
             std::copy(group[k]->imat.begin(), group[k]->imat.end(), all.begin() + (size_t)offs[k] * h->width);
-        rc = casphy_join_indel_batch(all.data(), offs.data(), (u32)group.size(), h->width, nullptr, 0, nullptr, 0,
-                                     2.0f, edges.data());
+        r = casphy_join_indel_batch(all.data(), offs.data(), (u32)group.size(), h->width, nullptr, 0, nullptr, 0,
+                                    2.0f, edges.data());
     }
-    if (rc) { fail(rc); return; }
+    if (r) return fail(r);
     size_t pos = 0;
     for (Job *j : group) {
         std::copy(edges.begin() + pos, edges.begin() + pos + j->out.size(), j->out.begin());
         pos += j->out.size();
-        j->status = 0;
     }
+    return rc;
 }
 
 namespace {

@@ -286,6 +286,84 @@ def ape_edges(parent: np.ndarray, leaves: np.ndarray):
     return np.array(edges, dtype=np.int32).reshape(-1, 2), int(nxt - ntip - 1), ntip
 
 
+def random_tree(rng: np.random.Generator, ntip: int, nint: int):
+    """Random rooted tree in ape's edge order: `nint` internal nodes (a random
+    recursive tree, so some have a single child), tips hung on random
+    internal nodes.  Returns (edge (nedge, 2) int32 1-based, nnode)."""
+    nall = nint + ntip
+    parent = np.full(nall, -1, np.int64)
+    for v in range(1, nint):
+        parent[v] = rng.integers(0, v)
+    parent[nint:] = rng.integers(0, nint, ntip)
+    # every internal node needs a child: childless ones take a tip of a node
+    # with at least two children
+    kids = np.bincount(parent[1:], minlength=nall)
+    for v in range(nint):
+        if kids[v] == 0:
+            donors = [t for t in range(nint, nall) if kids[parent[t]] >= 2]
+            t = donors[int(rng.integers(0, len(donors)))]
+            kids[parent[t]] -= 1
+            parent[t] = v
+            kids[v] += 1
+    leaves = rng.permutation(np.arange(nint, nall))
+    edge, nnode, nt = ape_edges(parent, leaves)
+    assert nt == ntip
+    return edge, nnode
+
+
+def caterpillar(ntip: int):
+    """((((1,2),3),4)...): the deepest binary tree."""
+    parent = np.full(2 * ntip - 1, -1, np.int64)
+    # internal nodes 0..ntip-2 in a chain, node k+1 child of k; tips hang off
+    for k in range(1, ntip - 1):
+        parent[k] = k - 1
+    tips = np.arange(ntip - 1, 2 * ntip - 1)
+    parent[tips[:ntip - 1]] = np.arange(ntip - 1)
+    parent[tips[-1]] = ntip - 2
+    edge, nnode, _ = ape_edges(parent, tips)
+    return edge, nnode
+
+
+_NATIVE = None
+
+
+def _native():
+    """The native tree generator (libcasphy_synth.so, built beside the HIP
+    library); raises when it has not been built."""
+    global _NATIVE
+    if _NATIVE is None:
+        import ctypes as C
+        import os
+        lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcasphy_synth.so"))
+        lib.casphy_synth_spr_edges.restype = C.c_int64
+        lib.casphy_synth_spr_edges.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_uint64,
+                                               C.c_void_p]
+        _NATIVE = lib
+    return _NATIVE
+
+
+def spr_edges_batch(parent: np.ndarray, leaves: np.ndarray, moves: int, nrep: int, seed: int, threads: int = 8):
+    """nrep bootstrap trees of a rooted binary tree, each `moves` random SPR
+    moves away from it (the moves of spr, drawn by a seeded native generator,
+    replicate b with seed (seed, b)), as R edge matrices numbered as
+    ape_edges numbers them: [(edge (nn - 1, 2) int32, nnode)]."""
+    from concurrent.futures import ThreadPoolExecutor
+    lib = _native()
+    par = np.ascontiguousarray(parent, dtype=np.int64)
+    lv = np.ascontiguousarray(leaves, dtype=np.int64)
+    nn = par.size
+
+    def one(b):
+        e = np.empty((nn - 1, 2), np.int32)
+        nnode = lib.casphy_synth_spr_edges(par.ctypes.data, nn, lv.ctypes.data, lv.size, int(moves),
+                                           (int(seed) << 32) ^ int(b), e.ctypes.data)
+        if nnode < 0:
+            raise ValueError("spr_edges_batch: the tree is not a rooted binary parent array")
+        return e, int(nnode)
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(one, range(nrep)))
+
+
 # --------------------------------------------------------------- root votes
 def vote_inputs(seed: int, ntip: int, nsample: int, ncol: int, mat: np.ndarray | None = None):
     """voteRoot2 inputs: a Yule tree on the first ntip rows of a JC69 alignment

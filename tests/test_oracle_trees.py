@@ -123,3 +123,30 @@ def test_reference_bootstrap_trees():
         for a, d in e:
             assert seen[a]
             seen[d] = True
+
+
+def test_native_spr_generator(orc):
+    """The native bootstrap-tree generator: no moves gives ape_edges of the
+    tree itself; with moves, every replicate is a valid cladewise binary tree
+    on the same tips (the oracle's nodeDesBig accepts it), replicates differ
+    from the reference and from each other, and a seed repeats its trees."""
+    rng = np.random.default_rng(30)
+    ntip = 3000
+    parent, leaves = synth.yule_parent(ntip, rng)
+    e0, n0, _ = synth.ape_edges(parent, leaves)
+    (z, nz), = synth.spr_edges_batch(parent, leaves, 0, 1, 7)
+    assert nz == n0 and np.array_equal(z, e0)
+    reps = synth.spr_edges_batch(parent, leaves, 150, 4, 7)
+    again = synth.spr_edges_batch(parent, leaves, 150, 4, 7)
+    Y, _ = orc.clade_bitsets(e0, n0, ntip)
+    seen = []
+    for (e, n), (e2, n2) in zip(reps, again):
+        assert n == ntip - 1 and e.shape == (2 * ntip - 2, 2)
+        assert np.array_equal(e, e2) and n == n2
+        assert sorted(e[e[:, 1] <= ntip, 1].tolist()) == list(range(1, ntip + 1))
+        X, sizes = orc.clade_bitsets(e, n, ntip)
+        assert X.shape == Y.shape and np.all(sizes >= 2)
+        ymin, _, _, _ = orc.transfer(Y, X, ntip)
+        assert ymin.max() > 0  # some reference clade was broken
+        seen.append(X.tobytes())
+    assert len(set(seen)) == len(seen)
