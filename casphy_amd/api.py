@@ -531,28 +531,48 @@ def _bitmask_result(row_list, n):
     return out.view(np.int32)
 
 
+# R calls the per-row functions once per query row (R/CasPhy_functions.R:
+# 160-208, 1730-1780); the lists of every row come from one device pass, kept
+# for the next call with the same matrix, weights and radius (one entry).
+_LISTS = {}
+
+
+def _row_lists(kind: str, m: np.ndarray, weight, radius: int):
+    import hashlib
+    wkey = None if weight is None else hashlib.sha1(np.ascontiguousarray(weight, dtype=np.float64).data).hexdigest()
+    key = (kind, m.shape, radius, hashlib.sha1(m.data).hexdigest(), wkey)
+    hit = _LISTS.get(key)
+    if hit is None:
+        counts, idx = radius_neighbor(m, radius, weight) if kind == "bit" else radius_neighbor_dna(m, radius)
+        off = np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)])
+        _LISTS.clear()
+        hit = _LISTS[key] = (off, idx)
+    return hit
+
+
 def radiusWeightedParallel(i: int, mat, maxChar, weight, radius: float):
     """src/functions.cpp:1825-1857.  mat: (W, n) bytes of the n x W R matrix;
-    returns IntegerVector(nbin+1) = [count, neighbour bitmask words...]."""
+    returns IntegerVector(nbin+1) = [count, neighbour bitmask words...].  The
+    first call for a matrix computes every row's list; later calls for its
+    other rows slice them."""
     m = _u32(mat)
     W, n = m.shape
     r = float(radius)
     if r != int(r):
         raise CasphyError(-1, "radius must be an integer for the exact path")
-    counts, idx = radius_neighbor(m, int(r), weight)
-    off = np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)])
+    off, idx = _row_lists("bit", m, weight, int(r))
     return _bitmask_result(idx[int(off[i]):int(off[i + 1])], n)
 
 
 def radiusWeightedDNAParallel(i: int, mat, weight, radius: float):
-    """src/functions.cpp:4021-4036 with unit weights.  mat: (n, nwRaw)."""
+    """src/functions.cpp:4021-4036 with unit weights.  mat: (n, nwRaw); lists
+    computed once per matrix and radius, as radiusWeightedParallel."""
     m = _u32(mat)
     n = m.shape[0]
     if weight is not None and not np.all(np.asarray(weight) == 1):
         raise CasphyError(-4, "DNA weights other than 1 are outside the exact path")
     r = float(radius)
-    counts, idx = radius_neighbor_dna(m, int(np.ceil(r)))
-    off = np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)])
+    off, idx = _row_lists("dna", m, None, int(np.ceil(r)))
     return _bitmask_result(idx[int(off[i]):int(off[i + 1])], n)
 
 

@@ -294,8 +294,6 @@ DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s, bool keep_va
     if (onehot || ambiguous) {
         m.metric = ambiguous ? M_NIB : M_AND;
         m.ldw = round_up(std::max<u32>(ceil_div(nwRaw, 2), 1), KC_BIT);
-        CASPHY_REQUIRE(!onehot || 32ull * m.ldw < (1u << 24), CASPHY_EARG,
-                       "DNA rows of more than 4,194,304 positions exceed the exact fp32 sums of the matrix cores");
         m.d.alloc((size_t)n * m.ldw * 4);
         launch_dna_pack_nib(raw.as<u32>(), n, nwRaw, m.d.as<u32>(), m.ldw, s);
         if (onehot && keep_valu && forced < 0) pack_dna2(raw.as<u32>(), n, nwRaw, m.d2, m.ldw2, s);
@@ -742,6 +740,26 @@ void transfer_batch_rows(const BitMat &my, const BitMat *mx, const uint32_t *xof
     d2h(ysize, dys.p, (size_t)ny * 4, s);
 }
 
+// Replicates in consecutive groups whose rows fit the device budget of the
+// bitset pair tiles (about 7 row copies per replicate row: upload, padded,
+// stage-major and its fp4 expansion); CASPHY_TBE_BUDGET_MB (default 8,192)
+// sets it.  A replicate larger than the budget forms a group of its own.
+std::vector<std::pair<u32, u32>> replicate_groups(const u32 *xoffsets, u32 nrep, u32 ldw)
+{
+    const char *e = std::getenv("CASPHY_TBE_BUDGET_MB");
+    const u64 budget = (e ? (u64)std::max(1, std::atoi(e)) : 8192ull) << 20;
+    const u64 per_row = 7ull * ldw * 4;
+    std::vector<std::pair<u32, u32>> g;
+    u32 b0 = 0;
+    while (b0 < nrep) {
+        u32 b1 = b0 + 1;
+        while (b1 < nrep && (u64)(xoffsets[b1 + 1] - xoffsets[b0]) * per_row <= budget) ++b1;
+        g.emplace_back(b0, b1);
+        b0 = b1;
+    }
+    return g;
+}
+
 // Clade rows of a tree in a BitMat (rows of ldw words, zero padding).
 BitMat clade_mat(const CladePlan &p, hipStream_t s)
 {
@@ -769,11 +787,24 @@ int casphy_transfer_batch(const uint32_t *Y, uint32_t ny, const uint32_t *X, con
         require(nx == 0 || (xmin && xsize), "casphy_transfer_batch: bad arguments");
         hipStream_t s = thread_stream();
         BitMat my = upload_bit(Y, ncol, ny, false, s);
-        if (nx) {
-            BitMat mx = upload_bit(X, ncol, nx, false, s);
-            transfer_batch_rows(my, &mx, xoffsets, nrep, ntip, 32u * ncol, ymin, ysize, xmin, xsize, s);
-        } else {
-            transfer_batch_rows(my, nullptr, xoffsets, nrep, ntip, 32u * ncol, ymin, ysize, xmin, xsize, s);
+        if (!nrep) {
+            transfer_batch_rows(my, nullptr, xoffsets, 0, ntip, 32u * ncol, ymin, ysize, xmin, xsize, s);
+            return;
+        }
+        // replicates stream through the device in groups; Y stays resident
+        for (const auto &g : replicate_groups(xoffsets, nrep, my.ldw)) {
+            const u32 r0 = xoffsets[g.first], r1 = xoffsets[g.second];
+            std::vector<u32> xo(g.second - g.first + 1);
+            for (u32 b = g.first; b <= g.second; ++b) xo[b - g.first] = xoffsets[b] - r0;
+            u32 *ym = ymin + (size_t)g.first * ny;
+            if (r1 > r0) {
+                BitMat mx = upload_bit(X + (size_t)r0 * ncol, ncol, r1 - r0, false, s);
+                transfer_batch_rows(my, &mx, xo.data(), g.second - g.first, ntip, 32u * ncol, ym, ysize, xmin + r0,
+                                    xsize + r0, s);
+            } else {
+                transfer_batch_rows(my, nullptr, xo.data(), g.second - g.first, ntip, 32u * ncol, ym, ysize, nullptr,
+                                    nullptr, s);
+            }
         }
     });
 }
@@ -813,15 +844,28 @@ int casphy_transfer_trees(const int32_t *y_anc, const int32_t *y_des, uint32_t y
         require(xoff[nrep] == 0 || (xmin && xsize), "casphy_transfer_trees: bad arguments");
         BitMat my = clade_mat(py, s);
         require(my.n > 0, "casphy_transfer_trees: the reference tree has no clade rows");
-        if (xoff[nrep]) {
-            BitMat mx;
-            mx.n = xoff[nrep]; mx.W = py.nbin; mx.ldw = my.ldw;
-            mx.d.alloc((size_t)mx.n * mx.ldw * 4);
-            for (u32 b = 0; b < nrep; ++b)
-                build_clade_rows(px[b], mx.d.as<u32>() + (size_t)xoff[b] * mx.ldw, mx.ldw, s);
-            transfer_batch_rows(my, &mx, xoff.data(), nrep, ntip, 32u * py.nbin, ymin, ysize, xmin, xsize, s);
-        } else {
-            transfer_batch_rows(my, nullptr, xoff.data(), nrep, ntip, 32u * py.nbin, ymin, ysize, xmin, xsize, s);
+        if (!nrep) {
+            transfer_batch_rows(my, nullptr, xoff.data(), 0, ntip, 32u * py.nbin, ymin, ysize, xmin, xsize, s);
+            return;
+        }
+        // replicates' clade rows are built on the device group by group
+        for (const auto &g : replicate_groups(xoff.data(), nrep, my.ldw)) {
+            const u32 r0 = xoff[g.first], r1 = xoff[g.second];
+            std::vector<u32> xo(g.second - g.first + 1);
+            for (u32 b = g.first; b <= g.second; ++b) xo[b - g.first] = xoff[b] - r0;
+            u32 *ym = ymin + (size_t)g.first * my.n;
+            if (r1 > r0) {
+                BitMat mx;
+                mx.n = r1 - r0; mx.W = py.nbin; mx.ldw = my.ldw;
+                mx.d.alloc((size_t)mx.n * mx.ldw * 4);
+                for (u32 b = g.first; b < g.second; ++b)
+                    build_clade_rows(px[b], mx.d.as<u32>() + (size_t)(xoff[b] - r0) * mx.ldw, mx.ldw, s);
+                transfer_batch_rows(my, &mx, xo.data(), g.second - g.first, ntip, 32u * py.nbin, ym, ysize,
+                                    xmin + r0, xsize + r0, s);
+            } else {
+                transfer_batch_rows(my, nullptr, xo.data(), g.second - g.first, ntip, 32u * py.nbin, ym, ysize,
+                                    nullptr, nullptr, s);
+            }
         }
     });
 }

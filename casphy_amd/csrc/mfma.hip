@@ -1384,8 +1384,13 @@ u32 log2_ceil(u32 x)
 
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s)
 {
-    const int forced = variant_from_env();
+    // the fp4 tiles sum in fp32, exact below 2^24: wider rows (every partial
+    // sum is at most the number of sites) take the int32 sums of the int8 tiles
+    const bool fp4_exact = (u64)a.stages * 128 < (1ull << 24);
+    int forced = variant_from_env();
+    if (forced >= kFp4 && !fp4_exact) forced = kDefaultVariant;
     if (forced >= 0) return launch_one(forced, epi, a, s);
+    if (!fp4_exact) return launch_one(kDefaultVariant, epi, a, s);
     const char *at = std::getenv("CASPHY_AUTOTUNE");
     if (at && at[0] == '0') return launch_one(kDefaultVariant, epi, a, s);
     const TuneKey key{epi, a.triangle, log2_ceil(a.nA), log2_ceil(a.nB), a.stages};

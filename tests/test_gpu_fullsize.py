@@ -128,3 +128,71 @@ def test_tbe_large_against_brute_rows(hip, orc):
     assert np.array_equal(xmin[sel], x2) and np.array_equal(xsize[sel], xs2)
     same = hip.transfer_dis(ref, ref, ntip)
     assert np.all(same[0] == 0) and np.all(same[2] == 0)
+
+
+@pytest.mark.timeout(900)
+def test_config3_dna_100k_lists_past_2_31(hip, orc, monkeypatch):
+    """The DNA configuration at full size (100,000 sequences x 1,000 columns):
+    the autoSplit search on the device picks the host rule's radius, and its
+    lists hold more than 2^31 ids (u64 offsets, staged read-back).  Checked:
+    counts sum to the list length, every list ascending with its own row in
+    it, symmetry and tryRadius counts on sampled rows, sampled rows (first,
+    middle and last, whose ids sit past 2^31) against the oracle, and the
+    VALU 2-bit form giving the same counts and id checksum as the one-hot
+    matrix-core form."""
+    monkeypatch.delenv("CASPHY_DNA_FORM", raising=False)
+    n = 100000
+    dna = synth.dna_matrix(seed=3, n=n, ncol=1000)
+    cells = hip.Cells(dna, "dna")
+    q = synth.query_rows(3, n, 1024)
+    bound = 9.5 - np.log10(n)
+    r, total = -1, 0
+    for start in range(100, 1000, 160):
+        radii = np.arange(start, start + 160, 10)
+        cnt = cells.try_radii(radii, q)
+        passing = [int(radii[k]) for k in range(len(radii)) if np.log10(max(cnt[k].mean(), 1e-30)) <= bound]
+        r_dev, t_dev = cells.autosplit(radii, q, bound)
+        if passing:
+            r = passing[0]
+            assert r_dev == r
+            total = t_dev
+            break
+        assert r_dev == -1 and t_dev == 0
+    assert r > 0 and total > 2 ** 31
+    counts, idx = cells.fetch()
+    assert idx.size == total == int(counts.sum(dtype=np.uint64))
+    off = csr(counts, idx)
+    assert off[-1] == total
+    # ascending within every list, own row present (>= 640 non-gap positions per row)
+    step = 10000
+    for i0 in range(0, n, step):
+        i1 = min(n, i0 + step)
+        seg = idx[off[i0]:off[i1]]
+        up = seg[1:] > seg[:-1]
+        starts = (off[i0 + 1:i1] - off[i0] - 1).astype(np.int64)  # last id of a row, then the next row
+        up[starts[(starts >= 0) & (starts < up.size)]] = True
+        assert up.all()
+        rows = np.repeat(np.arange(i0, i1, dtype=np.uint32), counts[i0:i1].astype(np.int64))
+        assert np.count_nonzero(seg == rows) == i1 - i0
+        del seg, up, rows
+    rng = np.random.default_rng(33)
+    sample = np.concatenate([[0, n // 2, n - 1], rng.integers(0, n, 5)])
+    for i in sample:
+        lst = idx[off[i]:off[i + 1]]
+        # symmetric: i is in the list of every j in its list
+        for j in lst[rng.integers(0, lst.size, 50)] if lst.size else []:
+            lj = idx[off[j]:off[j + 1]]
+            k = np.searchsorted(lj, i)
+            assert k < lj.size and lj[k] == i
+        c2, i2 = orc.radius_pass("dna", dna, r, (int(i), int(i) + 1))
+        assert c2[0] == counts[i] and np.array_equal(i2, lst), i
+    assert off[n - 1] > 2 ** 31  # the last rows' ids sit past 2^31
+    assert np.array_equal(cells.try_radii([r], q)[0], counts[q])
+    checksum = int(idx.sum(dtype=np.uint64))
+    cells.close()
+    del idx
+    monkeypatch.setenv("CASPHY_DNA_FORM", "valu")
+    cells = hip.Cells(dna, "dna")
+    assert cells.radius(r) == total
+    c3, i3 = cells.fetch()
+    assert np.array_equal(c3, counts) and int(i3.sum(dtype=np.uint64)) == checksum

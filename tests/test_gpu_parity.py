@@ -42,6 +42,24 @@ def test_score_bit_matrix(hip, orc, bitpath, n, W):
     assert np.array_equal(S, orc.score_bit_matrix(mat).astype(np.int32))
 
 
+def test_score_bit_past_fp32_exact_sums(hip, monkeypatch):
+    """Rows of 2^24 + 1,024 sites (sums above 2^24, where fp32 sums stop being
+    exact): the matrix-core tiles take the int8 form's int32 sums even when
+    an fp4 variant is asked for, and the scores equal the exact popcounts
+    (odd values above 2^24 included)."""
+    monkeypatch.setenv("CASPHY_MFMA", "1")
+    monkeypatch.setenv("CASPHY_MFMA_VARIANT", "fp4")
+    W = (1 << 19) + 32
+    mat = np.full((3, W), 0xFFFFFFFF, np.uint32)
+    mat[1, 5] = 0xFFFFFFFE          # one site fewer
+    mat[2, :W // 2] = 0x55555555    # half the sites in the first half
+    S = hip.score_bit_matrix(mat)
+    pop = lambda a: int(np.unpackbits(a.view(np.uint8)).sum())  # noqa: E731
+    want = np.array([[pop(mat[i] & mat[j]) for j in range(3)] for i in range(3)], np.int64)
+    assert want[0, 1] == (1 << 24) + 1023
+    assert np.array_equal(S.astype(np.int64), want)
+
+
 def test_autotuner_report(hip, monkeypatch):
     monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
     monkeypatch.setenv("CASPHY_MFMA", "1")
@@ -441,6 +459,34 @@ def test_transfer_batch(hip, orc, bitpath):
     assert all(np.array_equal(a, b) for a, b in zip(xm3, xm)) and all(np.array_equal(a, b) for a, b in zip(xs3, xs))
 
 
+def test_transfer_batch_streamed_groups(hip, orc, monkeypatch):
+    """Replicates streamed through the device one group at a time (a budget
+    that holds one replicate per group, empty replicates included) give the
+    oracle's minima, for bitsets in and for trees in on the bitset tiles."""
+    monkeypatch.setenv("CASPHY_TBE_BUDGET_MB", "1")
+    rng = np.random.default_rng(78)
+    ntip = 3000
+    parent, leaves = synth.yule_parent(ntip, rng)
+    ref = synth.clade_bitsets(parent, leaves, ntip)
+    boots = [synth.clade_bitsets(synth.spr(parent, leaves, k, rng), leaves, ntip) for k in (3, 30)]
+    boots.insert(1, boots[0][:0])
+    ymin, ysize, xm, xs = hip.transfer_dis_batch(ref, boots, ntip)
+    for b, X in enumerate(boots):
+        y2, ys2, x2, xs2 = orc.transfer(ref, X, ntip)
+        assert np.array_equal(ymin[b], y2) and np.array_equal(ysize, ys2), b
+        assert np.array_equal(xm[b], x2) and np.array_equal(xs[b], xs2), b
+    monkeypatch.setenv("CASPHY_TBE", "bitsets")
+    ref_e = synth.ape_edges(parent, leaves)[:2]
+    reps = [synth.ape_edges(synth.spr(parent, leaves, 50, rng), leaves)[:2] for _ in range(3)]
+    reps.insert(1, synth.ape_edges(np.array([-1] + [0] * ntip), np.arange(1, ntip + 1))[:2])  # no rows
+    y3, ys3, x3, xs3 = hip.transfer_trees(ref_e[0], ref_e[1], [e for e, _ in reps], [n for _, n in reps], ntip)
+    monkeypatch.delenv("CASPHY_TBE_BUDGET_MB")
+    monkeypatch.delenv("CASPHY_TBE")
+    y4, ys4, x4, xs4 = hip.transfer_trees(ref_e[0], ref_e[1], [e for e, _ in reps], [n for _, n in reps], ntip)
+    assert np.array_equal(y3, y4) and np.array_equal(ys3, ys4)
+    assert all(np.array_equal(a, b) for a, b in zip(x3, x4)) and all(np.array_equal(a, b) for a, b in zip(xs3, xs4))
+
+
 # ------------------------------------------------------------------ scan form
 @pytest.mark.parametrize("n,W", [(700, 256), (333, 300), (1025, 1024)])
 def test_try_radius_scan_form(hip, orc, monkeypatch, n, W):
@@ -477,3 +523,32 @@ def test_radius_full_rows(hip, orc, bitpath):
         counts, idx = cells.fetch()
         c2, i2 = orc.radius_bit(mat, r)
         assert r == radii[0] and np.array_equal(counts, c2) and np.array_equal(idx, i2)
+
+
+def test_rcpp_radius_shims_one_pass(hip, orc, monkeypatch):
+    """radiusWeightedParallel / radiusWeightedDNAParallel, called once per row
+    as R calls them: one device pass per matrix and radius, every row's
+    count and bitmask equal to the oracle's list."""
+    from casphy_amd import api
+    calls = []
+    real_bit, real_dna = api.radius_neighbor, api.radius_neighbor_dna
+    monkeypatch.setattr(api, "radius_neighbor", lambda *a, **k: calls.append("bit") or real_bit(*a, **k))
+    monkeypatch.setattr(api, "radius_neighbor_dna", lambda *a, **k: calls.append("dna") or real_dna(*a, **k))
+    mat = synth.binary_matrix(seed=61, n=300, nsite=512, p=0.01)
+    c2, i2 = orc.radius_bit(mat, 3)
+    off = np.concatenate([[0], np.cumsum(c2, dtype=np.int64)])
+    bmat = np.ascontiguousarray(mat.T)
+    for i in range(300):
+        out = api.radiusWeightedParallel(i, bmat, 1, None, 3.0)
+        lst = i2[off[i]:off[i + 1]]
+        assert out[0] == lst.size
+        bits = np.unpackbits(out[1:].view(np.uint8), bitorder="little")
+        assert np.array_equal(np.nonzero(bits)[0], lst), i
+    dna = synth.dna_matrix(seed=62, n=200, ncol=300)
+    c3, i3 = orc.radius_dna(dna, 200)
+    off = np.concatenate([[0], np.cumsum(c3, dtype=np.int64)])
+    for i in range(200):
+        out = api.radiusWeightedDNAParallel(i, dna, None, 200.0)
+        bits = np.unpackbits(out[1:].view(np.uint8), bitorder="little")
+        assert out[0] == off[i + 1] - off[i] and np.array_equal(np.nonzero(bits)[0], i3[off[i]:off[i + 1]]), i
+    assert calls == ["bit", "dna"]

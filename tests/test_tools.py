@@ -65,6 +65,11 @@ def test_radius_neighbor_tool(tmp_path, orc):
     out = formats.read_bin(pfx + ".nNeighbor")
     assert out.shape == (2, 100)
     assert np.array_equal(out[1], orc.try_radius_bit(mat, 4, out[0]))
+    # integer arguments read as atoi reads them: "2.9" and "2x" are 2
+    for arg in ("2.9", " 2x"):
+        assert run("radiusNeighbor", pfx, arg).returncode == 0
+        nn = formats.read_bin(pfx + ".nNeighbor")
+        assert np.array_equal(nn[0], c2) and np.array_equal(formats.read_idx(pfx + ".idx"), i2), arg
 
 
 @pytest.mark.gpu
