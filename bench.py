@@ -8,7 +8,7 @@ search of R/CasPhy_functions.R:1964-2025 as R runs it for DNA
 resident cells:
 
   1. tryRadiusDNA for 1,024 seeded query rows at the radii 100, 110, ...,
-     16 radii per device round, until a radius r passes the rule
+     64 radii per device round, until a radius r passes the rule
      log10(mean count) <= 9.5 - log10(n), chosen on the device;
   2. in the round where one passes, radiusNeighborDNA lists at r for all
      100,000 sequences (upper-triangle pair tiles -> neighbour bitmask ->
@@ -144,18 +144,21 @@ def autosplit_radius(counts: np.ndarray, n: int) -> int:
     return -1
 
 
+ROUND_RADII = 64  # radii per device round (counting passes of 16 over one score matrix)
+
+
 def dna_search(cells, q, n: int, bound: float, start: int = DNA_RADIUS_INITIAL, step: int = DNA_STEP,
                limit: int = NCOL3):
-    """R's autoSplit radius search on the device: rounds of 16 radii
+    """R's autoSplit radius search on the device: rounds of 64 radii
     (start, start + step, ...) until one passes; returns (radius, ids, rounds)."""
     rounds = 0
     while True:
-        radii = np.arange(start, start + 16 * step, step, dtype=np.int32)
+        radii = np.arange(start, start + ROUND_RADII * step, step, dtype=np.int32)
         r, total = cells.autosplit(radii, q, bound)
         rounds += 1
         if r >= 0 or radii[-1] >= limit:
             return r, total, rounds
-        start += 16 * step
+        start += ROUND_RADII * step
 
 
 def dna_search_host(cells, q, n: int, bound: float, start: int = DNA_RADIUS_INITIAL, step: int = DNA_STEP,
@@ -372,7 +375,7 @@ def dna_plain_vs_batched(cp, dna, dq, bound: float, batched: dict) -> dict:
     pairs = tries * R * n + n * (n + 1) // 2
     return {"plain": {"form": "VALU 2-bit + mask tiles, one tryRadius launch per radius", "tryradius_launches": tries,
                       "ms_per_search": el * 1e3, "mask_tiles_ms": tile_ms, "cell_pair_scores_per_s": pairs / el},
-            "batched_autotuned": {"form": f"device rounds of 16 radii, {batched['form']}",
+            "batched_autotuned": {"form": f"device rounds of 64 radii, {batched['form']}",
                                   "ms_per_search": batched["ms_per_step"],
                                   "mask_tiles_ms": batched["mask_tiles_ms"]},
             "speedup": el * 1e3 / batched["ms_per_step"]}
@@ -851,7 +854,7 @@ def main():
         "vs_baseline": None, "dtype": "u32" if valu else ("fp4" if roof["variant"].startswith("fp4") else "int8"),
         "data": "synthetic (Yule tree, Jukes-Cantor, 2% gaps, seed 3)",
         "config": {"workload": "DNA MSA 100,000 sequences x 1,000 columns: autoSplit radius search as R runs it "
-                               "(tryRadiusDNA 1,024 queries at radii 100, 110, ..., 16 per device round) + "
+                               "(tryRadiusDNA 1,024 queries at radii 100, 110, ..., 64 per device round) + "
                                "radiusNeighborDNA lists at the chosen radius",
                    "cells": N3, "columns": NCOL3, "queries": R, "radius": r_dev, "device_rounds": rounds,
                    "neighbour_ids": int(t_dev), "dna_form": form,

@@ -431,10 +431,11 @@ void radius_lists(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &pla
 
 // Few queries over wide unit-weight bit rows take the scan form (one HBM pass);
 // CASPHY_SCAN=0 keeps the tiles.
-bool use_scan(int metric, const Planes &planes, u32 R, u32 ldw)
+bool use_scan(int metric, const Planes &planes, u32 R, u32 ldw, u32 nradii)
 {
     const char *e = std::getenv("CASPHY_SCAN");
-    return metric == M_AND && planes.nplanes == 0 && R <= SCAN_Q && scan_fits(ldw) && !(e && e[0] == '0');
+    return metric == M_AND && planes.nplanes == 0 && R <= SCAN_Q && nradii <= MAX_RADII_K && scan_fits(ldw) &&
+           !(e && e[0] == '0');
 }
 
 // tryRadius: query x cell scores (matrix-core tiles for unit-weight bits,
@@ -451,7 +452,7 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
     h2d(dq.p, q, (size_t)R * 4, s);
     h2d(dr.p, radii, (size_t)nradii * 4, s);
     const u32 ntA = ceil_div(R, TILE), ntB = ceil_div(n, TILE);
-    if (use_scan(metric, planes, R, ldw)) {
+    if (use_scan(metric, planes, R, ldw, nradii)) {
         // few queries over wide rows: one HBM-bound pass over the cells
         DevBuf qr((size_t)R * ldw * 4);
         for (u32 k = 0; k < R; ++k)
@@ -602,7 +603,7 @@ int casphy_try_radii_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n, 
 {
     return guarded([&] {
         require(mat_wordmajor && radii && q && counts && n > 0 && W > 0, "casphy_try_radii_bit: bad arguments");
-        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_try_radii_bit: 1..16 radii");
+        require(nradii >= 1 && nradii <= MAX_ROUND_RADII, "casphy_try_radii_bit: 1..64 radii");
         if (R == 0) return;
         hipStream_t s = thread_stream();
         BitMat m = upload_bit(mat_wordmajor, W, n, true, s);
@@ -622,7 +623,7 @@ int casphy_try_radii_dna(const uint32_t *mat, uint32_t n, uint32_t nwRaw, const 
 {
     return guarded([&] {
         require(mat && radii && q && counts && n > 0 && nwRaw > 0, "casphy_try_radii_dna: bad arguments");
-        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_try_radii_dna: 1..16 radii");
+        require(nradii >= 1 && nradii <= MAX_ROUND_RADII, "casphy_try_radii_dna: 1..64 radii");
         if (R == 0) return;
         hipStream_t s = thread_stream();
         DnaMat m = upload_dna(mat, n, nwRaw, s);
@@ -1355,10 +1356,10 @@ int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradi
 {
     return guarded([&] {
         require(h && radii && q && counts, "casphy_cells_try_radii: bad arguments");
-        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_cells_try_radii: 1..16 radii");
+        require(nradii >= 1 && nradii <= MAX_ROUND_RADII, "casphy_cells_try_radii: 1..64 radii");
         if (R == 0) return;
         hipStream_t s = thread_stream();
-        const bool tiles = h->mfma() && !use_scan(h->metric, h->planes, R, h->ldw);
+        const bool tiles = h->mfma() && !use_scan(h->metric, h->planes, R, h->ldw, nradii);
         if (tiles) h->staged(s);
         h->timed[0] = true;
         try_radii(h->metric, h->mat.as<u32>(), h->n, h->ldw, h->planes, radii, nradii, q, R, counts, s,
@@ -1371,7 +1372,7 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
 {
     return guarded([&] {
         require(h && radii && q && radius && nidx && R > 0, "casphy_cells_autosplit: bad arguments");
-        require(nradii >= 1 && nradii <= MAX_RADII, "casphy_cells_autosplit: 1..16 radii");
+        require(nradii >= 1 && nradii <= MAX_ROUND_RADII, "casphy_cells_autosplit: 1..64 radii");
         const u32 n = h->n, ldw = h->ldw;
         for (u32 k = 0; k < R; ++k)
             CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
@@ -1385,7 +1386,7 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
 
         // every buffer the round touches is sized before anything is enqueued, so
         // the enqueued work allocates nothing and can be captured into a graph
-        const size_t up = (MAX_RADII + (size_t)R) * 4, res_off = (up + 15) / 16 * 16;
+        const size_t up = (MAX_ROUND_RADII + (size_t)R) * 4, res_off = (up + 15) / 16 * 16;
         const size_t cnt_bytes = (size_t)R * nradii * 4;
         h->pinned(res_off + 16 + cnt_bytes);
         h->a_in.reserve(up);
@@ -1404,15 +1405,15 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         const u64 cap = o.idx.bytes / 4;
 
         int32_t *pr = static_cast<int32_t *>(h->pin);
-        for (u32 k = 0; k < MAX_RADII; ++k) pr[k] = k < nradii ? radii[k] : 0x7FFFFFFF;
-        std::memcpy(pr + MAX_RADII, q, (size_t)R * 4);
+        for (u32 k = 0; k < MAX_ROUND_RADII; ++k) pr[k] = k < nradii ? radii[k] : 0x7FFFFFFF;
+        std::memcpy(pr + MAX_ROUND_RADII, q, (size_t)R * 4);
         char *res = static_cast<char *>(h->pin) + res_off;
         const bool want = counts != nullptr;
 
         // the inputs come in and the results go out through kernels reading and
         // writing the mapped pinned buffer (no copy engine in the round)
         const int32_t *dr = h->a_in.as<int32_t>();
-        const u32 *dq = h->a_in.as<u32>() + MAX_RADII;
+        const u32 *dq = h->a_in.as<u32>() + MAX_ROUND_RADII;
         int32_t *choice = h->a_misc.as<int32_t>();
         int32_t *dS = h->a_S.as<int32_t>();
         // phase 0: tryRadius counts of the query rows and the device-side choice

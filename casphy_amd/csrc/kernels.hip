@@ -192,7 +192,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, 
 // list, so the wave streams the chunk's ids to HBM as aligned 16-byte stores
 // (single ids at the two ends).  The next chunk's words are loaded while the
 // current one is placed.  Rows with an empty list are not read.
-constexpr int CWAVES = 4;
+constexpr int CWAVES = 1;  // one wave per workgroup: 4.44 ms against 4.80 with four on 2.8e9 ids
 __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words,
                                                         const u64 *offsets, u32 *idx, u64 cap)
 {
@@ -414,15 +414,15 @@ __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u3
     }
 }
 
-// One workgroup of 16 waves: wave k sums the R counts of radius k, then the
-// first radius whose sum is at most smax is chosen.
+// One workgroup of 16 waves: wave w sums the R counts of radii w, w + 16, ...,
+// then the first radius whose sum is at most smax is chosen.
 __global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii,
                                                            u32 nradii, u64 smax, int any, int32_t *out)
 {
-    __shared__ u64 sums[MAX_RADII_K];
-    const u32 lane = threadIdx.x & 63, k = threadIdx.x >> 6;
-    u64 c = 0;
-    if (k < nradii) {
+    __shared__ u64 sums[MAX_ROUND_RADII];
+    const u32 lane = threadIdx.x & 63;
+    for (u32 k = threadIdx.x >> 6; k < nradii; k += 16) {
+        u64 c = 0;
         const u32 *row = counts + (size_t)k * R;
         for (u32 q0 = 0; q0 < R; q0 += 512) {  // eight loads in flight per lane
             u32 v[8];
@@ -434,9 +434,9 @@ __global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u3
 #pragma unroll
             for (int u = 0; u < 8; ++u) c += v[u];
         }
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) sums[k] = c;
     }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if (lane == 0) sums[k] = c;
     __syncthreads();
     if (threadIdx.x != 0) return;
     int32_t r = 0x7FFFFFFF, kk = -1;
@@ -550,8 +550,11 @@ void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *
                         hipStream_t s)
 {
     if (!R) return;
-    hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii, nradii, counts);
-    CASPHY_HIP(hipGetLastError());
+    for (u32 g = 0; g < nradii; g += MAX_RADII_K) {  // one pass per group of 16 radii
+        hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii + g,
+                           std::min<u32>(MAX_RADII_K, nradii - g), counts + (size_t)g * R);
+        CASPHY_HIP(hipGetLastError());
+    }
 }
 
 void launch_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii, u32 nradii, u64 smax, int any,

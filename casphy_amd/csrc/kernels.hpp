@@ -29,12 +29,14 @@ void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s);
 void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *offsets, u32 *idx,
                     hipStream_t s, u64 cap = ~0ull);
 
-// counts[k*R + q] = #{j < n : S[q*ldS + j] >= radii[k]} for k < nradii (<= 16).
+// counts[k*R + q] = #{j < n : S[q*ldS + j] >= radii[k]} for k < nradii
+// (<= MAX_ROUND_RADII; one counting pass per group of MAX_RADII_K radii).
 constexpr int MAX_RADII_K = 16;
+constexpr int MAX_ROUND_RADII = 64;
 void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
                         hipStream_t s);
 
-// autoSplit choice: out[0] = radii[k] and out[1] = k for the first k with
+// autoSplit choice (nradii <= MAX_ROUND_RADII): out[0] = radii[k] and out[1] = k for the first k with
 // sum over q of counts[k*R + q] <= smax, or out[0] = INT32_MAX, out[1] = -1
 // when none (or !any).
 void launch_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii, u32 nradii, u64 smax, int any,

@@ -552,3 +552,39 @@ def test_rcpp_radius_shims_one_pass(hip, orc, monkeypatch):
         bits = np.unpackbits(out[1:].view(np.uint8), bitorder="little")
         assert out[0] == off[i + 1] - off[i] and np.array_equal(np.nonzero(bits)[0], i3[off[i]:off[i + 1]]), i
     assert calls == ["bit", "dna"]
+
+
+@pytest.mark.parametrize("kind", ["bit", "dna"])
+def test_cells_autosplit_64_radii(hip, orc, kind):
+    """Up to 64 radii in one device round (counting passes of 16 radii over
+    one score matrix): counts of every radius equal the oracle's, the choice
+    equals the host rule's, and the lists the oracle's at that radius; 40
+    radii through the plain tryRadius entry point as well."""
+    if kind == "bit":
+        mat = synth.binary_matrix(seed=71, n=700, nsite=1024, p=0.01)
+        cells = hip.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
+        radii = np.arange(1, 65)
+        ref_try = lambda r, q: orc.try_radius_bit(mat, r, q)  # noqa: E731
+        ref_lists = lambda r: orc.radius_bit(mat, r)  # noqa: E731
+        plain = lambda rr, q: hip.try_radii(np.ascontiguousarray(mat.T), rr, q)  # noqa: E731
+    else:
+        mat = synth.dna_matrix(seed=71, n=700, ncol=500)
+        cells = hip.Cells(mat, "dna")
+        radii = np.arange(100, 420, 5)
+        ref_try = lambda r, q: orc.try_radius_dna(mat, r, q)  # noqa: E731
+        ref_lists = lambda r: orc.radius_dna(mat, r)  # noqa: E731
+        plain = lambda rr, q: hip.try_radii_dna(mat, rr, q)  # noqa: E731
+    assert radii.size == 64
+    q = synth.query_rows(72, mat.shape[0], 150)
+    ref = np.stack([ref_try(int(r), q) for r in radii])
+    bound = float(np.log10(max(ref[45].mean(), 1.0)))
+    want = autosplit_rule(ref, radii, bound)
+    for rep in range(3):
+        r, total, cnt = cells.autosplit(radii, q, bound, counts=True)
+        assert np.array_equal(cnt, ref) and r == want, rep
+        if want >= 0:
+            counts, idx = cells.fetch()
+            c2, i2 = ref_lists(want)
+            assert np.array_equal(counts, c2) and np.array_equal(idx, i2), rep
+    assert np.array_equal(cells.try_radii(radii, q), ref)
+    assert np.array_equal(plain(radii[:40], q), ref[:40])
