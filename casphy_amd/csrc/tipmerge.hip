@@ -119,6 +119,7 @@ __global__ __launch_bounds__(256) void k_shared_sparse(const u64 *off, const u32
     const u32 np = nj - f;
     const u64 o = uoff[j];
     for (u32 w = tid; w < winw; w += 256) bm[w] = 0u;
+    __syncthreads();  // every word cleared before any thread sets bits in it
     const u64 span = (u64)winw * 32;
     for (u64 lo = 0; lo < n; lo += span) {
         const u64 hi = lo + span;
