@@ -31,6 +31,7 @@ SIGNATURES = [
     ("casphy_try_radius_dna", C.c_int, [vp, u32, u32, i32, vp, u32, vp]),
     ("casphy_try_radii_bit", C.c_int, [vp, u32, u32, vp, vp, u32, vp, u32, vp]),
     ("casphy_try_radii_dna", C.c_int, [vp, u32, u32, vp, u32, vp, u32, vp]),
+    ("casphy_try_radii_bit_noalloc", C.c_int, [vp, u32, u32, vp, u32, vp, u32, vp, vp]),
     ("casphy_join_bit", C.c_int, [vp, u32, u32, vp, C.c_int, vp, vp, vp]),
     ("casphy_join_indel", C.c_int, [vp, u32, u32, vp, u32, vp, u32, f32, vp, vp, vp]),
     ("casphy_cladewise", C.c_int, [vp, u32, vp, vp]),

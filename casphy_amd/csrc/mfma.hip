@@ -91,15 +91,18 @@ __device__ __forceinline__ bool mask_skipped(const MfmaArgs &a)
 
 constexpr int NTHR = 256;
 
-__global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 ldw, u32 kw, u32 nrows_pad, u32 *P)
+__global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 ldw, u32 kw, u32 nrows_pad, u32 *P,
+                                 u32 nsrc)
 {
-    // P[s][r][q] = mat[r][4s + q] for the first kw words of rows of stride ldw; rows >= n are zero
+    // P[s][r][q] = mat[r][4s + q] for the first kw words of rows of stride ldw;
+    // rows >= n, and gathered rows naming a row >= nsrc, are zero
     const u32 stages = kw / 4;
     const u64 total = (u64)stages * nrows_pad;
     for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
         const u32 s = (u32)(x / nrows_pad), r = (u32)(x - (u64)s * nrows_pad);
         uint4 v = make_uint4(0, 0, 0, 0);
-        const u32 src = r < n ? (rowidx ? rowidx[r] : r) : 0xFFFFFFFFu;
+        u32 src = r < n ? (rowidx ? rowidx[r] : r) : 0xFFFFFFFFu;
+        if (src >= nsrc) src = 0xFFFFFFFFu;
         if (src != 0xFFFFFFFFu) v = *reinterpret_cast<const uint4 *>(mat + (size_t)src * ldw + 4 * s);
         reinterpret_cast<uint4 *>(P)[x] = v;
     }
@@ -1036,11 +1039,12 @@ u32 grid_for(u64 items) { return (u32)std::min<u64>((items + 255) / 256, 65535ul
 
 }  // namespace
 
-void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s, const u32 *rowidx)
+void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s, const u32 *rowidx,
+                           u32 nsrc)
 {
     const u32 nrows_pad = stage_rows(n);
     hipLaunchKernelGGL(k_to_stage_major, dim3(grid_for((u64)(kw / 4) * nrows_pad)), dim3(256), 0, s, mat, rowidx, n,
-                       ldw, kw, nrows_pad, P);
+                       ldw, kw, nrows_pad, P, nsrc);
     CASPHY_HIP(hipGetLastError());
 }
 
@@ -1446,6 +1450,20 @@ void add_tuning_note(const std::string &json_object)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_notes.push_back(json_object);
+}
+
+void launch_mfma_tiles_untuned(int epi, const MfmaArgs &a, hipStream_t s)
+{
+    const bool fp4_exact = (u64)a.stages * 128 < (1ull << 24);
+    int v = variant_from_env();
+    if (v < 0) {
+        const TuneKey key{epi, a.triangle, log2_ceil(a.nA), log2_ceil(a.nB), a.stages};
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        v = it != g_tune.end() ? it->second.variant : kFp4;
+    }
+    if (v >= kFp4 && (!fp4_exact || !a.EA || !a.EB)) v = kDefaultVariant;
+    launch_one(v, epi, a, s);
 }
 
 std::string mfma_tuning_report()

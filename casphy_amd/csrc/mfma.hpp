@@ -49,7 +49,7 @@ struct MfmaArgs {
 // to a multiple of STAGE_PAD; kw a multiple of 32).  With rowidx, row r of P is
 // row rowidx[r] of mat.
 void launch_to_stage_major(const u32 *mat, u32 n, u32 ldw, u32 kw, u32 *P, hipStream_t s,
-                           const u32 *rowidx = nullptr);
+                           const u32 *rowidx = nullptr, u32 nsrc = 0xFFFFFFFFu);
 bool mfma_enabled();
 // Row popcounts of the first n rows of a stage-major matrix whose stages hold
 // `stride` rows.
@@ -60,6 +60,11 @@ void launch_stage_popc(const u32 *P, u32 n, u32 stride, u32 ldw, u32 *pop, hipSt
 inline size_t fp4_bytes(u32 pad, u32 stages) { return (size_t)stages * 2 * pad * 32; }
 void launch_expand_fp4(const u32 *P, u32 pad, u32 stages, u32 *E, hipStream_t s);
 void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
+// The same without timing anything: the autotuner's choice for the shape
+// class when it has one, else the fp4 tiles (the int8 tiles when the fp4
+// expansions EA / EB are not given or the sums could pass 2^24); creates no
+// events and allocates nothing.
+void launch_mfma_tiles_untuned(int epi, const MfmaArgs &a, hipStream_t s);
 // JSON report of the autotuner's choices (shape class -> variant, timings),
 // followed by the notes other tuners added (one JSON object each).
 std::string mfma_tuning_report();

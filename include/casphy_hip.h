@@ -105,13 +105,29 @@ int casphy_try_radius_dna(const uint32_t *mat_cellmajor, uint32_t n, uint32_t nw
 /* Batched tryRadius: all radii in one launch, counts[k*R + j] for radii[k]
  * (the autoSplit tuning loop of R/CasPhy_functions.R:2010-2021 runs one
  * tryRadius process per radius step; this answers every step at once).
- * nradii <= 16, radii ascending. */
+ * nradii <= 64, radii ascending. */
 int casphy_try_radii_bit(const uint32_t *mat_wordmajor, uint32_t W, uint32_t n,
                          const double *weight, const int32_t *radii, uint32_t nradii,
                          const uint32_t *q, uint32_t R, uint32_t *counts);
 int casphy_try_radii_dna(const uint32_t *mat_cellmajor, uint32_t n, uint32_t nwRaw,
                          const int32_t *radii, uint32_t nradii, const uint32_t *q,
                          uint32_t R, uint32_t *counts);
+
+/* tryRadius counts with no device allocation: casphy_try_radii_bit's
+ * matrix-core path on caller-owned DEVICE buffers -- mat_cellmajor (n x W
+ * u32 bit rows), radii (nradii <= 64, ascending), q (R row ids; an id of n
+ * or more scores as an all-zero row) and counts (nradii x R, counts[k*R + j])
+ * -- enqueued on `stream` (a hipStream_t; NULL = the null stream) with no
+ * host synchronisation, so the call can be captured into a graph.  Every
+ * intermediate buffer (padded rows, stage-major copies and their fp4
+ * expansions, the R x n score matrix) lives in one library array of 128 MiB
+ * per device, held until the process ends; sizes that need more return
+ * CASPHY_EARG with the amount.  Unit weights only.  Calls that overlap on one
+ * device share the array and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops with fixed memory). */
+int casphy_try_radii_bit_noalloc(const uint32_t *mat_cellmajor, uint32_t n, uint32_t W, const int32_t *radii,
+                                 uint32_t nradii, const uint32_t *q, uint32_t R, uint32_t *counts, void *stream);
 
 /* ------------------------------------------------------------------ joins */
 /* Greedy shared-mutation join of bit-packed cells (n >= 2).
@@ -330,11 +346,11 @@ void casphy_cells_destroy(casphy_cells *h);
 int casphy_cells_radius(casphy_cells *h, int32_t radius, uint64_t *nidx);
 /* Copies the lists of the last casphy_cells_radius call (nneighbor[n], idx[nidx]). */
 int casphy_cells_fetch(casphy_cells *h, uint32_t *nneighbor, uint32_t *idx);
-/* tryRadius counts for up to 16 ascending radii: counts[k*R + j]. */
+/* tryRadius counts for up to 64 ascending radii: counts[k*R + j]. */
 int casphy_cells_try_radii(casphy_cells *h, const int32_t *radii, uint32_t nradii, const uint32_t *q,
                            uint32_t R, uint32_t *counts);
 /* One autoSplit round with a single host synchronisation: tryRadius counts of
- * the R query rows at each of up to 16 ascending radii, then the first radius
+ * the R query rows at each of up to 64 ascending radii, then the first radius
  * whose mean count c (= sum / R in double) satisfies
  * log10(max(c, 1e-30)) <= bound -- the stopping rule of the autoSplit loop,
  * R/CasPhy_functions.R:2007-2021 (bound = nNeighborBound - log10(ntip)) --
