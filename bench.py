@@ -47,7 +47,7 @@ AUTOSPLIT_BOUND = 9.5  # nNeighborBound default of the autoSplit loop, R/CasPhy_
 N, NSITE = 10000, 4096
 RADII = np.arange(1, 17, dtype=np.int32)
 CHECK_REPLAYS = 50  # repeated device rounds checked against the first before timing (base-editor round)
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
+VALU_SPEC_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops (data sheet)
 MFMA_I8_PEAK_TOPS = 5000.0                # MI355X dense int8 matrix-core peak, 5.0 POP/s
 MFMA_FP4_PEAK_TOPS = 10100.0              # MI355X dense fp4 matrix-core peak, 10.1 PFLOP/s
 HBM_PEAK_GBS = 8000.0
@@ -176,6 +176,37 @@ def dna_search_host(cells, q, n: int, bound: float, start: int = DNA_RADIUS_INIT
     return -1, 0, tries
 
 
+@functools.lru_cache(maxsize=None)
+def ceilings() -> dict:
+    """Measured instruction-rate ceilings (casphy_amd/calib/calib.hip): the
+    32-bit integer VALU issue rate (8 independent xor / and / popcount-add
+    chains per lane, 8 waves per SIMD) and the fp4 matrix-core rate
+    (v_mfma_scale_f32_32x32x64_f8f6f4 on register operands, 4 accumulators per
+    wave, 4 waves per SIMD); the best of 3 timed launches after one warm-up."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "casphy_amd", "libcasphy_calib.so"))
+    hip = C.CDLL("libamdhip64.so")
+    vblocks, mblocks = 256 * 8, 256 * 4
+    buf = C.c_void_p()
+    assert hip.hipMalloc(C.byref(buf), C.c_size_t(max(vblocks, mblocks) * 256 * 4)) == 0
+    ms, work = C.c_float(), C.c_uint64()
+    best = {}
+    try:
+        for name, fn, blocks, iters in (("valu", lib.casphy_calib_valu, vblocks, 10000),
+                                        ("mfma_fp4", lib.casphy_calib_mfma_fp4, mblocks, 100000)):
+            rates = []
+            for k in range(4):
+                assert fn(buf, C.c_uint32(blocks), C.c_uint32(iters), None, C.byref(ms), C.byref(work)) == 0
+                if k:
+                    rates.append(work.value / (ms.value * 1e-3) / 1e12)
+            best[name] = max(rates)
+    finally:
+        hip.hipFree(buf)
+    return {"valu_tlane_ops_per_s": best["valu"], "valu_spec_tlane_ops_per_s": VALU_SPEC_TOPS,
+            "mfma_fp4_tops": best["mfma_fp4"], "mfma_fp4_spec_tops": MFMA_FP4_PEAK_TOPS,
+            "kernels": "k_valu_rate (192 VALU instructions per loop trip), k_mfma_fp4_rate"}
+
+
 def calibration_copy(nbytes: int = 256 << 20) -> None:
     """One device-to-device copy of `nbytes` (the HIP runtime's copy kernel),
     the known-size reference against which FETCH_SIZE / WRITE_SIZE counters of
@@ -230,7 +261,8 @@ def mask_roofline(cp, tile_ms: float, pairs: int, ops_per_pair: int, shape: tupl
     shape class (epilogue 0, triangle, log2 rows, sites)."""
     tile_s = tile_ms / 1000
     if valu:
-        roof = {"bound": "valu", "achieved": valu_ops_per_pair * pairs / tile_s / 1e12, "peak": VALU_PEAK_TOPS,
+        roof = {"bound": "valu", "achieved": valu_ops_per_pair * pairs / tile_s / 1e12,
+                "peak": ceilings()["valu_tlane_ops_per_s"], "peak_kind": "measured VALU issue ceiling",
                 "unit": "Tlane-op/s", "kernel": "k_tiles<DNA2, MASK>", "variant": "valu"}
     else:
         tuning = cp.mfma_tuning()
@@ -243,6 +275,8 @@ def mask_roofline(cp, tile_ms: float, pairs: int, ops_per_pair: int, shape: tupl
                 "peak": MFMA_FP4_PEAK_TOPS if fp4 else MFMA_I8_PEAK_TOPS, "unit": "TOP/s",
                 "kernel": VARIANT_KERNEL[variant], "variant": variant}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    if not valu and roof["variant"].startswith("fp4"):
+        roof["frac_of_measured_ceiling"] = roof["achieved"] / ceilings()["mfma_fp4_tops"]
     roof["traffic"] = measured_traffic(roof["kernel"], traffic_table)
     roof["kernel_ms"] = tile_ms
     roof["algorithmic"] = (f"{valu_ops_per_pair if valu else ops_per_pair} ops per cell pair x {pairs} pairs "
@@ -592,7 +626,44 @@ def fasta_step(cp, dna, with_cpu: bool) -> dict:
         assert all(np.array_equal(x, y) for x, y in zip((var, valid, out), w)), "CPU/GPU fasta2bin mismatch"
         res["cpu_baseline"] = {"value": seqs.size / cpu_s, "unit": "characters/s", "cores": 1, "kind": "port",
                                "sample": f"the full alignment, {cpu_s:.2f} s"}
+        if os.path.exists(REF_FASTA2BIN):
+            res["files_end_to_end"] = fasta_files(seqs)
     return res
+
+
+REF_FASTA2BIN = os.path.join(ROOT, "oracle", "_ref", "fasta2bin")
+
+
+def fasta_files(seqs: np.ndarray) -> dict:
+    """FASTA file in, the three output files out: the drop-in fasta2bin
+    executable against the reference's own fasta2bin (built from
+    cu/fasta2bin.cu into oracle/_ref/) on the same file, outputs compared
+    byte for byte."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="casphy_fasta_")
+    try:
+        src = os.path.join(d, "in.fa")
+        with open(src, "wb") as f:
+            for i in range(seqs.shape[0]):
+                f.write(b">seq%d\n" % i)
+                f.write(seqs[i].tobytes() + b"\n")
+        times = {}
+        for who, exe in (("drop_in", os.path.join(ROOT, "casphy_amd", "bin", "fasta2bin")), ("reference", REF_FASTA2BIN)):
+            t0 = time.perf_counter()
+            subprocess.run([exe, src, os.path.join(d, who)], check=True, stdout=subprocess.DEVNULL)
+            times[who] = time.perf_counter() - t0
+        same = all(open(os.path.join(d, "drop_in" + suf), "rb").read() == open(os.path.join(d, "reference" + suf), "rb").read()
+                   for suf in (".bin", "_names.txt", "_positions.txt"))
+        assert same, "drop-in / reference fasta2bin outputs differ"
+        mb = os.path.getsize(src) / 1e6
+        return {"fasta_mb": mb, "drop_in_s": times["drop_in"], "reference_s": times["reference"],
+                "speedup": times["reference"] / times["drop_in"], "identical_files": same,
+                "cpu_baseline": {"value": seqs.size / times["reference"], "unit": "characters/s", "cores": 1,
+                                 "kind": "reference", "sample": "the full alignment as a FASTA file, wall clock "
+                                 "of the reference executable built from cu/fasta2bin.cu"}}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 NODEKEEP_OUT = 1000    # outgroup rows (cells of the matrix)
@@ -766,7 +837,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
-    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe", "base", "indel"],
+    ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe", "base", "indel", "scan",
+                                       "dnavalu", "ceilings"],
                     help="run only one of the other configurations (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -800,6 +872,26 @@ def main():
         return
     if args.only == "indel":
         print(json.dumps(indel_config(cp, synth, with_cpu)))
+        return
+    if args.only == "scan":
+        print(json.dumps(scan_form(cp, synth)))
+        return
+    if args.only == "ceilings":
+        print(json.dumps(ceilings()))
+        return
+    if args.only == "dnavalu":
+        # the VALU 2-bit + mask tiles of the DNA configuration: one tryRadius
+        # and one radius pass at the headline's radius (counter profiles)
+        print(json.dumps(ceilings()))
+        dna = synth.dna_matrix(seed=3, n=N3, ncol=NCOL3)
+        os.environ["CASPHY_DNA_FORM"] = "valu"
+        cells = cp.Cells(dna, "dna")
+        q = synth.query_rows(3, N3, R)
+        counts = cells.try_radii(np.array([640], np.int32), q)
+        total = cells.radius(640)
+        print(json.dumps({"tryradius_mean": float(counts.mean()), "neighbour_ids": int(total),
+                          "mask_tiles_ms": cells.last_ms()[0], "pairs": N3 * (N3 + 1) // 2}))
+        cells.close()
         return
 
     t = time.perf_counter()
@@ -860,6 +952,7 @@ def main():
                    "neighbour_ids": int(t_dev), "dna_form": form,
                    "last_round_ms": float(np.mean(round_ms))},
         "roofline": roof,
+        "ceilings": ceilings(),
         "mfma_tuning": cp.mfma_tuning(),
     }
     if with_cpu:

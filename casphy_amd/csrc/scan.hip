@@ -12,9 +12,10 @@
 //   * the SCAN_Q per-lane partial sums are reduced with DPP row shifts and
 //     row broadcasts (no LDS traffic) and read out of lane 63;
 //   * lane (q, k = l & 15) counts the rows whose score reaches radius k; the
-//     workgroup's counts go to a per-workgroup slot and one small launch sums
-//     them (atomics from every workgroup on the same 64 counters would
-//     serialise on two cache lines).
+//     workgroup's counts go to per-counter slots (counter-major, so each
+//     counter's partials are contiguous) and a second launch of one
+//     workgroup per counter sums them (atomics from every workgroup on the
+//     same 64 counters would serialise on two cache lines).
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -103,30 +104,35 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void k_scan_bit(const u32 *mat, u3
             if (S && myk == 0) S[(size_t)myq * n + r] = (int32_t)v;
         }
     }
-    // per-workgroup counts (summed by k_scan_sum: one atomic per workgroup and
-    // counter would serialise 2,048 workgroups on the same two cache lines)
+    // per-workgroup counts, counter-major: part[counter * gridDim.x + workgroup]
     red[wv][lane] = cnt;
     __syncthreads();
     if (wv == 0) {
         u32 c = 0;
 #pragma unroll
         for (u32 k = 0; k < SCAN_WAVES; ++k) c += red[k][lane];
-        part[(size_t)blockIdx.x * 64 + lane] = c;
+        part[(size_t)lane * gridDim.x + blockIdx.x] = c;
     }
 }
 
-// counts[k*R + q] = sum over workgroups of lane (q, k)'s partial count.
+// counts[k*R + q] = sum over workgroups of counter (q, k)'s partials: one
+// workgroup per counter, its partials contiguous, every load of a thread
+// independent of the others (a single workgroup looping over all 64 counters
+// was bound by the latency of its dependent loads).
 __global__ __launch_bounds__(256) void k_scan_sum(const u32 *part, u32 nblocks, u32 Q, u32 nradii, u32 *counts,
                                                   u32 R)
 {
-    __shared__ u32 red[4][64];
-    const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ u32 red[4];
+    const u32 slot = blockIdx.x, q = (slot >> 4) & 3, k = slot & 15;
+    if (q >= Q || k >= nradii) return;  // uniform per workgroup
+    const u32 *p = part + (size_t)slot * nblocks;
     u32 c = 0;
-    for (u32 b = wv; b < nblocks; b += 4) c += part[(size_t)b * 64 + lane];
-    red[wv][lane] = c;
+#pragma unroll 8
+    for (u32 b = threadIdx.x; b < nblocks; b += 256) c += p[b];
+    c = __builtin_amdgcn_readlane(wave_sum_lane63(c), 63);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
-    const u32 q = (lane >> 4) & 3, k = lane & 15;
-    if (wv == 0 && q < Q && k < nradii) counts[(size_t)k * R + q] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    if (threadIdx.x == 0) counts[(size_t)k * R + q] = red[0] + red[1] + red[2] + red[3];
 }
 
 }  // namespace
@@ -147,7 +153,7 @@ void launch_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q, co
                        Q, radii, nradii, part, S);
     CASPHY_HIP(hipGetLastError());
     if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
-    hipLaunchKernelGGL(k_scan_sum, dim3(1), dim3(256), 0, s, part, grid, Q, nradii, counts, R);
+    hipLaunchKernelGGL(k_scan_sum, dim3(64), dim3(256), 0, s, part, grid, Q, nradii, counts, R);
     CASPHY_HIP(hipGetLastError());
 }
 
