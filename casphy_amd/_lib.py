@@ -55,6 +55,9 @@ SIGNATURES = [
     ("casphy_cells_radius", C.c_int, [vp, i32, C.POINTER(u64)]),
     ("casphy_cells_fetch", C.c_int, [vp, vp, vp]),
     ("casphy_cells_tip_merge", C.c_int, [vp, C.c_double, vp]),
+    ("casphy_cells_split_groups", C.c_int, [vp, vp, vp, u32, vp, u32, vp, vp, C.c_double, vp, vp]),
+    ("casphy_cells_group_fetch", C.c_int, [vp, u32, vp, vp]),
+    ("casphy_cells_group_tip_merge", C.c_int, [vp, u32, C.c_double, vp]),
     ("casphy_cells_try_radii", C.c_int, [vp, vp, u32, vp, u32, vp]),
     ("casphy_cells_autosplit", C.c_int, [vp, vp, u32, vp, u32, C.c_double, C.POINTER(i32), C.POINTER(u64), vp]),
     ("casphy_cells_last_ms", C.c_int, [vp, C.POINTER(f32), C.POINTER(f32)]),

@@ -663,6 +663,47 @@ class Cells:
         self.total = total.value
         return (rad.value, self.total, out) if counts else (rad.value, self.total)
 
+    def split_groups(self, groups, radii, queries, nneighbor_bound: float = 9.5):
+        """autoSplit's radius search for many groups of these cells at once
+        (casphy_cells_split_groups): groups = list of row-id arrays, queries
+        = list of group-local query-id arrays (one per group).  Returns
+        (radius per group, -1 where none passes; list length per group); the
+        lists stay on the device (group_fetch, group_tip_merge)."""
+        G = len(groups)
+        goff = np.zeros(G + 1, np.uint32)
+        goff[1:] = np.cumsum([len(g) for g in groups])
+        qoff = np.zeros(G + 1, np.uint32)
+        qoff[1:] = np.cumsum([len(x) for x in queries])
+        rows = _u32(np.concatenate([np.asarray(g, np.uint32) for g in groups]))
+        q = _u32(np.concatenate([np.asarray(x, np.uint32) for x in queries]))
+        r = np.ascontiguousarray(radii, dtype=np.int32)
+        rad = np.zeros(G, np.int32)
+        nidx = np.zeros(G, np.uint64)
+        check(load().casphy_cells_split_groups(self._h, _p(goff), _p(rows), G, _p(r), r.size, _p(qoff), _p(q),
+                                               float(nneighbor_bound), _p(rad), _p(nidx)))
+        self._group_n = [len(g) for g in groups]
+        self._group_total = [int(x) for x in nidx]
+        return rad, nidx
+
+    def _group_known(self, g: int):
+        if g >= len(getattr(self, "_group_n", [])):  # the library reports it
+            d = np.zeros(1, np.uint32)
+            check(load().casphy_cells_group_fetch(self._h, int(g), _p(d), _p(d)))
+
+    def group_fetch(self, g: int):
+        self._group_known(g)
+        n, t = self._group_n[g], self._group_total[g]
+        counts = np.empty(n, np.uint32)
+        idx = np.empty(max(t, 1), np.uint32)
+        check(load().casphy_cells_group_fetch(self._h, int(g), _p(counts), _p(idx)))
+        return counts, idx[:t]
+
+    def group_tip_merge(self, g: int, p: float) -> np.ndarray:
+        self._group_known(g)
+        member = np.empty(max(self._group_n[g], 1), np.int32)
+        check(load().casphy_cells_group_tip_merge(self._h, int(g), float(p), _p(member)))
+        return member[: self._group_n[g]]
+
     def last_ms(self):
         a, b = C.c_float(0), C.c_float(0)
         check(load().casphy_cells_last_ms(self._h, C.byref(a), C.byref(b)))

@@ -15,6 +15,7 @@
 #include <thread>
 #include <vector>
 
+#include "cells.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
@@ -305,14 +306,7 @@ DnaMat upload_dna(const u32 *host, u32 n, u32 nwRaw, hipStream_t s, bool keep_va
     return m;
 }
 
-// Radius lists of a device matrix: mask tiles -> counts -> scan -> compaction,
-// all kept on the device.
-struct RadiusOut {
-    DevBuf mask, counts, offsets, idx, pbuf, stage;
-    u64 total = 0;
-    u32 n = 0;
-};
-
+}  // namespace
 
 // Stage-major copy of a cell-major matrix for the MFMA tiles.
 void make_stage(const u32 *dmat, u32 n, u32 ldw, DevBuf &P, hipStream_t s)
@@ -378,8 +372,8 @@ void radius_mask(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &plan
 }
 
 void radius_compute(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, int32_t radius,
-                    RadiusOut &o, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                    const u32 *stage = nullptr, const u32 *estage = nullptr)
+                    RadiusOut &o, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const u32 *stage,
+                    const u32 *estage)
 {
     radius_mask(metric, dmat, n, ldw, planes, radius, nullptr, o, s, ev0, ev1, stage, estage);
     const u32 mstride = ceil_div(n, TILE) * 4;
@@ -441,8 +435,8 @@ bool use_scan(int metric, const Planes &planes, u32 R, u32 ldw, u32 nradii)
 // tryRadius: query x cell scores (matrix-core tiles for unit-weight bits,
 // VALU tiles otherwise) followed by one counting workgroup per query.
 void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, const int32_t *radii,
-               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage = nullptr,
-               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const u32 *estage = nullptr)
+               u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage, hipEvent_t ev0,
+               hipEvent_t ev1, const u32 *estage)
 {
     for (u32 k = 0; k < R; ++k)
         CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
@@ -497,6 +491,7 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
     d2h(counts, dc.p, (size_t)R * nradii * 4, s);
 }
 
+namespace {
 void require(bool c, const char *msg) { CASPHY_REQUIRE(c, CASPHY_EARG, msg); }
 
 }  // namespace
@@ -1142,75 +1137,7 @@ int casphy_cladewise(const uint32_t *edge_raw, uint32_t n, uint32_t *edge_out, u
     });
 }
 
-// ---------------------------------------------------------------- resident cells
-struct AutosplitKey {
-    u32 R = 0, nradii = 0;
-    u64 smax = 0;
-    int any = -1;
-    bool want = false;
-    u64 cap = 0;
-    const void *idx = nullptr, *mask = nullptr, *S = nullptr, *eq = nullptr, *in = nullptr, *pin = nullptr;
-    bool operator==(const AutosplitKey &o) const
-    {
-        return R == o.R && nradii == o.nradii && smax == o.smax && any == o.any && want == o.want && cap == o.cap &&
-               idx == o.idx && mask == o.mask && S == o.S && eq == o.eq && in == o.in && pin == o.pin;
-    }
-};
-
-struct casphy_cells {
-    int kind = CASPHY_KIND_BIT;
-    int metric = M_AND;
-    u32 n = 0, ldw = 0;
-    DevBuf mat;
-    Planes planes;
-    DevBuf stage;   // stage-major copy for the MFMA tiles (bit kind, unit weights)
-    DevBuf estage;  // its fp4 expansion (for the fp4 tile variant)
-    bool mfma() const { return metric == M_AND && planes.nplanes == 0 && mfma_enabled(); }
-    // stage-major copy and its fp4 expansion, made on first use
-    void staged(hipStream_t s)
-    {
-        if (!stage.p) make_stage(mat.as<u32>(), n, ldw, stage, s);
-        if (!estage.p) {
-            estage.alloc(fp4_bytes(stage_rows(n), ldw / 4));
-            launch_expand_fp4(stage.as<u32>(), stage_rows(n), ldw / 4, estage.as<u32>(), s);
-        }
-    }
-    RadiusOut out;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool timed[2] = {false, false};  // ev[0..1] / ev[2..3] recorded by some call
-    int device = 0;
-    // autoSplit round: uploads (radii | query rows), query stage copy, scores,
-    // counts and the choice; pinned host staging for one upload and
-    // the read-back of (choice, total)
-    DevBuf a_in, a_sq, a_eq, a_S, a_counts, a_misc, a_pbuf;
-    // captured autoSplit round and the shapes / buffers it was captured with
-    // (the round's two small-kernel phases; the mask launch between them is a
-    // plain launch bracketed by the handle's events)
-    hipGraphExec_t ag_exec[2] = {nullptr, nullptr};
-    AutosplitKey ag_key{};
-    bool ag_seen = false;
-    void drop_graph()
-    {
-        for (auto &g : ag_exec) {
-            if (g) (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
-    }
-    void *pin = nullptr;      // pinned host staging, mapped into the device's address space
-    void *pin_dev = nullptr;  // the device's address of it
-    size_t pin_bytes = 0;
-    void pinned(size_t b)
-    {
-        if (b <= pin_bytes) return;
-        if (pin) CASPHY_HIP(hipHostFree(pin));
-        pin = pin_dev = nullptr;
-        pin_bytes = 0;
-        CASPHY_HIP(hipHostMalloc(&pin, b, hipHostMallocMapped));
-        CASPHY_HIP(hipHostGetDevicePointer(&pin_dev, pin, 0));
-        pin_bytes = b;
-    }
-};
-
+// ---------------------------------------------------------------- resident cells (struct in cells.hpp)
 }  // extern "C"
 
 namespace {

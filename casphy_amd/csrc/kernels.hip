@@ -100,16 +100,20 @@ __global__ void k_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 l
 // words, four loads in flight per lane.
 // A device threshold of INT32_MAX (skip) means the mask launch scored nothing:
 // every count is 0 and the mask is not read.
-__global__ void k_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, const int32_t *skip)
+// With map.moff set, row r is the row of map (its own offset, width and group).
+__global__ void k_row_popc(const u32 *mask, u32 n, u32 mstride, u32 nwords, u32 *counts, const int32_t *skip,
+                           RowMap map)
 {
     const u32 lane = threadIdx.x & 63;
     const u32 row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (row >= n) return;
-    if (skip && *skip == 0x7FFFFFFF) {
+    const int32_t *sk = map.moff ? map.gthr + 2 * map.group[row] : skip;
+    if (sk && *sk == 0x7FFFFFFF) {
         if (lane == 0) counts[row] = 0;
         return;
     }
-    const u32 *m = mask + (size_t)row * mstride;
+    const u32 *m = map.moff ? mask + map.moff[row] : mask + (size_t)row * mstride;
+    const u32 words = map.moff ? map.words[row] : nwords;
     u32 c = 0, tail = 0;
     if ((reinterpret_cast<uintptr_t>(m) & 15) == 0) {
         const uint4 *m4 = reinterpret_cast<const uint4 *>(m);
@@ -193,17 +197,18 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const u32 *counts, u32 n, 
 // (single ids at the two ends).  The next chunk's words are loaded while the
 // current one is placed.  Rows with an empty list are not read.
 constexpr int CWAVES = 1;  // one wave per workgroup: 4.44 ms against 4.80 with four on 2.8e9 ids
-__global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 words,
-                                                        const u64 *offsets, u32 *idx, u64 cap)
+__global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n, u32 mstride, u32 nwords,
+                                                        const u64 *offsets, u32 *idx, u64 cap, RowMap map)
 {
     __shared__ __attribute__((aligned(16))) u32 stage[CWAVES][64 * 32 + 4];
     const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u32 row = blockIdx.x * CWAVES + wv;
     if (row >= n) return;
     u32 *buf = stage[wv];
-    const u32 *m = mask + (size_t)row * mstride;
     u64 base = offsets[row];
     if (offsets[row + 1] == base) return;  // empty list: the mask row is not read
+    const u32 *m = map.moff ? mask + map.moff[row] : mask + (size_t)row * mstride;
+    const u32 words = map.moff ? map.words[row] : nwords;
     u32 vnext = lane < words ? m[lane] : 0u;
     for (u32 w0 = 0; w0 < words; w0 += 64) {
         const u32 w = w0 + lane;
@@ -286,8 +291,8 @@ __global__ __launch_bounds__(64 * CWAVES) void k_compact(const u32 *mask, u32 n,
 // flight per lane.  The per-thread counters are summed through LDS.
 constexpr u32 COUNT_LUT = 1024;  // table entries: scores -1 .. COUNT_LUT - 2
 
-__global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii,
-                                                     u32 nradii, u32 *counts)
+__global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u32 nall, u64 ldS, const int32_t *radii,
+                                                     u32 nradii, u32 *counts, const u64 *rowoff, const u32 *rowlen)
 {
     __shared__ uint4 lds4[1088];  // the table, then 256 x 17 u32 for the reduction
     __shared__ u32 tot[MAX_RADII_K];
@@ -304,7 +309,8 @@ __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u3
         }
     }
     const bool table = rmin >= 0 && rmax <= (int32_t)COUNT_LUT - 2;
-    const int32_t *row = S + (size_t)q * ldS;
+    const int32_t *row = rowoff ? S + rowoff[q] : S + (size_t)q * ldS;
+    const u32 n = rowlen ? rowlen[q] : nall;
     const bool vec = (reinterpret_cast<uintptr_t>(row) & 15) == 0;
     const u32 n4 = vec ? n / 4 : 0;
     u32 c[MAX_RADII_K];
@@ -416,20 +422,21 @@ __global__ __launch_bounds__(256) void k_count_radii(const int32_t *S, u32 R, u3
 
 // One workgroup of 16 waves: wave w sums the R counts of radii w, w + 16, ...,
 // then the first radius whose sum is at most smax is chosen.
-__global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii,
-                                                           u32 nradii, u64 smax, int any, int32_t *out)
+// (the queries are the Rq columns from col0 of the R columns per radius)
+__device__ __forceinline__ void choose_body(const u32 *counts, u32 R, u32 col0, u32 Rq, const int32_t *radii,
+                                            u32 nradii, u64 smax, int any, int32_t *out)
 {
     __shared__ u64 sums[MAX_ROUND_RADII];
     const u32 lane = threadIdx.x & 63;
     for (u32 k = threadIdx.x >> 6; k < nradii; k += 16) {
         u64 c = 0;
-        const u32 *row = counts + (size_t)k * R;
-        for (u32 q0 = 0; q0 < R; q0 += 512) {  // eight loads in flight per lane
+        const u32 *row = counts + (size_t)k * R + col0;
+        for (u32 q0 = 0; q0 < Rq; q0 += 512) {  // eight loads in flight per lane
             u32 v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const u32 q = q0 + 64 * u + lane;
-                v[u] = q < R ? row[q] : 0u;
+                v[u] = q < Rq ? row[q] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) c += v[u];
@@ -449,6 +456,21 @@ __global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u3
             }
     out[0] = r;
     out[1] = kk;
+}
+
+__global__ __launch_bounds__(1024) void k_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii,
+                                                           u32 nradii, u64 smax, int any, int32_t *out)
+{
+    choose_body(counts, R, 0, R, radii, nradii, smax, any, out);
+}
+
+// One workgroup per group.
+__global__ __launch_bounds__(1024) void k_autosplit_choose_groups(const u32 *counts, u32 R, const u32 *qoff,
+                                                                  const int32_t *radii, u32 nradii, const u64 *smax,
+                                                                  const int32_t *any, int32_t *out)
+{
+    const u32 g = blockIdx.x;
+    choose_body(counts, R, qoff[g], qoff[g + 1] - qoff[g], radii, nradii, smax[g], any[g], out + 2 * g);
 }
 
 // Copies between device memory and mapped pinned host memory inside a kernel,
@@ -528,7 +550,15 @@ void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hi
 void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s,
                      const int32_t *skip)
 {
-    hipLaunchKernelGGL(k_row_popc, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, counts, skip);
+    hipLaunchKernelGGL(k_row_popc, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, mstride, words, counts, skip,
+                       RowMap{});
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_row_popc_map(const u32 *mask, u32 n, const RowMap &map, u32 *counts, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_row_popc, dim3(ceil_div(n, 4)), dim3(256), 0, s, mask, n, 0u, 0u, counts, nullptr, map);
     CASPHY_HIP(hipGetLastError());
 }
 
@@ -542,7 +572,16 @@ void launch_compact(const u32 *mask, u32 n, u32 mstride, u32 words, const u64 *o
                     u64 cap)
 {
     hipLaunchKernelGGL(k_compact, dim3(ceil_div(n, CWAVES)), dim3(64 * CWAVES), 0, s, mask, n, mstride, words, offsets,
-                       idx, cap);
+                       idx, cap, RowMap{});
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_compact_map(const u32 *mask, u32 n, const RowMap &map, const u64 *offsets, u32 *idx, hipStream_t s,
+                        u64 cap)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n, CWAVES)), dim3(64 * CWAVES), 0, s, mask, n, 0u, 0u, offsets, idx,
+                       cap, map);
     CASPHY_HIP(hipGetLastError());
 }
 
@@ -552,7 +591,18 @@ void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *
     if (!R) return;
     for (u32 g = 0; g < nradii; g += MAX_RADII_K) {  // one pass per group of 16 radii
         hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, n, ldS, radii + g,
-                           std::min<u32>(MAX_RADII_K, nradii - g), counts + (size_t)g * R);
+                           std::min<u32>(MAX_RADII_K, nradii - g), counts + (size_t)g * R, nullptr, nullptr);
+        CASPHY_HIP(hipGetLastError());
+    }
+}
+
+void launch_count_radii_rows(const int32_t *S, u32 R, const u64 *rowoff, const u32 *rowlen, const int32_t *radii,
+                             u32 nradii, u32 *counts, hipStream_t s)
+{
+    if (!R) return;
+    for (u32 g = 0; g < nradii; g += MAX_RADII_K) {
+        hipLaunchKernelGGL(k_count_radii, dim3(R), dim3(256), 0, s, S, R, 0u, (u64)0, radii + g,
+                           std::min<u32>(MAX_RADII_K, nradii - g), counts + (size_t)g * R, rowoff, rowlen);
         CASPHY_HIP(hipGetLastError());
     }
 }
@@ -561,6 +611,15 @@ void launch_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii, u32
                              int32_t *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_autosplit_choose, dim3(1), dim3(1024), 0, s, counts, R, radii, nradii, smax, any, out);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_autosplit_choose_groups(const u32 *counts, u32 R, const u32 *qoff, u32 ngroups, const int32_t *radii,
+                                    u32 nradii, const u64 *smax, const int32_t *any, int32_t *out, hipStream_t s)
+{
+    if (!ngroups) return;
+    hipLaunchKernelGGL(k_autosplit_choose_groups, dim3(ngroups), dim3(1024), 0, s, counts, R, qoff, radii, nradii,
+                       smax, any, out);
     CASPHY_HIP(hipGetLastError());
 }
 

@@ -23,6 +23,18 @@ void launch_dna_pack_nib(const u32 *raw, u32 n, u32 nwRaw, u32 *out, u32 ldw, hi
 // mask is not read (a radius-mask launch that scored nothing).
 void launch_row_popc(const u32 *mask, u32 n, u32 mstride, u32 words, u32 *counts, hipStream_t s,
                      const int32_t *skip = nullptr);
+// Rows of several independent masks (the groups of a group-mode search): row
+// r's mask starts at word moff[r] and has words[r] words; it counts 0 and is
+// not read when gthr[2 group[r]] is INT32_MAX (its group scored nothing).
+struct RowMap {
+    const u64 *moff = nullptr;
+    const u32 *words = nullptr;
+    const u32 *group = nullptr;
+    const int32_t *gthr = nullptr;
+};
+void launch_row_popc_map(const u32 *mask, u32 n, const RowMap &map, u32 *counts, hipStream_t s);
+void launch_compact_map(const u32 *mask, u32 n, const RowMap &map, const u64 *offsets, u32 *idx, hipStream_t s,
+                        u64 cap = ~0ull);
 // Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).
 void launch_scan_counts(const u32 *counts, u32 n, u64 *offsets, hipStream_t s);
 // Ids at positions >= cap are not written (cap = capacity of idx).
@@ -35,12 +47,21 @@ constexpr int MAX_RADII_K = 16;
 constexpr int MAX_ROUND_RADII = 64;
 void launch_count_radii(const int32_t *S, u32 R, u32 n, u64 ldS, const int32_t *radii, u32 nradii, u32 *counts,
                         hipStream_t s);
+// The same over rows of different lengths: row q is the rowlen[q] scores at
+// S + rowoff[q].
+void launch_count_radii_rows(const int32_t *S, u32 R, const u64 *rowoff, const u32 *rowlen, const int32_t *radii,
+                             u32 nradii, u32 *counts, hipStream_t s);
 
 // autoSplit choice (nradii <= MAX_ROUND_RADII): out[0] = radii[k] and out[1] = k for the first k with
 // sum over q of counts[k*R + q] <= smax, or out[0] = INT32_MAX, out[1] = -1
 // when none (or !any).
 void launch_autosplit_choose(const u32 *counts, u32 R, const int32_t *radii, u32 nradii, u64 smax, int any,
                              int32_t *out, hipStream_t s);
+// The choice of every group g at once: its queries are the columns
+// qoff[g] .. qoff[g+1] of counts (R columns per radius), its limits smax[g] and
+// any[g]; the result goes to out[2g], out[2g+1].
+void launch_autosplit_choose_groups(const u32 *counts, u32 R, const u32 *qoff, u32 ngroups, const int32_t *radii,
+                                    u32 nradii, const u64 *smax, const int32_t *any, int32_t *out, hipStream_t s);
 
 // Scan form of tryRadius for up to SCAN_Q query rows (qrows: Q x ldw, cell-major):
 // counts[k*R + q] = #{r < n : <mat_r, q-th query> >= radii[k]} for q < Q, k < nradii;

@@ -91,6 +91,34 @@ __device__ __forceinline__ bool mask_skipped(const MfmaArgs &a)
 
 constexpr int NTHR = 256;
 
+// Tile (I, J) of launch slot t and, in group mode, the group's view for the
+// epilogue: its row counts, output and threshold, and the origin that turns
+// rows of the gathered operands into the group's own rows.  False: the slot
+// holds no tile, or the tile's group scores nothing.
+template <int EPI>
+__device__ __forceinline__ bool resolve_tile(MfmaArgs &a, u32 t, u32 &I, u32 &J)
+{
+    if (!a.glist) return !mask_skipped<EPI>(a) && tile_of(a, t, I, J);
+    const uint4 d = a.glist[t];
+    const GroupDesc g = a.gdesc[d.z];
+    I = d.x;
+    J = d.y;
+    a.orgA = g.a0;
+    a.orgB = g.b0;
+    a.nA = g.nA;
+    a.nB = g.nB;
+    if (EPI == E_MASK) {
+        a.mask += g.off;
+        a.mstride = g.ld;
+        if (a.gthr) a.thr = a.gthr[2 * d.z];
+        a.thr_dev = nullptr;
+        return a.thr != 0x7FFFFFFF;
+    }
+    a.S += g.off;
+    a.ldS = g.ld;
+    return true;
+}
+
 __global__ void k_to_stage_major(const u32 *mat, const u32 *rowidx, u32 n, u32 ldw, u32 kw, u32 nrows_pad, u32 *P,
                                  u32 nsrc)
 {
@@ -311,9 +339,9 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
+    if (t >= a.ntiles) return;
     u32 I, J;
-    if (!tile_of(a, t, I, J)) return;
+    if (!resolve_tile<EPI>(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -369,12 +397,12 @@ __global__ __launch_bounds__(NTHR, OCC) void k_mfma_tiles(MfmaArgs a)
 
     const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const bool mirror = a.triangle && I != J;
+    const u32 er0 = I * TILE - a.orgA + wm * 64, ec0 = J * TILE - a.orgB + wn * 64;  // the group's own rows
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y)
-            block_epilogue<EPI>(a, acc[x][y], I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror,
-                                thr);
+            block_epilogue<EPI>(a, acc[x][y], er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
 }
 
 // ---------------------------------------------------------------------------
@@ -603,9 +631,9 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
+    if (t >= a.ntiles) return;
     u32 I, J;
-    if (!tile_of(a, t, I, J)) return;
+    if (!resolve_tile<EPI>(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -682,6 +710,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
 
     const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const bool mirror = a.triangle && I != J;
+    const u32 er0 = I * TILE - a.orgA + wm * 64, ec0 = J * TILE - a.orgB + wn * 64;  // the group's own rows
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -689,7 +718,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
             v16i ai;
 #pragma unroll
             for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror, thr);
+            block_epilogue<EPI>(a, ai, er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -704,9 +733,9 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
+    if (t >= a.ntiles) return;
     u32 I, J;
-    if (!tile_of(a, t, I, J)) return;
+    if (!resolve_tile<EPI>(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -792,6 +821,7 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
 
     const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const bool mirror = a.triangle && I != J;
+    const u32 er0 = I * TILE - a.orgA + wm * 64, ec0 = J * TILE - a.orgB + wn * 64;  // the group's own rows
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -799,7 +829,7 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
             v16i ai;
 #pragma unroll
             for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror, thr);
+            block_epilogue<EPI>(a, ai, er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -842,9 +872,9 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
 
     const u32 per = (a.ntiles + 7) / 8;
     const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
+    if (t >= a.ntiles) return;
     u32 I, J;
-    if (!tile_of(a, t, I, J)) return;
+    if (!resolve_tile<EPI>(a, t, I, J)) return;
     const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 wm = wave >> 1, wn = wave & 1;
 
@@ -905,6 +935,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
 
     const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const bool mirror = a.triangle && I != J;
+    const u32 er0 = I * TILE - a.orgA + wm * 64, ec0 = J * TILE - a.orgB + wn * 64;  // the group's own rows
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -912,7 +943,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
             v16i ai;
 #pragma unroll
             for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, I * TILE + wm * 64 + 32 * x, J * TILE + wn * 64 + 32 * y, lane, mirror, thr);
+            block_epilogue<EPI>(a, ai, er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -1464,6 +1495,32 @@ void launch_mfma_tiles_untuned(int epi, const MfmaArgs &a, hipStream_t s)
     }
     if (v >= kFp4 && (!fp4_exact || !a.EA || !a.EB)) v = kDefaultVariant;
     launch_one(v, epi, a, s);
+}
+
+void launch_mfma_groups(int epi, const MfmaArgs &in, u32 nslots, hipStream_t s)
+{
+    CASPHY_REQUIRE(in.glist && in.gdesc && (epi == E_MASK || epi == E_SCORES), CASPHY_EINTERNAL,
+                   "MFMA group launch: bad arguments");
+    if (!nslots) return;
+    MfmaArgs a = in;
+    a.ntiles = nslots;
+    a.sup = 0;
+    const bool fp4_exact = (u64)a.stages * 128 < (1ull << 24);
+    int v = variant_from_env();
+    if (v < 0 || v == 6 || v == kWide || v == kWide8 || v == kFp4w) v = kFp4;  // 128 x 128 tiles only
+    if (v >= kFp4 && (!fp4_exact || !a.EA || !a.EB)) v = kDefaultVariant;
+    switch (v) {
+    case 0: launch_variant<128, 1, 2>(epi, a, s); break;
+    case 1: launch_variant<128, 2, 2>(epi, a, s); break;
+    case 2: launch_variant<128, 4, 2>(epi, a, s); break;
+    case 3: launch_variant<64, 2, 4>(epi, a, s); break;
+    case 4: launch_variant<64, 4, 4>(epi, a, s); break;
+    case 5: launch_variant<64, 8, 4>(epi, a, s); break;
+    case kFp4: launch_fp4(epi, a, s, 0); break;
+    case kFp4b: launch_fp4(epi, a, s, 1); break;
+    default: launch_fp4(epi, a, s, 2); break;
+    }
+    CASPHY_HIP(hipGetLastError());
 }
 
 std::string mfma_tuning_report()

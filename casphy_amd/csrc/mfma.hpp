@@ -11,6 +11,18 @@ namespace casphy {
 constexpr u32 STAGE_PAD = 256;
 inline u32 stage_rows(u32 n) { return ceil_div(n, STAGE_PAD) * STAGE_PAD; }
 
+// One problem of a group-mode launch (many independent row sets in one
+// launch): its A and B rows start at rows a0 / b0 (multiples of 128) of the
+// gathered stage-major operands and number nA / nB; its output is the mask of
+// nA rows of `ld` words at word offset `off` (E_MASK) or the nA x nB scores
+// of row stride `ld` at offset `off` (E_SCORES).
+struct GroupDesc {
+    u32 a0 = 0, b0 = 0;
+    u32 nA = 0, nB = 0;
+    u64 off = 0;
+    u32 ld = 0, pad_ = 0;
+};
+
 struct MfmaArgs {
     const u32 *PA = nullptr;  // stage-major packed bits of the A rows: [stages][nrows_padA][4]
     const u32 *PB = nullptr;  // and of the B rows (== PA for self scoring)
@@ -43,6 +55,14 @@ struct MfmaArgs {
     const u32 *pq = nullptr;
     const u64 *prow = nullptr;
     u32 *pout = nullptr;
+    // Group mode (E_MASK, E_SCORES; 128 x 128 tiles): launch slot t is tile
+    // (glist[t].x, glist[t].y) of the gathered operands, of group glist[t].z;
+    // the epilogue sees the group's own rows, columns and output, and for
+    // E_MASK its threshold gthr[2 g] (INT32_MAX: the group scores nothing).
+    const uint4 *glist = nullptr;
+    const GroupDesc *gdesc = nullptr;
+    const int32_t *gthr = nullptr;
+    u32 orgA = 0, orgB = 0;   // set in the kernel: the group's first A / B row
 };
 
 // First kw words of cell-major rows of stride ldw -> stage-major P (rows padded
@@ -65,6 +85,10 @@ void launch_mfma_tiles(int epi, const MfmaArgs &a, hipStream_t s);
 // expansions EA / EB are not given or the sums could pass 2^24); creates no
 // events and allocates nothing.
 void launch_mfma_tiles_untuned(int epi, const MfmaArgs &a, hipStream_t s);
+// Group-mode launch over `nslots` entries of a.glist (E_MASK or E_SCORES):
+// the fp4 tiles when a.EA / a.EB are given and the sums stay exact, else the
+// int8 tiles; allocates nothing and times nothing.
+void launch_mfma_groups(int epi, const MfmaArgs &a, u32 nslots, hipStream_t s);
 // JSON report of the autotuner's choices (shape class -> variant, timings),
 // followed by the notes other tuners added (one JSON object each).
 std::string mfma_tuning_report();

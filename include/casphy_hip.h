@@ -368,6 +368,31 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
  * lists go through .idx / .nNeighbor files between radiusNeighbor and
  * tipMerge (R/CasPhy_functions.R:2024-2026). */
 int casphy_cells_tip_merge(casphy_cells *h, double p, int32_t *member);
+/* autoSplit's radius search for many groups of the resident cells at once:
+ * run.split (R/CasPhy_functions.R:2896-2937) calls autoSplit (:1964-2082) on
+ * mat[inIdx, ] for each group of a split layer, one tryRadius process per
+ * radius step and one radiusNeighbor process per group.  Group g is the
+ * n_g = goff[g+1] - goff[g] rows rows[goff[g] .. goff[g+1]) of h (goff[0] = 0,
+ * every group non-empty); row rows[goff[g] + i] is the group's row i.  Its
+ * queries are the group rows q[qoff[g] .. qoff[g+1]) (ids < n_g, qoff[0] = 0,
+ * at least one per group).  Per group: tryRadius counts of its queries at
+ * radii[0..nradii) (ascending, nradii <= 64), radius[g] = the first radius
+ * whose mean count passes log10(mean) <= nneighbor_bound - log10(n_g) (the
+ * rule of casphy_cells_autosplit with the group's own bound), or -1 when none
+ * does, and the group's radius lists at that radius, ids local to the group
+ * (nidx[g] ids; none for radius -1).  On the matrix cores every phase is one
+ * launch over all groups; cells on the VALU tiles run the groups in turn.
+ * The lists stay on the device for casphy_cells_group_fetch /
+ * casphy_cells_group_tip_merge until the next group search. */
+int casphy_cells_split_groups(casphy_cells *h, const uint32_t *goff, const uint32_t *rows, uint32_t ngroups,
+                              const int32_t *radii, uint32_t nradii, const uint32_t *qoff, const uint32_t *q,
+                              double nneighbor_bound, int32_t *radius, uint64_t *nidx);
+/* Group g's lists of the last group search: nneighbor[n_g], idx[nidx[g]]. */
+int casphy_cells_group_fetch(casphy_cells *h, uint32_t g, uint32_t *nneighbor, uint32_t *idx);
+/* casphy_tip_merge over group g's lists of the last group search (the rest
+ * of autoSplit's split step for that group, cu/tipMerge.cu:143-461);
+ * member[n_g] labels the group's rows. */
+int casphy_cells_group_tip_merge(casphy_cells *h, uint32_t g, double p, int32_t *member);
 /* Device time (ms, hipEvents on the handle's stream): tile_ms = the scoring
  * kernel of the last radius, try_radii or autosplit call (autosplit: the
  * radius-mask tiles); radius_ms = the whole last radius or autosplit call;
