@@ -332,9 +332,53 @@ def base_editor_round(cp, synth, with_cpu: bool, steps: int = 100, warmup: int =
     join_s, _ = timed(lambda: cp.join_bit(mat), 1)
     out["binary_join"] = {"plain_jobs_per_s": 1 / join_s, "ms_per_job": join_s * 1e3}
     out["nodekeep"] = node_keep_step(cp, mat, with_cpu)
-    out["tipmerge"] = tip_merge_step(cp, cells, q, with_cpu)
+    out["tipmerge"], member = tip_merge_step(cp, cells, q, with_cpu)
+    out["split_layer"] = split_layer_step(cp, mat, cells, member, out["tipmerge"]["radius"])
     cells.close()
     return out
+
+
+SPLIT_MIN = 10  # regularize.split's min_size default (R/CasPhy_functions.R:2896-2910)
+
+
+def split_layer_step(cp, mat, cells, member, radius: int) -> dict:
+    """The next split layer (run.split per group, R/CasPhy_functions.R:2896-2937):
+    every tipMerge cluster of at least 10 cells is a group whose autoSplit
+    search starts at the parent's radius (16 radii, step 20, nNeighborBound
+    9.5, up to 1,024 query rows per group).  Plain: the way R runs it, one
+    sub-matrix per group made resident and searched with its own device round;
+    batched: one group search over the resident parent matrix.  Every group's
+    radius and lists are compared between the two."""
+    rng = np.random.default_rng(8)
+    labels = np.unique(member)
+    groups = [np.flatnonzero(member == lab).astype(np.uint32) for lab in labels]
+    groups = [g for g in groups if g.size >= SPLIT_MIN]
+    queries = [rng.integers(0, g.size, min(g.size, R)).astype(np.uint32) for g in groups]
+    radii = (radius + 20 * np.arange(16)).astype(np.int32)
+    cells.split_groups(groups, radii, queries)
+    batch_s, (rad, nidx) = timed(lambda: cells.split_groups(groups, radii, queries), 5)
+    batched = [cells.group_fetch(g) for g in range(len(groups))]
+
+    def plain():
+        res = []
+        for g, rows in enumerate(groups):
+            sub = cp.Cells(np.ascontiguousarray(mat[rows].T), "bit", wordmajor=True)
+            r, total = sub.autosplit(radii, queries[g], 9.5 - float(np.log10(rows.size)))
+            res.append((r, sub.fetch() if total else None))
+            sub.close()
+        return res
+    plain()
+    plain_s, res = timed(plain, 1)
+    for g in range(len(groups)):
+        assert res[g][0] == rad[g], "plain / batched split layer radius mismatch"
+        if res[g][1] is not None:
+            assert np.array_equal(res[g][1][0], batched[g][0]) and np.array_equal(res[g][1][1], batched[g][1])
+    sizes = np.array([g.size for g in groups])
+    return {"groups": len(groups), "cells": int(sizes.sum()), "largest": int(sizes.max()),
+            "median": float(np.median(sizes)), "neighbour_ids": int(nidx.sum()),
+            "plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3, "speedup": plain_s / batch_s,
+            "plain_form": "one resident sub-matrix and one device autoSplit round per group",
+            "bit_exact_between_forms": True}
 
 
 def indel_config(cp, synth, with_cpu: bool) -> dict:
@@ -580,7 +624,7 @@ def tip_merge_step(cp, cells, q, with_cpu: bool) -> dict:
         out["cpu_baseline"] = {"value": n / cpu_s, "unit": "tips/s", "cores": 1, "kind": "port",
                                "sample": f"the full {n}-tip workload, {cpu_s:.2f} s"}
         out["speedup_vs_cpu"] = cpu_s / dev_s
-    return out
+    return out, member
 
 
 VOTE_TIPS = 10000      # voteRoot2: tree on 10,000 sequences of the DNA alignment
@@ -857,7 +901,9 @@ def main():
             print(json.dumps(node_keep_step(cp, mat, with_cpu)))
         else:
             cells = cp.Cells(np.ascontiguousarray(mat.T), "bit", wordmajor=True)
-            print(json.dumps(tip_merge_step(cp, cells, synth.query_rows(2, N, R), with_cpu)))
+            out, member = tip_merge_step(cp, cells, synth.query_rows(2, N, R), with_cpu)
+            out["split_layer"] = split_layer_step(cp, mat, cells, member, out["radius"])
+            print(json.dumps(out))
         return
     if args.only in ("voteroot", "fasta"):
         dna = synth.dna_matrix(seed=3, n=N3, ncol=NCOL3)
