@@ -65,6 +65,10 @@ SIGNATURES = [
     ("casphy_server_destroy", None, [vp]),
     ("casphy_server_submit_join", C.c_int, [vp, C.c_int, vp, u32, u32, C.c_int, C.POINTER(u64)]),
     ("casphy_server_submit_transfer", C.c_int, [vp, vp, u32, vp, u32, u32, u32, C.POINTER(u64)]),
+    ("casphy_server_put_matrix", C.c_int, [vp, C.c_int, vp, u32, u32, C.POINTER(u64)]),
+    ("casphy_server_drop_matrix", C.c_int, [vp, u64]),
+    ("casphy_server_submit_join_rows", C.c_int, [vp, u64, vp, u32, C.c_int, C.POINTER(u64)]),
+    ("casphy_server_submit_transfer_trees", C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp, u32, u32, C.POINTER(u64)]),
     ("casphy_server_wait", C.c_int, [vp, u64, vp, u64, C.POINTER(C.c_double)]),
     ("casphy_server_stats", C.c_int, [vp, C.POINTER(u64), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(u64)]),

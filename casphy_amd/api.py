@@ -715,9 +715,10 @@ __all__.append("Cells")
 
 # ------------------------------------------------------------------ serving
 class Server:
-    """Request loop over one GPU (casphy_server_* of the C ABI): FIFO admission
-    under an HBM budget, `workers` concurrent streams, small joins batched into
-    one persistent launch.  submit_* return a job id; wait(job) returns the
+    """Request loop over one GPU (casphy_server_* of the C ABI): admission
+    under an HBM budget without head-of-line blocking, `workers` concurrent
+    streams, small joins batched into one persistent launch, packed matrices
+    resident across jobs.  submit_* return a job id; wait(job) returns the
     job's output (raw edges for joins, (ymin, ysize, xmin, xsize) for
     transfers) and its latency in ms."""
 
@@ -755,14 +756,67 @@ class Server:
         self._shape[job.value] = ("transfer", Y.shape[0], X.shape[0])
         return job.value
 
+    def put_matrix(self, mat, kind: str = "bit") -> int:
+        """Makes a packed matrix resident on the server's device: n x W u32
+        bit rows (kind "bit") or n x ncol int32 states ("indel")."""
+        m = _u32(mat) if kind == "bit" else _i32(mat)
+        mid = C.c_uint64(0)
+        check(load().casphy_server_put_matrix(self._h, 0 if kind == "bit" else 2, _p(m), m.shape[0], m.shape[1],
+                                              C.byref(mid)))
+        return mid.value
+
+    def drop_matrix(self, mid: int):
+        check(load().casphy_server_drop_matrix(self._h, int(mid)))
+
+    def submit_join_rows(self, mid: int, rows, ancestral_type: int = 1) -> int:
+        r = _u32(rows)
+        job = C.c_uint64(0)
+        check(load().casphy_server_submit_join_rows(self._h, int(mid), _p(r), r.size, int(ancestral_type),
+                                                    C.byref(job)))
+        self._shape[job.value] = ("join", r.size)
+        return job.value
+
+    def submit_transfer_trees(self, ref_edge, ref_nnode: int, rep_edges, rep_nnodes, ntip: int) -> int:
+        """Transfer distances from trees (R 1-based edge matrices, as
+        transfer_trees); wait() returns (ymin (nrep, ny), ysize, [xmin_b], [xsize_b])."""
+        ye = np.ascontiguousarray(ref_edge, dtype=np.int32)
+        ya, yd = np.ascontiguousarray(ye[:, 0]), np.ascontiguousarray(ye[:, 1])
+        reps = [np.ascontiguousarray(e, dtype=np.int32) for e in rep_edges]
+        eoff = np.zeros(len(reps) + 1, np.uint32)
+        eoff[1:] = np.cumsum([e.shape[0] for e in reps])
+        allx = np.concatenate(reps) if reps else np.zeros((0, 2), np.int32)
+        xa, xd = np.ascontiguousarray(allx[:, 0]), np.ascontiguousarray(allx[:, 1])
+        xn = np.ascontiguousarray(rep_nnodes, dtype=np.uint32)
+        job = C.c_uint64(0)
+        check(load().casphy_server_submit_transfer_trees(
+            self._h, _p(ya), _p(yd), ye.shape[0], int(ref_nnode), _p(xa) if xa.size else None,
+            _p(xd) if xd.size else None, _p(eoff), _p(xn), len(reps), int(ntip), C.byref(job)))
+        self._shape[job.value] = ("trees", int(ref_nnode) - 1, [int(n) - 1 for n in rep_nnodes])
+        return job.value
+
     def wait(self, job: int):
         kind = self._shape.pop(job)
-        size = (2 * kind[1] - 2) * 2 if kind[0] == "join" else 2 * kind[1] + 2 * kind[2]
+        if kind[0] == "join":
+            size = (2 * kind[1] - 2) * 2
+        elif kind[0] == "trees":
+            ny, rows = kind[1], kind[2]
+            size = len(rows) * ny + ny + 2 * sum(rows)
+        else:
+            size = 2 * kind[1] + 2 * kind[2]
         out = np.empty(max(size, 1), np.uint32)
         lat = C.c_double(0)
         check(load().casphy_server_wait(self._h, int(job), _p(out), size, C.byref(lat)))
         if kind[0] == "join":
             return out[:size].reshape(-1, 2), lat.value
+        if kind[0] == "trees":
+            nrep, tot = len(rows), sum(rows)
+            ymin = out[:nrep * ny].reshape(nrep, ny)
+            ysize = out[nrep * ny:nrep * ny + ny]
+            xs = out[nrep * ny + ny:]
+            off = np.concatenate([[0], np.cumsum(rows)]).astype(np.int64)
+            xmin = [xs[off[b]:off[b + 1]] for b in range(nrep)]
+            xsize = [xs[tot + off[b]:tot + off[b + 1]] for b in range(nrep)]
+            return (ymin, ysize, xmin, xsize), lat.value
         ny, nx = kind[1], kind[2]
         return (out[:ny], out[ny:2 * ny], out[2 * ny:2 * ny + nx], out[2 * ny + nx:2 * ny + 2 * nx]), lat.value
 

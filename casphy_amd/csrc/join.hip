@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "join.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
 #include "tiles.hpp"
@@ -842,6 +843,57 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
 void require(bool c, const char *msg) { CASPHY_REQUIRE(c, CASPHY_EARG, msg); }
 
 }  // namespace
+
+void join_bit_dev(const u32 *raw, u32 n, u32 W, int ancestral_type, u32 *edge_host, hipStream_t s)
+{
+    const u32 ldw = round_up(W, KC_BIT);
+    Tree t;
+    tree_alloc(t, n, ldw, ancestral_type);
+    hipLaunchKernelGGL(k_bit_slots, dim3(grid_for((u64)n * 2 * ldw)), dim3(256), 0, s, raw, n, W, t.rows.as<u32>(),
+                       ldw);
+    CASPHY_HIP(hipGetLastError());
+    tree_run(t, Planes{}, nullptr, 0, 2.0f, s);
+    d2h(edge_host, t.edge.p, (size_t)(2 * n - 2) * 2 * 4, s);
+}
+
+void join_indel_dev(const int32_t *raw, u32 n, u32 ncol, u32 *edge_host, hipStream_t s)
+{
+    IndelLayout L = indel_layout(raw, n, ncol, s);
+    Tree t;
+    tree_alloc(t, n, L.ldw, 2);
+    launch_fill_u32(t.rows.as<u32>(), (u64)n * 2 * L.ldw, 0, s);
+    hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)n * ncol)), dim3(256), 0, s, raw, n, ncol,
+                       L.dloci.as<u32>(), L.dwidth.as<u32>(), t.rows.as<u32>(), L.ldw);
+    CASPHY_HIP(hipGetLastError());
+    tree_run(t, Planes{}, nullptr, 0, 2.0f, s);
+    d2h(edge_host, t.edge.p, (size_t)(2 * n - 2) * 2 * 4, s);
+}
+
+void join_bit_batch_dev(const u32 *raw, const std::vector<u32> &offs, u32 W, int ancestral_type, u32 *edge_host,
+                        hipStream_t s)
+{
+    const u32 ldw = round_up(W, KC_BIT), total = offs.back();
+    DevBuf rows((size_t)total * 2 * ldw * 4);
+    hipLaunchKernelGGL(k_bit_slots, dim3(grid_for((u64)total * 2 * ldw)), dim3(256), 0, s, raw, total, W,
+                       rows.as<u32>(), ldw);
+    CASPHY_HIP(hipGetLastError());
+    batch_run(rows, offs, ldw, ancestral_type, Planes{}, nullptr, 0, 2.0f, edge_host, s);
+}
+
+void join_indel_batch_dev(const int32_t *raw, const std::vector<u32> &offs, u32 ncol, u32 *edge_host,
+                          hipStream_t s)
+{
+    const u32 total = offs.back();
+    // one layout for every job: per-site allele groups sized by the maximum over all jobs
+    IndelLayout L = indel_layout(raw, total, ncol, s);
+    DevBuf rows((size_t)total * 2 * L.ldw * 4);
+    launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
+    hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, raw, total, ncol,
+                       L.dloci.as<u32>(), L.dwidth.as<u32>(), rows.as<u32>(), L.ldw);
+    CASPHY_HIP(hipGetLastError());
+    batch_run(rows, offs, L.ldw, 2, Planes{}, nullptr, 0, 2.0f, edge_host, s);
+}
+
 }  // namespace casphy
 
 using namespace casphy;

@@ -1,20 +1,28 @@
 // serve.hip -- request loop for tree-build and transfer-distance jobs.
 //
 // A casphy_server owns a pool of worker threads, each with its own stream on
-// the server's device.  Jobs are submitted from any thread (inputs are copied,
-// so callers may reuse their buffers) and admitted in FIFO order while their
-// estimated device footprint fits the HBM budget.  A free worker takes the
-// admitted head of the queue; when the head is a small join it also takes
-// every other small join of the same kind and width that has arrived by then
-// and runs them as one persistent work-queue launch (continuous batching:
-// requests that arrive while others are on the GPU join the next launch on
-// the next free worker, and launches of different workers overlap on the
-// device).  Large joins and transfer-distance jobs run on their own worker.
+// the server's device, so the launches of different workers overlap on the
+// device.  Jobs are submitted from any thread; their inputs are copied, so
+// callers may reuse their buffers.
+//   * Packed matrices can be made resident once (casphy_server_put_matrix)
+//     and joined over any subset of their rows (casphy_server_submit_join_rows):
+//     the worker gathers the rows on the device, nothing crosses PCIe per job.
+//   * Admission by HBM budget without head-of-line blocking: a worker takes the
+//     first queued job whose estimated footprint fits next to the running
+//     jobs; once the oldest job has waited `hol_ms` without fitting, later jobs
+//     start only if they leave room for it, so it starts as soon as enough
+//     running jobs finish.
+//   * Continuous batching: a worker that takes a small join also takes every
+//     other small join of the same kind, width and rule queued by then and runs
+//     them as one persistent work-queue launch.  With two or more workers, one
+//     of them takes small joins only, so they never wait behind large ones.
+//   * Transfer jobs come as clade bitsets or as trees (edge matrices: the
+//     transfer distances are computed from the trees on the device).
 // Each job's output equals the corresponding plain entry point's.
 #include <algorithm>
 #include <chrono>
-#include <cstdlib>
 #include <condition_variable>
+#include <cstdlib>
 #include <deque>
 #include <memory>
 #include <mutex>
@@ -22,13 +30,15 @@
 #include <unordered_map>
 #include <vector>
 
-#include "common.hpp"
+#include "context.hpp"
+#include "join.hpp"
 
 namespace casphy {
 namespace {
 
 constexpr u32 BATCH_MAX_CELLS = 2048;  // joins up to this size are batched
 constexpr u32 BATCH_MAX_JOBS = 1024;
+constexpr int JOB_TREES = 4;           // transfer distances from edge matrices
 
 using clock_t_ = std::chrono::steady_clock;
 
@@ -37,43 +47,78 @@ double ms_between(clock_t_::time_point a, clock_t_::time_point b)
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// A packed matrix kept on the server's device.
+struct Resident {
+    int kind = CASPHY_KIND_BIT;
+    u32 n = 0, width = 0;
+    u32 onehot_bits = 0;  // indel: one-hot width of the whole matrix (bounds any row subset's)
+    DevBuf d;
+};
+
+// out[i][0 .. width) = mat[rows[i]][0 .. width) (32-bit words)
+__global__ void k_gather_words(const u32 *mat, u32 width, const u32 *rows, u32 n, u32 *out)
+{
+    const u64 total = (u64)n * width;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 i = (u32)(x / width), w = (u32)(x - (u64)i * width);
+        out[x] = mat[(size_t)rows[i] * width + w];
+    }
+}
+
 struct Job {
     u64 id = 0;
-    int kind = 0;  // CASPHY_KIND_BIT / CASPHY_KIND_INDEL joins, CASPHY_JOB_TRANSFER
+    int kind = 0;  // CASPHY_KIND_BIT / CASPHY_KIND_INDEL joins, CASPHY_JOB_TRANSFER, JOB_TREES
     u32 n = 0, width = 0;
     int anc = 1;
     std::vector<u32> mat;     // bit join input, or Y then X for transfer
     std::vector<int32_t> imat;
+    std::shared_ptr<Resident> res;  // join over rows of a resident matrix
+    std::vector<u32> rows;
     u32 ny = 0, nx = 0, ncol = 0, ntip = 0;
+    // JOB_TREES: reference and replicate edge matrices
+    std::vector<int32_t> yanc, ydes, xanc, xdes;
+    std::vector<u32> xoff, xnnode;
+    u32 ynedge = 0, ynnode = 0, nrep = 0;
     std::vector<u32> out;     // join: raw edges; transfer: ymin|ysize|xmin|xsize
     size_t dev_bytes = 0;
     int status = 1;           // 1 pending, 0 done, < 0 error
     std::string err;
     clock_t_::time_point t_submit, t_done;
-    bool batchable() const
-    {
-        return (kind == CASPHY_KIND_BIT || kind == CASPHY_KIND_INDEL) && n <= BATCH_MAX_CELLS;
-    }
+    bool join() const { return kind == CASPHY_KIND_BIT || kind == CASPHY_KIND_INDEL; }
+    bool batchable() const { return join() && n <= BATCH_MAX_CELLS; }
+    bool same_batch(const Job &o) const { return kind == o.kind && width == o.width && anc == o.anc; }
 };
 
 size_t stage_pad_rows(u32 n) { return ((size_t)n + 255) / 256 * 256; }
 
+size_t join_bytes(u32 n, size_t ldw, u32 width)
+{
+    return (size_t)n * (ldw * 4 * 5 + width * 4 * 2 + 96) + stage_pad_rows(n) * ldw * 4;
+}
+
+u32 onehot_bits(const int32_t *m, u32 n, u32 width)
+{
+    size_t bits = 0;
+    for (u32 s = 0; s < width; ++s) {
+        int32_t mx = 1;
+        for (u32 r = 0; r < n; ++r) mx = std::max(mx, m[(size_t)r * width + s]);
+        bits += (size_t)mx;
+    }
+    return (u32)std::min<size_t>(bits, 0xFFFFFFFFu);
+}
+
 size_t estimate_bytes(const Job &j)
 {
-    if (j.kind == CASPHY_KIND_BIT) {
-        const size_t ldw = (j.width + 31) / 32 * 32;
-        return (size_t)j.n * (ldw * 4 * 5 + j.width * 4 + 96) + (size_t)stage_pad_rows(j.n) * ldw * 4;
-    }
+    if (j.kind == CASPHY_KIND_BIT) return join_bytes(j.n, (j.width + 31) / 32 * 32, j.width);
     if (j.kind == CASPHY_KIND_INDEL) {
-        // one-hot width: per site max(max allele, 1) bits
-        size_t bits = 0;
-        for (u32 s = 0; s < j.width; ++s) {
-            int32_t m = 1;
-            for (u32 r = 0; r < j.n; ++r) m = std::max(m, j.imat[(size_t)r * j.width + s]);
-            bits += (size_t)m;
-        }
-        const size_t ldw = ((bits + 31) / 32 + 31) / 32 * 32;
-        return (size_t)j.n * (ldw * 4 * 5 + j.width * 4 + 96) + (size_t)stage_pad_rows(j.n) * ldw * 4;
+        const u32 bits = j.res ? j.res->onehot_bits : onehot_bits(j.imat.data(), j.n, j.width);
+        return join_bytes(j.n, ((size_t)(bits + 31) / 32 + 31) / 32 * 32, j.width);
+    }
+    if (j.kind == JOB_TREES) {
+        // per tree: the walk programs and positions (about 16 B per node) and the
+        // outputs; the clade plans are host memory
+        const size_t nodes = (size_t)j.ynedge + j.xanc.size() + j.nrep + 1;
+        return nodes * 48 + (size_t)j.nrep * j.ynnode * 8 + (64ull << 20);
     }
     const size_t ldw = (j.ncol + 31) / 32 * 32;
     return (size_t)(j.ny + j.nx + 512) * ldw * 4 * 4 + (size_t)(j.ny + j.nx) * 32;
@@ -91,7 +136,9 @@ struct casphy_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Job *> queue;
     std::unordered_map<u64, std::unique_ptr<Job>> jobs;
-    size_t in_use = 0;
+    std::unordered_map<u64, std::shared_ptr<Resident>> resident;
+    size_t running = 0;        // estimated footprint of the running jobs
+    size_t resident_bytes = 0; // the resident matrices
     bool stop = false;
     u64 next_id = 1;
     std::vector<std::thread> workers;
@@ -100,15 +147,63 @@ struct casphy_server {
     clock_t_::time_point t_first, t_last;
     u64 launches = 0, batched_jobs = 0;
     bool batching = true;  // CASPHY_SERVE_BATCH=0: one job per launch (queue-only form)
+    double hol_ms = 500.0; // CASPHY_SERVE_HOL_MS: how long the oldest job may be passed over
 
-    void worker_main();
+    void worker_main(bool small_lane);
+    // The jobs a worker may take now (empty: none); called with m held.
+    std::vector<Job *> pick(bool small_lane, size_t &bytes);
     // Runs a group's launch and fills each job's `out`; returns the status of
     // every job (0 or a CASPHY_E* code) and the error message.  Job status,
     // error and completion time are published by worker_main under `m`.
     std::vector<int> run_group(std::vector<Job *> &group, std::string &err);
 };
 
-void casphy_server::worker_main()
+std::vector<Job *> casphy_server::pick(bool small_lane, size_t &bytes)
+{
+    std::vector<Job *> group;
+    if (queue.empty()) return group;
+    const auto now = clock_t_::now();
+    Job *head = queue.front();
+    // `extra`: what the group being formed already holds.  A first job that
+    // fits nowhere still runs once nothing else is running.
+    auto can_start = [&](size_t need, size_t extra) {
+        return resident_bytes + running + extra + need <= budget || (running == 0 && extra == 0);
+    };
+    // the oldest job, once it has waited hol_ms without fitting, keeps its room
+    const bool reserve = !can_start(head->dev_bytes, 0) && ms_between(head->t_submit, now) >= hol_ms;
+    auto admissible = [&](const Job *j, size_t extra) {
+        if (j == head || !reserve) return can_start(j->dev_bytes, extra);
+        return resident_bytes + running + extra + j->dev_bytes + head->dev_bytes <= budget;
+    };
+    auto first = queue.end();
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+        if (small_lane && !(*it)->batchable()) continue;
+        if (admissible(*it, 0)) {
+            first = it;
+            break;
+        }
+    }
+    if (first == queue.end()) return group;
+    Job *h = *first;
+    group.push_back(h);
+    bytes = h->dev_bytes;
+    queue.erase(first);
+    if (h->batchable() && batching) {
+        for (auto it = queue.begin(); it != queue.end() && group.size() < BATCH_MAX_JOBS;) {
+            Job *j = *it;
+            if (j->batchable() && j->same_batch(*h) && admissible(j, bytes)) {
+                group.push_back(j);
+                bytes += j->dev_bytes;
+                it = queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    return group;
+}
+
+void casphy_server::worker_main(bool small_lane)
 {
     if (casphy_set_device(device) != CASPHY_OK) return;
     for (;;) {
@@ -118,30 +213,12 @@ void casphy_server::worker_main()
             std::unique_lock<std::mutex> lk(m);
             for (;;) {
                 if (stop) return;
-                if (!queue.empty()) {
-                    Job *h = queue.front();
-                    if (in_use == 0 || in_use + h->dev_bytes <= budget) break;
-                }
-                cv_work.wait(lk);
+                group = pick(small_lane, bytes);
+                if (!group.empty()) break;
+                // a reserved head may become admissible by the clock alone
+                cv_work.wait_for(lk, std::chrono::milliseconds(50));
             }
-            Job *h = queue.front();
-            queue.pop_front();
-            group.push_back(h);
-            bytes = h->dev_bytes;
-            if (h->batchable() && batching) {
-                for (auto it = queue.begin(); it != queue.end() && group.size() < BATCH_MAX_JOBS;) {
-                    Job *j = *it;
-                    if (j->batchable() && j->kind == h->kind && j->width == h->width && j->anc == h->anc &&
-                        in_use + bytes + j->dev_bytes <= std::max(budget, h->dev_bytes)) {
-                        group.push_back(j);
-                        bytes += j->dev_bytes;
-                        it = queue.erase(it);
-                    } else {
-                        ++it;
-                    }
-                }
-            }
-            in_use += bytes;
+            running += bytes;
             ++launches;
             if (group.size() > 1) batched_jobs += group.size();
         }
@@ -158,11 +235,12 @@ void casphy_server::worker_main()
             // the worker writes into a job is written here, before the notify,
             // and nothing after
             std::lock_guard<std::mutex> lk(m);
-            in_use -= bytes;
+            running -= bytes;
             const auto now = clock_t_::now();
             for (size_t k = 0; k < group.size(); ++k) {
                 Job *j = group[k];
                 j->t_done = now;
+                j->res.reset();  // the resident matrix may be dropped now
                 if (rc[k] == 0) {
                     lat_ms.push_back(ms_between(j->t_submit, now));
                 } else {
@@ -194,38 +272,61 @@ std::vector<int> casphy_server::run_group(std::vector<Job *> &group, std::string
                                       o.data() + 2 * h->ny, o.data() + 2 * h->ny + h->nx);
         return r ? fail(r) : rc;
     }
-    for (Job *j : group) j->out.assign((size_t)(2 * j->n - 2) * 2, 0);
-    if (group.size() == 1) {
-        const int r = h->kind == CASPHY_KIND_BIT
-                          ? casphy_join_bit(h->mat.data(), h->n, h->width, nullptr, h->anc, h->out.data(), nullptr,
-                                            nullptr)
-                          : casphy_join_indel(h->imat.data(), h->n, h->width, nullptr, 0, nullptr, 0, 2.0f,
-                                              h->out.data(), nullptr, nullptr);
+    if (h->kind == JOB_TREES) {
+        const u32 ny = h->ynnode - 1;
+        size_t nxr = 0;
+        for (u32 b = 0; b < h->nrep; ++b) nxr += h->xnnode[b] - 1;
+        std::vector<u32> &o = h->out;
+        o.assign((size_t)h->nrep * ny + ny + 2 * nxr, 0);
+        const int r = casphy_transfer_trees(h->yanc.data(), h->ydes.data(), h->ynedge, h->ynnode,
+                                            h->xanc.empty() ? nullptr : h->xanc.data(),
+                                            h->xdes.empty() ? nullptr : h->xdes.data(), h->xoff.data(),
+                                            h->xnnode.data(), h->nrep, h->ntip, o.data(),
+                                            o.data() + (size_t)h->nrep * ny, o.data() + (size_t)h->nrep * ny + ny,
+                                            o.data() + (size_t)h->nrep * ny + ny + nxr);
         return r ? fail(r) : rc;
     }
-    std::vector<u32> offs(group.size() + 1, 0);
-    for (size_t k = 0; k < group.size(); ++k) offs[k + 1] = offs[k] + group[k]->n;
-    std::vector<u32> edges((size_t)(2 * (u64)offs.back() - 2 * group.size()) * 2);
-    int r;
-    if (h->kind == CASPHY_KIND_BIT) {
-        std::vector<u32> all((size_t)offs.back() * h->width);
-        for (size_t k = 0; k < group.size(); ++k)
-            std::copy(group[k]->mat.begin(), group[k]->mat.end(), all.begin() + (size_t)offs[k] * h->width);
-        r = casphy_join_bit_batch(all.data(), offs.data(), (u32)group.size(), h->width, nullptr, h->anc, edges.data());
-    } else {
-        std::vector<int32_t> all((size_t)offs.back() * h->width);
-        for (size_t k = 0; k < group.size(); ++k)
-            std::copy(group[k]->imat.begin(), group[k]->imat.end(), all.begin() + (size_t)offs[k] * h->width);
-        r = casphy_join_indel_batch(all.data(), offs.data(), (u32)group.size(), h->width, nullptr, 0, nullptr, 0,
-                                    2.0f, edges.data());
-    }
-    if (r) return fail(r);
-    size_t pos = 0;
-    for (Job *j : group) {
-        std::copy(edges.begin() + pos, edges.begin() + pos + j->out.size(), j->out.begin());
-        pos += j->out.size();
-    }
-    return rc;
+    // joins: the group's rows assembled on the device (uploaded from the
+    // copies taken at submission, or gathered from a resident matrix)
+    const int r = guarded([&] {
+               hipStream_t s = thread_stream();
+               std::vector<u32> offs(group.size() + 1, 0);
+               for (size_t k = 0; k < group.size(); ++k) offs[k + 1] = offs[k] + group[k]->n;
+               const u32 W = h->width;
+               DevBuf raw((size_t)offs.back() * W * 4), didx;
+               for (size_t k = 0; k < group.size(); ++k) {
+                   Job *j = group[k];
+                   u32 *dst = raw.as<u32>() + (size_t)offs[k] * W;
+                   if (j->res) {
+                       didx.reserve((size_t)j->n * 4);
+                       h2d(didx.p, j->rows.data(), (size_t)j->n * 4, s);
+                       const u64 words = (u64)j->n * W;
+                       hipLaunchKernelGGL(k_gather_words, dim3((u32)std::min<u64>((words + 255) / 256, 4096)),
+                                          dim3(256), 0, s, j->res->d.as<u32>(), W, didx.as<u32>(), j->n, dst);
+                       CASPHY_HIP(hipGetLastError());
+                   } else if (j->kind == CASPHY_KIND_BIT) {
+                       h2d(dst, j->mat.data(), (size_t)j->n * W * 4, s);
+                   } else {
+                       h2d(dst, j->imat.data(), (size_t)j->n * W * 4, s);
+                   }
+                   j->out.assign((size_t)(2 * j->n - 2) * 2, 0);
+               }
+               const bool bit = h->kind == CASPHY_KIND_BIT;
+               if (group.size() == 1) {
+                   if (bit) join_bit_dev(raw.as<u32>(), h->n, W, h->anc, h->out.data(), s);
+                   else join_indel_dev(raw.as<int32_t>(), h->n, W, h->out.data(), s);
+                   return;
+               }
+               std::vector<u32> edges((size_t)(2 * (u64)offs.back() - 2 * group.size()) * 2);
+               if (bit) join_bit_batch_dev(raw.as<u32>(), offs, W, h->anc, edges.data(), s);
+               else join_indel_batch_dev(raw.as<int32_t>(), offs, W, edges.data(), s);
+               size_t pos = 0;
+               for (Job *j : group) {
+                   std::copy(edges.begin() + pos, edges.begin() + pos + j->out.size(), j->out.begin());
+                   pos += j->out.size();
+               }
+           });
+    return r == CASPHY_OK ? rc : fail(r);
 }
 
 namespace {
@@ -241,7 +342,7 @@ int submit(casphy_server *s, std::unique_ptr<Job> j, uint64_t *id)
         s->queue.push_back(j.get());
         s->jobs.emplace(j->id, std::move(j));
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return CASPHY_OK;
 }
 }  // namespace
@@ -259,7 +360,10 @@ int casphy_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers
         s->budget = hbm_budget_bytes;
         const char *b = std::getenv("CASPHY_SERVE_BATCH");
         s->batching = !(b && b[0] == '0');
-        for (u32 w = 0; w < workers; ++w) s->workers.emplace_back([p = s.get()] { p->worker_main(); });
+        const char *hol = std::getenv("CASPHY_SERVE_HOL_MS");
+        if (hol) s->hol_ms = std::atof(hol);
+        for (u32 w = 0; w < workers; ++w)
+            s->workers.emplace_back([p = s.get(), lane = workers >= 2 && w == 0] { p->worker_main(lane); });
         *out = s.release();
     });
 }
@@ -274,6 +378,53 @@ void casphy_server_destroy(casphy_server *s)
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
     delete s;
+}
+
+int casphy_server_put_matrix(casphy_server *s, int kind, const void *mat, uint32_t n, uint32_t width,
+                             uint64_t *matrix)
+{
+    return guarded([&] {
+        CASPHY_REQUIRE(s && mat && matrix && n > 0 && width > 0, CASPHY_EARG,
+                       "casphy_server_put_matrix: bad arguments");
+        CASPHY_REQUIRE(kind == CASPHY_KIND_BIT || kind == CASPHY_KIND_INDEL, CASPHY_EARG,
+                       "casphy_server_put_matrix: kind must be CASPHY_KIND_BIT or CASPHY_KIND_INDEL");
+        auto r = std::make_shared<Resident>();
+        r->kind = kind;
+        r->n = n;
+        r->width = width;
+        if (kind == CASPHY_KIND_INDEL) r->onehot_bits = onehot_bits(static_cast<const int32_t *>(mat), n, width);
+        const size_t bytes = (size_t)n * width * 4;
+        // upload on the server's device from this thread, then restore the thread's device
+        const int prev = thread_device();
+        int rc = casphy_set_device(s->device);
+        if (rc) throw Error(rc, casphy_last_error());
+        try {
+            hipStream_t st = thread_stream();
+            r->d.alloc(bytes);
+            h2d(r->d.p, mat, bytes, st);
+            CASPHY_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            (void)casphy_set_device(prev);
+            throw;
+        }
+        (void)casphy_set_device(prev);
+        std::lock_guard<std::mutex> lk(s->m);
+        *matrix = s->next_id++;
+        s->resident.emplace(*matrix, std::move(r));
+        s->resident_bytes += bytes;
+    });
+}
+
+int casphy_server_drop_matrix(casphy_server *s, uint64_t matrix)
+{
+    return guarded([&] {
+        CASPHY_REQUIRE(s, CASPHY_EARG, "casphy_server_drop_matrix: bad arguments");
+        std::lock_guard<std::mutex> lk(s->m);
+        auto it = s->resident.find(matrix);
+        CASPHY_REQUIRE(it != s->resident.end(), CASPHY_EARG, "casphy_server_drop_matrix: unknown matrix");
+        s->resident_bytes -= (size_t)it->second->n * it->second->width * 4;
+        s->resident.erase(it);  // queued and running jobs keep their reference
+    });
 }
 
 int casphy_server_submit_join(casphy_server *s, int kind, const void *mat, uint32_t n, uint32_t width,
@@ -298,6 +449,31 @@ int casphy_server_submit_join(casphy_server *s, int kind, const void *mat, uint3
     });
 }
 
+int casphy_server_submit_join_rows(casphy_server *s, uint64_t matrix, const uint32_t *rows, uint32_t nrows,
+                                   int ancestral_type, uint64_t *job)
+{
+    return guarded([&] {
+        CASPHY_REQUIRE(s && rows && job && nrows >= 2, CASPHY_EARG, "casphy_server_submit_join_rows: bad arguments");
+        CASPHY_REQUIRE(ancestral_type == 0 || ancestral_type == 1, CASPHY_EARG,
+                       "casphy_server_submit_join_rows: ancestralType must be 0 or 1");
+        std::shared_ptr<Resident> r;
+        {
+            std::lock_guard<std::mutex> lk(s->m);
+            auto it = s->resident.find(matrix);
+            CASPHY_REQUIRE(it != s->resident.end(), CASPHY_EARG, "casphy_server_submit_join_rows: unknown matrix");
+            r = it->second;
+        }
+        for (u32 i = 0; i < nrows; ++i)
+            CASPHY_REQUIRE(rows[i] < r->n, CASPHY_EARG,
+                           "casphy_server_submit_join_rows: row " + std::to_string(rows[i]) + " out of range");
+        auto j = std::make_unique<Job>();
+        j->kind = r->kind; j->n = nrows; j->width = r->width; j->anc = ancestral_type;
+        j->res = std::move(r);
+        j->rows.assign(rows, rows + nrows);
+        submit(s, std::move(j), job);
+    });
+}
+
 int casphy_server_submit_transfer(casphy_server *s, const uint32_t *Y, uint32_t ny, const uint32_t *X, uint32_t nx,
                                   uint32_t ncol, uint32_t ntip, uint64_t *job)
 {
@@ -308,6 +484,35 @@ int casphy_server_submit_transfer(casphy_server *s, const uint32_t *Y, uint32_t 
         j->kind = CASPHY_JOB_TRANSFER; j->ny = ny; j->nx = nx; j->ncol = ncol; j->ntip = ntip;
         j->mat.assign(Y, Y + (size_t)ny * ncol);
         j->mat.insert(j->mat.end(), X, X + (size_t)nx * ncol);
+        submit(s, std::move(j), job);
+    });
+}
+
+int casphy_server_submit_transfer_trees(casphy_server *s, const int32_t *y_anc, const int32_t *y_des,
+                                        uint32_t y_nedge, uint32_t y_nnode, const int32_t *x_anc,
+                                        const int32_t *x_des, const uint32_t *x_edge_offsets,
+                                        const uint32_t *x_nnode, uint32_t nrep, uint32_t ntip, uint64_t *job)
+{
+    return guarded([&] {
+        CASPHY_REQUIRE(s && y_anc && y_des && x_edge_offsets && x_nnode && job && nrep > 0 && y_nnode >= 2 &&
+                           ntip >= 2,
+                       CASPHY_EARG, "casphy_server_submit_transfer_trees: bad arguments");
+        const u32 xe = x_edge_offsets[nrep];
+        CASPHY_REQUIRE(x_edge_offsets[0] == 0 && (xe == 0 || (x_anc && x_des)), CASPHY_EARG,
+                       "casphy_server_submit_transfer_trees: bad replicate edges");
+        for (u32 b = 0; b < nrep; ++b)
+            CASPHY_REQUIRE(x_edge_offsets[b + 1] >= x_edge_offsets[b] && x_nnode[b] >= 1, CASPHY_EARG,
+                           "casphy_server_submit_transfer_trees: bad replicate " + std::to_string(b));
+        auto j = std::make_unique<Job>();
+        j->kind = JOB_TREES; j->ntip = ntip; j->nrep = nrep; j->ynedge = y_nedge; j->ynnode = y_nnode;
+        j->yanc.assign(y_anc, y_anc + y_nedge);
+        j->ydes.assign(y_des, y_des + y_nedge);
+        if (xe) {
+            j->xanc.assign(x_anc, x_anc + xe);
+            j->xdes.assign(x_des, x_des + xe);
+        }
+        j->xoff.assign(x_edge_offsets, x_edge_offsets + nrep + 1);
+        j->xnnode.assign(x_nnode, x_nnode + nrep);
         submit(s, std::move(j), job);
     });
 }

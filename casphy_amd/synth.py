@@ -435,3 +435,30 @@ def newick(parent: np.ndarray, leaves: np.ndarray, labels=None, rng: np.random.G
         if rng is not None and v != 0:
             out.append(f":{rng.random():.5f}")
     return "".join(out) + ";"
+
+
+# --------------------------------------------------------------- serving mix
+def serving_jobs(seed: int, njobs: int, min_cells: int, max_cells: int, nbin_rows: int, nindel_rows: int,
+                 tbe_every: int = 4, nrep: int = 3):
+    """The serving mix: job sizes log-uniform in [min_cells, max_cells]; every
+    tbe_every-th job is a transfer-distance job on a Yule tree of that many
+    tips against nrep replicates 5% SPR moves away (as R edge matrices), the
+    others alternate binary and indel tree builds over that many rows, drawn
+    without replacement and ascending, of resident matrices of nbin_rows /
+    nindel_rows rows.  Returns [(kind, data)]: ("bit" | "indel", rows) or
+    ("trees", (ref_edge, ref_nnode, rep_edges, rep_nnodes, ntip))."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(njobs):
+        n = int(round(np.exp(rng.uniform(np.log(min_cells), np.log(max_cells)))))
+        if tbe_every and k % tbe_every == tbe_every - 1:
+            parent, leaves = yule_parent(max(n, 4), rng)
+            ref_e, ref_n, ntip = ape_edges(parent, leaves)
+            reps = spr_edges_batch(parent, leaves, int(np.ceil(0.05 * (ntip - 3))), nrep, seed * 100003 + k)
+            jobs.append(("trees", (ref_e, ref_n, [e for e, _ in reps], [m for _, m in reps], ntip)))
+        else:
+            bit = len(jobs) % 2 == 0
+            pool = nbin_rows if bit else nindel_rows
+            rows = np.sort(rng.choice(pool, min(n, pool), replace=False)).astype(np.uint32)
+            jobs.append(("bit" if bit else "indel", rows))
+    return jobs

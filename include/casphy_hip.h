@@ -401,13 +401,17 @@ int casphy_cells_last_ms(casphy_cells *h, float *tile_ms, float *radius_ms);
 
 /* ------------------------------------------------------------------ serving */
 /* A request loop over one GPU: jobs of mixed sizes are submitted from any
- * thread, admitted in FIFO order while their estimated device footprint fits
- * `hbm_budget_bytes`, and run by `workers` worker threads (one stream each, so
- * their launches overlap on the device).  Joins of at most 2,048 cells that are
- * waiting when a worker frees up are batched into one persistent launch.  Every
- * job's output equals the plain entry point's (casphy_join_bit / _indel with
- * unit weights, casphy_transfer).  No counterpart in the reference (R forks
- * one executable per job, R/CasPhy_functions.R:2389-2392). */
+ * thread and run by `workers` worker threads (one stream each, so their
+ * launches overlap on the device).  A worker takes the first queued job whose
+ * estimated device footprint fits `hbm_budget_bytes` next to the running jobs
+ * and the resident matrices; once the oldest job has waited
+ * CASPHY_SERVE_HOL_MS (default 500) without fitting, later jobs start only if
+ * they leave room for it.  Joins of at most 2,048 cells that are waiting when
+ * a worker frees up are batched into one persistent launch; with two or more
+ * workers one of them takes only such joins.  Every job's output equals the
+ * plain entry point's (casphy_join_bit / _indel with unit weights,
+ * casphy_transfer, casphy_transfer_trees).  No counterpart in the reference
+ * (R forks one executable per job, R/CasPhy_functions.R:2389-2392). */
 #define CASPHY_KIND_INDEL 2
 #define CASPHY_JOB_TRANSFER 3
 typedef struct casphy_server casphy_server;
@@ -421,6 +425,27 @@ int casphy_server_submit_join(casphy_server *s, int kind, const void *mat, uint3
                               int ancestral_type, uint64_t *job);
 int casphy_server_submit_transfer(casphy_server *s, const uint32_t *Y, uint32_t ny, const uint32_t *X,
                                   uint32_t nx, uint32_t ncol, uint32_t ntip, uint64_t *job);
+/* A packed matrix made resident on the server's device (kind CASPHY_KIND_BIT:
+ * n x width u32 cell-major bits; CASPHY_KIND_INDEL: n x width int32 states),
+ * uploaded before the call returns and counted against the HBM budget until
+ * dropped; *matrix names it for casphy_server_submit_join_rows. */
+int casphy_server_put_matrix(casphy_server *s, int kind, const void *mat, uint32_t n, uint32_t width,
+                             uint64_t *matrix);
+/* Drops a resident matrix; jobs already submitted on it still run. */
+int casphy_server_drop_matrix(casphy_server *s, uint64_t matrix);
+/* A join over rows rows[0..nrows) of a resident matrix (row i of the job is
+ * row rows[i]; the output equals casphy_join_bit / _indel on those rows, the
+ * subsets R's h.partial.join joins, R/CasPhy_functions.R:2124-2131).  The
+ * rows are gathered on the device. */
+int casphy_server_submit_join_rows(casphy_server *s, uint64_t matrix, const uint32_t *rows, uint32_t nrows,
+                                   int ancestral_type, uint64_t *job);
+/* Transfer distances from trees, the arguments of casphy_transfer_trees; the
+ * output is ymin[nrep * ny], ysize[ny], then xmin and xsize of the
+ * replicates' clade rows back to back (ny = y_nnode - 1). */
+int casphy_server_submit_transfer_trees(casphy_server *s, const int32_t *y_anc, const int32_t *y_des,
+                                        uint32_t y_nedge, uint32_t y_nnode, const int32_t *x_anc,
+                                        const int32_t *x_des, const uint32_t *x_edge_offsets,
+                                        const uint32_t *x_nnode, uint32_t nrep, uint32_t ntip, uint64_t *job);
 /* Blocks until `job` is done and copies its output: joins (2n-2)*2 raw edge
  * values, transfers ymin[ny] ysize[ny] xmin[nx] xsize[nx].  A job is waited
  * once. */

@@ -65,3 +65,63 @@ def test_server_errors_are_per_job(hip):
     e, _ = srv.wait(j)
     assert e.shape == (8, 2)
     srv.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_serving_mix_mid_size_equal_standalone(hip):
+    """The serving mix at mid size: 200 jobs of 1k..30k cells -- binary and
+    indel tree builds over rows of resident matrices (the config-2 and config-1
+    site models) and transfer distances from trees at the job's size --
+    submitted from four threads into three workers under a 512 MiB budget,
+    with the resident matrices dropped before the results are read; every
+    job's output equals the same computation alone."""
+    bmat = synth.binary_matrix(seed=51, n=30000, nsite=4096)
+    imat = synth.indel_matrix(seed=52, depth=15, nsite=200)
+    jobs = synth.serving_jobs(53, 200, 1000, 30000, bmat.shape[0], imat.shape[0])
+    srv = hip.Server(hbm_budget_bytes=512 << 20, workers=3)
+    mids = {"bit": srv.put_matrix(bmat, "bit"), "indel": srv.put_matrix(imat, "indel")}
+    ids = [None] * len(jobs)
+
+    def submit(idx):
+        for k in idx:
+            kind, data = jobs[k]
+            ids[k] = srv.submit_transfer_trees(*data) if kind == "trees" else srv.submit_join_rows(mids[kind], data)
+
+    th = [threading.Thread(target=submit, args=(range(t, len(jobs), 4),)) for t in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for mid in mids.values():
+        srv.drop_matrix(mid)  # submitted jobs keep their matrix
+    for k, (kind, data) in enumerate(jobs):
+        got, lat = srv.wait(ids[k])
+        if kind == "trees":
+            want = hip.transfer_trees(*data)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), k
+            for g, w in zip(got[2] + got[3], want[2] + want[3]):
+                assert np.array_equal(g, w), k
+        else:
+            want, _, _ = (hip.join_bit if kind == "bit" else hip.join_indel)((bmat if kind == "bit" else imat)[data])
+            assert np.array_equal(got, want), (k, kind, data.size)
+    st = srv.stats()
+    assert st["completed"] == len(jobs) and st["p99_ms"] >= st["p50_ms"] > 0
+    with pytest.raises(hip.CasphyError):
+        srv.submit_join_rows(mids["bit"], np.arange(4, dtype=np.uint32))  # dropped
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_server_resident_rows_errors(hip):
+    srv = hip.Server(workers=2)
+    m = srv.put_matrix(np.ones((10, 3), np.uint32), "bit")
+    with pytest.raises(hip.CasphyError, match="out of range"):
+        srv.submit_join_rows(m, np.array([0, 10], np.uint32))
+    with pytest.raises(hip.CasphyError):
+        srv.submit_join_rows(m, np.array([3], np.uint32))  # fewer than two cells
+    with pytest.raises(hip.CasphyError, match="unknown matrix"):
+        srv.submit_join_rows(m + 1000, np.array([0, 1], np.uint32))
+    e, _ = srv.wait(srv.submit_join_rows(m, np.array([9, 2, 5], np.uint32)))
+    assert e.shape == (4, 2)
+    srv.close()
