@@ -784,43 +784,31 @@ def scan_form(cp, synth, n: int = 100000, W: int = 2560, R: int = 4) -> dict:
             "kernel": "k_scan_bit", "bit_exact_vs_tiles": True}
 
 
-def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 8000) -> dict:
-    """Reduced form of the serving configuration: a Poisson stream of tree-build
-    (50/50 indel 200 sites / binary 4,096 sites) and TBE jobs, cells log-uniform
-    in [1k, max_cells], through the request loop in two forms: queue-only (one
-    worker, one job per launch) and continuous batching (4 workers whose
-    launches overlap on the device, small joins batched).  Each form runs
-    once with every job submitted at once (capacity) and once with Poisson
-    arrivals at 80% of the queue-only capacity (latency).  Every result is
-    compared with the first queue-only run."""
+def serving_mix(cp, synth, njobs: int = 60, seed: int = 5, max_cells: int = 30000) -> dict:
+    """The serving configuration: tree builds (binary 4,096 sites and indel
+    200 sites, alternating) over rows of resident matrices and transfer
+    distances from trees (every fourth job: a Yule tree of the job's size
+    against 3 replicates 5% SPR moves away), sizes log-uniform in
+    [1k, max_cells], through the request loop in two forms:
+      * plain: one worker, one job per launch, every job's input copied from
+        the host (what one process per job does);
+      * served: 4 workers whose launches overlap, small joins batched,
+        matrices resident (rows gathered on the device), no head-of-line
+        blocking.
+    Each form runs once with every job submitted at once (capacity); then
+    Poisson arrivals at 80% of each form's capacity give its p50 / p99
+    latency.  Every result is compared with the plain capacity run."""
     import threading
-    rng = np.random.default_rng(seed)
-    # pools the jobs draw their cells from
-    bin_pool = synth.binary_matrix(seed=seed, n=16384, nsite=4096, p=0.005, missing=0.05)
-    ind_pool = synth.indel_matrix(seed=seed, depth=14, nsite=200)
-    trees = {}
-    jobs = []
-    for k in range(njobs):
-        n = int(np.exp(rng.uniform(np.log(1000), np.log(max_cells))))
-        u = rng.random()
-        if u < 0.4:
-            rows = np.sort(rng.choice(bin_pool.shape[0], n, replace=False))
-            jobs.append(("bit", bin_pool[rows]))
-        elif u < 0.8:
-            rows = np.sort(rng.choice(ind_pool.shape[0], n, replace=False))
-            jobs.append(("indel", ind_pool[rows]))
-        else:
-            ntip = int(2 ** int(np.clip(np.round(np.log2(n)), 10, 14)))
-            if ntip not in trees:
-                par, lv = synth.yule_parent(ntip, rng)
-                trees[ntip] = (synth.clade_bitsets(par, lv, ntip), par, lv)
-            Y, par, lv = trees[ntip]
-            X = synth.clade_bitsets(synth.spr(par, lv, int(np.ceil(0.05 * ntip)), rng), lv, ntip)
-            jobs.append(("tbe", (Y, X, ntip)))
+    bmat = synth.binary_matrix(seed=seed, n=max_cells, nsite=4096, p=0.005, missing=0.05)
+    depth = int(np.ceil(np.log2(max_cells)))
+    imat = synth.indel_matrix(seed=seed, depth=depth, nsite=200)
+    jobs = synth.serving_jobs(seed, njobs, 1000, max_cells, bmat.shape[0], imat.shape[0])
+    print(f"# serving inputs ready ({njobs} jobs)", file=sys.stderr, flush=True)
 
-    def run(workers, batch, rate):
-        os.environ["CASPHY_SERVE_BATCH"] = "1" if batch else "0"
-        srv = cp.Server(hbm_budget_bytes=32 << 30, workers=workers)
+    def run(served, rate):
+        os.environ["CASPHY_SERVE_BATCH"] = "1" if served else "0"
+        srv = cp.Server(hbm_budget_bytes=64 << 30, workers=4 if served else 1)
+        mids = {"bit": srv.put_matrix(bmat, "bit"), "indel": srv.put_matrix(imat, "indel")} if served else None
         gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, len(jobs)) if rate else np.zeros(len(jobs))
         ids = [None] * len(jobs)
 
@@ -832,7 +820,12 @@ def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 800
                 d = t0 + due - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                ids[k] = srv.submit_transfer(*data) if kind == "tbe" else srv.submit_join(data, kind)
+                if kind == "trees":
+                    ids[k] = srv.submit_transfer_trees(*data)
+                elif served:
+                    ids[k] = srv.submit_join_rows(mids[kind], data)
+                else:
+                    ids[k] = srv.submit_join((bmat if kind == "bit" else imat)[data], kind)
         t0 = time.perf_counter()
         th = threading.Thread(target=feeder)
         th.start()
@@ -842,35 +835,39 @@ def serving_mix(cp, synth, njobs: int = 120, seed: int = 5, max_cells: int = 800
         st = srv.stats()
         srv.close()
         os.environ.pop("CASPHY_SERVE_BATCH", None)
-        print(f"# serving run: workers={workers} batching={batch} rate={rate:.1f}/s {el:.1f} s", file=sys.stderr,
-              flush=True)
+        print(f"# serving run: served={served} rate={rate:.2f}/s {el:.1f} s", file=sys.stderr, flush=True)
         return outs, el, st
 
     def same(xs, ys):
         for a, b in zip(xs, ys):
             if isinstance(a, tuple):
-                assert all(np.array_equal(x, y) for x, y in zip(a, b))
+                for x, y in zip(a, b):
+                    if isinstance(x, list):
+                        assert all(np.array_equal(u, v) for u, v in zip(x, y))
+                    else:
+                        assert np.array_equal(x, y)
             else:
                 assert np.array_equal(a, b)
 
-    # capacity: every job submitted at once
-    outs_q, el_qb, st_qb = run(1, False, 0)
-    outs_b, el_bb, st_bb = run(4, True, 0)
-    same(outs_q, outs_b)
-    # Poisson arrivals at 80% of the queue-only capacity
-    rate = 0.8 * len(jobs) / el_qb
-    outs_q2, _, st_q = run(1, False, rate)
-    outs_b2, _, st_b = run(4, True, rate)
-    same(outs_q, outs_q2)
-    same(outs_q, outs_b2)
+    outs_p, el_p, st_pb = run(False, 0)
+    outs_s, el_s, st_sb = run(True, 0)
+    same(outs_p, outs_s)
+    rate_p, rate_s = 0.8 * len(jobs) / el_p, 0.8 * len(jobs) / el_s
+    outs_p2, _, st_p = run(False, rate_p)
+    outs_s2, _, st_s = run(True, rate_s)
+    same(outs_p, outs_p2)
+    same(outs_p, outs_s2)
+    sizes = np.array([d[4] if k == "trees" else d.size for k, d in jobs])
 
-    def form(burst, el, poisson):
-        return {"saturated_jobs_per_s": len(jobs) / el, "saturated_launches": burst["launches"],
-                "poisson_jobs_per_s": poisson["jobs_per_s"], "p50_ms": poisson["p50_ms"], "p99_ms": poisson["p99_ms"],
-                "launches": poisson["launches"]}
-    return {"jobs": len(jobs), "arrival_rate_per_s": rate, "cells": f"log-uniform 1k..{max_cells // 1000}k",
-            "queue_only": form(st_qb, el_qb, st_q), "continuous_batching": form(st_bb, el_bb, st_b),
-            "saturated_speedup": el_qb / el_bb, "bit_exact_between_forms": True}
+    def form(el, burst, poisson, rate):
+        return {"capacity_jobs_per_s": len(jobs) / el, "capacity_launches": burst["launches"],
+                "poisson_rate_per_s": rate, "poisson_jobs_per_s": poisson["jobs_per_s"],
+                "p50_ms": poisson["p50_ms"], "p99_ms": poisson["p99_ms"], "launches": poisson["launches"]}
+    return {"jobs": len(jobs), "cells": f"log-uniform 1k..{max_cells // 1000}k",
+            "mean_cells": float(sizes.mean()), "largest": int(sizes.max()),
+            "tree_builds": sum(k != "trees" for k, _ in jobs), "tbe_jobs": sum(k == "trees" for k, _ in jobs),
+            "plain": form(el_p, st_pb, st_p, rate_p), "served": form(el_s, st_sb, st_s, rate_s),
+            "capacity_speedup": el_p / el_s, "bit_exact_between_forms": True}
 
 
 def main():
@@ -881,8 +878,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
+    ap.add_argument("--serving-jobs", type=int, default=60, help="jobs of the serving mix")
+    ap.add_argument("--serving-max", type=int, default=30000, help="largest job of the serving mix (cells)")
     ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe", "base", "indel", "scan",
-                                       "dnavalu", "ceilings"],
+                                       "dnavalu", "ceilings", "serving"],
                     help="run only one of the other configurations (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -924,6 +923,9 @@ def main():
         return
     if args.only == "ceilings":
         print(json.dumps(ceilings()))
+        return
+    if args.only == "serving":
+        print(json.dumps(serving_mix(cp, synth, njobs=args.serving_jobs, max_cells=args.serving_max)))
         return
     if args.only == "dnavalu":
         # the VALU 2-bit + mask tiles of the DNA configuration: one tryRadius
