@@ -51,6 +51,7 @@ struct Ctl {
     u32 err;
     u32 steps;
     u32 arrive;  // k_step workgroups that finished their rescoring
+    u32 nlive;   // live slots: slots 0 .. nlive-1 hold the live nodes
     unsigned long long key;
 };
 
@@ -131,7 +132,7 @@ __device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long lo
             best = k > best ? k : best;
         }
     } else {
-        for (u32 s = tid; s < n; s += NT) {
+        for (u32 s = tid; s < ctl->nlive; s += NT) {
             const int32_t v = s == fslot ? (int32_t)fin[1] : a.lmax[s];
             const u32 id = a.slot2node[s];
             const unsigned long long k = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - id);
@@ -187,7 +188,12 @@ __device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long lo
             a.lmax[c1] = -1; a.lmax[c2] = -1;
             a.size[nc] = sa + sb;
             ctl->nodeCount = nc + 1;
-            ctl->c1 = c1;
+            // slot c2 is freed: the last live slot moves into it (after the
+            // merge below), so the live slots stay 0 .. nlive-1
+            const u32 last = ctl->nlive - 1;
+            sh[5] = last;
+            ctl->nlive = last;
+            ctl->c1 = c1 == last ? c2 : c1;
             ctl->newnode = nc;
             ctl->mode = MODE_RESCORE;
             if (nc + 1 == nnode) {
@@ -199,7 +205,7 @@ __device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long lo
     }
     __syncthreads();
     if (sh[0] != 2) return;
-    const u32 c1 = sh[1], c2 = sh[2], choice = sh[3], nc = sh[4];
+    const u32 c1 = sh[1], c2 = sh[2], choice = sh[3], nc = sh[4], last = sh[5];
     const u32 ldw = a.ldw;
     u32 *ra = a.rows + (size_t)c1 * 2 * ldw;
     const u32 *rb = a.rows + (size_t)c2 * 2 * ldw;
@@ -213,6 +219,20 @@ __device__ __forceinline__ void select_phase(const JoinArgs &a, unsigned long lo
         ra[w] = r0; ra[ldw + w] = r1;
         out[w] = r0; out[ldw + w] = r1;
     }
+    if (c2 == last) return;
+    __syncthreads();  // the merged row is complete (it may be the one that moves)
+    const u32 *src = a.rows + (size_t)last * 2 * ldw;
+    u32 *dst = a.rows + (size_t)c2 * 2 * ldw;
+    for (u32 w = tid; w < 2 * ldw; w += NT) dst[w] = src[w];
+    if (tid == 0) {
+        const u32 node = a.slot2node[last];
+        a.slot2node[c2] = node;
+        a.node2slot[node] = c2;
+        a.lmax[c2] = a.lmax[last];
+        a.lidx[c2] = a.lidx[last];
+        a.slot2node[last] = 2 * a.n - 1;
+        a.lmax[last] = -1;
+    }
 }
 
 // PARTS = false: full scan over the slots (first selection after the initial
@@ -221,59 +241,70 @@ template <bool PARTS>
 __global__ __launch_bounds__(1024) void k_select(JoinArgs a)
 {
     __shared__ unsigned long long red[16];
-    __shared__ u32 sh[5];
+    __shared__ u32 sh[6];
     __shared__ u32 fin[3];  // recompute result: slot, value, partner (LDS copy)
     select_phase<1024, PARTS>(a, red, sh, fin, a.nparts);
 }
 
-// 16 lanes per target slot, 16 slots per 256-thread workgroup.  Returns (to
-// every thread) the workgroup's argmax key (lmax + 1, smallest node id) over
-// its live slots after the update, leaving out a slot under recompute.
+// 16 lanes per target slot, 16 slots per 256-thread workgroup and group of
+// slots; the workgroups loop over the groups of live slots (0 .. nlive-1).
+// Returns (to every thread) the workgroup's argmax key (lmax + 1, smallest
+// node id) over its live slots after the update, leaving out a slot under
+// recompute.
 __device__ __forceinline__ unsigned long long rescore_phase(const JoinArgs &a, unsigned long long *red)
 {
     const Ctl *ctl = a.ctl;
     const u32 mode = ctl->mode;  // uniform over the grid
-    const u32 c1 = ctl->c1, newnode = ctl->newnode, n = a.n, nnode = 2 * n - 1, ldw = a.ldw;
+    const u32 c1 = ctl->c1, newnode = ctl->newnode, nlive = ctl->nlive, nnode = 2 * a.n - 1, ldw = a.ldw;
     const u32 tid = threadIdx.x, l16 = tid & 15;
-    const u32 k = blockIdx.x * 16 + (tid >> 4);
-    const u32 node = k < n ? a.slot2node[k] : nnode;
-    const bool alive = node < nnode;
     const bool active = mode == MODE_RESCORE || mode == MODE_RECOMPUTE;
-    const bool live = active && alive && k != c1;
-    u32 S = 0;
-    if (live) {
-        const u32 *q = a.rows + (size_t)c1 * 2 * ldw;
-        const u32 *r = a.rows + (size_t)k * 2 * ldw;
-        const u32 np = a.nplanes ? a.nplanes : 1;
-        for (u32 p = 0; p < np; ++p) {
-            if (p) S <<= a.pshift[p];
-            u32 part = 0;
-            for (u32 w = 4 * l16; w < ldw; w += 64) {
-                uint4 x = *reinterpret_cast<const uint4 *>(q + w);
-                const uint4 y = *reinterpret_cast<const uint4 *>(r + w);
-                if (a.nplanes) {
-                    const uint4 m = *reinterpret_cast<const uint4 *>(a.pmask + (size_t)p * ldw + w);
-                    x.x &= m.x; x.y &= m.y; x.z &= m.z; x.w &= m.w;
-                }
-                part += __builtin_popcount(x.x & y.x) + __builtin_popcount(x.y & y.y) +
-                        __builtin_popcount(x.z & y.z) + __builtin_popcount(x.w & y.w);
-            }
-            for (int o = 1; o < 16; o <<= 1) part += __shfl_xor(part, o);
-            S += part;
-        }
-        if (a.nplanes) S <<= a.ptail;
-    }
+    const u32 *q = a.rows + (size_t)c1 * 2 * ldw;
+    const u32 np = a.nplanes ? a.nplanes : 1;
     unsigned long long pk = 0, rk = 0;
-    if (l16 == 0 && alive && !(mode == MODE_RECOMPUTE && k == c1)) {
-        int32_t v = a.lmax[k];
-        if (mode == MODE_RESCORE && live && (int32_t)S > v) {
-            v = (int32_t)S;
-            a.lmax[k] = v;
-            a.lidx[k] = newnode;
+    for (u32 g = blockIdx.x; g * 16 < nlive; g += gridDim.x) {
+        const u32 k = g * 16 + (tid >> 4);
+        // the live slots are 0 .. nlive-1, so the row loads need not wait for
+        // the node id or the current maximum, which load alongside them
+        const bool alive = k < nlive;
+        const bool live = active && alive && k != c1;
+        const u32 node = alive ? a.slot2node[k] : nnode;
+        const bool keyed = l16 == 0 && alive && !(mode == MODE_RECOMPUTE && k == c1);
+        int32_t v = keyed ? a.lmax[k] : -1;
+        u32 S = 0;
+        if (live) {
+            const u32 *r = a.rows + (size_t)k * 2 * ldw;
+            for (u32 p = 0; p < np; ++p) {
+                if (p) S <<= a.pshift[p];
+                u32 part = 0;
+                for (u32 w = 4 * l16; w < ldw; w += 64) {
+                    uint4 x = *reinterpret_cast<const uint4 *>(q + w);
+                    const uint4 y = *reinterpret_cast<const uint4 *>(r + w);
+                    if (a.nplanes) {
+                        const uint4 m = *reinterpret_cast<const uint4 *>(a.pmask + (size_t)p * ldw + w);
+                        x.x &= m.x; x.y &= m.y; x.z &= m.z; x.w &= m.w;
+                    }
+                    part += __builtin_popcount(x.x & y.x) + __builtin_popcount(x.y & y.y) +
+                            __builtin_popcount(x.z & y.z) + __builtin_popcount(x.w & y.w);
+                }
+                for (int o = 1; o < 16; o <<= 1) part += __shfl_xor(part, o);
+                S += part;
+            }
+            if (a.nplanes) S <<= a.ptail;
         }
-        pk = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - node);
+        if (keyed) {
+            if (mode == MODE_RESCORE && live && (int32_t)S > v) {
+                v = (int32_t)S;
+                a.lmax[k] = v;
+                a.lidx[k] = newnode;
+            }
+            const unsigned long long key = ((unsigned long long)(u32)(v + 1) << 32) | (0xFFFFFFFFu - node);
+            pk = key > pk ? key : pk;
+        }
+        if (mode == MODE_RECOMPUTE && live) {
+            const unsigned long long key = ((unsigned long long)S << 32) | (0xFFFFFFFFu - node);
+            rk = key > rk ? key : rk;
+        }
     }
-    if (mode == MODE_RECOMPUTE && live) rk = ((unsigned long long)S << 32) | (0xFFFFFFFFu - node);
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long x = __shfl_xor(pk, o), y = __shfl_xor(rk, o);
         pk = x > pk ? x : pk;
@@ -304,7 +335,7 @@ __global__ __launch_bounds__(256) void k_rescore(JoinArgs a)
 __global__ __launch_bounds__(256) void k_step(JoinArgs a)
 {
     __shared__ unsigned long long red[16];
-    __shared__ u32 sh[5];
+    __shared__ u32 sh[6];
     __shared__ u32 fin[3];
     __shared__ u32 last;
     if (a.ctl->done) return;
@@ -340,7 +371,7 @@ __global__ void k_join_init(JoinArgs a, const u64 *keys, const u32 *tip_size)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Ctl *c = a.ctl;
         c->nodeCount = n; c->mode = MODE_NONE; c->c1 = 0; c->newnode = 0;
-        c->done = n < 2 ? 1u : 0u; c->err = 0; c->steps = 0; c->arrive = 0; c->key = 0;
+        c->done = n < 2 ? 1u : 0u; c->err = 0; c->steps = 0; c->arrive = 0; c->key = 0; c->nlive = n;
     }
 }
 
@@ -738,7 +769,12 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
 
     // Join loop: CH steps per captured graph, poll the done flag per chunk.
     const u32 CH = 64;
-    const u32 rescore_grid = ceil_div(n, 16);
+    // one group of 16 slots per workgroup up to 2,048 workgroups (8 per CU),
+    // then the workgroups loop over the live slots
+    // (CASPHY_JOIN_RESCORE_WG lowers the cap, so small joins take the loop too)
+    const char *cap_env = std::getenv("CASPHY_JOIN_RESCORE_WG");
+    const u32 cap = cap_env ? std::max(1, std::atoi(cap_env)) : 2048;
+    const u32 rescore_grid = std::min<u32>(ceil_div(n, 16), cap);
     a.nparts = rescore_grid;
     const char *env = std::getenv("CASPHY_JOIN_GRAPH");
     const bool use_graph = !(env && env[0] == '0');

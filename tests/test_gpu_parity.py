@@ -330,10 +330,16 @@ def test_join_bit(hip, orc, bitpath, n, W, p, anc):
     assert np.array_equal(ns, ns2) and np.array_equal(nb, nb2)
 
 
-@pytest.fixture(params=["fused", "two-kernel"])
+@pytest.fixture(params=["fused", "two-kernel", "fused-looped", "two-kernel-looped"])
 def joinstep(request, monkeypatch):
-    """Single-tree join with one launch per step or select + rescore launches."""
-    monkeypatch.setenv("CASPHY_JOIN_FUSED", "1" if request.param == "fused" else "0")
+    """Single-tree join with one launch per step or select + rescore launches;
+    "looped": 5 rescoring workgroups, so each one loops over several groups
+    of live slots (the form joins of more than 32,768 cells take)."""
+    monkeypatch.setenv("CASPHY_JOIN_FUSED", "1" if request.param.startswith("fused") else "0")
+    if request.param.endswith("looped"):
+        monkeypatch.setenv("CASPHY_JOIN_RESCORE_WG", "5")
+    else:
+        monkeypatch.delenv("CASPHY_JOIN_RESCORE_WG", raising=False)
     return request.param
 
 
