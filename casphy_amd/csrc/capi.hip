@@ -74,16 +74,27 @@ constexpr unsigned STAGE_THREADS = 4;
 std::mutex g_stage_mu;
 void *g_stage[2] = {nullptr, nullptr};
 
-void parallel_copy(void *dst, const void *src, size_t bytes)
+// Never throws: a part whose thread cannot be started is copied by the
+// calling thread, and every started thread is joined, so a staged copy cannot
+// unwind (and release the staging buffers) while its DMA is in flight.
+void parallel_copy(void *dst, const void *src, size_t bytes) noexcept
 {
     const size_t part = (bytes / STAGE_THREADS + 4095) / 4096 * 4096;
+    auto copy = [=](unsigned k) {
+        const size_t off = std::min(bytes, k * part), len = std::min(bytes - off, part);
+        std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len);
+    };
     std::thread t[STAGE_THREADS - 1];
     for (unsigned k = 1; k < STAGE_THREADS; ++k) {
-        const size_t off = std::min(bytes, k * part), len = std::min(bytes - off, part);
-        t[k - 1] = std::thread([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len); });
+        try {
+            t[k - 1] = std::thread(copy, k);
+        } catch (...) {
+            copy(k);
+        }
     }
-    std::memcpy(dst, src, std::min(bytes, part));
-    for (auto &x : t) x.join();
+    copy(0);
+    for (auto &x : t)
+        if (x.joinable()) x.join();
 }
 
 void staged_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
