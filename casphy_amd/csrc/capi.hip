@@ -443,8 +443,16 @@ bool use_scan(int metric, const Planes &planes, u32 R, u32 ldw, u32 nradii)
            !(e && e[0] == '0');
 }
 
+// r_max - r_min + 1 of ascending radii (the count table's span).
+u64 counts_span(const int32_t *radii, u32 nradii)
+{
+    return (u64)((int64_t)radii[nradii - 1] - radii[0] + 1);
+}
+
 // tryRadius: query x cell scores (matrix-core tiles for unit-weight bits,
-// VALU tiles otherwise) followed by one counting workgroup per query.
+// VALU tiles otherwise) followed by one counting workgroup per query; on the
+// fp4 tiles the counts come straight from the tiles when the radii fit the
+// count table.
 void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes, const int32_t *radii,
                u32 nradii, const u32 *q, u32 R, u32 *counts, hipStream_t s, const u32 *stage, hipEvent_t ev0,
                hipEvent_t ev1, const u32 *estage)
@@ -453,7 +461,7 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
         CASPHY_REQUIRE(q[k] < n, CASPHY_EARG, "query row " + std::to_string(q[k]) + " out of range");
     for (u32 k = 1; k < nradii; ++k)
         CASPHY_REQUIRE(radii[k - 1] <= radii[k], CASPHY_EARG, "radii must be ascending");
-    DevBuf dq((size_t)R * 4), dc((size_t)R * nradii * 4), dr((size_t)nradii * 4), dS((size_t)R * n * 4), pbuf;
+    DevBuf dq((size_t)R * 4), dc((size_t)R * nradii * 4), dr((size_t)nradii * 4), dS, pbuf;
     h2d(dq.p, q, (size_t)R * 4, s);
     h2d(dr.p, radii, (size_t)nradii * 4, s);
     const u32 ntA = ceil_div(R, TILE), ntB = ceil_div(n, TILE);
@@ -480,6 +488,34 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
         m.PA = sq.as<u32>(); m.PB = stage; m.padA = stage_rows(R); m.padB = stage_rows(n); m.stages = ldw / 4;
         m.EB = estage;
         m.nA = R; m.nB = n; m.triangle = 0;
+        // counts straight from the tiles (no score matrix) when the radii fit
+        // the count table
+        MfmaArgs f = m;
+        f.EA = f.EB = sq.as<u32>();  // stand-ins: only their presence is checked
+        const bool fused = fused_counts_ok(f, nradii) && counts_span(radii, nradii) <= COUNT_SPAN_MAX;
+        if (fused) {
+            const u32 stages = ldw / 4, ntB = ceil_div(n, TILE);
+            DevBuf ea(fp4_bytes(m.padA, stages)), eb, tab(COUNT_SPAN_MAX + 64 + 16),
+                cpart((size_t)ntB * nradii * m.padA);
+            launch_expand_fp4(sq.as<u32>(), m.padA, stages, ea.as<u32>(), s);
+            m.EA = ea.as<u32>();
+            if (!m.EB) {
+                eb.alloc(fp4_bytes(m.padB, stages));
+                launch_expand_fp4(stage, m.padB, stages, eb.as<u32>(), s);
+                m.EB = eb.as<u32>();
+            }
+            unsigned char *t = tab.as<unsigned char>();
+            m.ctab = t; m.cneed = t + COUNT_SPAN_MAX; m.cpar = reinterpret_cast<const int32_t *>(t + COUNT_SPAN_MAX + 64);
+            m.nrad = nradii; m.cpart = cpart.as<unsigned char>();
+            launch_count_table(dr.as<int32_t>(), nradii, t, t + COUNT_SPAN_MAX,
+                               reinterpret_cast<int32_t *>(t + COUNT_SPAN_MAX + 64), s);
+            if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
+            launch_mfma_counts(m, dc.as<u32>(), R, s);
+            if (ev1) CASPHY_HIP(hipEventRecord(ev1, s));
+            d2h(counts, dc.p, (size_t)R * nradii * 4, s);
+            return;
+        }
+        dS.alloc((size_t)R * n * 4);
         m.S = dS.as<int32_t>(); m.ldS = n;
         if (ev0) CASPHY_HIP(hipEventRecord(ev0, s));
         launch_mfma_tiles(E_SCORES, m, s);
@@ -488,6 +524,7 @@ void try_radii(int metric, const u32 *dmat, u32 n, u32 ldw, const Planes &planes
         d2h(counts, dc.p, (size_t)R * nradii * 4, s);
         return;
     }
+    dS.alloc((size_t)R * n * 4);
     TileArgs a;
     a.A = dmat; a.Aidx = dq.as<u32>(); a.nA = R; a.B = dmat; a.nB = n;
     a.ldw = ldw; a.kw = ldw;
@@ -1328,12 +1365,23 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         const size_t cnt_bytes = (size_t)R * nradii * 4;
         h->pinned(res_off + 16 + cnt_bytes);
         h->a_in.reserve(up);
-        h->a_S.reserve((size_t)R * n * 4);
         h->a_counts.reserve(cnt_bytes);
         h->a_misc.reserve(16);
+        // on the fp4 tiles the counts come straight from the tiles when the radii
+        // fit the count table (no score matrix)
+        bool fused = false;
         if (mfma) {
             h->a_sq.reserve((size_t)stage_rows(R) * ldw * 4);
             h->a_eq.reserve(fp4_bytes(stage_rows(R), ldw / 4));
+            MfmaArgs f;
+            f.EA = h->a_eq.as<u32>(); f.EB = h->estage.as<u32>(); f.stages = ldw / 4;
+            fused = fused_counts_ok(f, nradii) && counts_span(radii, nradii) <= COUNT_SPAN_MAX;
+        }
+        if (fused) {
+            h->a_tab.reserve(COUNT_SPAN_MAX + 64 + 16);
+            h->a_cpart.reserve((size_t)ceil_div(n, TILE) * nradii * stage_rows(R));
+        } else {
+            h->a_S.reserve((size_t)R * n * 4);
         }
         RadiusOut &o = h->out;
         const u32 nt = ceil_div(n, TILE), mstride = nt * 4;
@@ -1364,6 +1412,17 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
                 m.PA = h->a_sq.as<u32>(); m.PB = h->stage.as<u32>(); m.padA = stage_rows(R); m.padB = stage_rows(n);
                 m.stages = ldw / 4; m.nA = R; m.nB = n; m.triangle = 0;
                 m.EA = h->a_eq.as<u32>(); m.EB = h->estage.as<u32>();
+                if (fused) {
+                    unsigned char *t = h->a_tab.as<unsigned char>();
+                    m.ctab = t; m.cneed = t + COUNT_SPAN_MAX;
+                    m.cpar = reinterpret_cast<const int32_t *>(t + COUNT_SPAN_MAX + 64);
+                    m.nrad = nradii; m.cpart = h->a_cpart.as<unsigned char>();
+                    launch_count_table(dr, nradii, t, t + COUNT_SPAN_MAX,
+                                       reinterpret_cast<int32_t *>(t + COUNT_SPAN_MAX + 64), s);
+                    launch_mfma_counts(m, h->a_counts.as<u32>(), R, s);
+                    launch_autosplit_choose(h->a_counts.as<u32>(), R, dr, nradii, smax, any, choice, s);
+                    return;
+                }
                 m.S = dS; m.ldS = n;
                 launch_mfma_tiles(E_SCORES, m, s);
             } else {
@@ -1400,8 +1459,9 @@ int casphy_cells_autosplit(casphy_cells *h, const int32_t *radii, uint32_t nradi
         // the same shapes and buffers captures each into a graph (the first one
         // also runs the autotuner), later calls replay them.
         // CASPHY_AUTOSPLIT_GRAPH=0 keeps plain launches.
-        const AutosplitKey key{R, nradii, smax, any, want, cap, o.idx.p, o.mask.p, h->a_S.p, h->a_eq.p, h->a_in.p,
-                               h->pin_dev};
+        const AutosplitKey key{R,          nradii,     smax,       any,        want,   cap,
+                               fused,      o.idx.p,    o.mask.p,   h->a_S.p,   h->a_eq.p, h->a_in.p,
+                               h->pin_dev, h->a_tab.p, h->a_cpart.p};
         const char *genv = std::getenv("CASPHY_AUTOSPLIT_GRAPH");
         const bool graphs = !(genv && genv[0] == '0');
         h->timed[0] = h->timed[1] = true;

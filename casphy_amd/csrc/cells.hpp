@@ -42,11 +42,14 @@ struct AutosplitKey {
     int any = -1;
     bool want = false;
     u64 cap = 0;
+    bool fused = false;
     const void *idx = nullptr, *mask = nullptr, *S = nullptr, *eq = nullptr, *in = nullptr, *pin = nullptr;
+    const void *tab = nullptr, *cpart = nullptr;
     bool operator==(const AutosplitKey &o) const
     {
         return R == o.R && nradii == o.nradii && smax == o.smax && any == o.any && want == o.want && cap == o.cap &&
-               idx == o.idx && mask == o.mask && S == o.S && eq == o.eq && in == o.in && pin == o.pin;
+               fused == o.fused && idx == o.idx && mask == o.mask && S == o.S && eq == o.eq && in == o.in &&
+               pin == o.pin && tab == o.tab && cpart == o.cpart;
     }
 };
 
@@ -92,6 +95,7 @@ struct casphy_cells {
     // counts and the choice; pinned host staging for one upload and
     // the read-back of (choice, total)
     casphy::DevBuf a_in, a_sq, a_eq, a_S, a_counts, a_misc, a_pbuf;
+    casphy::DevBuf a_tab, a_cpart;  // fused counts: count table, per-tile byte counts
     // captured autoSplit round and the shapes / buffers it was captured with
     // (the round's two small-kernel phases; the mask launch between them is a
     // plain launch bracketed by the handle's events)

@@ -618,6 +618,69 @@ __global__ void k_expand_fp4(const u32 *P, u32 pad, u32 stages, u32 *E)
     }
 }
 
+// tryRadius counts of one 128 x 128 tile without storing its scores: each
+// score's bin idx = #{k : radii[k] <= score} comes from the count table
+// (copied into the B operand's LDS), every row's bins are counted in the A
+// operand's LDS (two 16-bit bins per word; a bin holds at most 128), then one
+// thread per row turns its bins into suffix sums and writes, per radius k, the
+// row's count of columns with idx >= cneed[k] as one byte of cpart.  The
+// operand LDS is free: every wave has left the k-loop.
+__device__ __forceinline__ void count_epilogue(const MfmaArgs &a, const v16f (&acc)[2][2], u32 I, u32 J, u32 wm,
+                                               u32 wn, u32 lane, u32 tid, u32 *H, unsigned char *T)
+{
+    constexpr u32 HW = 33;  // words per row: bins 0 .. 65 as 16-bit halves
+    const int32_t cmin = a.cpar[0];
+    const u32 cspan = min((u32)a.cpar[1], COUNT_SPAN_MAX), nrad = a.nrad;
+    __syncthreads();
+    for (u32 w = tid; w < TILE * HW; w += NTHR) H[w] = 0u;
+    for (u32 x = tid; x < cspan; x += NTHR) T[x] = a.ctab[x];
+    __syncthreads();
+    const u32 half = lane >> 5, col = lane & 31;
+    const u32 topw = nrad >> 1, tops = 16 * (nrad & 1);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const bool jv = J * TILE + wn * 64 + 32 * y + col < a.nB;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const u32 rl0 = wm * 64 + 32 * x + 8 * (v / 4) + (v % 4), rl = rl0 + 4 * half;
+                const int32_t d = (int32_t)acc[x][y][v] - cmin;
+                const u32 idx = d < 0 ? 0u : (u32)d >= cspan ? nrad : (u32)T[d];
+                const bool valid = jv && idx && I * TILE + rl < a.nA;
+                // scores above every radius (the common case) are counted per
+                // half-wave row with one ballot instead of 32 conflicting atomics
+                const unsigned long long top = __ballot(valid && idx == nrad);
+                if (lane == 0 && top) {
+                    const u32 lo = (u32)__builtin_popcountll(top & 0xFFFFFFFFull), hi = (u32)__builtin_popcountll(top >> 32);
+                    if (lo) atomicAdd(&H[rl0 * HW + topw], lo << tops);
+                    if (hi) atomicAdd(&H[(rl0 + 4) * HW + topw], hi << tops);
+                }
+                if (valid && idx != nrad) atomicAdd(&H[rl * HW + (idx >> 1)], 1u << (16 * (idx & 1)));
+            }
+        }
+    __syncthreads();
+    if (tid < TILE) {
+        // the row's bins in registers, the suffix sum from the top bin down; the
+        // radii (cneed non-increasing from the last one) are uniform over the row
+        // threads, so the inner loop does not diverge
+        u32 bins[HW];
+#pragma unroll
+        for (int w = 0; w < (int)HW; ++w) bins[w] = H[tid * HW + w];
+        unsigned char *out = a.cpart + (size_t)J * nrad * a.padA + I * TILE + tid;
+        u32 run = 0;
+        int k = (int)nrad - 1;
+#pragma unroll
+        for (int b = 64; b >= 1; --b) {
+            run += (bins[b >> 1] >> (16 * (b & 1))) & 0xFFFFu;
+            while (k >= 0 && a.cneed[k] == (u32)b) {
+                out[(size_t)k * a.padA] = (unsigned char)run;
+                --k;
+            }
+        }
+    }
+}
+
 // 128 x 128 pair tile, 4 waves of 2 x 2 blocks, LDS stages of 128 sites (two
 // 64-site chunks, rows of 64 + 16 bytes), one stage of register prefetch.
 // PA / PB point at the expanded nibble arrays.
@@ -708,6 +771,10 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
         __syncthreads();
     }
 
+    if constexpr (EPI == E_COUNTS) {
+        count_epilogue(a, acc, I, J, wm, wn, lane, tid, reinterpret_cast<u32 *>(&As[0][0]), &Bs[0][0]);
+        return;
+    }
     const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
     const bool mirror = a.triangle && I != J;
     const u32 er0 = I * TILE - a.orgA + wm * 64, ec0 = J * TILE - a.orgB + wn * 64;  // the group's own rows
@@ -1495,6 +1562,110 @@ void launch_mfma_tiles_untuned(int epi, const MfmaArgs &a, hipStream_t s)
     }
     if (v >= kFp4 && (!fp4_exact || !a.EA || !a.EB)) v = kDefaultVariant;
     launch_one(v, epi, a, s);
+}
+
+namespace {
+
+// ctab[x] = #{k : radii[k] <= radii[0] + x} for x < span = radii[nrad-1] -
+// radii[0] + 1 (at most COUNT_SPAN_MAX), cneed[k] = #{j : radii[j] <=
+// radii[k]}, cpar = (radii[0], span).  Radii ascending.
+__global__ void k_count_table(const int32_t *radii, u32 nrad, unsigned char *ctab, unsigned char *cneed,
+                              int32_t *cpar)
+{
+    const int32_t rmin = radii[0];
+    const u32 span = (u32)min((int64_t)radii[nrad - 1] - rmin + 1, (int64_t)COUNT_SPAN_MAX);
+    for (u32 x = threadIdx.x; x < span; x += blockDim.x) {
+        const int64_t sc = (int64_t)rmin + x;
+        u32 c = 0;
+        for (u32 k = 0; k < nrad; ++k) c += radii[k] <= sc;
+        ctab[x] = (unsigned char)c;
+    }
+    for (u32 k = threadIdx.x; k < nrad; k += blockDim.x) {
+        u32 c = 0;
+        for (u32 j = 0; j < nrad; ++j) c += radii[j] <= radii[k];
+        cneed[k] = (unsigned char)c;
+    }
+    if (threadIdx.x == 0) {
+        cpar[0] = rmin;
+        cpar[1] = (int32_t)span;
+    }
+}
+
+// counts[k * R + q] = sum over the ntB column tiles of cpart[(J * nrad + k) * padA + q];
+// one wave per radius and 4 consecutive rows, its lanes over the column tiles
+// (independent loads), then a wave sum.
+__global__ __launch_bounds__(256) void k_count_sum(const unsigned char *cpart, u32 ntB, u32 nrad, u32 padA, u32 R,
+                                                   u32 *counts)
+{
+    const u32 lane = threadIdx.x & 63;
+    const u32 q4 = padA / 4;
+    const u64 items = (u64)nrad * q4;
+    const size_t stride = (size_t)nrad * padA / 4;
+    for (u64 it = blockIdx.x * 4ull + (threadIdx.x >> 6); it < items; it += gridDim.x * 4ull) {
+        const u32 k = (u32)(it / q4), q = 4 * (u32)(it - (u64)k * q4);
+        if (q >= R) continue;  // wave-uniform
+        const u32 *p = reinterpret_cast<const u32 *>(cpart + (size_t)k * padA + q);
+        u32 c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        for (u32 J = lane; J < ntB; J += 64) {
+            const u32 w = p[(size_t)J * stride];
+            c0 += w & 0xFFu; c1 += (w >> 8) & 0xFFu; c2 += (w >> 16) & 0xFFu; c3 += w >> 24;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            c0 += __shfl_xor(c0, o); c1 += __shfl_xor(c1, o); c2 += __shfl_xor(c2, o); c3 += __shfl_xor(c3, o);
+        }
+        if (lane == 0) {
+            u32 *out = counts + (size_t)k * R + q;
+            out[0] = c0;
+            if (q + 1 < R) out[1] = c1;
+            if (q + 2 < R) out[2] = c2;
+            if (q + 3 < R) out[3] = c3;
+        }
+    }
+}
+
+}  // namespace
+
+bool fused_counts_ok(const MfmaArgs &a, u32 nradii)
+{
+    const char *e = std::getenv("CASPHY_FUSED_COUNTS");
+    if (e && e[0] == '0') return false;
+    const int v = variant_from_env();
+    if (v >= 0 && v != kFp4) return false;  // a forced variant keeps the score matrix
+    return a.EA && a.EB && (u64)a.stages * 128 < (1ull << 24) && nradii >= 1 && nradii <= MAX_ROUND_RADII_FUSED;
+}
+
+void launch_count_table(const int32_t *radii, u32 nrad, unsigned char *ctab, unsigned char *cneed, int32_t *cpar,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_count_table, dim3(1), dim3(256), 0, s, radii, nrad, ctab, cneed, cpar);
+    CASPHY_HIP(hipGetLastError());
+}
+
+void launch_mfma_counts(const MfmaArgs &in, u32 *counts, u32 R, hipStream_t s)
+{
+    MfmaArgs a = in;
+    CASPHY_REQUIRE(a.EA && a.EB && a.ctab && a.cneed && a.cpar && a.cpart && a.nrad >= 1 &&
+                       a.nrad <= MAX_ROUND_RADII_FUSED && a.triangle == 0 && a.padA % 4 == 0,
+                   CASPHY_EINTERNAL, "fused tryRadius counts: bad arguments");
+    a.ntA = ceil_div(a.nA, TILE);
+    a.ntB = ceil_div(a.nB, TILE);
+    if (a.ntA == 0 || a.ntB == 0) return;
+    a.sup = super_edge();
+    if (a.sup) {
+        const u64 nSA = ceil_div(a.ntA, a.sup), nSB = ceil_div(a.ntB, a.sup);
+        a.ntiles = (u32)(nSA * nSB * a.sup * a.sup);
+    } else {
+        a.ntiles = a.ntA * a.ntB;
+    }
+    a.PA = a.EA;
+    a.PB = a.EB;
+    const u32 grid = 8 * ((a.ntiles + 7) / 8);
+    hipLaunchKernelGGL(k_mfma_fp4<E_COUNTS>, dim3(grid), dim3(NTHR), 0, s, a);
+    CASPHY_HIP(hipGetLastError());
+    const u64 items = (u64)a.nrad * (a.padA / 4);
+    hipLaunchKernelGGL(k_count_sum, dim3((u32)std::min<u64>((items + 3) / 4, 16384)), dim3(256), 0, s, a.cpart,
+                       a.ntB, a.nrad, a.padA, R, counts);
+    CASPHY_HIP(hipGetLastError());
 }
 
 void launch_mfma_groups(int epi, const MfmaArgs &in, u32 nslots, hipStream_t s)

@@ -63,7 +63,20 @@ struct MfmaArgs {
     const GroupDesc *gdesc = nullptr;
     const int32_t *gthr = nullptr;
     u32 orgA = 0, orgB = 0;   // set in the kernel: the group's first A / B row
+    // E_COUNTS (tryRadius counts without the score matrix; fp4 128 x 128
+    // tiles): cpart[(J * nrad + k) * padA + i] = #{j of column tile J :
+    // S(i, j) >= radii[k]} as a byte, from the count table of
+    // launch_count_table (ctab / cneed / cpar).
+    const unsigned char *ctab = nullptr, *cneed = nullptr;
+    const int32_t *cpar = nullptr;
+    u32 nrad = 0;
+    unsigned char *cpart = nullptr;
 };
+
+// Largest span r_max - r_min + 1 of the radii the fused counts take (the
+// count table lives in the tile's operand LDS after the k-loop).
+constexpr u32 COUNT_SPAN_MAX = 20480;
+constexpr u32 MAX_ROUND_RADII_FUSED = 64;
 
 // First kw words of cell-major rows of stride ldw -> stage-major P (rows padded
 // to a multiple of STAGE_PAD; kw a multiple of 32).  With rowidx, row r of P is
@@ -89,6 +102,16 @@ void launch_mfma_tiles_untuned(int epi, const MfmaArgs &a, hipStream_t s);
 // the fp4 tiles when a.EA / a.EB are given and the sums stay exact, else the
 // int8 tiles; allocates nothing and times nothing.
 void launch_mfma_groups(int epi, const MfmaArgs &a, u32 nslots, hipStream_t s);
+// tryRadius counts without the score matrix: the fp4 tiles (a.EA / a.EB given,
+// sums exact, nradii <= 64) with the E_COUNTS epilogue into a.cpart
+// (ceil(nB / 128) x nrad x padA bytes), then their sum over the column tiles
+// into counts[k * R + q].  Allocates nothing.
+bool fused_counts_ok(const MfmaArgs &a, u32 nradii);
+// The count table of radii (ascending, nradii <= 64): ctab (COUNT_SPAN_MAX
+// bytes), cneed (64 bytes), cpar (2 int32), all device memory.
+void launch_count_table(const int32_t *radii, u32 nradii, unsigned char *ctab, unsigned char *cneed, int32_t *cpar,
+                        hipStream_t s);
+void launch_mfma_counts(const MfmaArgs &a, u32 *counts, u32 R, hipStream_t s);
 // JSON report of the autotuner's choices (shape class -> variant, timings),
 // followed by the notes other tuners added (one JSON object each).
 std::string mfma_tuning_report();

@@ -18,7 +18,7 @@
 namespace casphy {
 
 enum Metric : int { M_AND = 0, M_XOR = 1, M_DNA2 = 2, M_NIB = 3 };
-enum Epi : int { E_MASK = 0, E_SCORES = 1, E_HIST = 2, E_LOCALMAX = 3, E_TBE = 4, E_PAIRS = 5, E_ARGMAX = 6 };
+enum Epi : int { E_MASK = 0, E_SCORES = 1, E_HIST = 2, E_LOCALMAX = 3, E_TBE = 4, E_PAIRS = 5, E_ARGMAX = 6, E_COUNTS = 7 };
 
 constexpr int TILE = 128;      // rows per tile side
 constexpr int KC_BIT = 32;     // words per LDS stage (AND / XOR / NIB)
