@@ -182,7 +182,9 @@ def ceilings() -> dict:
     32-bit integer VALU issue rate (8 independent xor / and / popcount-add
     chains per lane, 8 waves per SIMD) and the fp4 matrix-core rate
     (v_mfma_scale_f32_32x32x64_f8f6f4 on register operands, 4 accumulators per
-    wave, 4 waves per SIMD); the best of 3 timed launches after one warm-up."""
+    wave, 4 waves per SIMD), with the same operands every instruction and with
+    operands that change before every instruction as a tile kernel's do; the
+    best of 3 timed launches after one warm-up."""
     import ctypes as C
     lib = C.CDLL(os.path.join(ROOT, "casphy_amd", "libcasphy_calib.so"))
     hip = C.CDLL("libamdhip64.so")
@@ -193,7 +195,8 @@ def ceilings() -> dict:
     best = {}
     try:
         for name, fn, blocks, iters in (("valu", lib.casphy_calib_valu, vblocks, 10000),
-                                        ("mfma_fp4", lib.casphy_calib_mfma_fp4, mblocks, 100000)):
+                                        ("mfma_fp4", lib.casphy_calib_mfma_fp4, mblocks, 100000),
+                                        ("mfma_fp4_data", lib.casphy_calib_mfma_fp4_data, mblocks, 100000)):
             rates = []
             for k in range(4):
                 assert fn(buf, C.c_uint32(blocks), C.c_uint32(iters), None, C.byref(ms), C.byref(work)) == 0
@@ -203,8 +206,9 @@ def ceilings() -> dict:
     finally:
         hip.hipFree(buf)
     return {"valu_tlane_ops_per_s": best["valu"], "valu_spec_tlane_ops_per_s": VALU_SPEC_TOPS,
-            "mfma_fp4_tops": best["mfma_fp4"], "mfma_fp4_spec_tops": MFMA_FP4_PEAK_TOPS,
-            "kernels": "k_valu_rate (192 VALU instructions per loop trip), k_mfma_fp4_rate"}
+            "mfma_fp4_tops": best["mfma_fp4"], "mfma_fp4_switching_operands_tops": best["mfma_fp4_data"],
+            "mfma_fp4_spec_tops": MFMA_FP4_PEAK_TOPS,
+            "kernels": "k_valu_rate (192 VALU instructions per loop trip), k_mfma_fp4_rate<false / true>"}
 
 
 def calibration_copy(nbytes: int = 256 << 20) -> None:
@@ -228,7 +232,8 @@ VARIANT_KERNEL = {"128x1": "k_mfma_tiles<0, 128, 1, 2>", "128x2": "k_mfma_tiles<
                   "64x4": "k_mfma_tiles<0, 64, 4, 4>", "64x8": "k_mfma_tiles<0, 64, 8, 4>",
                   "reg": "k_mfma_reg<0>", "wide": "k_mfma_wide<0>", "wide8": "k_mfma_wide8<0>",
                   "fp4": "k_mfma_fp4<0>", "fp4b": "k_mfma_fp4b<0>", "fp4d": "k_mfma_fp4d<0>",
-                  "fp4w": "k_mfma_fp4w<0>"}
+                  "fp4w": "k_mfma_fp4w<0>", "fp4x": "k_mfma_fp4x<0>",
+                  "fp4y": "k_mfma_fp4y<0>"}
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_mfma_tiles_mask_10000x4096_pmc.md")
 TRAFFIC_TABLE_DNA = os.path.join(ROOT, "profiles", "k_mfma_mask_dna_100000x1000_pmc.md")
 
@@ -277,6 +282,8 @@ def mask_roofline(cp, tile_ms: float, pairs: int, ops_per_pair: int, shape: tupl
     roof["frac"] = roof["achieved"] / roof["peak"]
     if not valu and roof["variant"].startswith("fp4"):
         roof["frac_of_measured_ceiling"] = roof["achieved"] / ceilings()["mfma_fp4_tops"]
+        roof["frac_of_switching_operand_ceiling"] = (roof["achieved"] /
+                                                     ceilings()["mfma_fp4_switching_operands_tops"])
     roof["traffic"] = measured_traffic(roof["kernel"], traffic_table)
     roof["kernel_ms"] = tile_ms
     roof["algorithmic"] = (f"{valu_ops_per_pair if valu else ops_per_pair} ops per cell pair x {pairs} pairs "

@@ -10,6 +10,12 @@
 //   casphy_calib_mfma_fp4: v_mfma_scale_f32_32x32x64_f8f6f4 with fp4
 //     operands held in registers, 4 independent accumulators per wave,
 //     4 waves per SIMD (131,072 ops per instruction).
+//   casphy_calib_mfma_fp4_data: the same with operands that change before
+//     every instruction, as a tile kernel's do: each MFMA's A and B words are
+//     XORed with a fresh mask of the 1.0 bits (0b0010 per nibble; about half
+//     of them flip), 2 VALU instructions per MFMA.  Switching operands draw
+//     more power than repeated ones, so this is the rate a matrix-core kernel
+//     on real data can reach if the part lowers its clock under that load.
 // Each reports the kernel's duration from HIP events on the given stream and
 // the work it did; the result words are stored so the loops are not removed.
 #include <hip/hip_runtime.h>
@@ -48,6 +54,7 @@ __global__ __launch_bounds__(256) void k_valu_rate(uint32_t *out, uint32_t iters
     out[blockIdx.x * blockDim.x + threadIdx.x] = r;
 }
 
+template <bool VARY>
 __global__ __launch_bounds__(256, 4) void k_mfma_fp4_rate(float *out, uint32_t iters, uint32_t seed)
 {
     const int lane = threadIdx.x & 63;
@@ -56,10 +63,17 @@ __global__ __launch_bounds__(256, 4) void k_mfma_fp4_rate(float *out, uint32_t i
     v16f acc[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[k] = v16f{};
+    uint32_t x = seed * 2654435761u + (uint32_t)lane * 40503u;
     for (uint32_t i = 0; i < iters; ++i) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (VARY) {
+                x = x * 1664525u + 1013904223u;
+                a[k] ^= (int)(x & 0x22222222u);
+                b[k] ^= (int)((x >> 1) & 0x22222222u);
+            }
             acc[k] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, acc[k], 4, 4, 0, 127, 0, 127);
+        }
     }
     float r = 0.f;
 #pragma unroll
@@ -110,7 +124,18 @@ int casphy_calib_mfma_fp4(float *out, uint32_t blocks, uint32_t iters, void *str
     if (!out || !ms || !ops || blocks == 0 || iters == 0) return 1;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int rc = timed_launch(
-        [&] { hipLaunchKernelGGL(k_mfma_fp4_rate, dim3(blocks), dim3(256), 0, s, out, iters, 5u); }, s, ms);
+        [&] { hipLaunchKernelGGL(k_mfma_fp4_rate<false>, dim3(blocks), dim3(256), 0, s, out, iters, 5u); }, s, ms);
+    *ops = (uint64_t)blocks * 4 /* waves */ * iters * 4 * (2ull * 32 * 32 * 64);
+    return rc;
+}
+
+// As casphy_calib_mfma_fp4, with operands that change before every MFMA.
+int casphy_calib_mfma_fp4_data(float *out, uint32_t blocks, uint32_t iters, void *stream, float *ms, uint64_t *ops)
+{
+    if (!out || !ms || !ops || blocks == 0 || iters == 0) return 1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int rc = timed_launch(
+        [&] { hipLaunchKernelGGL(k_mfma_fp4_rate<true>, dim3(blocks), dim3(256), 0, s, out, iters, 5u); }, s, ms);
     *ops = (uint64_t)blocks * 4 /* waves */ * iters * 4 * (2ull * 32 * 32 * 64);
     return rc;
 }

@@ -46,6 +46,23 @@ int guarded(F &&body)
 // Per-thread stream on the thread's selected device (defined in capi.hip).
 hipStream_t thread_stream();
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves only (no LDS
+// crossbar round trips): scans within each 16-lane row (row_shr 1, 2, 4, 8,
+// zero filled), then row 0's total into row 1 and row 2's into row 3
+// (row_bcast:15), then rows 0-1's total into rows 2-3 (row_bcast:31).  Every
+// lane of the wave must be active.  Lane l ends with the sum over lanes 0 .. l,
+// lane 63 with the wave's total.
+__device__ __forceinline__ u32 wave_inclusive_scan(u32 v)
+{
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
 // Device buffer owned by the library (RAII).  Allocation and release are
 // stream-ordered (hipMallocAsync / hipFreeAsync on the allocating thread's
 // stream, from a pool that keeps freed memory), so concurrent callers on

@@ -29,6 +29,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace casphy {
@@ -1121,6 +1122,315 @@ __global__ __launch_bounds__(512, 1) void k_mfma_fp4w(MfmaArgs a)
         }
 }
 
+// fp4 with 256 x 256 pair tiles of 4 waves, one per SIMD, each 128 x 128 pairs
+// as 4 x 4 MFMA blocks (256 fp32 accumulators per lane): per pair a quarter of
+// the LDS fragment reads of the 128 x 128 tiles and half their L2 reads.  With
+// no second wave on its SIMD, a wave keeps its matrix core fed by software
+// pipelining instead of occupancy:
+//   * four LDS stage buffers (the swizzled 64-byte rows of k_mfma_fp4w), filled
+//     by LDS DMA three stages ahead of the stage being multiplied;
+//   * the fragments of a stage's second 64-site chunk are read while the first
+//     chunk's 16 MFMAs run (two register sets of 8 fragments);
+//   * the wait for the next stage's DMA and the barrier that publishes it come
+//     before the last 4 MFMAs of a stage, so the next stage's first fragment
+//     reads overlap them.
+// The buffers are four arrays used in a fixed order (the loop is unrolled by
+// four), so the compiler sees which buffer each read and DMA touches.
+constexpr u32 FP4X_STAGE = 2 * WT * 64;  // bytes of one stage buffer: A rows then B rows
+
+template <int EPI>
+__global__ __launch_bounds__(256, 1) void k_mfma_fp4x(MfmaArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char S0[FP4X_STAGE];
+    __shared__ __attribute__((aligned(16))) unsigned char S1[FP4X_STAGE];
+    __shared__ __attribute__((aligned(16))) unsigned char S2[FP4X_STAGE];
+    __shared__ __attribute__((aligned(16))) unsigned char S3[FP4X_STAGE];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
+    u32 I, J;
+    if (!tile_of(a, t, I, J)) return;
+    const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 wm = wave >> 1, wn = wave & 1, h = lane >> 5;
+
+    v16f acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0.0f;
+
+    // DMA: waves 0-1 move the A rows, waves 2-3 the B rows; lane granule q of
+    // instruction k is slot q & 3 of row q >> 2 and holds granule
+    // g = slot ^ ((row >> 2) & 3) (chunk g >> 1, half g & 1) of that row
+    const bool opB = wave >= 2;
+    const unsigned char *E = reinterpret_cast<const unsigned char *>(opB ? a.PB : a.PA);
+    const size_t pad = opB ? a.padB : a.padA;
+    const size_t rbase = (size_t)(opB ? J : I) * WT;
+    size_t off[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const u32 q = ((wave & 1) * 8 + k) * 64 + lane, row = q >> 2, g = (q & 3) ^ ((row >> 2) & 3);
+        off[k] = ((size_t)(g >> 1) * pad + rbase + row) * 32 + (g & 1) * 16;
+    }
+    const size_t sstride = 64 * pad;  // bytes between stages: two 32-byte chunks per row
+    const u32 dbase = wave * 8 * 1024;
+    auto dma = [&](unsigned char *buf, u32 s) {
+        const unsigned char *src = E + (size_t)s * sstride;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + off[k]),
+                                             (__attribute__((address_space(3))) void *)(buf + dbase + k * 1024), 16,
+                                             0, 0);
+    };
+    // fragment of row r, chunk c: the row's slot (2c + h) ^ ((r >> 2) & 3); with
+    // r = 32 m + (lane & 31) the swizzle depends on the lane only
+    const u32 lo0 = (lane & 31) * 64 + ((h ^ ((lane >> 2) & 3)) * 16), lo1 = lo0 ^ 32;
+    const u32 fa0 = wm * 128 * 64, fb0 = WT * 64 + wn * 128 * 64;
+    v4i F0[8], F1[8];
+    auto frags = [&](const unsigned char *buf, u32 lo, v4i(&F)[8]) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) F[x] = *reinterpret_cast<const v4i *>(buf + fa0 + x * 32 * 64 + lo);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) F[4 + y] = *reinterpret_cast<const v4i *>(buf + fb0 + y * 32 * 64 + lo);
+    };
+    auto mma = [&](const v4i(&F)[8], int b) {
+        const int x = b >> 2, y = b & 3;
+        const v8i fa = v8i{F[x].x, F[x].y, F[x].z, F[x].w, 0, 0, 0, 0};
+        const v8i fb = v8i{F[4 + y].x, F[4 + y].y, F[4 + y].z, F[4 + y].w, 0, 0, 0, 0};
+        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, acc[x][y], 4, 4, 0, 127, 0, 127);
+    };
+    // one stage out of buffer `cur`: PRE issues the DMA of stage s + 3 into
+    // `pre`, NEXT publishes stage s + 1 (buffer `nxt`) and reads its first
+    // fragments.  Waits: before the barrier this wave's DMA of stage s + 1 has
+    // landed -- the DMAs of s + 2 and s + 3 (16 instructions) may still be in
+    // flight when PRE, none otherwise.
+    auto step = [&](u32 s, unsigned char *cur, unsigned char *nxt, unsigned char *pre, auto PRE, auto NEXT) {
+        frags(cur, lo1, F1);
+        if constexpr (decltype(PRE)::value) dma(pre, s + 3);
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            mma(F0, b);
+            if (b < 8) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            if (b < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            if (b >= 8) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            if (b >= 8 && decltype(PRE)::value) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // DMA
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the second chunk's MFMAs stay behind the first's
+#pragma unroll
+        for (int b = 0; b < 12; ++b) mma(F1, b);
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (decltype(NEXT)::value) {
+            if constexpr (decltype(PRE)::value)
+                __builtin_amdgcn_s_waitcnt(0x4F70);  // vmcnt(16)
+            else
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            asm volatile("" ::: "memory");  // no compiler motion of LDS accesses across the barrier
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            frags(nxt, lo0, F0);
+        }
+#pragma unroll
+        for (int b = 12; b < 16; ++b) {
+            mma(F1, b);
+            if constexpr (decltype(NEXT)::value) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
+                __builtin_amdgcn_sched_group_barrier(0x100, 2, 1);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    // stages come in multiples of 8
+    const u32 nst = a.stages;
+    dma(S0, 0);
+    dma(S1, 1);
+    dma(S2, 2);
+    __builtin_amdgcn_s_waitcnt(0x4F70);  // vmcnt(16): stage 0 has landed
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    frags(S0, lo0, F0);
+    u32 s = 0;
+    for (; s + 4 < nst; s += 4) {
+        step(s, S0, S1, S3, T_{}, T_{});
+        step(s + 1, S1, S2, S0, T_{}, T_{});
+        step(s + 2, S2, S3, S1, T_{}, T_{});
+        step(s + 3, S3, S0, S2, T_{}, T_{});
+    }
+    step(s, S0, S1, S3, T_{}, T_{});
+    step(s + 1, S1, S2, S0, F_{}, T_{});
+    step(s + 2, S2, S3, S1, F_{}, T_{});
+    step(s + 3, S3, S0, S2, F_{}, F_{});
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const bool mirror = a.triangle && I != J;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            v16i ai;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
+            block_epilogue<EPI>(a, ai, I * WT + wm * 128 + 32 * x, J * WT + wn * 128 + 32 * y, lane, mirror, thr);
+        }
+}
+
+// fp4 with 256 x 128 pair tiles (256 A rows, 128 B rows) of 4 waves, each
+// 128 x 64 pairs as 4 x 2 MFMA blocks (128 fp32 accumulators per lane), so two
+// workgroups share a CU: while one runs its epilogue (about 100 VALU
+// instructions per 32 x 32 block for the radius mask) or waits for its first
+// stage, the other's MFMAs keep the matrix cores busy.  Inside a workgroup the
+// pipeline of k_mfma_fp4x: LDS DMA two stages ahead into three rotating
+// buffers (24 KB each: A rows then B rows, swizzled 64-byte rows), the second
+// chunk's fragments read during the first chunk's MFMAs, the next stage
+// published before the last two MFMAs of a stage.  Triangles run on 256-row blocks of A
+// against the two 128-row halves of the matching 256-row blocks of B; a 32 x 32
+// block below the diagonal is skipped (its mirror is written by the block above
+// it) and one on the diagonal writes its own pairs only.
+constexpr u32 FP4Y_STAGE = (WT + TILE) * 64;  // 24 KB
+
+template <int EPI>
+__global__ __launch_bounds__(256, 2) void k_mfma_fp4y(MfmaArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char S[3 * FP4Y_STAGE];
+
+    const u32 per = (a.ntiles + 7) / 8;
+    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= a.ntiles || mask_skipped<EPI>(a)) return;
+    u32 I, J;  // A block of 256 rows, B block of 128 rows
+    if (a.triangle) {
+        u32 I2, J2;
+        tri_index(t >> 1, I2, J2);
+        I = I2;
+        J = 2 * J2 + (t & 1);
+        if (J >= a.ntB) return;  // the second half of the last 256-row block of B
+    } else {
+        I = t / a.ntB;
+        J = t - I * a.ntB;
+    }
+    const u32 tid = threadIdx.x, lane = tid & 63;
+    const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the DMA bases stay scalar
+    const u32 wm = wave >> 1, wn = wave & 1, h = lane >> 5;
+
+    v16f acc[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[x][y][v] = 0.0f;
+
+    // DMA: 1,536 granules per stage, 6 instructions per wave; instruction
+    // w * 6 + k covers granules 64 (w * 6 + k) .. + 63, A rows (granules below
+    // 1,024) or B rows, uniformly per instruction; granule q of a row block is
+    // slot q & 3 of row q >> 2 and holds granule g = slot ^ ((row >> 2) & 3)
+    // the A rows of the tile start at EA + 32 I * 256, the B rows at EB + 32 J * 128
+    // (chunk 0); a lane's offset from there is below 64 pad bytes
+    const unsigned char *baseA = reinterpret_cast<const unsigned char *>(a.PA) + (size_t)I * WT * 32;
+    const unsigned char *baseB = reinterpret_cast<const unsigned char *>(a.PB) + (size_t)J * TILE * 32;
+    const size_t strA = 64 * (size_t)a.padA, strB = 64 * (size_t)a.padB;  // bytes per stage
+    u32 off[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const bool opB = wave * 6 + k >= 16;  // uniform
+        const u32 q2 = (wave * 6 + k) * 64 + lane - (opB ? WT * 4 : 0u), row = q2 >> 2;
+        const u32 g = (q2 & 3) ^ ((row >> 2) & 3);
+        off[k] = ((g >> 1) * (opB ? a.padB : a.padA) + row) * 32 + (g & 1) * 16;
+    }
+    const u32 dbase = wave * 6 * 1024;
+    auto dma = [&](unsigned char *buf, u32 s) {
+        const unsigned char *sa = baseA + s * strA, *sb = baseB + s * strB;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)((wave * 6 + k >= 16 ? sb : sa) + off[k]),
+                (__attribute__((address_space(3))) void *)(buf + dbase + k * 1024), 16, 0, 0);
+    };
+    const u32 lo0 = (lane & 31) * 64 + ((h ^ ((lane >> 2) & 3)) * 16), lo1 = lo0 ^ 32;
+    const u32 fa0 = wm * 128 * 64, fb0 = WT * 64 + wn * 64 * 64;
+    v4i F0[6], F1[6];
+    auto frags = [&](const unsigned char *buf, u32 lo, v4i(&F)[6]) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) F[x] = *reinterpret_cast<const v4i *>(buf + fa0 + x * 32 * 64 + lo);
+#pragma unroll
+        for (int y = 0; y < 2; ++y) F[4 + y] = *reinterpret_cast<const v4i *>(buf + fb0 + y * 32 * 64 + lo);
+    };
+    auto mma = [&](const v4i(&F)[6], int b) {
+        const int x = b >> 1, y = b & 1;
+        const v8i fa = v8i{F[x].x, F[x].y, F[x].z, F[x].w, 0, 0, 0, 0};
+        const v8i fb = v8i{F[4 + y].x, F[4 + y].y, F[4 + y].z, F[4 + y].w, 0, 0, 0, 0};
+        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, acc[x][y], 4, 4, 0, 127, 0, 127);
+    };
+    // one stage out of buffer `cur`: the DMA of stage s + 2 (the last stage
+    // again past the end, into a buffer no one reads) goes into `pre`; then
+    // stage s + 1 (in `nxt`) is published -- this wave's DMA of it has landed
+    // (vmcnt(6) leaves the DMA of s + 2 in flight) and every wave has passed
+    // the barrier -- and its first fragments are read.  The three buffers
+    // rotate, so every stage runs the same code.
+    const u32 nst = a.stages, last = nst - 1;
+    dma(S, 0);
+    dma(S + FP4Y_STAGE, 1);
+    __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt(6): stage 0 has landed
+    asm volatile("" ::: "memory");  // no compiler motion of LDS accesses across the barrier
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    frags(S, lo0, F0);
+    u32 cur = 0, nxt = FP4Y_STAGE, pre = 2 * FP4Y_STAGE;
+    for (u32 s = 0; s < nst; ++s) {
+        frags(S + cur, lo1, F1);
+        dma(S + pre, min(s + 2, last));
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            mma(F0, b);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            if (b < 6) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            if (b >= 2) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // DMA
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the second chunk's MFMAs stay behind the first's
+#pragma unroll
+        for (int b = 0; b < 6; ++b) mma(F1, b);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt(6)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        frags(S + nxt, lo0, F0);
+#pragma unroll
+        for (int b = 6; b < 8; ++b) {
+            mma(F1, b);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
+            __builtin_amdgcn_sched_group_barrier(0x100, 3, 1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const u32 c = cur;
+        cur = nxt;
+        nxt = pre;
+        pre = c;
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no DMA still writes this workgroup's LDS
+
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const u32 r0 = I * WT + wm * 128 + 32 * x, c0 = J * TILE + wn * 64 + 32 * y;
+            if (a.triangle && c0 < r0) continue;  // wave-uniform
+            v16i ai;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
+            block_epilogue<EPI>(a, ai, r0, c0, lane, a.triangle && c0 > r0, thr);
+        }
+}
+
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1284,7 +1594,8 @@ void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
 // Expands the packed operands to fp4 nibbles (stream-ordered scratch freed
 // after the launch) and runs the fp4 tiles.
 // mode 0: LDS staging through registers, 1: B operands from global, 2: LDS DMA,
-// 3: LDS DMA with 256 x 256 tiles
+// 3: LDS DMA with 256 x 256 tiles, 4: 256 x 256 tiles of 4 software-pipelined waves,
+// 5: 256 x 128 tiles of 4 pipelined waves, two workgroups per CU
 void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
 {
     MfmaArgs a = in;
@@ -1308,7 +1619,33 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
     a.PA = EA;
     a.PB = EB;
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
-    if (mode == 3) {
+    if ((mode == 4 || mode == 5) && (stages % 4 != 0 || stages < 4))
+        throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: the pipelined tiles need a multiple of 4 stages");
+    if (mode == 5) {
+        if (epi == E_MASK)
+            hipLaunchKernelGGL(k_mfma_fp4y<E_MASK>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_SCORES)
+            hipLaunchKernelGGL(k_mfma_fp4y<E_SCORES>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_LOCALMAX)
+            hipLaunchKernelGGL(k_mfma_fp4y<E_LOCALMAX>, dim3(grid), dim3(256), 0, s, a);
+        else
+            throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    } else if (mode == 4) {
+        if (epi == E_MASK)
+            hipLaunchKernelGGL(k_mfma_fp4x<E_MASK>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_SCORES)
+            hipLaunchKernelGGL(k_mfma_fp4x<E_SCORES>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_LOCALMAX)
+            hipLaunchKernelGGL(k_mfma_fp4x<E_LOCALMAX>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_TBE)
+            hipLaunchKernelGGL(k_mfma_fp4x<E_TBE>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_PAIRS)
+            hipLaunchKernelGGL(k_mfma_fp4x<E_PAIRS>, dim3(grid), dim3(256), 0, s, a);
+        else if (epi == E_ARGMAX)
+            hipLaunchKernelGGL(k_mfma_fp4x<E_ARGMAX>, dim3(grid), dim3(256), 0, s, a);
+        else
+            throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    } else if (mode == 3) {
         if (epi == E_MASK)
             hipLaunchKernelGGL(k_mfma_fp4w<E_MASK>, dim3(grid), dim3(512), 0, s, a);
         else if (epi == E_SCORES)
@@ -1393,12 +1730,12 @@ namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
 const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4",
-                                 "fp4b", "fp4d", "fp4w"};
-constexpr int kNumVariants = 13;
+                                 "fp4b", "fp4d", "fp4w", "fp4x", "fp4y"};
+constexpr int kNumVariants = 15;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11, 12};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
-constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11, kFp4w = 12;
+constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11, kFp4w = 12, kFp4x = 13, kFp4y = 14;  // fp4y: mask, scores, localmax
 // super-tile edge for 128 x 128 tiles: 8 (8 x 8 tiles = 16 row blocks of 128);
 // CASPHY_MFMA_SUPER overrides, 0 keeps the plain column order
 u32 super_edge()
@@ -1407,7 +1744,11 @@ u32 super_edge()
     return e ? (u32)std::atoi(e) : 8u;
 }
 
-bool variant_supports(int v, int epi) { return v < kWide || v >= kFp4 || epi == E_MASK || epi == E_SCORES; }
+bool variant_supports(int v, int epi)
+{
+    if (v == kFp4y) return epi == E_MASK || epi == E_SCORES || epi == E_LOCALMAX;
+    return v < kWide || v >= kFp4 || epi == E_MASK || epi == E_SCORES;
+}
 
 int variant_from_env()
 {
@@ -1422,11 +1763,19 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
     if (!variant_supports(v, epi)) v = kDefaultVariant;
-    const u32 T = v == 6 ? RT : (v == kWide || v == kWide8 || v == kFp4w) ? WT : TILE;
+    const u32 T = v == 6 ? RT : (v == kWide || v == kWide8 || v == kFp4w || v == kFp4x || v == kFp4y) ? WT : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntA = ntA;
-    a.ntB = ceil_div(a.nB, T);
+    a.ntB = ceil_div(a.nB, v == kFp4y ? (u32)TILE : T);  // fp4y: 256 A rows x 128 B rows
     if (ntA == 0 || a.ntB == 0) return;
+    if (v == kFp4y) {
+        // triangle: the two 128-row halves of every 256-row block pair I <= J
+        a.sup = 0;
+        a.ntiles = a.triangle ? (u32)((u64)ntA * (ntA + 1)) : ntA * a.ntB;
+        launch_fp4(epi, a, s, 5);
+        CASPHY_HIP(hipGetLastError());
+        return;
+    }
     // 128 x 128 tiles run in 8 x 8 super-tiles (launch slots without a tile exit)
     a.sup = T == TILE ? super_edge() : 0;
     if (a.sup) {
@@ -1450,6 +1799,8 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case kFp4b: launch_fp4(epi, a, s, 1); break;
     case kFp4d: launch_fp4(epi, a, s, 2); break;
     case kFp4w: launch_fp4(epi, a, s, 3); break;
+    case kFp4x: launch_fp4(epi, a, s, 4); break;
+    case kFp4y: launch_fp4(epi, a, s, 5); break;
     default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());

@@ -26,21 +26,6 @@ namespace {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// Wave sum with DPP moves only (no LDS crossbar, which the query reads keep
-// busy): inclusive scans within each 16-lane row (row_shr 1, 2, 4, 8), then
-// row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then rows
-// 0-1's total into rows 2-3 (row_bcast:31).  Lane 63 ends with the sum.
-__device__ __forceinline__ u32 wave_sum_lane63(u32 v)
-{
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, false);  // row_shr:1
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, false);  // row_shr:2
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false);  // row_shr:4
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, false);  // row_shr:8
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
-    return v;
-}
-
 constexpr u32 SCAN_WAVES = 4;  // waves per workgroup share one copy of the queries in LDS
 
 __global__ __launch_bounds__(64 * SCAN_WAVES) void k_scan_bit(const u32 *mat, u32 n, u32 ldw, const u32 *qrows, u32 Q,
@@ -94,10 +79,10 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void k_scan_bit(const u32 *mat, u3
             }
         }
         if (r + stride < n) load_chunk(r + stride, 0);
-        const u32 t0 = __builtin_amdgcn_readlane(wave_sum_lane63(p0), 63);
-        const u32 t1 = __builtin_amdgcn_readlane(wave_sum_lane63(p1), 63);
-        const u32 t2 = __builtin_amdgcn_readlane(wave_sum_lane63(p2), 63);
-        const u32 t3 = __builtin_amdgcn_readlane(wave_sum_lane63(p3), 63);
+        const u32 t0 = __builtin_amdgcn_readlane(wave_inclusive_scan(p0), 63);
+        const u32 t1 = __builtin_amdgcn_readlane(wave_inclusive_scan(p1), 63);
+        const u32 t2 = __builtin_amdgcn_readlane(wave_inclusive_scan(p2), 63);
+        const u32 t3 = __builtin_amdgcn_readlane(wave_inclusive_scan(p3), 63);
         const u32 v = myq == 0 ? t0 : myq == 1 ? t1 : myq == 2 ? t2 : t3;
         if (myq < Q) {
             cnt += (int32_t)v >= rk;
@@ -129,7 +114,7 @@ __global__ __launch_bounds__(256) void k_scan_sum(const u32 *part, u32 nblocks, 
     u32 c = 0;
 #pragma unroll 8
     for (u32 b = threadIdx.x; b < nblocks; b += 256) c += p[b];
-    c = __builtin_amdgcn_readlane(wave_sum_lane63(c), 63);
+    c = __builtin_amdgcn_readlane(wave_inclusive_scan(c), 63);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) counts[(size_t)k * R + q] = red[0] + red[1] + red[2] + red[3];

@@ -60,7 +60,7 @@ def test_config2_radius_properties(hip, orc, cfg2, monkeypatch):
         c3, i3 = hip.radius_neighbor(bmat, r)
         monkeypatch.setenv("CASPHY_MFMA", "1")
         assert np.array_equal(c3, counts) and np.array_equal(i3, idx)
-        for variant in ("64x2", "128x1", "fp4", "fp4b", "fp4d", "fp4w", "wide", "wide8", "reg"):
+        for variant in ("64x2", "128x1", "fp4", "fp4b", "fp4d", "fp4w", "fp4x", "fp4y", "wide", "wide8", "reg"):
             monkeypatch.setenv("CASPHY_MFMA_VARIANT", variant)
             c3, i3 = hip.radius_neighbor(bmat, r)
             assert np.array_equal(c3, counts) and np.array_equal(i3, idx), variant

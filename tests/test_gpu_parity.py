@@ -16,8 +16,8 @@ def rand_bits(rng, n, W, p):
 
 
 # ------------------------------------------------------------------ scores
-@pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-fp4", "mfma-fp4b", "mfma-fp4d", "mfma-fp4w", "mfma-wide",
-                        "mfma-wide8", "mfma-reg", "valu"])
+@pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-fp4", "mfma-fp4b", "mfma-fp4d", "mfma-fp4w", "mfma-fp4x",
+                        "mfma-fp4y", "mfma-wide", "mfma-wide8", "mfma-reg", "valu"])
 def bitpath(request, monkeypatch):
     """Runs a test on the autotuned int8 matrix-core path (every candidate
     variant runs on the first call of a shape class), on the fixed variants
@@ -29,7 +29,8 @@ def bitpath(request, monkeypatch):
         monkeypatch.delenv("CASPHY_AUTOTUNE", raising=False)
     else:
         variant = {"mfma-reg": "reg", "mfma-wide": "wide", "mfma-wide8": "wide8", "mfma-fp4": "fp4",
-                   "mfma-fp4b": "fp4b", "mfma-fp4d": "fp4d", "mfma-fp4w": "fp4w"}.get(request.param, "64x2")
+                   "mfma-fp4b": "fp4b", "mfma-fp4d": "fp4d", "mfma-fp4w": "fp4w",
+                   "mfma-fp4x": "fp4x", "mfma-fp4y": "fp4y"}.get(request.param, "64x2")
         monkeypatch.setenv("CASPHY_MFMA_VARIANT", variant)
     return request.param
 
