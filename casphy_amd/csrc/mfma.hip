@@ -38,6 +38,7 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
 // (j, i) with i <= j of the t-th entry of the upper triangle in column order.
 __device__ __forceinline__ void tri_index(u32 t, u32 &i, u32 &j)
@@ -205,6 +206,31 @@ __device__ __forceinline__ void mask_rows(const v16i &acc, int32_t thr, unsigned
     if constexpr (V + 1 < 16) mask_rows<V + 1>(acc, thr, jmask, rowword, colbits);
 }
 
+// Radius mask of one 32 x 32 block from fp32 accumulators (the fp4 tiles):
+// every sum is an exact integer below 2^24, so `sum >= (float)thr` decides as
+// the integer compare does (a threshold beyond +-2^24 rounds to a value that
+// still lies beyond every sum).  Register V holds rows t and t + 4 of the block
+// (t = 8 (V / 4) + V % 4) in the two half-waves: one ballot is both rows' words;
+// they gather into lanes t and t + 4 of two registers (two chains of
+// writelanes, so no writelane waits on the one before it).  The writelanes
+// read the ballot through the scalar AND with the column mask: on this target
+// a writelane that reads a scalar register straight after a vector compare
+// wrote it reads the old value, so the scalar instruction between them is
+// part of the contract (a compiler that dropped the AND would break it; the
+// column mask is a runtime ballot, so it cannot).
+template <int V>
+__device__ __forceinline__ void mask_rows_f(const v16f &acc, float thr, unsigned long long jmask, u32 &rw0, u32 &rw1,
+                                            u32 &colbits)
+{
+    constexpr int t = 8 * (V / 4) + (V % 4);
+    const bool hit = acc[V] >= thr;
+    const unsigned long long b = __ballot(hit) & jmask;
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(rw0) : "s"((u32)b), "i"(t));
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(rw1) : "s"((u32)(b >> 32)), "i"(t + 4));
+    colbits |= hit ? 1u << t : 0u;
+    if constexpr (V + 1 < 16) mask_rows_f<V + 1>(acc, thr, jmask, rw0, rw1, colbits);
+}
+
 // Epilogue of one 32 x 32 block whose rows start at r0 and columns at c0
 // (mirror: the tile is off the diagonal of a triangle, so the transposed
 // pairs are written too).  Row blocks of the transfer-distance batches are
@@ -326,6 +352,35 @@ __device__ __forceinline__ void block_epilogue(const MfmaArgs &a, const v16i &ac
         const u32 q = __shfl_xor(cmin, 32);
         cmin = q < cmin ? q : cmin;
         if (half == 0 && j < a.nB) atomicMin(&a.bmin[bbase + j], cmin);
+    }
+}
+
+// Epilogue of one 32 x 32 block of the fp4 tiles (fp32 accumulators): the
+// radius mask compares the sums as floats; every other epilogue takes them as
+// the integers they are.
+template <int EPI>
+__device__ __forceinline__ void fp4_epilogue(const MfmaArgs &a, const v16f &acc, u32 r0, u32 c0, u32 lane, bool mirror,
+                                             int32_t thr)
+{
+    if constexpr (EPI == E_MASK) {
+        const u32 half = lane >> 5, j = c0 + (lane & 31);
+        const float fthr = (float)thr;
+        u32 rw0 = 0, rw1 = 0, colbits = 0;
+        mask_rows_f<0>(acc, fthr, __ballot(j < a.nB), rw0, rw1, colbits);
+        const u32 rowword = rw0 | rw1, wc = c0 / 32;
+        if (lane < 32 && r0 + lane < a.nA && wc < a.mstride) a.mask[(size_t)(r0 + lane) * a.mstride + wc] = rowword;
+        if (mirror) {
+            colbits <<= 4 * half;
+            const u32 rows = r0 < a.nA ? a.nA - r0 : 0u;  // rows past nA score the zero padding
+            colbits &= rows >= 32 ? 0xFFFFFFFFu : (1u << rows) - 1u;
+            colbits |= __shfl_xor(colbits, 32);
+            if (half == 0 && j < a.nB && r0 / 32 < a.mstride) a.mask[(size_t)j * a.mstride + r0 / 32] = colbits;
+        }
+    } else {
+        v16i ai;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) ai[v] = (int)acc[v];
+        block_epilogue<EPI>(a, ai, r0, c0, lane, mirror, thr);
     }
 }
 
@@ -591,7 +646,6 @@ __global__ __launch_bounds__(512, 1) void k_mfma_wide8(MfmaArgs a)
 // The expansion to nibbles runs once per launch into HBM (stage-major, 64-site
 // chunks of 32 bytes per row), so the tile loop only copies bytes to LDS.
 typedef int v8i __attribute__((ext_vector_type(8)));
-typedef float v16f __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ u32 nib8(u32 b)  // bits 0..7 -> nibbles 0..7 of value 2 or 0
 {
@@ -783,10 +837,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4(MfmaArgs a)
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            v16i ai;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
+            fp4_epilogue<EPI>(a, acc[x][y], er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -894,10 +945,7 @@ __global__ __launch_bounds__(NTHR, 3) void k_mfma_fp4b(MfmaArgs a)
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            v16i ai;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
+            fp4_epilogue<EPI>(a, acc[x][y], er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -1008,10 +1056,7 @@ __global__ __launch_bounds__(NTHR, 4) void k_mfma_fp4d(MfmaArgs a)
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            v16i ai;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
+            fp4_epilogue<EPI>(a, acc[x][y], er0 + 32 * x, ec0 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -1115,10 +1160,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_fp4w(MfmaArgs a)
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            v16i ai;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, I * WT + wm * 128 + 32 * x, J * WT + wn * 64 + 32 * y, lane, mirror, thr);
+            fp4_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 64 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -1274,10 +1316,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_fp4x(MfmaArgs a)
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) {
-            v16i ai;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, I * WT + wm * 128 + 32 * x, J * WT + wn * 128 + 32 * y, lane, mirror, thr);
+            fp4_epilogue<EPI>(a, acc[x][y], I * WT + wm * 128 + 32 * x, J * WT + wn * 128 + 32 * y, lane, mirror, thr);
         }
 }
 
@@ -1424,10 +1463,7 @@ __global__ __launch_bounds__(256, 2) void k_mfma_fp4y(MfmaArgs a)
         for (int y = 0; y < 2; ++y) {
             const u32 r0 = I * WT + wm * 128 + 32 * x, c0 = J * TILE + wn * 64 + 32 * y;
             if (a.triangle && c0 < r0) continue;  // wave-uniform
-            v16i ai;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ai[v] = (int)acc[x][y][v];
-            block_epilogue<EPI>(a, ai, r0, c0, lane, a.triangle && c0 > r0, thr);
+            fp4_epilogue<EPI>(a, acc[x][y], r0, c0, lane, a.triangle && c0 > r0, thr);
         }
 }
 
