@@ -18,20 +18,19 @@
 //     of them takes small joins only, so they never wait behind large ones.
 //   * Transfer jobs come as clade bitsets or as trees (edge matrices: the
 //     transfer distances are computed from the trees on the device).
+// The queue, admission, batching and completion bookkeeping are the host-only
+// Scheduler (sched.hpp); this file holds the jobs and their device work.
 // Each job's output equals the corresponding plain entry point's.
 #include <algorithm>
-#include <chrono>
-#include <condition_variable>
 #include <cstdlib>
-#include <deque>
 #include <memory>
 #include <mutex>
-#include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "context.hpp"
 #include "join.hpp"
+#include "sched.hpp"
 
 namespace casphy {
 namespace {
@@ -39,13 +38,6 @@ namespace {
 constexpr u32 BATCH_MAX_CELLS = 2048;  // joins up to this size are batched
 constexpr u32 BATCH_MAX_JOBS = 1024;
 constexpr int JOB_TREES = 4;           // transfer distances from edge matrices
-
-using clock_t_ = std::chrono::steady_clock;
-
-double ms_between(clock_t_::time_point a, clock_t_::time_point b)
-{
-    return std::chrono::duration<double, std::milli>(b - a).count();
-}
 
 // A packed matrix kept on the server's device.
 struct Resident {
@@ -65,8 +57,7 @@ __global__ void k_gather_words(const u32 *mat, u32 width, const u32 *rows, u32 n
     }
 }
 
-struct Job {
-    u64 id = 0;
+struct Job : SchedJob {
     int kind = 0;  // CASPHY_KIND_BIT / CASPHY_KIND_INDEL joins, CASPHY_JOB_TRANSFER, JOB_TREES
     u32 n = 0, width = 0;
     int anc = 1;
@@ -80,13 +71,8 @@ struct Job {
     std::vector<u32> xoff, xnnode;
     u32 ynedge = 0, ynnode = 0, nrep = 0;
     std::vector<u32> out;     // join: raw edges; transfer: ymin|ysize|xmin|xsize
-    size_t dev_bytes = 0;
-    int status = 1;           // 1 pending, 0 done, < 0 error
-    std::string err;
-    clock_t_::time_point t_submit, t_done;
     bool join() const { return kind == CASPHY_KIND_BIT || kind == CASPHY_KIND_INDEL; }
-    bool batchable() const { return join() && n <= BATCH_MAX_CELLS; }
-    bool same_batch(const Job &o) const { return kind == o.kind && width == o.width && anc == o.anc; }
+    void on_done() override { res.reset(); }  // the resident matrix may be dropped now
 };
 
 size_t stage_pad_rows(u32 n) { return ((size_t)n + 255) / 256 * 256; }
@@ -131,132 +117,19 @@ using namespace casphy;
 
 struct casphy_server {
     int device = 0;
-    size_t budget = 0;
-    std::mutex m;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Job *> queue;
-    std::unordered_map<u64, std::unique_ptr<Job>> jobs;
+    std::mutex rm;  // guards `resident`
     std::unordered_map<u64, std::shared_ptr<Resident>> resident;
-    size_t running = 0;        // estimated footprint of the running jobs
-    size_t resident_bytes = 0; // the resident matrices
-    bool stop = false;
-    u64 next_id = 1;
-    std::vector<std::thread> workers;
-    std::vector<double> lat_ms;
-    bool have_first = false;
-    clock_t_::time_point t_first, t_last;
-    u64 launches = 0, batched_jobs = 0;
-    bool batching = true;  // CASPHY_SERVE_BATCH=0: one job per launch (queue-only form)
-    double hol_ms = 500.0; // CASPHY_SERVE_HOL_MS: how long the oldest job may be passed over
-
-    void worker_main(bool small_lane);
-    // The jobs a worker may take now (empty: none); called with m held.
-    std::vector<Job *> pick(bool small_lane, size_t &bytes);
-    // Runs a group's launch and fills each job's `out`; returns the status of
-    // every job (0 or a CASPHY_E* code) and the error message.  Job status,
-    // error and completion time are published by worker_main under `m`.
-    std::vector<int> run_group(std::vector<Job *> &group, std::string &err);
+    std::unique_ptr<Scheduler> sched;
 };
 
-std::vector<Job *> casphy_server::pick(bool small_lane, size_t &bytes)
-{
-    std::vector<Job *> group;
-    if (queue.empty()) return group;
-    const auto now = clock_t_::now();
-    Job *head = queue.front();
-    // `extra`: what the group being formed already holds.  A first job that
-    // fits nowhere still runs once nothing else is running.
-    auto can_start = [&](size_t need, size_t extra) {
-        return resident_bytes + running + extra + need <= budget || (running == 0 && extra == 0);
-    };
-    // the oldest job, once it has waited hol_ms without fitting, keeps its room
-    const bool reserve = !can_start(head->dev_bytes, 0) && ms_between(head->t_submit, now) >= hol_ms;
-    auto admissible = [&](const Job *j, size_t extra) {
-        if (j == head || !reserve) return can_start(j->dev_bytes, extra);
-        return resident_bytes + running + extra + j->dev_bytes + head->dev_bytes <= budget;
-    };
-    auto first = queue.end();
-    for (auto it = queue.begin(); it != queue.end(); ++it) {
-        if (small_lane && !(*it)->batchable()) continue;
-        if (admissible(*it, 0)) {
-            first = it;
-            break;
-        }
-    }
-    if (first == queue.end()) return group;
-    Job *h = *first;
-    group.push_back(h);
-    bytes = h->dev_bytes;
-    queue.erase(first);
-    if (h->batchable() && batching) {
-        for (auto it = queue.begin(); it != queue.end() && group.size() < BATCH_MAX_JOBS;) {
-            Job *j = *it;
-            if (j->batchable() && j->same_batch(*h) && admissible(j, bytes)) {
-                group.push_back(j);
-                bytes += j->dev_bytes;
-                it = queue.erase(it);
-            } else {
-                ++it;
-            }
-        }
-    }
-    return group;
-}
+namespace {
 
-void casphy_server::worker_main(bool small_lane)
+// The device work of one group (one job, or a batch of small joins of one
+// class): fills each job's `out`, returns each job's status.
+std::vector<int> run_group(std::vector<SchedJob *> &sgroup, std::string &err)
 {
-    if (casphy_set_device(device) != CASPHY_OK) return;
-    for (;;) {
-        std::vector<Job *> group;
-        size_t bytes = 0;
-        {
-            std::unique_lock<std::mutex> lk(m);
-            for (;;) {
-                if (stop) return;
-                group = pick(small_lane, bytes);
-                if (!group.empty()) break;
-                // a reserved head may become admissible by the clock alone
-                cv_work.wait_for(lk, std::chrono::milliseconds(50));
-            }
-            running += bytes;
-            ++launches;
-            if (group.size() > 1) batched_jobs += group.size();
-        }
-        std::string err;
-        std::vector<int> rc;
-        try {
-            rc = run_group(group, err);
-        } catch (const std::exception &e) {  // host allocation of the outputs
-            rc.assign(group.size(), CASPHY_ENOMEM);
-            err = e.what();
-        }
-        {
-            // a waiter may free a job as soon as it sees its status: everything
-            // the worker writes into a job is written here, before the notify,
-            // and nothing after
-            std::lock_guard<std::mutex> lk(m);
-            running -= bytes;
-            const auto now = clock_t_::now();
-            for (size_t k = 0; k < group.size(); ++k) {
-                Job *j = group[k];
-                j->t_done = now;
-                j->res.reset();  // the resident matrix may be dropped now
-                if (rc[k] == 0) {
-                    lat_ms.push_back(ms_between(j->t_submit, now));
-                } else {
-                    j->err = err;
-                }
-                j->status = rc[k];
-            }
-            t_last = now;
-        }
-        cv_done.notify_all();
-        cv_work.notify_all();
-    }
-}
-
-std::vector<int> casphy_server::run_group(std::vector<Job *> &group, std::string &err)
-{
+    std::vector<Job *> group(sgroup.size());
+    for (size_t k = 0; k < sgroup.size(); ++k) group[k] = static_cast<Job *>(sgroup[k]);
     std::vector<int> rc(group.size(), 0);
     auto fail = [&](int code) {
         err = casphy_last_error();
@@ -329,22 +202,15 @@ std::vector<int> casphy_server::run_group(std::vector<Job *> &group, std::string
     return r == CASPHY_OK ? rc : fail(r);
 }
 
-namespace {
 int submit(casphy_server *s, std::unique_ptr<Job> j, uint64_t *id)
 {
     j->dev_bytes = estimate_bytes(*j);
-    j->t_submit = clock_t_::now();
-    {
-        std::lock_guard<std::mutex> lk(s->m);
-        if (!s->have_first) { s->t_first = j->t_submit; s->have_first = true; }
-        j->id = s->next_id++;
-        *id = j->id;
-        s->queue.push_back(j.get());
-        s->jobs.emplace(j->id, std::move(j));
-    }
-    s->cv_work.notify_all();
+    j->batchable = j->join() && j->n <= BATCH_MAX_CELLS;
+    j->batch_class = ((u64)(u32)j->kind << 34) | ((u64)(u32)(j->anc & 3) << 32) | j->width;
+    *id = s->sched->submit(std::move(j));
     return CASPHY_OK;
 }
+
 }  // namespace
 
 extern "C" {
@@ -357,13 +223,16 @@ int casphy_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers
         if (rc) throw Error(rc, casphy_last_error());
         auto s = std::make_unique<casphy_server>();
         s->device = device;
-        s->budget = hbm_budget_bytes;
+        Scheduler::Options o;
+        o.budget = hbm_budget_bytes;
+        o.workers = workers;
         const char *b = std::getenv("CASPHY_SERVE_BATCH");
-        s->batching = !(b && b[0] == '0');
+        o.batching = !(b && b[0] == '0');
         const char *hol = std::getenv("CASPHY_SERVE_HOL_MS");
-        if (hol) s->hol_ms = std::atof(hol);
-        for (u32 w = 0; w < workers; ++w)
-            s->workers.emplace_back([p = s.get(), lane = workers >= 2 && w == 0] { p->worker_main(lane); });
+        if (hol) o.hol_ms = std::atof(hol);
+        o.batch_max_jobs = BATCH_MAX_JOBS;
+        s->sched = std::make_unique<Scheduler>(
+            o, [device] { return casphy_set_device(device) == CASPHY_OK; }, run_group);
         *out = s.release();
     });
 }
@@ -371,12 +240,7 @@ int casphy_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers
 void casphy_server_destroy(casphy_server *s)
 {
     if (!s) return;
-    {
-        std::lock_guard<std::mutex> lk(s->m);
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    for (auto &t : s->workers) t.join();
+    s->sched.reset();  // stops and joins the workers
     delete s;
 }
 
@@ -408,10 +272,13 @@ int casphy_server_put_matrix(casphy_server *s, int kind, const void *mat, uint32
             throw;
         }
         (void)casphy_set_device(prev);
-        std::lock_guard<std::mutex> lk(s->m);
-        *matrix = s->next_id++;
-        s->resident.emplace(*matrix, std::move(r));
-        s->resident_bytes += bytes;
+        const u64 id = s->sched->new_id();
+        {
+            std::lock_guard<std::mutex> lk(s->rm);
+            s->resident.emplace(id, std::move(r));
+        }
+        s->sched->add_resident(bytes);
+        *matrix = id;
     });
 }
 
@@ -419,11 +286,15 @@ int casphy_server_drop_matrix(casphy_server *s, uint64_t matrix)
 {
     return guarded([&] {
         CASPHY_REQUIRE(s, CASPHY_EARG, "casphy_server_drop_matrix: bad arguments");
-        std::lock_guard<std::mutex> lk(s->m);
-        auto it = s->resident.find(matrix);
-        CASPHY_REQUIRE(it != s->resident.end(), CASPHY_EARG, "casphy_server_drop_matrix: unknown matrix");
-        s->resident_bytes -= (size_t)it->second->n * it->second->width * 4;
-        s->resident.erase(it);  // queued and running jobs keep their reference
+        size_t bytes;
+        {
+            std::lock_guard<std::mutex> lk(s->rm);
+            auto it = s->resident.find(matrix);
+            CASPHY_REQUIRE(it != s->resident.end(), CASPHY_EARG, "casphy_server_drop_matrix: unknown matrix");
+            bytes = (size_t)it->second->n * it->second->width * 4;
+            s->resident.erase(it);  // queued and running jobs keep their reference
+        }
+        s->sched->remove_resident(bytes);
     });
 }
 
@@ -458,7 +329,7 @@ int casphy_server_submit_join_rows(casphy_server *s, uint64_t matrix, const uint
                        "casphy_server_submit_join_rows: ancestralType must be 0 or 1");
         std::shared_ptr<Resident> r;
         {
-            std::lock_guard<std::mutex> lk(s->m);
+            std::lock_guard<std::mutex> lk(s->rm);
             auto it = s->resident.find(matrix);
             CASPHY_REQUIRE(it != s->resident.end(), CASPHY_EARG, "casphy_server_submit_join_rows: unknown matrix");
             r = it->second;
@@ -521,18 +392,14 @@ int casphy_server_wait(casphy_server *s, uint64_t job, uint32_t *out, uint64_t o
 {
     return guarded([&] {
         CASPHY_REQUIRE(s, CASPHY_EARG, "casphy_server_wait: bad arguments");
-        std::unique_lock<std::mutex> lk(s->m);
-        auto it = s->jobs.find(job);
-        CASPHY_REQUIRE(it != s->jobs.end(), CASPHY_EARG, "casphy_server_wait: unknown job");
-        Job *j = it->second.get();
-        s->cv_done.wait(lk, [&] { return j->status != 1; });
-        std::unique_ptr<Job> owned = std::move(it->second);
-        s->jobs.erase(it);
+        std::unique_ptr<SchedJob> owned = s->sched->wait(job);
+        CASPHY_REQUIRE(owned, CASPHY_EARG, "casphy_server_wait: unknown job (or another thread waits for it)");
+        const Job *j = static_cast<const Job *>(owned.get());
         if (j->status < 0) throw Error(j->status, j->err);
         CASPHY_REQUIRE(out && out_len >= j->out.size(), CASPHY_EARG,
                        "casphy_server_wait: output buffer needs " + std::to_string(j->out.size()) + " values");
         std::copy(j->out.begin(), j->out.end(), out);
-        if (latency_ms) *latency_ms = ms_between(j->t_submit, j->t_done);
+        if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(j->t_done - j->t_submit).count();
     });
 }
 
@@ -542,16 +409,12 @@ int casphy_server_stats(casphy_server *s, uint64_t *completed, double *p50_ms, d
     return guarded([&] {
         CASPHY_REQUIRE(s && completed && p50_ms && p99_ms && jobs_per_s && launches, CASPHY_EARG,
                        "casphy_server_stats: bad arguments");
-        std::lock_guard<std::mutex> lk(s->m);
-        std::vector<double> v = s->lat_ms;
-        std::sort(v.begin(), v.end());
-        *completed = v.size();
-        auto pct = [&](double q) { return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5))]; };
-        *p50_ms = pct(0.50);
-        *p99_ms = pct(0.99);
-        const double span = s->have_first ? ms_between(s->t_first, s->t_last) / 1000.0 : 0.0;
-        *jobs_per_s = span > 0 ? v.size() / span : 0.0;
-        *launches = s->launches;
+        const SchedStats st = s->sched->stats();
+        *completed = st.completed;
+        *p50_ms = st.p50_ms;
+        *p99_ms = st.p99_ms;
+        *jobs_per_s = st.jobs_per_s;
+        *launches = st.launches;
     });
 }
 
