@@ -578,10 +578,13 @@ def radiusWeightedDNAParallel(i: int, mat, weight, radius: float):
 
 # ------------------------------------------------------------------ resident
 def mfma_tuning() -> list:
-    """Choices of the MFMA tile autotuner so far: one dict per shape class
-    (epilogue, triangle, log2 row counts, sites) with the chosen variant and
-    the time in ms of every candidate.  CASPHY_MFMA_VARIANT forces a variant
-    and CASPHY_AUTOTUNE=0 keeps the default one."""
+    """Choices of the tile autotuners so far: one dict per shape class with
+    the chosen variant and the time in ms of every candidate -- the
+    matrix-core tiles (epilogue, triangle, log2 row counts, sites), then the
+    VALU tiles ("form": "valu"; metric, epilogue, triangle, log2 row counts,
+    words, weight planes; variants kKCxCB: KC words staged per step, CB
+    columns per thread).  CASPHY_MFMA_VARIANT / CASPHY_VALU_VARIANT force a
+    variant and CASPHY_AUTOTUNE=0 keeps the defaults."""
     import json
     lib = load()
     n = C.c_size_t(0)

@@ -575,7 +575,13 @@ int casphy_mfma_tuning(char *buf, size_t cap, size_t *len)
 {
     return guarded([&] {
         require(len && (buf || cap == 0), "casphy_mfma_tuning: bad arguments");
-        const std::string r = mfma_tuning_report();
+        std::string r = mfma_tuning_report();
+        const std::string v = valu_tuning_json();
+        if (v.size() > 2) {  // both lists in one
+            r.pop_back();
+            if (r.size() > 1) r += ",";
+            r += v.substr(1);
+        }
         *len = r.size();
         if (cap) {
             const size_t k = std::min(cap - 1, r.size());

@@ -1,13 +1,18 @@
 // tiles.hip -- exact integer pair-scoring tile kernel with fused epilogues.
 //
 // Work decomposition (gfx950, wave64):
-//   * a workgroup of 256 threads (4 waves) owns a 128 x 128 tile of pairs;
-//     thread (ty, tx) of the 16 x 16 thread grid owns rows ty*8..ty*8+7 and
-//     columns tx*8..tx*8+7, i.e. 64 exact u32 accumulators;
-//   * row words are staged through LDS k-major (As[k][row]) in 32-word (128 B,
-//     one cache line per row) stages, double buffered, so each thread reads
-//     its 8 rows / 8 columns of a word with two 16-byte LDS reads each and
-//     spends 2 VALU ops (AND/XOR + v_bcnt_u32_b32 accumulate) per pair-word;
+//   * a workgroup of 256 threads (4 waves) owns a 128-row tile of A rows
+//     against a tile of 16 CB B rows (CB = 8: 128 x 128; CB = 4: 128 x 64, the
+//     callers' 128 x 128 tile run as two column halves); thread (ty, tx) of the
+//     16 x 16 thread grid owns rows ty*8..ty*8+7 and columns tx*CB..tx*CB+CB-1,
+//     i.e. 8 CB exact u32 accumulators;
+//   * row words are staged through LDS k-major (As[k][row]) in stages of KC
+//     words, double buffered, so each thread reads its 8 rows / CB columns of
+//     a word with 16-byte LDS reads and spends 2 VALU ops (AND/XOR + v_bcnt
+//     accumulate) per pair-word;
+//   * (CB, KC) -- the tile shape and the words each lane stages per step -- is
+//     picked per shape class by an autotuner that times every candidate on the
+//     real data (every tuned epilogue is idempotent);
 //   * tiles are enumerated (upper triangle for self-scoring) and remapped so
 //     that consecutive tiles, which share their A rows, land on the same XCD
 //     (workgroups are dealt round-robin over the 8 XCDs).
@@ -20,6 +25,14 @@
 //   E_LOCALMAX  per row max over j > i of (S, smallest j) as a packed u64 key;
 //   E_TBE       min(H, ntip - H) row and column minima (XOR metric).
 #include "tiles.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 namespace casphy {
 
@@ -38,14 +51,6 @@ __device__ __forceinline__ u32 nib_nonzero(u32 x)
     return pop(x & 0x11111111u);
 }
 
-template <int METRIC>
-struct Stage {
-    static constexpr int KC = METRIC == M_DNA2 ? KC_DNA2 : KC_BIT;
-    static constexpr int STEP = METRIC == M_DNA2 ? 3 : 1;
-    static constexpr int HALF = KC / 2;      // words loaded per thread per operand
-    static constexpr int V4 = HALF / 4;      // 16-byte loads per thread per operand
-};
-
 __device__ __forceinline__ void tile_coords(const TileArgs &a, u32 t, u32 &I, u32 &J)
 {
     if (a.triangle) {
@@ -60,29 +65,44 @@ __device__ __forceinline__ void tile_coords(const TileArgs &a, u32 t, u32 &I, u3
     }
 }
 
-template <int METRIC, int EPI>
+// A variant: CB columns per thread (8: 128 x 128 tiles, 4: 128 x 64) and KC
+// words per stage.  STEP words make one pair-word (3 for the 2-bit DNA form).
+template <int METRIC, int KC, int CB>
+struct Shape {
+    static constexpr int STEP = METRIC == M_DNA2 ? 3 : 1;
+    static constexpr int TC = 16 * CB;        // B rows per tile
+    static constexpr int WA = KC / 2;         // words per thread per A row (2 threads a row)
+    static constexpr int WB = KC * TC / 256;  // words per thread per B row
+    static constexpr int V4A = WA / 4, V4B = WB / 4;
+    static_assert(KC % STEP == 0 && WA % 4 == 0 && WB % 4 == 0, "stage width must split into 16-byte loads");
+};
+
+template <int METRIC, int EPI, int KC, int CB>
 __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
 {
-    using ST = Stage<METRIC>;
-    constexpr int KC = ST::KC;
+    using SH = Shape<METRIC, KC, CB>;
+    constexpr int TC = SH::TC, NSUB = TILE / TC;
     __shared__ __attribute__((aligned(16))) u32 As[2][KC][TILE];
-    __shared__ __attribute__((aligned(16))) u32 Bs[2][KC][TILE];
+    __shared__ __attribute__((aligned(16))) u32 Bs[2][KC][TC];
 
-    const u32 per = (a.ntiles + 7) / 8;
-    const u32 t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (t >= a.ntiles) return;
+    const u32 nsub = a.ntiles * NSUB;
+    const u32 per = (nsub + 7) / 8;
+    const u32 ts = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (ts >= nsub) return;
     // a device threshold of INT32_MAX (autoSplit round with no qualifying
     // radius): nothing to score; the row counts skip the mask as well
     if (EPI == E_MASK && a.thr_dev && *a.thr_dev == 0x7FFFFFFF) return;
     u32 I, J;
-    tile_coords(a, t, I, J);
+    tile_coords(a, ts / NSUB, I, J);
+    const u32 col0 = J * TILE + (ts % NSUB) * TC;
 
     const u32 tid = threadIdx.x;
     const u32 ty = tid >> 4, tx = tid & 15;
 
-    // ---- loader setup: thread loads HALF words of one A row and one B row
-    const u32 lrow = tid & (TILE - 1), lpart = tid >> 7;
-    const u32 ga = I * TILE + lrow, gb = J * TILE + lrow;
+    // ---- loader setup: a thread loads WA words of one A row and WB of one B row
+    const u32 lrowA = tid & (TILE - 1), lpartA = tid / TILE;
+    const u32 lrowB = tid & (TC - 1), lpartB = tid / TC;
+    const u32 ga = I * TILE + lrowA, gb = col0 + lrowB;
     const bool va = ga < a.nA, vb = gb < a.nB;
     const u32 *arow = nullptr, *brow = nullptr;
     if (va) arow = a.A + (size_t)(a.Aidx ? a.Aidx[ga] : ga) * a.ldw;
@@ -92,19 +112,21 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
     const u32 stages_per_plane = a.kw / KC;
     const u32 nstages = stages_per_plane * nplanes;
 
-    uint4 ra[ST::V4], rb[ST::V4];
+    uint4 ra[SH::V4A], rb[SH::V4B];
     auto gload = [&](u32 s) {
         const u32 plane = s / stages_per_plane;
-        const u32 k0 = (s - plane * stages_per_plane) * KC + lpart * ST::HALF;
+        const u32 kbase = (s - plane * stages_per_plane) * KC;
+        const u32 ka = kbase + lpartA * SH::WA, kb = kbase + lpartB * SH::WB;
 #pragma unroll
-        for (int v = 0; v < ST::V4; ++v) {
-            ra[v] = va ? *reinterpret_cast<const uint4 *>(arow + k0 + 4 * v) : make_uint4(0, 0, 0, 0);
-            rb[v] = vb ? *reinterpret_cast<const uint4 *>(brow + k0 + 4 * v) : make_uint4(0, 0, 0, 0);
-        }
+        for (int v = 0; v < SH::V4A; ++v)
+            ra[v] = va ? *reinterpret_cast<const uint4 *>(arow + ka + 4 * v) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < SH::V4B; ++v)
+            rb[v] = vb ? *reinterpret_cast<const uint4 *>(brow + kb + 4 * v) : make_uint4(0, 0, 0, 0);
         if (METRIC == M_AND && a.nplanes) {
-            const u32 *pm = a.pmask + (size_t)plane * a.kw + k0;
+            const u32 *pm = a.pmask + (size_t)plane * a.kw + ka;
 #pragma unroll
-            for (int v = 0; v < ST::V4; ++v) {
+            for (int v = 0; v < SH::V4A; ++v) {
                 uint4 m = *reinterpret_cast<const uint4 *>(pm + 4 * v);
                 ra[v].x &= m.x; ra[v].y &= m.y; ra[v].z &= m.z; ra[v].w &= m.w;
             }
@@ -112,20 +134,24 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int v = 0; v < ST::V4; ++v) {
-            const int k = lpart * ST::HALF + 4 * v;
-            As[buf][k + 0][lrow] = ra[v].x; As[buf][k + 1][lrow] = ra[v].y;
-            As[buf][k + 2][lrow] = ra[v].z; As[buf][k + 3][lrow] = ra[v].w;
-            Bs[buf][k + 0][lrow] = rb[v].x; Bs[buf][k + 1][lrow] = rb[v].y;
-            Bs[buf][k + 2][lrow] = rb[v].z; Bs[buf][k + 3][lrow] = rb[v].w;
+        for (int v = 0; v < SH::V4A; ++v) {
+            const int k = lpartA * SH::WA + 4 * v;
+            As[buf][k + 0][lrowA] = ra[v].x; As[buf][k + 1][lrowA] = ra[v].y;
+            As[buf][k + 2][lrowA] = ra[v].z; As[buf][k + 3][lrowA] = ra[v].w;
+        }
+#pragma unroll
+        for (int v = 0; v < SH::V4B; ++v) {
+            const int k = lpartB * SH::WB + 4 * v;
+            Bs[buf][k + 0][lrowB] = rb[v].x; Bs[buf][k + 1][lrowB] = rb[v].y;
+            Bs[buf][k + 2][lrowB] = rb[v].z; Bs[buf][k + 3][lrowB] = rb[v].w;
         }
     };
 
-    u32 acc[RB][RB];
+    u32 acc[RB][CB];
 #pragma unroll
     for (int r = 0; r < RB; ++r)
 #pragma unroll
-        for (int c = 0; c < RB; ++c) acc[r][c] = 0;
+        for (int c = 0; c < CB; ++c) acc[r][c] = 0;
 
     gload(0);
     lstore(0);
@@ -138,26 +164,27 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
 #pragma unroll
             for (int r = 0; r < RB; ++r)
 #pragma unroll
-                for (int c = 0; c < RB; ++c) acc[r][c] <<= sh;
+                for (int c = 0; c < CB; ++c) acc[r][c] <<= sh;
         }
 #pragma unroll 4
-        for (int k = 0; k < KC; k += ST::STEP) {
-            u32 av[ST::STEP][RB], bv[ST::STEP][RB];
+        for (int k = 0; k < KC; k += SH::STEP) {
+            u32 av[SH::STEP][RB], bv[SH::STEP][CB];
 #pragma unroll
-            for (int q = 0; q < ST::STEP; ++q) {
-                uint4 x0 = *reinterpret_cast<const uint4 *>(&As[buf][k + q][ty * RB]);
-                uint4 x1 = *reinterpret_cast<const uint4 *>(&As[buf][k + q][ty * RB + 4]);
-                uint4 y0 = *reinterpret_cast<const uint4 *>(&Bs[buf][k + q][tx * RB]);
-                uint4 y1 = *reinterpret_cast<const uint4 *>(&Bs[buf][k + q][tx * RB + 4]);
+            for (int q = 0; q < SH::STEP; ++q) {
+                const uint4 x0 = *reinterpret_cast<const uint4 *>(&As[buf][k + q][ty * RB]);
+                const uint4 x1 = *reinterpret_cast<const uint4 *>(&As[buf][k + q][ty * RB + 4]);
                 av[q][0] = x0.x; av[q][1] = x0.y; av[q][2] = x0.z; av[q][3] = x0.w;
                 av[q][4] = x1.x; av[q][5] = x1.y; av[q][6] = x1.z; av[q][7] = x1.w;
-                bv[q][0] = y0.x; bv[q][1] = y0.y; bv[q][2] = y0.z; bv[q][3] = y0.w;
-                bv[q][4] = y1.x; bv[q][5] = y1.y; bv[q][6] = y1.z; bv[q][7] = y1.w;
+#pragma unroll
+                for (int c4 = 0; c4 < CB / 4; ++c4) {
+                    const uint4 y = *reinterpret_cast<const uint4 *>(&Bs[buf][k + q][tx * CB + 4 * c4]);
+                    bv[q][4 * c4 + 0] = y.x; bv[q][4 * c4 + 1] = y.y; bv[q][4 * c4 + 2] = y.z; bv[q][4 * c4 + 3] = y.w;
+                }
             }
 #pragma unroll
             for (int r = 0; r < RB; ++r)
 #pragma unroll
-                for (int c = 0; c < RB; ++c) {
+                for (int c = 0; c < CB; ++c) {
                     if (METRIC == M_AND) {
                         acc[r][c] += pop(av[0][r] & bv[0][c]);
                     } else if (METRIC == M_XOR) {
@@ -177,29 +204,33 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
-            for (int c = 0; c < RB; ++c) acc[r][c] <<= a.ptail;
+            for (int c = 0; c < CB; ++c) acc[r][c] <<= a.ptail;
     }
 
-    const u32 i0 = I * TILE + ty * RB, j0 = J * TILE + tx * RB;
+    const u32 i0 = I * TILE + ty * RB, j0 = col0 + tx * CB;
 
     if (EPI == E_MASK) {
         const int32_t thr = a.thr_dev ? *a.thr_dev : a.thr;
-        // direct words: row i, columns J*128 + 32w .. +31
+        // direct words: row i0 + r, word j0 / 32 holds this thread's CB columns
+        // at bit j0 % 32; the TPW threads of one word combine by shuffles and
+        // one of them, chosen by r, stores it
+        constexpr u32 TPW = 32 / CB;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
-            u32 byte = 0;
+            u32 bits = 0;
 #pragma unroll
-            for (int c = 0; c < RB; ++c)
-                if ((int32_t)acc[r][c] >= thr && j0 + c < a.nB) byte |= 1u << c;
-            u32 piece = byte << ((tx & 3) * 8);
-            piece |= __shfl_xor(piece, 1);
-            piece |= __shfl_xor(piece, 2);
-            if ((tx & 3) == (u32)(r >> 1) && i0 + r < a.nA)
-                a.mask[(size_t)(i0 + r) * a.mstride + J * 4 + (tx >> 2)] = piece;
+            for (int c = 0; c < CB; ++c)
+                if ((int32_t)acc[r][c] >= thr && j0 + c < a.nB) bits |= 1u << c;
+            u32 piece = bits << ((tx % TPW) * CB);
+#pragma unroll
+            for (u32 o = 1; o < TPW; o <<= 1) piece |= __shfl_xor(piece, o);
+            if ((tx % TPW) == (u32)r * TPW / RB && i0 + r < a.nA)
+                a.mask[(size_t)(i0 + r) * a.mstride + (j0 >> 5)] = piece;
         }
         if (a.triangle && I != J) {
+            // mirrored words: row j0 + c, word i0 / 32 from the four ty rows of a wave
 #pragma unroll
-            for (int c = 0; c < RB; ++c) {
+            for (int c = 0; c < CB; ++c) {
                 u32 byte = 0;
 #pragma unroll
                 for (int r = 0; r < RB; ++r)
@@ -207,15 +238,15 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
                 u32 piece = byte << ((ty & 3) * 8);
                 piece |= __shfl_xor(piece, 16);
                 piece |= __shfl_xor(piece, 32);
-                if ((ty & 3) == (u32)(c >> 1) && j0 + c < a.nB)
-                    a.mask[(size_t)(j0 + c) * a.mstride + I * 4 + (ty >> 2)] = piece;
+                if ((ty & 3) == (u32)c * 4 / CB && j0 + c < a.nB)
+                    a.mask[(size_t)(j0 + c) * a.mstride + (i0 >> 5)] = piece;
             }
         }
     } else if (EPI == E_SCORES) {
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
-            for (int c = 0; c < RB; ++c) {
+            for (int c = 0; c < CB; ++c) {
                 const u32 i = i0 + r, j = j0 + c;
                 if (i < a.nA && j < a.nB) {
                     a.S[(size_t)i * a.ldS + j] = (int32_t)acc[r][c];
@@ -225,13 +256,14 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
     } else if (EPI == E_HIST) {
         u32 *hist = &As[0][0][0];  // TILE x (MAX_RADII + 1)
         constexpr int HB = MAX_RADII + 1;
+        static_assert(2 * KC * TILE >= TILE * HB, "histogram does not fit the stage buffer");
         for (u32 x = tid; x < TILE * HB; x += NTHR) hist[x] = 0;
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
             if (i0 + r >= a.nA) continue;
 #pragma unroll
-            for (int c = 0; c < RB; ++c) {
+            for (int c = 0; c < CB; ++c) {
                 if (j0 + c >= a.nB) continue;
                 u32 bin = 0;
                 for (u32 k = 0; k < a.nradii; ++k) bin += (int32_t)acc[r][c] >= a.radii[k];
@@ -255,7 +287,7 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
             const u32 i = i0 + r;
             u64 best = 0;
 #pragma unroll
-            for (int c = 0; c < RB; ++c) {
+            for (int c = 0; c < CB; ++c) {
                 const u32 j = j0 + c;
                 if (j < a.nB && j > i) {
                     const u64 key = ((u64)acc[r][c] << 32) | (u64)(0xFFFFFFFFu - j);
@@ -270,15 +302,15 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
             if (tx == 0 && i < a.nA && best) atomicMax((unsigned long long *)&a.keys[i], (unsigned long long)best);
         }
     } else if (EPI == E_TBE) {
-        u32 cmin[RB];
+        u32 cmin[CB];
 #pragma unroll
-        for (int c = 0; c < RB; ++c) cmin[c] = 0xFFFFFFFFu;
+        for (int c = 0; c < CB; ++c) cmin[c] = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
             u32 rmin = 0xFFFFFFFFu;
             const bool vi = i0 + r < a.nA;
 #pragma unroll
-            for (int c = 0; c < RB; ++c) {
+            for (int c = 0; c < CB; ++c) {
                 const u32 h = acc[r][c], hc = a.ntip - h;
                 const u32 d = h < hc ? h : hc;
                 if (vi && j0 + c < a.nB) {
@@ -294,7 +326,7 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
             if (tx == 0 && vi) atomicMin(&a.amin[i0 + r], rmin);
         }
 #pragma unroll
-        for (int c = 0; c < RB; ++c) {
+        for (int c = 0; c < CB; ++c) {
             u32 v = cmin[c];
             u32 o = __shfl_xor(v, 16); v = o < v ? o : v;
             o = __shfl_xor(v, 32); v = o < v ? o : v;
@@ -303,17 +335,122 @@ __global__ __launch_bounds__(NTHR) void k_tiles(TileArgs a)
     }
 }
 
-template <int METRIC>
-void launch_m(int epi, const TileArgs &a, hipStream_t s)
+// Variants: (KC, CB) per metric, named kKCxCB; index 0 is the default.
+constexpr int kNumValu = 4;
+struct ValuShape {
+    int kc, cb;
+    const char *name;
+};
+ValuShape valu_shape(int metric, int v)
 {
-    const u32 grid = 8 * ((a.ntiles + 7) / 8);
-    switch (epi) {
-    case E_MASK: hipLaunchKernelGGL((k_tiles<METRIC, E_MASK>), dim3(grid), dim3(NTHR), 0, s, a); break;
-    case E_SCORES: hipLaunchKernelGGL((k_tiles<METRIC, E_SCORES>), dim3(grid), dim3(NTHR), 0, s, a); break;
-    case E_HIST: hipLaunchKernelGGL((k_tiles<METRIC, E_HIST>), dim3(grid), dim3(NTHR), 0, s, a); break;
-    case E_LOCALMAX: hipLaunchKernelGGL((k_tiles<METRIC, E_LOCALMAX>), dim3(grid), dim3(NTHR), 0, s, a); break;
-    default: throw Error(CASPHY_EINTERNAL, "bad epilogue for metric");
+    if (metric == M_DNA2) {
+        static const ValuShape d[kNumValu] = {{24, 8, "k24x8"}, {48, 8, "k48x8"}, {0, 0, "-"}, {48, 4, "k48x4"}};
+        return d[v];
     }
+    static const ValuShape b[kNumValu] = {{32, 8, "k32x8"}, {16, 8, "k16x8"}, {32, 4, "k32x4"}, {16, 4, "k16x4"}};
+    return b[v];
+}
+bool valu_supported(int metric, int v, const TileArgs &a)
+{
+    const ValuShape sh = valu_shape(metric, v);
+    return sh.kc > 0 && a.kw % (u32)sh.kc == 0;
+}
+
+template <int METRIC, int KC, int CB>
+void launch_shape(int epi, const TileArgs &a, hipStream_t s)
+{
+    const u32 nsub = a.ntiles * (TILE / (16 * CB));
+    const u32 grid = 8 * ((nsub + 7) / 8);
+    if constexpr (METRIC == M_XOR) {  // Hamming distances feed the transfer-distance epilogue only
+        CASPHY_REQUIRE(epi == E_TBE, CASPHY_EINTERNAL, "XOR metric only feeds the TBE epilogue");
+        hipLaunchKernelGGL((k_tiles<METRIC, E_TBE, KC, CB>), dim3(grid), dim3(NTHR), 0, s, a);
+        return;
+    } else {
+        switch (epi) {
+        case E_MASK: hipLaunchKernelGGL((k_tiles<METRIC, E_MASK, KC, CB>), dim3(grid), dim3(NTHR), 0, s, a); break;
+        case E_SCORES: hipLaunchKernelGGL((k_tiles<METRIC, E_SCORES, KC, CB>), dim3(grid), dim3(NTHR), 0, s, a); break;
+        case E_HIST: hipLaunchKernelGGL((k_tiles<METRIC, E_HIST, KC, CB>), dim3(grid), dim3(NTHR), 0, s, a); break;
+        case E_LOCALMAX:
+            hipLaunchKernelGGL((k_tiles<METRIC, E_LOCALMAX, KC, CB>), dim3(grid), dim3(NTHR), 0, s, a);
+            break;
+        default: throw Error(CASPHY_EINTERNAL, "bad epilogue for metric");
+        }
+    }
+}
+
+template <int METRIC>
+void launch_metric(int v, int epi, const TileArgs &a, hipStream_t s)
+{
+    if constexpr (METRIC == M_DNA2) {
+        switch (v) {
+        case 0: launch_shape<METRIC, 24, 8>(epi, a, s); break;
+        case 1: launch_shape<METRIC, 48, 8>(epi, a, s); break;
+        case 3: launch_shape<METRIC, 48, 4>(epi, a, s); break;
+        default: throw Error(CASPHY_EINTERNAL, "VALU tiles: variant not built for the DNA form");
+        }
+    } else {
+        switch (v) {
+        case 0: launch_shape<METRIC, 32, 8>(epi, a, s); break;
+        case 1: launch_shape<METRIC, 16, 8>(epi, a, s); break;
+        case 2: launch_shape<METRIC, 32, 4>(epi, a, s); break;
+        case 3: launch_shape<METRIC, 16, 4>(epi, a, s); break;
+        default: throw Error(CASPHY_EINTERNAL, "VALU tiles: unknown variant");
+        }
+    }
+}
+
+void launch_variant(int v, int metric, int epi, const TileArgs &a, hipStream_t s)
+{
+    switch (metric) {
+    case M_AND: launch_metric<M_AND>(v, epi, a, s); break;
+    case M_XOR: launch_metric<M_XOR>(v, epi, a, s); break;
+    case M_DNA2: launch_metric<M_DNA2>(v, epi, a, s); break;
+    case M_NIB: launch_metric<M_NIB>(v, epi, a, s); break;
+    default: throw Error(CASPHY_EINTERNAL, "bad metric");
+    }
+    CASPHY_HIP(hipGetLastError());
+}
+
+// CASPHY_VALU_VARIANT: force a variant by name (k32x8, k16x8, k32x4, k16x4
+// for the bit metrics; k24x8, k48x8, k48x4 for the 2-bit DNA form); a name
+// the metric does not have leaves its launches to the autotuner.
+int valu_variant_from_env(int metric)
+{
+    const char *e = std::getenv("CASPHY_VALU_VARIANT");
+    if (!e) return -1;
+    for (int v = 0; v < kNumValu; ++v)
+        if (valu_shape(metric, v).kc > 0 && std::strcmp(e, valu_shape(metric, v).name) == 0) return v;
+    return -1;
+}
+
+// Autotuner: the first launch of each shape class (metric, epilogue,
+// triangle, log2 of both row counts, words per row, weight planes) runs every
+// candidate twice on the real data -- the mask, score, local-maximum and
+// transfer-distance epilogues are idempotent (stores, atomic max / min of
+// identical values) -- times the second run with events and keeps the
+// fastest.  tryRadius counts (E_HIST, atomic adds) and launches under stream
+// capture take the default.
+struct ValuKey {
+    int metric, epi, tri;
+    u32 la, lb, kw, planes;
+    bool operator<(const ValuKey &o) const
+    {
+        return std::tie(metric, epi, tri, la, lb, kw, planes) <
+               std::tie(o.metric, o.epi, o.tri, o.la, o.lb, o.kw, o.planes);
+    }
+};
+struct ValuEntry {
+    int variant;
+    float ms[kNumValu];
+};
+std::mutex g_valu_mu;
+std::map<ValuKey, ValuEntry> g_valu;
+
+u32 log2_ceil_u(u32 x)
+{
+    u32 l = 0;
+    while ((1u << l) < x && l < 31) ++l;
+    return l;
 }
 
 }  // namespace
@@ -323,17 +460,89 @@ void launch_tiles(int metric, int epi, const TileArgs &a, hipStream_t s)
     if (a.ntiles == 0) return;
     CASPHY_REQUIRE(a.kw > 0 && a.kw % stage_words(metric) == 0 && a.kw <= a.ldw, CASPHY_EINTERNAL,
                    "tile row width not a multiple of the stage width");
-    switch (metric) {
-    case M_AND: launch_m<M_AND>(epi, a, s); break;
-    case M_XOR:
-        CASPHY_REQUIRE(epi == E_TBE, CASPHY_EINTERNAL, "XOR metric only feeds the TBE epilogue");
-        hipLaunchKernelGGL((k_tiles<M_XOR, E_TBE>), dim3(8 * ((a.ntiles + 7) / 8)), dim3(NTHR), 0, s, a);
-        break;
-    case M_DNA2: launch_m<M_DNA2>(epi, a, s); break;
-    case M_NIB: launch_m<M_NIB>(epi, a, s); break;
-    default: throw Error(CASPHY_EINTERNAL, "bad metric");
+    CASPHY_REQUIRE(metric != M_XOR || epi == E_TBE, CASPHY_EINTERNAL, "XOR metric only feeds the TBE epilogue");
+    const int forced = valu_variant_from_env(metric);
+    if (forced >= 0 && valu_supported(metric, forced, a)) return launch_variant(forced, metric, epi, a, s);
+    const char *at = std::getenv("CASPHY_AUTOTUNE");
+    if (epi == E_HIST || (at && at[0] == '0')) return launch_variant(0, metric, epi, a, s);
+    const ValuKey key{metric, epi, a.triangle, log2_ceil_u(a.nA), log2_ceil_u(a.nB), a.kw, a.nplanes};
+    {
+        std::lock_guard<std::mutex> lk(g_valu_mu);
+        auto it = g_valu.find(key);
+        if (it != g_valu.end()) return launch_variant(it->second.variant, metric, epi, a, s);
     }
-    CASPHY_HIP(hipGetLastError());
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    CASPHY_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return launch_variant(0, metric, epi, a, s);
+    // a radius mask whose threshold lives in device memory may score nothing
+    // (INT32_MAX): the candidates are timed with a host threshold, then the
+    // real launch follows with the choice
+    TileArgs t = a;
+    const bool relaunch = epi == E_MASK && a.thr_dev;
+    if (relaunch) {
+        t.thr_dev = nullptr;
+        t.thr = 1;
+    }
+    ValuEntry e{};
+    for (float &m : e.ms) m = -1.0f;
+    e.variant = 0;
+    hipEvent_t ev[2];
+    CASPHY_HIP(hipEventCreate(&ev[0]));
+    CASPHY_HIP(hipEventCreate(&ev[1]));
+    float best = 1e30f;
+    try {
+        for (int v = 0; v < kNumValu; ++v) {
+            if (!valu_supported(metric, v, a)) continue;
+            launch_variant(v, metric, epi, t, s);  // untimed: first-use costs
+            CASPHY_HIP(hipEventRecord(ev[0], s));
+            launch_variant(v, metric, epi, t, s);
+            CASPHY_HIP(hipEventRecord(ev[1], s));
+            CASPHY_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            CASPHY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            e.ms[v] = ms;
+            if (ms < best) {
+                best = ms;
+                e.variant = v;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (relaunch) launch_variant(e.variant, metric, epi, a, s);
+    std::lock_guard<std::mutex> lk(g_valu_mu);
+    g_valu.emplace(key, e);
+}
+
+std::string valu_tuning_json()
+{
+    std::lock_guard<std::mutex> lk(g_valu_mu);
+    std::string out = "[";
+    bool first = true;
+    for (const auto &kv : g_valu) {
+        const ValuKey &k = kv.first;
+        const ValuEntry &e = kv.second;
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "%s{\"form\": \"valu\", \"metric\": %d, \"epilogue\": %d, \"triangle\": %d, \"log2_rows_a\": %u, "
+                      "\"log2_rows_b\": %u, \"words\": %u, \"planes\": %u, ",
+                      first ? "" : ", ", k.metric, k.epi, k.tri, k.la, k.lb, k.kw, k.planes);
+        out += buf;
+        out += std::string("\"chosen\": \"") + valu_shape(k.metric, e.variant).name + "\", \"ms\": {";
+        bool f2 = true;
+        for (int v = 0; v < kNumValu; ++v) {
+            if (e.ms[v] < 0) continue;
+            std::snprintf(buf, sizeof buf, "%s\"%s\": %.6f", f2 ? "" : ", ", valu_shape(k.metric, v).name, e.ms[v]);
+            out += buf;
+            f2 = false;
+        }
+        out += "}}";
+        first = false;
+    }
+    return out + "]";
 }
 
 }  // namespace casphy

@@ -13,6 +13,8 @@
 // shifted between planes (Horner), so the sum stays exact in int32.
 #pragma once
 
+#include <string>
+
 #include "common.hpp"
 
 namespace casphy {
@@ -57,8 +59,14 @@ struct TileArgs {
     u32 *amin = nullptr, *bmin = nullptr;
 };
 
-// Launches the tile kernel for (metric, epilogue) on `stream`.
+// Launches the tile kernel for (metric, epilogue) on `stream`; the tile shape
+// and stage width come from the autotuner (or CASPHY_VALU_VARIANT).  The
+// caller's tile grid (ntA, ntB, ntiles) is in 128 x 128 tiles.
 void launch_tiles(int metric, int epi, const TileArgs &a, hipStream_t stream);
+
+// JSON list of the VALU tile autotuner's choices so far (one entry per shape
+// class, with the time of every candidate).
+std::string valu_tuning_json();
 
 // Stage width (words) of a metric: rows must be padded to a multiple of it.
 inline u32 stage_words(int metric) { return metric == M_DNA2 ? KC_DNA2 : KC_BIT; }

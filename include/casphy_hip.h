@@ -52,8 +52,9 @@ void casphy_free(void *p);
 int casphy_set_device(int device);
 /* Blocks until this thread's stream is idle. */
 int casphy_synchronize(void);
-/* JSON list of the MFMA tile variants the autotuner chose so far (one entry
- * per shape class, with the time of every candidate).  Writes at most cap
+/* JSON list of the tile variants the autotuners chose so far (one entry per
+ * shape class, with the time of every candidate): the matrix-core tiles, then
+ * the VALU tiles (entries with "form": "valu").  Writes at most cap
  * bytes including the terminating NUL and stores the full length in *len.
  * No counterpart in the reference, which has one kernel shape per path. */
 int casphy_mfma_tuning(char *buf, size_t cap, size_t *len);

@@ -17,13 +17,24 @@ def rand_bits(rng, n, W, p):
 
 # ------------------------------------------------------------------ scores
 @pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-fp4", "mfma-fp4b", "mfma-fp4d", "mfma-fp4w", "mfma-fp4x",
-                        "mfma-fp4y", "mfma-wide", "mfma-wide8", "mfma-reg", "valu"])
+                        "mfma-fp4y", "mfma-wide", "mfma-wide8", "mfma-reg", "valu", "valu-k16x8", "valu-k32x4",
+                        "valu-k16x4"])
 def bitpath(request, monkeypatch):
     """Runs a test on the autotuned int8 matrix-core path (every candidate
     variant runs on the first call of a shape class), on the fixed variants
     (LDS-staged bytes with 128 x 128 or 256 x 256 workgroup tiles, fp4
-    nibbles, register-resident bits) and on the VALU popcount tiles."""
-    monkeypatch.setenv("CASPHY_MFMA", "0" if request.param == "valu" else "1")
+    nibbles, register-resident bits), on the autotuned VALU popcount tiles and
+    on each VALU tile shape forced (kKCxCB: KC words staged per step, CB
+    columns per thread)."""
+    valu = request.param.startswith("valu")
+    monkeypatch.setenv("CASPHY_MFMA", "0" if valu else "1")
+    if valu:
+        if request.param == "valu":
+            monkeypatch.delenv("CASPHY_VALU_VARIANT", raising=False)
+            monkeypatch.delenv("CASPHY_AUTOTUNE", raising=False)
+        else:
+            monkeypatch.setenv("CASPHY_VALU_VARIANT", request.param[5:])
+        return request.param
     if request.param == "mfma-auto":
         monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
         monkeypatch.delenv("CASPHY_AUTOTUNE", raising=False)
@@ -144,14 +155,19 @@ def test_radius_bit_synthetic_config(hip, orc, bitpath):
         assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
-@pytest.fixture(params=["auto", "mfma", "valu"])
+@pytest.fixture(params=["auto", "mfma", "valu", "valu-k48x8", "valu-k48x4"])
 def dnaform(request, monkeypatch):
     """ACGT data on the one-hot matrix-core form, on the 2-bit + mask VALU
-    form, or on the form the resident cells' tuner picks."""
+    form (autotuned, or with the 48-word stage in 128 x 128 or 128 x 64 tiles
+    forced), or on the form the resident cells' tuner picks."""
     monkeypatch.setenv("CASPHY_MFMA", "1")
     monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
+    monkeypatch.delenv("CASPHY_VALU_VARIANT", raising=False)
     if request.param == "auto":
         monkeypatch.delenv("CASPHY_DNA_FORM", raising=False)
+    elif request.param.startswith("valu-"):
+        monkeypatch.setenv("CASPHY_DNA_FORM", "valu")
+        monkeypatch.setenv("CASPHY_VALU_VARIANT", request.param[5:])
     else:
         monkeypatch.setenv("CASPHY_DNA_FORM", request.param)
     return request.param
