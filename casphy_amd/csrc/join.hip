@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "context.hpp"
@@ -437,6 +438,375 @@ __global__ void k_indel_decode(const u32 *rows, u32 count, u32 ldw, const u32 *l
 
 
 // ---------------------------------------------------------------------------
+// One-launch join step (single tree, the default form).  Every workgroup of a
+// step repeats the selection from the previous step's per-workgroup results,
+// so no workgroup waits for another and no device-scope fence is needed inside
+// a launch: the per-slot state a step reads (parity p) and the state it writes
+// (parity 1 - p) are different buffers, a new node's row goes to a fresh row
+// of `nstate` (tip rows are never written), and the few node -> slot changes
+// of a step travel in the step's control block: workgroup 0 of the next step
+// writes them into node2slot while every reader overrides node2slot with them.
+// Each workgroup also leaves, for its best slot, the partner node and that
+// node's slot, so the next selection needs no dependent lookups.
+// Semantics are those of k_select / k_rescore (and of the CPU twins): the
+// maximal (score, smallest node id) pair, stale partners recomputed over every
+// live slot, strict '>' on rescoring, live slots kept compact (the last live
+// slot moves into the second child's slot).
+constexpr u32 FS_INIT = 0, FS_JOIN = 1, FS_RECOMPUTE = 2;
+constexpr u32 FS_LPS = 8;             // lanes per slot
+constexpr u32 FS_SPP = 256 / FS_LPS;  // slots per workgroup pass
+
+struct FCtl {
+    u32 nodeCount, nlive;
+    u32 mode;      // what the writing step did (FS_*)
+    u32 c, cnode;  // FS_RECOMPUTE: the slot recomputed and its node
+    u32 done, err, steps;
+    u32 nupd;      // node -> slot changes of the writing step
+    u32 upd_node[4], upd_slot[4];
+};
+
+struct FMeta {  // a workgroup's best slot, its partner node and that node's slot
+    u32 slot, j2, c2, pad;
+};
+
+struct FArgs {
+    const u32 *rows;  // tip rows: n x (2 planes x ldw)
+    u32 *nstate;      // internal node rows: (n - 1) x (2 planes x ldw)
+    int32_t *lmax[2];
+    u32 *lidx[2], *s2n[2];
+    unsigned long long *key[2];   // per workgroup: best (lmax + 1, ~node) over its slots
+    FMeta *meta[2];
+    unsigned long long *rkey[2];  // recompute: per workgroup best (score, ~node)
+    u32 *rslot[2];                // ... and that node's slot
+    u32 *node2slot, *size, *edge;
+    FCtl *ctl[2];
+    u32 n, ldw, nwg;
+    int rule;
+    u32 min_size;
+    float min_ratio;
+    u32 nplanes;
+    const u32 *pmask;
+    u32 pshift[MAX_PLANES];
+    u32 ptail;
+};
+
+__device__ __forceinline__ const u32 *frow(const FArgs &a, u32 node)
+{
+    return node < a.n ? a.rows + (size_t)node * 2 * a.ldw : a.nstate + (size_t)(node - a.n) * 2 * a.ldw;
+}
+
+// Sum over each aligned group of 8 lanes, in every lane of the group (DPP:
+// quad swaps, then the half-row mirror joins the two quads).
+__device__ __forceinline__ u32 sum8(u32 v)
+{
+    v += __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return v;
+}
+
+// Sum over each aligned group of LPS lanes (8 .. 64), in every lane of it.
+template <u32 LPS>
+__device__ __forceinline__ u32 sum_lanes(u32 v)
+{
+    v = sum8(v);
+    if constexpr (LPS >= 16) v += __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    if constexpr (LPS >= 32) v += __shfl_xor(v, 16);
+    if constexpr (LPS >= 64) v += __shfl_xor(v, 32);
+    return v;
+}
+
+// Score of the LDS row q against the global row r (plane 0, weight planes) by
+// the LPS lanes of a slot (lane l reads words 4 l + 4 LPS i); every lane of
+// the group returns the total.
+template <u32 LPS>
+__device__ __forceinline__ u32 fscore(const FArgs &a, const u32 *q, const u32 *r, u32 l)
+{
+    const u32 np = a.nplanes ? a.nplanes : 1;
+    u32 S = 0;
+    for (u32 p = 0; p < np; ++p) {
+        if (p) S <<= a.pshift[p];
+        u32 part = 0;
+#pragma unroll 4
+        for (u32 w = 4 * l; w < a.ldw; w += 4 * LPS) {
+            uint4 x = *reinterpret_cast<const uint4 *>(q + w);
+            const uint4 y = *reinterpret_cast<const uint4 *>(r + w);
+            if (a.nplanes) {
+                const uint4 m = *reinterpret_cast<const uint4 *>(a.pmask + (size_t)p * a.ldw + w);
+                x.x &= m.x; x.y &= m.y; x.z &= m.z; x.w &= m.w;
+            }
+            part += __builtin_popcount(x.x & y.x) + __builtin_popcount(x.y & y.y) +
+                    __builtin_popcount(x.z & y.z) + __builtin_popcount(x.w & y.w);
+        }
+        S += sum_lanes<LPS>(part);
+    }
+    if (a.nplanes) S <<= a.ptail;
+    return S;
+}
+
+// Maximum of a u64 over the wave (every lane active): a prefix maximum by DPP
+// row shifts and row broadcasts (zero filled), read from lane 63.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k)
+{
+#define CASPHY_MAX_STEP(ctrl, rmask)                                                                 \
+    {                                                                                                \
+        const u32 lo_ = __builtin_amdgcn_update_dpp(0u, (u32)k, ctrl, rmask, 0xF, false);            \
+        const u32 hi_ = __builtin_amdgcn_update_dpp(0u, (u32)(k >> 32), ctrl, rmask, 0xF, false);    \
+        const unsigned long long o_ = ((unsigned long long)hi_ << 32) | lo_;                         \
+        k = o_ > k ? o_ : k;                                                                         \
+    }
+    CASPHY_MAX_STEP(0x111, 0xF)  // row_shr:1
+    CASPHY_MAX_STEP(0x112, 0xF)  // row_shr:2
+    CASPHY_MAX_STEP(0x114, 0xF)  // row_shr:4
+    CASPHY_MAX_STEP(0x118, 0xF)  // row_shr:8
+    CASPHY_MAX_STEP(0x142, 0xA)  // row_bcast:15 -> rows 1, 3
+    CASPHY_MAX_STEP(0x143, 0xC)  // row_bcast:31 -> rows 2, 3
+#undef CASPHY_MAX_STEP
+    // readlane returns int: through u32, so the low word is not sign-extended
+    return ((unsigned long long)(u32)__builtin_amdgcn_readlane((u32)(k >> 32), 63) << 32) |
+           (unsigned long long)(u32)__builtin_amdgcn_readlane((u32)k, 63);
+}
+
+// Workgroup (256 threads) argmax of a u64 key carrying three u32 payloads; the
+// result lands in every thread.  Keys are distinct or carry equal payloads.
+struct FRed {
+    unsigned long long k[4];
+    u32 x[4], y[4], z[4];
+};
+__device__ __forceinline__ void fmax3(unsigned long long &k, u32 &x, u32 &y, u32 &z, FRed &r)
+{
+    const unsigned long long wm = wave_max_u64(k);
+    const u32 src = (u32)__builtin_ctzll(__ballot(k == wm));
+    const u32 wx = __builtin_amdgcn_readlane(x, src), wy = __builtin_amdgcn_readlane(y, src),
+              wz = __builtin_amdgcn_readlane(z, src);
+    __syncthreads();  // the previous use of r is over
+    if ((threadIdx.x & 63) == 0) {
+        const u32 w = threadIdx.x >> 6;
+        r.k[w] = wm; r.x[w] = wx; r.y[w] = wy; r.z[w] = wz;
+    }
+    __syncthreads();
+    k = r.k[0]; x = r.x[0]; y = r.y[0]; z = r.z[0];
+    for (int w = 1; w < 4; ++w)
+        if (r.k[w] > k) { k = r.k[w]; x = r.x[w]; y = r.y[w]; z = r.z[w]; }
+}
+
+// Initial per-workgroup keys and partners (state in parity 0, written by
+// k_join_init) and the first control block.
+__global__ __launch_bounds__(256) void k_fstep_init(FArgs a)
+{
+    __shared__ FRed red;
+    const u32 tid = threadIdx.x, n = a.n;
+    unsigned long long bk = 0;
+    u32 bs = n, bj = 0xFFFFFFFFu, bc = n;
+    for (u32 base = blockIdx.x * FS_SPP; base < n; base += a.nwg * FS_SPP) {
+        const u32 k = base + (tid / FS_LPS);
+        if (k < n && tid % FS_LPS == 0) {
+            const unsigned long long key = ((unsigned long long)(u32)(a.lmax[0][k] + 1) << 32) |
+                                           (0xFFFFFFFFu - a.s2n[0][k]);
+            if (key > bk) {
+                bk = key; bs = k; bj = a.lidx[0][k];
+                bc = bj < 2 * n - 1 ? a.node2slot[bj] : n;
+            }
+        }
+    }
+    fmax3(bk, bs, bj, bc, red);
+    if (tid == 0) {
+        a.key[0][blockIdx.x] = bk;
+        a.meta[0][blockIdx.x] = FMeta{bs, bj, bc, 0};
+        if (blockIdx.x == 0) {
+            FCtl c{};
+            c.nodeCount = n; c.nlive = n; c.mode = FS_INIT; c.c = n; c.cnode = 0;
+            c.done = n < 2 ? 1u : 0u;
+            *a.ctl[0] = c;
+        }
+    }
+}
+
+// One step; LPS lanes score a slot.
+template <u32 LPS>
+__global__ __launch_bounds__(256) void k_fstep(FArgs a, u32 p)
+{
+    extern __shared__ __attribute__((aligned(16))) u32 fq[];  // the step's query row: plane 0 (and 1 for a join)
+    __shared__ FRed red;
+    constexpr u32 SPP = 256 / LPS;  // slots per workgroup pass
+    const u32 tid = threadIdx.x, R = p, W = 1 - p;
+    const u32 n = a.n, ldw = a.ldw, NONE = 0xFFFFFFFFu, nnode = 2 * n - 1;
+    const u32 l = tid % LPS;
+    // ---- selection, the same in every workgroup: each workgroup's best key
+    // comes with its slot, partner and partner slot
+    unsigned long long best = 0xFFFFFFFFull;  // value -1, node 0: no pair
+    u32 c1 = n, j2 = NONE, c2 = n;
+    for (u32 b = tid; b < a.nwg; b += 256) {
+        const unsigned long long k = a.key[R][b];
+        const FMeta m = a.meta[R][b];
+        if (k > best) { best = k; c1 = m.slot; j2 = m.j2; c2 = m.c2; }
+    }
+    const FCtl C = *a.ctl[R];
+    if (C.done) {
+        if (blockIdx.x == 0 && tid == 0) *a.ctl[W] = C;
+        return;
+    }
+    fmax3(best, c1, j2, c2, red);
+    const u32 fslot = C.mode == FS_RECOMPUTE ? C.c : n;  // the slot whose maximum the last step recomputed
+    unsigned long long fk = 0;
+    u32 fks = n, d0 = 0, d1 = 0;
+    if (fslot < n) {
+        for (u32 b = tid; b < a.nwg; b += 256) {
+            const unsigned long long k = a.rkey[R][b];
+            const u32 rs = a.rslot[R][b];
+            if (k > fk) { fk = k; fks = rs; }
+        }
+        fmax3(fk, fks, d0, d1, red);
+    }
+    const int32_t fv = fk ? (int32_t)(u32)(fk >> 32) : -1;
+    const u32 fj = fk ? 0xFFFFFFFFu - (u32)fk : NONE;
+    if (fslot < n) {
+        const unsigned long long kf = ((unsigned long long)(u32)(fv + 1) << 32) | (0xFFFFFFFFu - C.cnode);
+        if (kf > best) {
+            best = kf;
+            c1 = fslot; j2 = fj; c2 = fk ? fks : n;
+        }
+    }
+    const int32_t v = (int32_t)(u32)(best >> 32) - 1;
+    const u32 j1 = 0xFFFFFFFFu - (u32)best;
+    const u32 verdict = (v < 0 || c1 >= n) ? 0u : (c2 < n ? FS_JOIN : FS_RECOMPUTE);
+    if (verdict == 0) {  // no live pair: cannot happen with >= 2 live nodes
+        if (blockIdx.x == 0 && tid == 0) {
+            FCtl c = C;
+            c.done = 1; c.err = 1; c.nupd = 0;
+            *a.ctl[W] = c;
+        }
+        return;
+    }
+    const int32_t *lmR = a.lmax[R];
+    const u32 *liR = a.lidx[R], *snR = a.s2n[R];
+    int32_t *lmW = a.lmax[W];
+    u32 *liW = a.lidx[W], *snW = a.s2n[W];
+    const u32 last = C.nlive - 1;
+    unsigned long long bk = 0, rbk = 0;
+    u32 bs = n, bj = NONE, bc = n, rbs = n;
+    // slot of a partner node once the changes of this step (cn/cs) and of the
+    // previous one (C) are in, else node2slot
+    auto slot_of = [&](u32 node, const u32 *cn, const u32 *cs, u32 ncur) {
+        if (node >= nnode) return n;
+        for (u32 i = 0; i < ncur; ++i)
+            if (cn[i] == node) return cs[i];
+        for (u32 i = 0; i < C.nupd; ++i)
+            if (C.upd_node[i] == node) return C.upd_slot[i];
+        return a.node2slot[node];
+    };
+    if (verdict == FS_JOIN) {
+        const u32 nc = C.nodeCount, nl = last;
+        const u32 s_nc = c1 == last ? c2 : c1;
+        const u32 mv = (c2 != last && c1 != last) ? snR[last] : NONE;  // node moving into slot c2
+        u32 cn[4] = {j1, j2, nc, mv}, cs[4] = {n, n, s_nc, c2};
+        const u32 ncur = mv != NONE ? 4u : 3u;
+        // merged row of the new node (cu/joinEdgeIndel.cu:407-413 size rule)
+        u32 choice = 0;
+        if (a.rule == 2) {
+            const u32 sa = a.size[j1], sb = a.size[j2];
+            if (sa < a.min_size && (float)sa * a.min_ratio < (float)sb) choice = 1;
+            else if (sb < a.min_size && (float)sb * a.min_ratio < (float)sa) choice = 2;
+        }
+        const u32 *ra = frow(a, j1), *rb = frow(a, j2);
+        for (u32 w = tid; w < ldw; w += 256) {
+            const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
+            u32 r0, r1;
+            if (choice == 1) { r0 = b0; r1 = b1; }
+            else if (choice == 2) { r0 = a0; r1 = a1; }
+            else merge_words(a.rule, a0, a1, b0, b1, r0, r1);
+            fq[w] = r0;
+            fq[ldw + w] = r1;
+        }
+        __syncthreads();
+        if (blockIdx.x == 0) {
+            u32 *out = a.nstate + (size_t)(nc - n) * 2 * ldw;
+            for (u32 w = tid; w < 2 * ldw; w += 256) out[w] = fq[w];
+            if (tid == 0) {
+                const u32 e = nc - n;
+                a.edge[4 * e + 0] = nc; a.edge[4 * e + 1] = j1;
+                a.edge[4 * e + 2] = nc; a.edge[4 * e + 3] = j2;
+                a.size[nc] = a.size[j1] + a.size[j2];
+                for (u32 i = 0; i < C.nupd; ++i) a.node2slot[C.upd_node[i]] = C.upd_slot[i];
+                FCtl c{};
+                c.nodeCount = nc + 1; c.nlive = nl; c.mode = FS_JOIN; c.c = n; c.cnode = 0;
+                c.done = nc + 1 == 2 * n - 1 ? 1u : 0u; c.err = 0; c.steps = C.steps + 1; c.nupd = ncur;
+                for (u32 i = 0; i < 4; ++i) { c.upd_node[i] = cn[i]; c.upd_slot[i] = cs[i]; }
+                *a.ctl[W] = c;
+            }
+        }
+        // the new node against every live slot; the state moves to parity W
+        for (u32 base = blockIdx.x * SPP; base < nl; base += a.nwg * SPP) {
+            const u32 k = base + tid / LPS;
+            if (k >= nl) continue;  // the whole slot group
+            const u32 o = (k == c2 && c2 != last) ? last : k;  // old slot of the node now in k
+            u32 node, li;
+            int32_t lm;
+            if (o == c1) {
+                node = nc; lm = -1; li = NONE;
+            } else {
+                node = snR[o];
+                if (o == fslot) { lm = fv; li = fj; }
+                else { lm = lmR[o]; li = liR[o]; }
+                const u32 S = fscore<LPS>(a, fq, frow(a, node), l);
+                if ((int32_t)S > lm) { lm = (int32_t)S; li = nc; }
+            }
+            if (l == 0) { lmW[k] = lm; liW[k] = li; snW[k] = node; }
+            const unsigned long long key = ((unsigned long long)(u32)(lm + 1) << 32) | (0xFFFFFFFFu - node);
+            if (key > bk) {
+                bk = key; bs = k; bj = li;
+                bc = slot_of(li, cn, cs, ncur);
+            }
+        }
+        fmax3(bk, bs, bj, bc, red);
+        if (tid == 0) {
+            a.key[W][blockIdx.x] = bk;
+            a.meta[W][blockIdx.x] = FMeta{bs, bj, bc, 0};
+        }
+        return;
+    }
+    // ---- recompute: the maximum of slot c1 (node j1) over every other live slot
+    const u32 *rq = frow(a, j1);
+    for (u32 w = tid; w < ldw; w += 256) fq[w] = rq[w];
+    __syncthreads();
+    if (blockIdx.x == 0 && tid == 0) {
+        for (u32 i = 0; i < C.nupd; ++i) a.node2slot[C.upd_node[i]] = C.upd_slot[i];
+        FCtl c{};
+        c.nodeCount = C.nodeCount; c.nlive = C.nlive; c.mode = FS_RECOMPUTE; c.c = c1; c.cnode = j1;
+        c.done = 0; c.err = 0; c.steps = C.steps + 1; c.nupd = 0;
+        *a.ctl[W] = c;
+    }
+    for (u32 base = blockIdx.x * SPP; base < C.nlive; base += a.nwg * SPP) {
+        const u32 k = base + tid / LPS;
+        if (k >= C.nlive) continue;
+        const u32 node = snR[k];
+        int32_t lm;
+        u32 li;
+        if (k == fslot) { lm = fv; li = fj; }
+        else { lm = lmR[k]; li = liR[k]; }
+        if (l == 0) { lmW[k] = lm; liW[k] = li; snW[k] = node; }
+        if (k == c1) continue;
+        const u32 S = fscore<LPS>(a, fq, frow(a, node), l);
+        const unsigned long long r = ((unsigned long long)S << 32) | (0xFFFFFFFFu - node);
+        if (r > rbk) { rbk = r; rbs = k; }
+        const unsigned long long key = ((unsigned long long)(u32)(lm + 1) << 32) | (0xFFFFFFFFu - node);
+        if (key > bk) {
+            bk = key; bs = k; bj = li;
+            bc = slot_of(li, nullptr, nullptr, 0);
+        }
+    }
+    u32 e0 = 0, e1 = 0;
+    fmax3(rbk, rbs, e0, e1, red);
+    fmax3(bk, bs, bj, bc, red);
+    if (tid == 0) {
+        a.rkey[W][blockIdx.x] = rbk;
+        a.rslot[W][blockIdx.x] = rbs;
+        a.key[W][blockIdx.x] = bk;
+        a.meta[W][blockIdx.x] = FMeta{bs, bj, bc, 0};
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched form: one persistent workgroup per tree, jobs taken from an atomic
 // work queue, the whole join (initial local maxima, argmax, stale recompute,
 // ancestral merge, rescoring) inside the workgroup with LDS broadcasts and
@@ -719,6 +1089,91 @@ void tree_alloc(Tree &t, u32 n, u32 ldw, int rule)
     t.part.alloc((size_t)ceil_div(n, 16) * 8);
 }
 
+// The join loop in one launch per step (k_fstep), after k_join_init has put
+// the initial maxima into parity 0 of the per-slot state.
+void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hipStream_t s)
+{
+    const u32 n = t.n, ldw = t.ldw;
+    const char *wenv = std::getenv("CASPHY_JOIN_WG");
+    const u32 cap = wenv ? std::max(1, std::atoi(wenv)) : 1024;
+    // 8 lanes per slot, 32 slots per workgroup pass: measured faster than 16,
+    // 32 or 64 lanes per slot for every row width (more slots in flight beat
+    // fewer serial 16-byte loads per lane)
+    const u32 lps = 8;
+    const u32 nwg = std::min<u32>(ceil_div(n, 256 / lps), cap);
+    DevBuf lm1((size_t)n * 4), li1((size_t)n * 4), sn1((size_t)n * 4), keys(2 * (size_t)nwg * 8),
+        meta(2 * (size_t)nwg * sizeof(FMeta)), rkeys(2 * (size_t)nwg * 8), rslots(2 * (size_t)nwg * 4),
+        ctl(2 * sizeof(FCtl));
+    FArgs f{};
+    f.rows = t.rows.as<u32>(); f.nstate = t.nstate.as<u32>();
+    f.lmax[0] = t.lmax.as<int32_t>(); f.lmax[1] = lm1.as<int32_t>();
+    f.lidx[0] = t.lidx.as<u32>(); f.lidx[1] = li1.as<u32>();
+    f.s2n[0] = t.s2n.as<u32>(); f.s2n[1] = sn1.as<u32>();
+    for (int q = 0; q < 2; ++q) {
+        f.key[q] = keys.as<unsigned long long>() + (size_t)q * nwg;
+        f.meta[q] = reinterpret_cast<FMeta *>(meta.p) + (size_t)q * nwg;
+        f.rkey[q] = rkeys.as<unsigned long long>() + (size_t)q * nwg;
+        f.rslot[q] = rslots.as<u32>() + (size_t)q * nwg;
+        f.ctl[q] = reinterpret_cast<FCtl *>(ctl.p) + q;
+    }
+    f.node2slot = t.n2s.as<u32>(); f.size = t.size.as<u32>(); f.edge = t.edge.as<u32>();
+    f.n = n; f.ldw = ldw; f.nwg = nwg;
+    f.rule = t.rule; f.min_size = min_size; f.min_ratio = min_ratio;
+    f.nplanes = planes.nplanes;
+    f.pmask = t.args.pmask;
+    for (u32 q = 0; q < planes.nplanes; ++q) f.pshift[q] = planes.shift[q];
+    f.ptail = planes.tail;
+    hipLaunchKernelGGL(k_fstep_init, dim3(nwg), dim3(256), 0, s, f);
+    CASPHY_HIP(hipGetLastError());
+    const u32 CH = 64;  // steps per captured graph: even, so every chunk starts on parity 0
+    const size_t lds = (size_t)2 * ldw * 4;
+    auto step = [&](u32 par) {
+        hipLaunchKernelGGL((k_fstep<8>), dim3(nwg), dim3(256), lds, s, f, par);
+    };
+    auto enqueue_chunk = [&]() {
+        for (u32 i = 0; i < CH; ++i) step(i & 1u);
+    };
+    const char *genv = std::getenv("CASPHY_JOIN_GRAPH");
+    const bool use_graph = !(genv && genv[0] == '0');
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph) {
+        CASPHY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_chunk();
+        CASPHY_HIP(hipStreamEndCapture(s, &graph));
+        CASPHY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    FCtl h{};
+    const u64 max_steps = 64ull * n + 4096;
+    u64 launched = 0;
+    try {
+        for (;;) {
+            if (use_graph) CASPHY_HIP(hipGraphLaunch(exec, s));
+            else enqueue_chunk();
+            CASPHY_HIP(hipGetLastError());
+            launched += CH;
+            // poll every few chunks (the chunk's last step wrote parity 0); the
+            // steps after completion only carry the control block over
+            if ((launched / CH) % 4 == 0 || launched >= n) {
+                d2h(&h, f.ctl[0], sizeof(FCtl), s);
+                if (h.done) break;
+                CASPHY_REQUIRE(launched < max_steps, CASPHY_EINTERNAL, "join did not converge");
+            }
+        }
+    } catch (...) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    if (exec) CASPHY_HIP(hipGraphExecDestroy(exec));
+    if (graph) CASPHY_HIP(hipGraphDestroy(graph));
+    CASPHY_REQUIRE(!h.err, CASPHY_EINTERNAL, "join found no live pair");
+    const char *st = std::getenv("CASPHY_JOIN_STATS");
+    if (st && st[0] == '1')
+        std::fprintf(stderr, "join stats: n=%u selections=%u launched=%llu (one launch per step, %u workgroups)\n", n,
+                     h.steps, (unsigned long long)launched, nwg);
+}
+
 // Initial local maxima (all-pairs upper triangle with the LOCALMAX epilogue)
 // and the join loop.
 void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size, float min_ratio,
@@ -787,6 +1242,17 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     // than the second launch on this part, so it is not the default.
     const char *fenv = std::getenv("CASPHY_JOIN_FUSED");
     const bool fused = fenv && fenv[0] == '1';
+    // One launch per step (k_fstep) unless CASPHY_JOIN_STEP=two asks for the
+    // rescoring + selection pair.  Measured per join on MI355X (ms, one-launch
+    // vs two-launch step): binary rows of 128 words 11.5 / 12.7 at 1,024 cells,
+    // 575 / 654 at 30,000, 3,163 / 3,291 at 100,000; one-hot indel rows of 384
+    // words 216 / 228 at 10,000 but 1,208 / 1,085 at 30,000 and 7,481 / 6,502
+    // at 100,000, where the 2,048 rescoring workgroups of the two-launch step
+    // keep more rows in flight.  So wide rows of large joins take two launches.
+    const char *senv = std::getenv("CASPHY_JOIN_STEP");
+    const bool force_one = senv && !std::strcmp(senv, "one");
+    const bool two = (senv && !std::strcmp(senv, "two")) || fused || (!force_one && ldw > 256 && n > 16384);
+    if (!two && (size_t)2 * ldw * 4 <= 64 * 1024) return fstep_run(t, planes, min_size, min_ratio, s);
     hipLaunchKernelGGL(k_select<false>, dim3(1), dim3(1024), 0, s, a);
     CASPHY_HIP(hipGetLastError());
     auto enqueue_chunk = [&]() {

@@ -347,16 +347,25 @@ def test_join_bit(hip, orc, bitpath, n, W, p, anc):
     assert np.array_equal(ns, ns2) and np.array_equal(nb, nb2)
 
 
-@pytest.fixture(params=["fused", "two-kernel", "fused-looped", "two-kernel-looped"])
+@pytest.fixture(params=["fstep", "fused", "two-kernel", "fstep-looped", "fused-looped", "two-kernel-looped"])
 def joinstep(request, monkeypatch):
-    """Single-tree join with one launch per step or select + rescore launches;
-    "looped": 5 rescoring workgroups, so each one loops over several groups
-    of live slots (the form joins of more than 32,768 cells take)."""
-    monkeypatch.setenv("CASPHY_JOIN_FUSED", "1" if request.param.startswith("fused") else "0")
+    """Single-tree join in its three step forms: one launch per step that
+    repeats the selection in every workgroup (the default, "fstep"), one
+    launch whose last workgroup selects ("fused"), or rescore + select
+    launches ("two-kernel"); "looped": 5 workgroups, so each one loops over
+    several groups of live slots (the form large joins take)."""
+    form = request.param.replace("-looped", "")
+    monkeypatch.setenv("CASPHY_JOIN_FUSED", "1" if form == "fused" else "0")
+    if form == "two-kernel":
+        monkeypatch.setenv("CASPHY_JOIN_STEP", "two")
+    else:
+        monkeypatch.delenv("CASPHY_JOIN_STEP", raising=False)
     if request.param.endswith("looped"):
         monkeypatch.setenv("CASPHY_JOIN_RESCORE_WG", "5")
+        monkeypatch.setenv("CASPHY_JOIN_WG", "5")
     else:
         monkeypatch.delenv("CASPHY_JOIN_RESCORE_WG", raising=False)
+        monkeypatch.delenv("CASPHY_JOIN_WG", raising=False)
     return request.param
 
 
