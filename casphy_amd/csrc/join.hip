@@ -469,9 +469,18 @@ struct FMeta {  // a workgroup's best slot, its partner node and that node's slo
     u32 slot, j2, c2, pad;
 };
 
+// Row formats of the one-launch step: FMT_PLANES, two bit planes of ldw words
+// (binary: state and bistate; indel: one-hot alleles and missing flags);
+// FMT_CODES (indel, unit weights), one byte per site of ldw words: 0 unmutated,
+// 1..254 the allele, 255 missing.  A node's indel state holds at most one
+// allele per site (a tip one; a merge keeps the common allele or takes the
+// other child's where one is missing), so the byte is the one-hot group.
+constexpr int FMT_PLANES = 0, FMT_CODES = 1;
+constexpr u32 CODE_MISSING = 0xFFu;
+
 struct FArgs {
-    const u32 *rows;  // tip rows: n x (2 planes x ldw)
-    u32 *nstate;      // internal node rows: (n - 1) x (2 planes x ldw)
+    const u32 *rows;  // tip rows: n x rstride words
+    u32 *nstate;      // internal node rows: (n - 1) x rstride words
     int32_t *lmax[2];
     u32 *lidx[2], *s2n[2];
     unsigned long long *key[2];   // per workgroup: best (lmax + 1, ~node) over its slots
@@ -481,6 +490,7 @@ struct FArgs {
     u32 *node2slot, *size, *edge;
     FCtl *ctl[2];
     u32 n, ldw, nwg;
+    u32 rstride;  // words per row: 2 ldw (planes) or ldw (codes)
     int rule;
     u32 min_size;
     float min_ratio;
@@ -492,7 +502,30 @@ struct FArgs {
 
 __device__ __forceinline__ const u32 *frow(const FArgs &a, u32 node)
 {
-    return node < a.n ? a.rows + (size_t)node * 2 * a.ldw : a.nstate + (size_t)(node - a.n) * 2 * a.ldw;
+    return node < a.n ? a.rows + (size_t)node * a.rstride : a.nstate + (size_t)(node - a.n) * a.rstride;
+}
+
+// Per byte of a word: 0x80 where the byte is zero, else 0 (no carries across bytes).
+__device__ __forceinline__ u32 zero_bytes(u32 t)
+{
+    return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t | 0x7F7F7F7Fu);
+}
+
+// Sites of a code word where a and b hold the same allele (neither
+// unmutated nor missing).
+__device__ __forceinline__ u32 code_matches(u32 x, u32 y)
+{
+    return (u32)__builtin_popcount(zero_bytes(x ^ y) & ~zero_bytes(x) & ~zero_bytes(~x));
+}
+
+// Merge of two code words (the indel rule of src/functions.cpp:1965-1974 on
+// one-hot groups): a missing site takes the other child's state, two present
+// sites keep their common allele or become unmutated.
+__device__ __forceinline__ u32 code_merge(u32 x, u32 y)
+{
+    const u32 mx = (zero_bytes(~x) >> 7) * 0xFFu, my = (zero_bytes(~y) >> 7) * 0xFFu;
+    const u32 eq = (zero_bytes(x ^ y) >> 7) * 0xFFu;
+    return (mx & y) | (~mx & my & x) | (~mx & ~my & x & eq);
 }
 
 // Sum over each aligned group of 8 lanes, in every lane of the group (DPP:
@@ -519,9 +552,20 @@ __device__ __forceinline__ u32 sum_lanes(u32 v)
 // Score of the LDS row q against the global row r (plane 0, weight planes) by
 // the LPS lanes of a slot (lane l reads words 4 l + 4 LPS i); every lane of
 // the group returns the total.
-template <u32 LPS>
+template <u32 LPS, int FMT>
 __device__ __forceinline__ u32 fscore(const FArgs &a, const u32 *q, const u32 *r, u32 l)
 {
+    if constexpr (FMT == FMT_CODES) {
+        u32 part = 0;
+#pragma unroll 4
+        for (u32 w = 4 * l; w < a.ldw; w += 4 * LPS) {
+            const uint4 x = *reinterpret_cast<const uint4 *>(q + w);
+            const uint4 y = *reinterpret_cast<const uint4 *>(r + w);
+            part += code_matches(x.x, y.x) + code_matches(x.y, y.y) + code_matches(x.z, y.z) +
+                    code_matches(x.w, y.w);
+        }
+        return sum_lanes<LPS>(part);
+    }
     const u32 np = a.nplanes ? a.nplanes : 1;
     u32 S = 0;
     for (u32 p = 0; p < np; ++p) {
@@ -623,7 +667,7 @@ __global__ __launch_bounds__(256) void k_fstep_init(FArgs a)
 }
 
 // One step; LPS lanes score a slot.
-template <u32 LPS>
+template <u32 LPS, int FMT>
 __global__ __launch_bounds__(256) void k_fstep(FArgs a, u32 p)
 {
     extern __shared__ __attribute__((aligned(16))) u32 fq[];  // the step's query row: plane 0 (and 1 for a join)
@@ -709,19 +753,26 @@ __global__ __launch_bounds__(256) void k_fstep(FArgs a, u32 p)
             else if (sb < a.min_size && (float)sb * a.min_ratio < (float)sa) choice = 2;
         }
         const u32 *ra = frow(a, j1), *rb = frow(a, j2);
-        for (u32 w = tid; w < ldw; w += 256) {
-            const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
-            u32 r0, r1;
-            if (choice == 1) { r0 = b0; r1 = b1; }
-            else if (choice == 2) { r0 = a0; r1 = a1; }
-            else merge_words(a.rule, a0, a1, b0, b1, r0, r1);
-            fq[w] = r0;
-            fq[ldw + w] = r1;
+        if constexpr (FMT == FMT_CODES) {
+            for (u32 w = tid; w < ldw; w += 256) {
+                const u32 x = ra[w], y = rb[w];
+                fq[w] = choice == 1 ? y : choice == 2 ? x : code_merge(x, y);
+            }
+        } else {
+            for (u32 w = tid; w < ldw; w += 256) {
+                const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
+                u32 r0, r1;
+                if (choice == 1) { r0 = b0; r1 = b1; }
+                else if (choice == 2) { r0 = a0; r1 = a1; }
+                else merge_words(a.rule, a0, a1, b0, b1, r0, r1);
+                fq[w] = r0;
+                fq[ldw + w] = r1;
+            }
         }
         __syncthreads();
         if (blockIdx.x == 0) {
-            u32 *out = a.nstate + (size_t)(nc - n) * 2 * ldw;
-            for (u32 w = tid; w < 2 * ldw; w += 256) out[w] = fq[w];
+            u32 *out = a.nstate + (size_t)(nc - n) * a.rstride;
+            for (u32 w = tid; w < a.rstride; w += 256) out[w] = fq[w];
             if (tid == 0) {
                 const u32 e = nc - n;
                 a.edge[4 * e + 0] = nc; a.edge[4 * e + 1] = j1;
@@ -748,7 +799,7 @@ __global__ __launch_bounds__(256) void k_fstep(FArgs a, u32 p)
                 node = snR[o];
                 if (o == fslot) { lm = fv; li = fj; }
                 else { lm = lmR[o]; li = liR[o]; }
-                const u32 S = fscore<LPS>(a, fq, frow(a, node), l);
+                const u32 S = fscore<LPS, FMT>(a, fq, frow(a, node), l);
                 if ((int32_t)S > lm) { lm = (int32_t)S; li = nc; }
             }
             if (l == 0) { lmW[k] = lm; liW[k] = li; snW[k] = node; }
@@ -786,7 +837,7 @@ __global__ __launch_bounds__(256) void k_fstep(FArgs a, u32 p)
         else { lm = lmR[k]; li = liR[k]; }
         if (l == 0) { lmW[k] = lm; liW[k] = li; snW[k] = node; }
         if (k == c1) continue;
-        const u32 S = fscore<LPS>(a, fq, frow(a, node), l);
+        const u32 S = fscore<LPS, FMT>(a, fq, frow(a, node), l);
         const unsigned long long r = ((unsigned long long)S << 32) | (0xFFFFFFFFu - node);
         if (r > rbk) { rbk = r; rbs = k; }
         const unsigned long long key = ((unsigned long long)(u32)(lm + 1) << 32) | (0xFFFFFFFFu - node);
@@ -1065,10 +1116,47 @@ Planes indel_planes(const IndelLayout &L, const double *weight, u32 nstate, u32 
     return make_planes_bits(wb, L.ldw);
 }
 
+// Indel states -> byte codes (0 unmutated, 1..254 allele, 255 missing), 4
+// sites a word, rows of cw words (zero past ncol).
+__global__ void k_indel_codes(const int32_t *mat, u32 n, u32 ncol, u32 *out, u32 cw)
+{
+    const u64 total = (u64)n * cw;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 r = (u32)(x / cw), w = (u32)(x - (u64)r * cw);
+        u32 word = 0;
+        for (u32 b = 0; b < 4; ++b) {
+            const u32 s = 4 * w + b;
+            if (s < ncol) {
+                const int32_t v = mat[(size_t)r * ncol + s];
+                const u32 c = v > 0 ? (u32)v : v < 0 ? CODE_MISSING : 0u;
+                word |= c << (8 * b);
+            }
+        }
+        out[x] = word;
+    }
+}
+
+// Byte codes of `count` rows back to states (-1 missing).
+__global__ void k_codes_decode(const u32 *codes, u32 count, u32 cw, u32 ncol, int32_t *out)
+{
+    const u64 total = (u64)count * ncol;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u32 r = (u32)(x / ncol), s = (u32)(x - (u64)r * ncol);
+        const u32 c = (codes[(size_t)r * cw + s / 4] >> (8 * (s % 4))) & 0xFFu;
+        out[x] = c == CODE_MISSING ? -1 : (int32_t)c;
+    }
+}
+
 struct Tree {
     u32 n = 0, ldw = 0;
     int rule = 1;
     DevBuf rows, lmax, lidx, s2n, n2s, size, edge, nstate, ctl, keys, pmask, part;
+    // indel joins on byte codes (FMT_CODES): the source states, their width,
+    // and the code rows of the tips and of the internal nodes
+    const int32_t *code_src = nullptr;
+    u32 ncol = 0, cw = 0;
+    bool codes = false;
+    DevBuf ctips, cnodes;
     JoinArgs args{};
 };
 
@@ -1093,7 +1181,16 @@ void tree_alloc(Tree &t, u32 n, u32 ldw, int rule)
 // the initial maxima into parity 0 of the per-slot state.
 void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hipStream_t s)
 {
-    const u32 n = t.n, ldw = t.ldw;
+    const u32 n = t.n;
+    if (t.codes) {  // indel rows as byte codes: one word of 4 sites, 16-byte multiples
+        t.cw = round_up(ceil_div(t.ncol, 4), 4u);
+        t.ctips.alloc((size_t)n * t.cw * 4);
+        t.cnodes.alloc((size_t)std::max(n - 1, 1u) * t.cw * 4);
+        hipLaunchKernelGGL(k_indel_codes, dim3(grid_for((u64)n * t.cw)), dim3(256), 0, s, t.code_src, n, t.ncol,
+                           t.ctips.as<u32>(), t.cw);
+        CASPHY_HIP(hipGetLastError());
+    }
+    const u32 ldw = t.codes ? t.cw : t.ldw;
     const char *wenv = std::getenv("CASPHY_JOIN_WG");
     const u32 cap = wenv ? std::max(1, std::atoi(wenv)) : 1024;
     // 8 lanes per slot, 32 slots per workgroup pass: measured faster than 16,
@@ -1105,7 +1202,11 @@ void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hip
         meta(2 * (size_t)nwg * sizeof(FMeta)), rkeys(2 * (size_t)nwg * 8), rslots(2 * (size_t)nwg * 4),
         ctl(2 * sizeof(FCtl));
     FArgs f{};
-    f.rows = t.rows.as<u32>(); f.nstate = t.nstate.as<u32>();
+    if (t.codes) {
+        f.rows = t.ctips.as<u32>(); f.nstate = t.cnodes.as<u32>(); f.rstride = ldw;
+    } else {
+        f.rows = t.rows.as<u32>(); f.nstate = t.nstate.as<u32>(); f.rstride = 2 * ldw;
+    }
     f.lmax[0] = t.lmax.as<int32_t>(); f.lmax[1] = lm1.as<int32_t>();
     f.lidx[0] = t.lidx.as<u32>(); f.lidx[1] = li1.as<u32>();
     f.s2n[0] = t.s2n.as<u32>(); f.s2n[1] = sn1.as<u32>();
@@ -1126,9 +1227,10 @@ void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hip
     hipLaunchKernelGGL(k_fstep_init, dim3(nwg), dim3(256), 0, s, f);
     CASPHY_HIP(hipGetLastError());
     const u32 CH = 64;  // steps per captured graph: even, so every chunk starts on parity 0
-    const size_t lds = (size_t)2 * ldw * 4;
+    const size_t lds = (size_t)f.rstride * 4;
     auto step = [&](u32 par) {
-        hipLaunchKernelGGL((k_fstep<8>), dim3(nwg), dim3(256), lds, s, f, par);
+        if (t.codes) hipLaunchKernelGGL((k_fstep<8, FMT_CODES>), dim3(nwg), dim3(256), lds, s, f, par);
+        else hipLaunchKernelGGL((k_fstep<8, FMT_PLANES>), dim3(nwg), dim3(256), lds, s, f, par);
     };
     auto enqueue_chunk = [&]() {
         for (u32 i = 0; i < CH; ++i) step(i & 1u);
@@ -1170,8 +1272,8 @@ void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hip
     CASPHY_REQUIRE(!h.err, CASPHY_EINTERNAL, "join found no live pair");
     const char *st = std::getenv("CASPHY_JOIN_STATS");
     if (st && st[0] == '1')
-        std::fprintf(stderr, "join stats: n=%u selections=%u launched=%llu (one launch per step, %u workgroups)\n", n,
-                     h.steps, (unsigned long long)launched, nwg);
+        std::fprintf(stderr, "join stats: n=%u selections=%u launched=%llu (one launch per step, %u workgroups%s)\n",
+                     n, h.steps, (unsigned long long)launched, nwg, t.codes ? ", byte codes" : "");
 }
 
 // Initial local maxima (all-pairs upper triangle with the LOCALMAX epilogue)
@@ -1251,8 +1353,16 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
     // keep more rows in flight.  So wide rows of large joins take two launches.
     const char *senv = std::getenv("CASPHY_JOIN_STEP");
     const bool force_one = senv && !std::strcmp(senv, "one");
-    const bool two = (senv && !std::strcmp(senv, "two")) || fused || (!force_one && ldw > 256 && n > 16384);
-    if (!two && (size_t)2 * ldw * 4 <= 64 * 1024) return fstep_run(t, planes, min_size, min_ratio, s);
+    // Indel joins of unit weights whose alleles fit a byte take byte codes
+    // (t.code_src set by the caller): rows of 4 sites a word instead of the
+    // one-hot planes, so the step reads a fraction of the bytes (200 sites:
+    // 52 words against 352-608 one-hot words a plane).
+    // (CASPHY_JOIN_CODES=0 keeps the one-hot planes)
+    const char *cenv = std::getenv("CASPHY_JOIN_CODES");
+    t.codes = t.code_src && t.rule == 2 && planes.nplanes == 0 && !(cenv && cenv[0] == '0') &&
+              !(senv && !std::strcmp(senv, "two")) && !fused;
+    const bool two = !t.codes && ((senv && !std::strcmp(senv, "two")) || fused || (!force_one && ldw > 256 && n > 16384));
+    if (t.codes || (!two && (size_t)2 * ldw * 4 <= 64 * 1024)) return fstep_run(t, planes, min_size, min_ratio, s);
     hipLaunchKernelGGL(k_select<false>, dim3(1), dim3(1024), 0, s, a);
     CASPHY_HIP(hipGetLastError());
     auto enqueue_chunk = [&]() {
@@ -1358,11 +1468,20 @@ void join_bit_dev(const u32 *raw, u32 n, u32 W, int ancestral_type, u32 *edge_ho
     d2h(edge_host, t.edge.p, (size_t)(2 * n - 2) * 2 * 4, s);
 }
 
+// Byte codes need every allele below the missing code.
+bool codes_fit(const IndelLayout &L)
+{
+    for (int32_t k : L.kmax)
+        if (k >= (int32_t)CODE_MISSING) return false;
+    return true;
+}
+
 void join_indel_dev(const int32_t *raw, u32 n, u32 ncol, u32 *edge_host, hipStream_t s)
 {
     IndelLayout L = indel_layout(raw, n, ncol, s);
     Tree t;
     tree_alloc(t, n, L.ldw, 2);
+    if (codes_fit(L)) { t.code_src = raw; t.ncol = ncol; }
     launch_fill_u32(t.rows.as<u32>(), (u64)n * 2 * L.ldw, 0, s);
     hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)n * ncol)), dim3(256), 0, s, raw, n, ncol,
                        L.dloci.as<u32>(), L.dwidth.as<u32>(), t.rows.as<u32>(), L.ldw);
@@ -1449,8 +1568,16 @@ int casphy_join_indel(const int32_t *mat, uint32_t n, uint32_t ncol, const doubl
         h2d(dmat.p, mat, (size_t)n * ncol * 4, s);
         IndelLayout L = indel_layout(dmat.as<int32_t>(), n, ncol, s);
         Planes planes = indel_planes(L, weight, nstate, ncol);
+        // all-one weights (what R passes by default) are unit weights
+        bool unit = weight == nullptr;
+        if (weight) {
+            unit = true;
+            for (u64 i = 0; i < (u64)nstate * ncol && unit; ++i) unit = weight[i] == 1.0;
+        }
+        if (unit) planes = Planes{};
         Tree t;
         tree_alloc(t, n, L.ldw, 2);
+        if (codes_fit(L)) { t.code_src = dmat.as<int32_t>(); t.ncol = ncol; }
         launch_fill_u32(t.rows.as<u32>(), (u64)n * 2 * L.ldw, 0, s);
         hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)n * ncol)), dim3(256), 0, s, dmat.as<int32_t>(), n,
                            ncol, L.dloci.as<u32>(), L.dwidth.as<u32>(), t.rows.as<u32>(), L.ldw);
@@ -1464,9 +1591,13 @@ int casphy_join_indel(const int32_t *mat, uint32_t n, uint32_t ncol, const doubl
         d2h(edge, t.edge.p, (size_t)(2 * n - 2) * 2 * 4, s);
         if (node_state || branch) {
             DevBuf dec((size_t)(n - 1) * ncol * 4);
-            hipLaunchKernelGGL(k_indel_decode, dim3(grid_for((u64)(n - 1) * ncol)), dim3(256), 0, s,
-                               t.nstate.as<u32>(), n - 1, L.ldw, L.dloci.as<u32>(), L.dwidth.as<u32>(), ncol,
-                               dec.as<int32_t>());
+            if (t.codes)
+                hipLaunchKernelGGL(k_codes_decode, dim3(grid_for((u64)(n - 1) * ncol)), dim3(256), 0, s,
+                                   t.cnodes.as<u32>(), n - 1, t.cw, ncol, dec.as<int32_t>());
+            else
+                hipLaunchKernelGGL(k_indel_decode, dim3(grid_for((u64)(n - 1) * ncol)), dim3(256), 0, s,
+                                   t.nstate.as<u32>(), n - 1, L.ldw, L.dloci.as<u32>(), L.dwidth.as<u32>(), ncol,
+                                   dec.as<int32_t>());
             CASPHY_HIP(hipGetLastError());
             std::vector<int32_t> ns((size_t)(2 * n - 1) * ncol);
             std::copy(mat, mat + (size_t)n * ncol, ns.begin());

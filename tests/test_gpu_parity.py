@@ -347,14 +347,17 @@ def test_join_bit(hip, orc, bitpath, n, W, p, anc):
     assert np.array_equal(ns, ns2) and np.array_equal(nb, nb2)
 
 
-@pytest.fixture(params=["fstep", "fused", "two-kernel", "fstep-looped", "fused-looped", "two-kernel-looped"])
+@pytest.fixture(params=["fstep", "fstep-onehot", "fused", "two-kernel", "fstep-looped", "fstep-onehot-looped",
+                        "fused-looped", "two-kernel-looped"])
 def joinstep(request, monkeypatch):
     """Single-tree join in its three step forms: one launch per step that
-    repeats the selection in every workgroup (the default, "fstep"), one
-    launch whose last workgroup selects ("fused"), or rescore + select
-    launches ("two-kernel"); "looped": 5 workgroups, so each one loops over
-    several groups of live slots (the form large joins take)."""
-    form = request.param.replace("-looped", "")
+    repeats the selection in every workgroup (the default, "fstep"; indel rows
+    as byte codes, or as the one-hot planes with "onehot"), one launch whose
+    last workgroup selects ("fused"), or rescore + select launches
+    ("two-kernel"); "looped": 5 workgroups, so each one loops over several
+    groups of live slots (the form large joins take)."""
+    monkeypatch.setenv("CASPHY_JOIN_CODES", "0" if "onehot" in request.param else "1")
+    form = request.param.replace("-looped", "").replace("-onehot", "")
     monkeypatch.setenv("CASPHY_JOIN_FUSED", "1" if form == "fused" else "0")
     if form == "two-kernel":
         monkeypatch.setenv("CASPHY_JOIN_STEP", "two")
@@ -394,6 +397,25 @@ def test_join_indel(hip, orc, bitpath, min_size, ratio):
     e, ns, br = hip.join_indel(mat, w, ts, min_size, ratio)
     e2, ns2, br2 = orc.join_indel(mat, w.astype(np.float32), ts, min_size, ratio)
     assert np.array_equal(e, e2) and np.array_equal(ns, ns2) and np.array_equal(br, br2)
+
+
+@pytest.mark.parametrize("kmax,ncol", [(3, 37), (254, 41), (300, 33)])
+def test_join_indel_codes_edges(hip, orc, kmax, ncol):
+    """Indel joins on byte codes: many missing states, alleles up to 254 (the
+    largest a code holds; 300 falls back to the one-hot planes), site counts
+    that are not a multiple of 4; unit and all-one weights (what R passes),
+    and node states decoded from the codes, against the oracle."""
+    rng = np.random.default_rng(kmax + ncol)
+    n = 300
+    mat = rng.integers(1, kmax + 1, (n, ncol)).astype(np.int32)
+    mat[rng.random((n, ncol)) < 0.45] = 0
+    mat[rng.random((n, ncol)) < 0.3] = -1
+    mat[:, 0] = kmax  # the largest allele occurs
+    w = np.ones((kmax, ncol), np.float32)
+    for weight in (None, w.astype(np.float64)):
+        e, ns, br = hip.join_indel(mat, weight)
+        e2, ns2, br2 = orc.join_indel(mat, w)
+        assert np.array_equal(e, e2) and np.array_equal(ns, ns2) and np.array_equal(br, br2)
 
 
 def test_join_indel_config1(hip, orc, joinstep):
