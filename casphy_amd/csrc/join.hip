@@ -528,21 +528,14 @@ __device__ __forceinline__ u32 code_merge(u32 x, u32 y)
     return (mx & y) | (~mx & my & x) | (~mx & ~my & x & eq);
 }
 
-// Sum over each aligned group of 8 lanes, in every lane of the group (DPP:
-// quad swaps, then the half-row mirror joins the two quads).
-__device__ __forceinline__ u32 sum8(u32 v)
-{
-    v += __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-    v += __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
-    return v;
-}
-
-// Sum over each aligned group of LPS lanes (8 .. 64), in every lane of it.
+// Sum over each aligned group of LPS lanes (4 .. 64), in every lane of it:
+// quad swaps, the half-row mirror, a row rotation, then cross-row shuffles.
 template <u32 LPS>
 __device__ __forceinline__ u32 sum_lanes(u32 v)
 {
-    v = sum8(v);
+    v += __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);                            // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);                            // quad_perm [2,3,0,1]
+    if constexpr (LPS >= 8) v += __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, false);   // row_half_mirror
     if constexpr (LPS >= 16) v += __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xF, 0xF, false);  // row_ror:8
     if constexpr (LPS >= 32) v += __shfl_xor(v, 16);
     if constexpr (LPS >= 64) v += __shfl_xor(v, 32);
@@ -871,6 +864,7 @@ struct BatchArgs {
     const u32 *offsets;
     const u32 *tip_size;  // sum n_k or nullptr
     u32 njobs, ldw;
+    u32 rstride;          // words per row: 2 ldw (planes) or ldw (byte codes)
     int rule;
     u32 min_size;
     float min_ratio;
@@ -882,39 +876,63 @@ struct BatchArgs {
     u32 ptail;
 };
 
-// Score of rows q and r (ldw words each, plane 0), by 16 cooperating lanes;
-// every lane of the group returns the total.
-__device__ __forceinline__ u32 group16_score(const BatchArgs &b, const u32 *q, const u32 *r, u32 l16)
+// Scores of row q against NR rows r[] (ldw words each, plane 0), by LPS
+// cooperating lanes; every lane of the group gets the totals.  A lane loads
+// its (up to) four 16-byte chunks of every row before it counts, so 4 NR loads
+// are in flight (rows of more than 16 LPS words take several rounds).
+template <int FMT, u32 LPS, u32 NR>
+__device__ __forceinline__ void group_scores(const BatchArgs &b, const u32 *q, const u32 *const (&r)[NR], u32 l,
+                                             u32 (&S)[NR])
 {
-    const u32 np = b.nplanes ? b.nplanes : 1;
-    u32 S = 0;
+    const u32 np = FMT == FMT_CODES ? 1u : (b.nplanes ? b.nplanes : 1);
+#pragma unroll
+    for (u32 k = 0; k < NR; ++k) S[k] = 0;
     for (u32 p = 0; p < np; ++p) {
-        if (p) S <<= b.pshift[p];
-        u32 part = 0;
-        for (u32 w = 4 * l16; w < b.ldw; w += 64) {
-            uint4 x = *reinterpret_cast<const uint4 *>(q + w);
-            const uint4 y = *reinterpret_cast<const uint4 *>(r + w);
-            if (b.nplanes) {
-                const uint4 m = *reinterpret_cast<const uint4 *>(b.pmask + (size_t)p * b.ldw + w);
-                x.x &= m.x; x.y &= m.y; x.z &= m.z; x.w &= m.w;
+        u32 part[NR];
+#pragma unroll
+        for (u32 k = 0; k < NR; ++k) part[k] = 0;
+        for (u32 w0 = 4 * l; w0 < b.ldw; w0 += 16 * LPS) {
+            uint4 x[4], y[NR][4];
+#pragma unroll
+            for (u32 i = 0; i < 4; ++i) {
+                const u32 w = w0 + 4 * LPS * i;
+                const bool in = w < b.ldw;
+                x[i] = in ? *reinterpret_cast<const uint4 *>(q + w) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (u32 k = 0; k < NR; ++k)
+                    y[k][i] = in ? *reinterpret_cast<const uint4 *>(r[k] + w) : make_uint4(0, 0, 0, 0);
+                if (FMT == FMT_PLANES && b.nplanes && in) {
+                    const uint4 m = *reinterpret_cast<const uint4 *>(b.pmask + (size_t)p * b.ldw + w);
+                    x[i].x &= m.x; x[i].y &= m.y; x[i].z &= m.z; x[i].w &= m.w;
+                }
             }
-            part += __builtin_popcount(x.x & y.x) + __builtin_popcount(x.y & y.y) +
-                    __builtin_popcount(x.z & y.z) + __builtin_popcount(x.w & y.w);
+#pragma unroll
+            for (u32 i = 0; i < 4; ++i)
+#pragma unroll
+                for (u32 k = 0; k < NR; ++k) {
+                    if constexpr (FMT == FMT_CODES)
+                        part[k] += code_matches(x[i].x, y[k][i].x) + code_matches(x[i].y, y[k][i].y) +
+                                   code_matches(x[i].z, y[k][i].z) + code_matches(x[i].w, y[k][i].w);
+                    else
+                        part[k] += __builtin_popcount(x[i].x & y[k][i].x) + __builtin_popcount(x[i].y & y[k][i].y) +
+                                   __builtin_popcount(x[i].z & y[k][i].z) + __builtin_popcount(x[i].w & y[k][i].w);
+                }
         }
-        for (int o = 1; o < 16; o <<= 1) part += __shfl_xor(part, o);
-        S += part;
+#pragma unroll
+        for (u32 k = 0; k < NR; ++k) {
+            if (FMT == FMT_PLANES && b.nplanes && p) S[k] <<= b.pshift[p];
+            S[k] += sum_lanes<LPS>(part[k]);
+        }
     }
-    if (b.nplanes) S <<= b.ptail;
-    return S;
+    if (FMT == FMT_PLANES && b.nplanes)
+#pragma unroll
+        for (u32 k = 0; k < NR; ++k) S[k] <<= b.ptail;
 }
 
 // Workgroup-wide max of a u64 key (1024 threads).
 __device__ __forceinline__ unsigned long long wg_max(unsigned long long k, unsigned long long *red)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long x = __shfl_xor(k, o);
-        k = x > k ? x : k;
-    }
+    k = wave_max_u64(k);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = k;
     __syncthreads();
@@ -923,11 +941,15 @@ __device__ __forceinline__ unsigned long long wg_max(unsigned long long k, unsig
     return m;
 }
 
+// LPS lanes score a pair of rows (G = 1024 / LPS groups); the row loops are
+// unrolled so that every group keeps several rows' loads in flight.
+template <int FMT, u32 LPS>
 __global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
 {
     __shared__ unsigned long long red[16];
     __shared__ u32 sh[8];
-    const u32 tid = threadIdx.x, l16 = tid & 15, grp = tid >> 4;  // 64 groups of 16 lanes
+    constexpr u32 G = 1024 / LPS;
+    const u32 tid = threadIdx.x, l = tid % LPS, grp = tid / LPS;
     const u32 ldw = b.ldw;
     for (;;) {
         __syncthreads();
@@ -936,13 +958,14 @@ __global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
         const u32 job = sh[0];
         if (job >= b.njobs) return;
         const u32 off = b.offsets[job], n = b.offsets[job + 1] - off, nnode = 2 * n - 1;
-        u32 *rows = b.rows + (size_t)off * 2 * ldw;
+        const u32 rs = b.rstride;
+        u32 *rows = b.rows + (size_t)off * rs;
         int32_t *lmax = b.lmax + off;
         u32 *lidx = b.lidx + off, *s2n = b.slot2node + off;
         u32 *n2s = b.node2slot + (2 * (size_t)off - job);
         u32 *size = b.size + (2 * (size_t)off - job);
         u32 *edge = b.edge + 2 * (2 * (size_t)off - 2 * (size_t)job);
-        u32 *nst = b.nstate + ((size_t)off - job) * 2 * ldw;
+        u32 *nst = b.nstate + ((size_t)off - job) * rs;
 
         // maps and sizes
         for (u32 i = tid; i < nnode; i += 1024) {
@@ -954,16 +977,21 @@ __global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
                 n2s[i] = n;
             }
         }
-        // initial local maxima: group g scores rows i = g, g+64, ... against all j > i
-        for (u32 i = grp; i < n; i += 64) {
+        // initial local maxima: group g scores rows i = g, g+G, ... against all j > i
+        for (u32 i = grp; i < n; i += G) {
             unsigned long long best = 0;
-            const u32 *qi = rows + (size_t)i * 2 * ldw;
-            for (u32 j = i + 1; j < n; ++j) {
-                const u32 S = group16_score(b, qi, rows + (size_t)j * 2 * ldw, l16);
-                const unsigned long long key = ((unsigned long long)S << 32) | (0xFFFFFFFFu - j);
-                best = key > best ? key : best;
+            const u32 *qi = rows + (size_t)i * rs;
+            for (u32 j = i + 1; j < n; j += 2) {  // two rows at a time (the second may be past n)
+                const u32 j1 = j + 1 < n ? j + 1 : j;
+                const u32 *rr[2] = {rows + (size_t)j * rs, rows + (size_t)j1 * rs};
+                u32 S[2];
+                group_scores<FMT, LPS, 2>(b, qi, rr, l, S);
+                const unsigned long long k0 = ((unsigned long long)S[0] << 32) | (0xFFFFFFFFu - j);
+                const unsigned long long k1 = ((unsigned long long)S[1] << 32) | (0xFFFFFFFFu - j1);
+                best = k0 > best ? k0 : best;
+                best = k1 > best ? k1 : best;
             }
-            if (l16 == 0) {
+            if (l == 0) {
                 lmax[i] = best ? (int32_t)(best >> 32) : -1;
                 lidx[i] = best ? 0xFFFFFFFFu - (u32)best : 0xFFFFFFFFu;
             }
@@ -995,16 +1023,22 @@ __global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
             __syncthreads();
             const u32 verdict = sh[1], c1 = sh[2], c2 = sh[3], j1 = sh[4], j2 = sh[5];
             if (verdict == 0) { status = 2; break; }
-            const u32 *q = rows + (size_t)c1 * 2 * ldw;
+            const u32 *q = rows + (size_t)c1 * rs;
             if (verdict == 1) {
                 // recompute row c1 over every live slot
                 unsigned long long kb = 0;
-                for (u32 k = grp; k < n; k += 64) {
-                    const u32 id = s2n[k];
-                    if (k == c1 || id >= nnode) continue;
-                    const u32 S = group16_score(b, q, rows + (size_t)k * 2 * ldw, l16);
-                    const unsigned long long key = ((unsigned long long)S << 32) | (0xFFFFFFFFu - id);
-                    kb = key > kb ? key : kb;
+                for (u32 k = grp; k < n; k += 2 * G) {  // slots k and k + G together
+                    const u32 k1 = k + G;
+                    const u32 id0 = s2n[k], id1 = k1 < n ? s2n[k1] : nnode;
+                    const bool v0 = k != c1 && id0 < nnode, v1 = k1 != c1 && id1 < nnode;
+                    if (!v0 && !v1) continue;
+                    const u32 *rr[2] = {rows + (size_t)(v0 ? k : k1) * rs, rows + (size_t)(v1 ? k1 : k) * rs};
+                    u32 S[2];
+                    group_scores<FMT, LPS, 2>(b, q, rr, l, S);
+                    const unsigned long long k0 = v0 ? ((unsigned long long)S[0] << 32) | (0xFFFFFFFFu - id0) : 0;
+                    const unsigned long long kk = v1 ? ((unsigned long long)S[1] << 32) | (0xFFFFFFFFu - id1) : 0;
+                    kb = k0 > kb ? k0 : kb;
+                    kb = kk > kb ? kk : kb;
                 }
                 kb = wg_max(kb, red);
                 if (tid == 0) {
@@ -1021,17 +1055,24 @@ __global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
                 if (sa < b.min_size && (float)sa * b.min_ratio < (float)sb) choice = 1;
                 else if (sb < b.min_size && (float)sb * b.min_ratio < (float)sa) choice = 2;
             }
-            u32 *ra = rows + (size_t)c1 * 2 * ldw;
-            const u32 *rb = rows + (size_t)c2 * 2 * ldw;
-            u32 *out = nst + (size_t)(nodeCount - n) * 2 * ldw;
+            u32 *ra = rows + (size_t)c1 * rs;
+            const u32 *rb = rows + (size_t)c2 * rs;
+            u32 *out = nst + (size_t)(nodeCount - n) * rs;
             for (u32 w = tid; w < ldw; w += 1024) {
-                const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
-                u32 r0, r1;
-                if (choice == 1) { r0 = b0; r1 = b1; }
-                else if (choice == 2) { r0 = a0; r1 = a1; }
-                else merge_words(b.rule, a0, a1, b0, b1, r0, r1);
-                ra[w] = r0; ra[ldw + w] = r1;
-                out[w] = r0; out[ldw + w] = r1;
+                if constexpr (FMT == FMT_CODES) {
+                    const u32 x = ra[w], y = rb[w];
+                    const u32 r = choice == 1 ? y : choice == 2 ? x : code_merge(x, y);
+                    ra[w] = r;
+                    out[w] = r;
+                } else {
+                    const u32 a0 = ra[w], a1 = ra[ldw + w], b0 = rb[w], b1 = rb[ldw + w];
+                    u32 r0, r1;
+                    if (choice == 1) { r0 = b0; r1 = b1; }
+                    else if (choice == 2) { r0 = a0; r1 = a1; }
+                    else merge_words(b.rule, a0, a1, b0, b1, r0, r1);
+                    ra[w] = r0; ra[ldw + w] = r1;
+                    out[w] = r0; out[ldw + w] = r1;
+                }
             }
             if (tid == 0) {
                 const u32 k = nodeCount - n;
@@ -1044,12 +1085,20 @@ __global__ __launch_bounds__(1024) void k_join_batch(BatchArgs b)
             }
             __syncthreads();
             // rescore the new node against every live slot (strict '>')
-            for (u32 k = grp; k < n; k += 64) {
-                if (k == c1 || s2n[k] >= nnode) continue;
-                const u32 S = group16_score(b, q, rows + (size_t)k * 2 * ldw, l16);
-                if (l16 == 0 && (int32_t)S > lmax[k]) {
-                    lmax[k] = (int32_t)S;
+            for (u32 k = grp; k < n; k += 2 * G) {  // slots k and k + G together
+                const u32 k1 = k + G;
+                const bool v0 = k != c1 && s2n[k] < nnode, v1 = k1 < n && k1 != c1 && s2n[k1] < nnode;
+                if (!v0 && !v1) continue;
+                const u32 *rr[2] = {rows + (size_t)(v0 ? k : k1) * rs, rows + (size_t)(v1 ? k1 : k) * rs};
+                u32 S[2];
+                group_scores<FMT, LPS, 2>(b, q, rr, l, S);
+                if (l == 0 && v0 && (int32_t)S[0] > lmax[k]) {
+                    lmax[k] = (int32_t)S[0];
                     lidx[k] = nodeCount;
+                }
+                if (l == 0 && v1 && (int32_t)S[1] > lmax[k1]) {
+                    lmax[k1] = (int32_t)S[1];
+                    lidx[k1] = nodeCount;
                 }
             }
             ++nodeCount;
@@ -1413,14 +1462,15 @@ void tree_run(Tree &t, const Planes &planes, const u32 *dtip_size, u32 min_size,
 
 
 // Runs the persistent batch kernel on encoded rows (sum n_k slots, 2 planes).
+// codes: the rows are indel byte codes of ldw words (FMT_CODES), else two planes.
 void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, const Planes &planes,
-               const u32 *dtip, u32 min_size, float min_ratio, u32 *edge_host, hipStream_t s)
+               const u32 *dtip, u32 min_size, float min_ratio, u32 *edge_host, hipStream_t s, bool codes = false)
 {
     const u32 njobs = (u32)offs.size() - 1;
     const u64 total = offs.back();
     const u64 nodes = 2 * total - njobs, edges = 2 * total - 2 * (u64)njobs, internal = total - njobs;
     DevBuf lmax(total * 4), lidx(total * 4), s2n(total * 4), n2s(nodes * 4), size(nodes * 4),
-        dedge(std::max<u64>(edges, 1) * 8), nst(std::max<u64>(internal, 1) * 2 * ldw * 4),
+        dedge(std::max<u64>(edges, 1) * 8), nst(std::max<u64>(internal, 1) * (codes ? 1 : 2) * ldw * 4),
         doff(offs.size() * 4), ctr(4), status((size_t)njobs * 4), pm;
     h2d(doff.p, offs.data(), offs.size() * 4, s);
     launch_fill_u32(ctr.as<u32>(), 1, 0, s);
@@ -1429,7 +1479,7 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
     b.rows = rows.as<u32>(); b.lmax = lmax.as<int32_t>(); b.lidx = lidx.as<u32>();
     b.slot2node = s2n.as<u32>(); b.node2slot = n2s.as<u32>(); b.size = size.as<u32>();
     b.edge = dedge.as<u32>(); b.nstate = nst.as<u32>(); b.offsets = doff.as<u32>();
-    b.tip_size = dtip; b.njobs = njobs; b.ldw = ldw; b.rule = rule;
+    b.tip_size = dtip; b.njobs = njobs; b.ldw = ldw; b.rstride = codes ? ldw : 2 * ldw; b.rule = rule;
     b.min_size = min_size; b.min_ratio = min_ratio; b.next_job = ctr.as<u32>(); b.status = status.as<u32>();
     b.nplanes = planes.nplanes;
     if (planes.nplanes) {
@@ -1443,7 +1493,26 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
     CASPHY_HIP(hipGetDevice(&dev));
     CASPHY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const u32 grid = std::min<u32>(njobs, (u32)std::max(cus, 1) * 2);
-    hipLaunchKernelGGL(k_join_batch, dim3(grid), dim3(1024), 0, s, b);
+    // lanes per pair: the fewest whose 16-byte loads of a row are at most four a lane
+    const u32 lps = ldw <= 64 ? 4u : ldw <= 128 ? 8u : ldw <= 256 ? 16u : ldw <= 512 ? 32u : 64u;
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), 0, s, b); };
+    if (codes) {
+        switch (lps) {
+        case 4: go(k_join_batch<FMT_CODES, 4>); break;
+        case 8: go(k_join_batch<FMT_CODES, 8>); break;
+        case 16: go(k_join_batch<FMT_CODES, 16>); break;
+        case 32: go(k_join_batch<FMT_CODES, 32>); break;
+        default: go(k_join_batch<FMT_CODES, 64>); break;
+        }
+    } else {
+        switch (lps) {
+        case 4: go(k_join_batch<FMT_PLANES, 4>); break;
+        case 8: go(k_join_batch<FMT_PLANES, 8>); break;
+        case 16: go(k_join_batch<FMT_PLANES, 16>); break;
+        case 32: go(k_join_batch<FMT_PLANES, 32>); break;
+        default: go(k_join_batch<FMT_PLANES, 64>); break;
+        }
+    }
     CASPHY_HIP(hipGetLastError());
     std::vector<u32> st(njobs);
     d2h(st.data(), status.p, (size_t)njobs * 4, s);
@@ -1473,6 +1542,22 @@ bool codes_fit(const IndelLayout &L)
 {
     for (int32_t k : L.kmax)
         if (k >= (int32_t)CODE_MISSING) return false;
+    return true;
+}
+
+// CASPHY_JOIN_CODES=0 keeps indel joins on the one-hot planes.
+bool no_codes_env()
+{
+    const char *e = std::getenv("CASPHY_JOIN_CODES");
+    return e && e[0] == '0';
+}
+
+// No weights or all-one weights (what R passes by default): unit weights.
+bool unit_weights(const double *weight, u32 nstate, u32 ncol)
+{
+    if (!weight) return true;
+    for (u64 i = 0; i < (u64)nstate * ncol; ++i)
+        if (weight[i] != 1.0) return false;
     return true;
 }
 
@@ -1507,6 +1592,15 @@ void join_indel_batch_dev(const int32_t *raw, const std::vector<u32> &offs, u32 
     const u32 total = offs.back();
     // one layout for every job: per-site allele groups sized by the maximum over all jobs
     IndelLayout L = indel_layout(raw, total, ncol, s);
+    if (codes_fit(L) && !no_codes_env()) {  // byte codes
+        const u32 cw = round_up(ceil_div(ncol, 4), 4u);
+        DevBuf rows((size_t)total * cw * 4);
+        hipLaunchKernelGGL(k_indel_codes, dim3(grid_for((u64)total * cw)), dim3(256), 0, s, raw, total, ncol,
+                           rows.as<u32>(), cw);
+        CASPHY_HIP(hipGetLastError());
+        batch_run(rows, offs, cw, 2, Planes{}, nullptr, 0, 2.0f, edge_host, s, true);
+        return;
+    }
     DevBuf rows((size_t)total * 2 * L.ldw * 4);
     launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
     hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, raw, total, ncol,
@@ -1568,13 +1662,7 @@ int casphy_join_indel(const int32_t *mat, uint32_t n, uint32_t ncol, const doubl
         h2d(dmat.p, mat, (size_t)n * ncol * 4, s);
         IndelLayout L = indel_layout(dmat.as<int32_t>(), n, ncol, s);
         Planes planes = indel_planes(L, weight, nstate, ncol);
-        // all-one weights (what R passes by default) are unit weights
-        bool unit = weight == nullptr;
-        if (weight) {
-            unit = true;
-            for (u64 i = 0; i < (u64)nstate * ncol && unit; ++i) unit = weight[i] == 1.0;
-        }
-        if (unit) planes = Planes{};
+        if (unit_weights(weight, nstate, ncol)) planes = Planes{};  // all-one weights are unit weights
         Tree t;
         tree_alloc(t, n, L.ldw, 2);
         if (codes_fit(L)) { t.code_src = dmat.as<int32_t>(); t.ncol = ncol; }
@@ -1687,15 +1775,26 @@ int casphy_join_indel_batch(const int32_t *mat, const uint32_t *offsets, uint32_
         // one layout for every job: per-site allele groups sized by the maximum over all jobs
         IndelLayout L = indel_layout(dmat.as<int32_t>(), total, ncol, s);
         Planes planes = indel_planes(L, weight, nstate, ncol);
-        DevBuf rows((size_t)total * 2 * L.ldw * 4), dts;
-        launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
-        hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, dmat.as<int32_t>(),
-                           total, ncol, L.dloci.as<u32>(), L.dwidth.as<u32>(), rows.as<u32>(), L.ldw);
-        CASPHY_HIP(hipGetLastError());
+        DevBuf dts;
         if (tip_size) {
             dts.alloc((size_t)total * 4);
             h2d(dts.p, tip_size, (size_t)total * 4, s);
         }
+        if (unit_weights(weight, nstate, ncol) && codes_fit(L) && !no_codes_env()) {  // byte codes
+            const u32 cw = round_up(ceil_div(ncol, 4), 4u);
+            DevBuf crows((size_t)total * cw * 4);
+            hipLaunchKernelGGL(k_indel_codes, dim3(grid_for((u64)total * cw)), dim3(256), 0, s, dmat.as<int32_t>(),
+                               total, ncol, crows.as<u32>(), cw);
+            CASPHY_HIP(hipGetLastError());
+            batch_run(crows, offs, cw, 2, Planes{}, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge,
+                      s, true);
+            return;
+        }
+        DevBuf rows((size_t)total * 2 * L.ldw * 4);
+        launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
+        hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, dmat.as<int32_t>(),
+                           total, ncol, L.dloci.as<u32>(), L.dwidth.as<u32>(), rows.as<u32>(), L.ldw);
+        CASPHY_HIP(hipGetLastError());
         batch_run(rows, offs, L.ldw, 2, planes, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge, s);
     });
 }

@@ -416,6 +416,13 @@ def test_join_indel_codes_edges(hip, orc, kmax, ncol):
         e, ns, br = hip.join_indel(mat, weight)
         e2, ns2, br2 = orc.join_indel(mat, w)
         assert np.array_equal(e, e2) and np.array_equal(ns, ns2) and np.array_equal(br, br2)
+    # the same rows as three jobs of one persistent launch, with the size rule
+    parts = [mat[:120], mat[120:121 + 60], mat[181:]]
+    ts = [np.random.default_rng(k).integers(1, 6, p.shape[0]).astype(np.uint32) for k, p in enumerate(parts)]
+    batch = hip.join_indel_batch(parts, w.astype(np.float64), ts, 3, 1.5)
+    for p, t, e in zip(parts, ts, batch):
+        e2, _, _ = orc.join_indel(p, w, t, 3, 1.5)
+        assert np.array_equal(e, e2)
 
 
 def test_join_indel_config1(hip, orc, joinstep):
@@ -453,9 +460,12 @@ def test_join_batch_weighted_and_types(hip):
             assert np.array_equal(e, e1)
 
 
-def test_join_indel_batch_bootstrap(hip, orc):
+@pytest.mark.parametrize("rows", ["codes", "onehot"])
+def test_join_indel_batch_bootstrap(hip, orc, monkeypatch, rows):
     """Bootstrap replicates of the indel configuration (sites resampled with
-    replacement) in one persistent launch, each equal to its own oracle join."""
+    replacement) in one persistent launch, each equal to its own oracle join;
+    rows as byte codes (the default) or as the one-hot planes."""
+    monkeypatch.setenv("CASPHY_JOIN_CODES", "1" if rows == "codes" else "0")
     base = synth.indel_matrix(seed=1, depth=8)
     rng = np.random.default_rng(42)
     reps = [base[:, rng.integers(0, base.shape[1], base.shape[1])] for _ in range(6)]
