@@ -233,7 +233,7 @@ VARIANT_KERNEL = {"128x1": "k_mfma_tiles<0, 128, 1, 2>", "128x2": "k_mfma_tiles<
                   "reg": "k_mfma_reg<0>", "wide": "k_mfma_wide<0>", "wide8": "k_mfma_wide8<0>",
                   "fp4": "k_mfma_fp4<0>", "fp4b": "k_mfma_fp4b<0>", "fp4d": "k_mfma_fp4d<0>",
                   "fp4w": "k_mfma_fp4w<0>", "fp4x": "k_mfma_fp4x<0>",
-                  "fp4y": "k_mfma_fp4y<0>"}
+                  "fp4y": "k_mfma_fp4y<0>", "fp4p": "k_mfma_fp4p<0>"}
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "k_mfma_tiles_mask_10000x4096_pmc.md")
 TRAFFIC_TABLE_DNA = os.path.join(ROOT, "profiles", "k_mfma_mask_dna_100000x1000_pmc.md")
 

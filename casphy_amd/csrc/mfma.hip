@@ -217,18 +217,22 @@ __device__ __forceinline__ void mask_rows(const v16i &acc, int32_t thr, unsigned
 // a writelane that reads a scalar register straight after a vector compare
 // wrote it reads the old value, so the scalar instruction between them is
 // part of the contract (a compiler that dropped the AND would break it; the
-// column mask is a runtime ballot, so it cannot).
+// column mask is a runtime ballot, so it cannot).  Each lane's own compares
+// (its column of the block, for the mirrored words) collect in colbits by one
+// add with carry per register: colbits = 2 colbits + compare, registers 15
+// down to 0, so bit V of colbits is register V's compare.
 template <int V>
 __device__ __forceinline__ void mask_rows_f(const v16f &acc, float thr, unsigned long long jmask, u32 &rw0, u32 &rw1,
                                             u32 &colbits)
 {
     constexpr int t = 8 * (V / 4) + (V % 4);
-    const bool hit = acc[V] >= thr;
-    const unsigned long long b = __ballot(hit) & jmask;
+    const unsigned long long hit = __ballot(acc[V] >= thr);
+    const unsigned long long b = hit & jmask;
     asm("v_writelane_b32 %0, %1, %2" : "+v"(rw0) : "s"((u32)b), "i"(t));
     asm("v_writelane_b32 %0, %1, %2" : "+v"(rw1) : "s"((u32)(b >> 32)), "i"(t + 4));
-    colbits |= hit ? 1u << t : 0u;
-    if constexpr (V + 1 < 16) mask_rows_f<V + 1>(acc, thr, jmask, rw0, rw1, colbits);
+    unsigned long long carry;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(colbits), "=s"(carry) : "v"(colbits), "s"(hit));
+    if constexpr (V > 0) mask_rows_f<V - 1>(acc, thr, jmask, rw0, rw1, colbits);
 }
 
 // Epilogue of one 32 x 32 block whose rows start at r0 and columns at c0
@@ -366,10 +370,13 @@ __device__ __forceinline__ void fp4_epilogue(const MfmaArgs &a, const v16f &acc,
         const u32 half = lane >> 5, j = c0 + (lane & 31);
         const float fthr = (float)thr;
         u32 rw0 = 0, rw1 = 0, colbits = 0;
-        mask_rows_f<0>(acc, fthr, __ballot(j < a.nB), rw0, rw1, colbits);
+        mask_rows_f<15>(acc, fthr, __ballot(j < a.nB), rw0, rw1, colbits);
         const u32 rowword = rw0 | rw1, wc = c0 / 32;
         if (lane < 32 && r0 + lane < a.nA && wc < a.mstride) a.mask[(size_t)(r0 + lane) * a.mstride + wc] = rowword;
         if (mirror) {
+            // bit V -> row 8 (V / 4) + V % 4: nibble k of the 16 bits to byte k
+            colbits = (colbits | (colbits << 8)) & 0x00FF00FFu;
+            colbits = (colbits | (colbits << 4)) & 0x0F0F0F0Fu;
             colbits <<= 4 * half;
             const u32 rows = r0 < a.nA ? a.nA - r0 : 0u;  // rows past nA score the zero padding
             colbits &= rows >= 32 ? 0xFFFFFFFFu : (1u << rows) - 1u;
@@ -1467,6 +1474,184 @@ __global__ __launch_bounds__(256, 2) void k_mfma_fp4y(MfmaArgs a)
         }
 }
 
+// fp4 with 256 x 256 pair tiles and one persistent workgroup per CU.  Each
+// XCD (blockIdx.x & 7) owns a contiguous run of tiles, which its workgroups
+// take round-robin; the LDS DMA never drains between tiles: a workgroup's next
+// tile streams its first four stages in during the last four stages of the
+// current tile and its epilogue.  A 256 x 256 tile moves 32 KB per stage
+// (128 sites) for 65,536 pairs, against 48 KB for the two 256 x 128 tiles of
+// k_mfma_fp4y, and the DMA into LDS is what bounds those tiles.  Four waves of
+// 128 x 128 pairs (4 x 4 MFMA blocks, 256 fp32 accumulators per lane); five
+// rotating LDS buffers of 32 KB (A rows then B rows, swizzled 64-byte rows:
+// the whole 160 KB), DMA four stages ahead.  Per stage: the second chunk's fragments and the
+// stage's DMA are issued under the first chunk's 16 MFMAs, then the barrier
+// (this wave's DMA of the next stage has landed and its reads of this stage
+// are done), then the next stage's first fragments are read under the second
+// chunk's 16 MFMAs.
+constexpr u32 FP4P_STAGE = 2 * WT * 64;  // 32 KB
+constexpr u32 FP4P_NBUF = 5;
+
+template <int EPI>
+__global__ __launch_bounds__(256, 1) void k_mfma_fp4p(MfmaArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char S[FP4P_NBUF * FP4P_STAGE];
+
+    if (mask_skipped<EPI>(a)) return;
+    const u32 per = (a.ntiles + 7) / 8, xcd = blockIdx.x & 7;
+    const u32 tend = min((xcd + 1) * per, a.ntiles), tstep = gridDim.x >> 3;
+    // (slot, I, J) of the first tile at or after slot t, slot tend when none
+    auto next_tile = [&](u32 t) {
+        for (; t < tend; t += tstep) {
+            u32 i, j;
+            if (tile_of(a, t, i, j)) return make_uint3(t, i, j);
+        }
+        return make_uint3(tend, 0u, 0u);
+    };
+    uint3 tile = next_tile(xcd * per + (blockIdx.x >> 3));
+    if (tile.x >= tend) return;
+
+    const u32 tid = threadIdx.x, lane = tid & 63;
+    const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the DMA bases stay scalar
+    const u32 wm = wave >> 1, wn = wave & 1, h = lane >> 5;
+
+    // DMA: waves 0-1 move the A rows, waves 2-3 the B rows, 8 instructions of
+    // 64 x 16 bytes each; lane granule q of instruction k is slot q & 3 of row
+    // q >> 2 and holds granule g = slot ^ ((row >> 2) & 3) (chunk g >> 1, half
+    // g & 1) of that row
+    const bool opB = wave >= 2;
+    const unsigned char *E = reinterpret_cast<const unsigned char *>(opB ? a.PB : a.PA);
+    const u32 pad = opB ? a.padB : a.padA;  // below 2^26 (checked on the host)
+    u32 off[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const u32 q = ((wave & 1) * 8 + k) * 64 + lane, row = q >> 2, g = (q & 3) ^ ((row >> 2) & 3);
+        off[k] = ((g >> 1) * pad + row) * 32 + (g & 1) * 16;
+    }
+    const size_t sstride = 64 * (size_t)pad;  // bytes between stages: two 32-byte chunks per row
+    const u32 dbase = wave * 8 * 1024;
+    auto base_of = [&](const uint3 &tl) {
+        return E + (size_t)__builtin_amdgcn_readfirstlane(opB ? tl.z : tl.y) * WT * 32;
+    };
+    auto dma = [&](u32 buf, const unsigned char *base, u32 st) {
+        const unsigned char *src = base + (size_t)st * sstride;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + off[k]),
+                                             (__attribute__((address_space(3))) void *)(S + buf + dbase + k * 1024),
+                                             16, 0, 0);
+    };
+    // fragment of row r, chunk c: the row's slot (2c + h) ^ ((r >> 2) & 3); with
+    // r = 32 m + (lane & 31) the swizzle depends on the lane only
+    const u32 lo0 = (lane & 31) * 64 + ((h ^ ((lane >> 2) & 3)) * 16), lo1 = lo0 ^ 32;
+    const u32 fa0 = wm * 128 * 64, fb0 = WT * 64 + wn * 128 * 64;
+    v4i F0[8], F1[8];
+    auto frags = [&](u32 buf, u32 lo, v4i(&F)[8]) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) F[x] = *reinterpret_cast<const v4i *>(S + buf + fa0 + x * 32 * 64 + lo);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) F[4 + y] = *reinterpret_cast<const v4i *>(S + buf + fb0 + y * 32 * 64 + lo);
+    };
+    v16f acc[4][4];
+    auto mma = [&](const v4i(&F)[8], int b) {
+        const int x = b >> 2, y = b & 3;
+        const v8i fa = v8i{F[x].x, F[x].y, F[x].z, F[x].w, 0, 0, 0, 0};
+        const v8i fb = v8i{F[4 + y].x, F[4 + y].y, F[4 + y].z, F[4 + y].w, 0, 0, 0, 0};
+        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, acc[x][y], 4, 4, 0, 127, 0, 127);
+    };
+    // one stage out of buffer `cur`; its DMA (base db, stage sd) goes into `pre`;
+    // NEXT: the next stage's first fragments are read (the last stage of a
+    // tile leaves them until after the epilogue, whose registers they would
+    // crowd)
+    auto stage = [&](u32 cur, u32 nxt, u32 pre, const unsigned char *db, u32 sd, auto NEXT) {
+        frags(cur, lo1, F1);
+        dma(pre, db, sd);
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            mma(F0, b);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            if (b < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            if (b >= 8) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // DMA
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0x4078);  // vmcnt(24) lgkmcnt(0): the next stage's DMA has landed
+        asm volatile("" ::: "memory");  // no compiler motion of LDS accesses across the barrier
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (decltype(NEXT)::value) frags(nxt, lo0, F0);
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            mma(F1, b);
+            if (decltype(NEXT)::value && b < 8) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 1);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    const u32 nst = a.stages;  // at least 4 (checked on the host)
+    auto bnext = [](u32 b) { return b + 1 == FP4P_NBUF ? 0u : b + 1; };
+    const int32_t thr = EPI == E_MASK && a.thr_dev ? *a.thr_dev : a.thr;
+    const unsigned char *base = base_of(tile);
+    dma(0, base, 0);
+    dma(FP4P_STAGE, base, 1);
+    dma(2 * FP4P_STAGE, base, 2);
+    dma(3 * FP4P_STAGE, base, 3);
+    __builtin_amdgcn_s_waitcnt(0x4F78);  // vmcnt(24): stage 0 has landed
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    frags(0, lo0, F0);
+    u32 cur = 0;
+    for (;;) {
+        const uint3 nt = next_tile(tile.x + tstep);
+        const bool more = nt.x < tend;
+        const unsigned char *nbase = more ? base_of(nt) : base;
+        // stage s prefetches stage s + 4 of this tile or, past its end, the
+        // next tile's first stages (the last workgroup tile reloads its own
+        // last stage into a buffer no one reads)
+        // the accumulators are zeroed in their registers, out of the
+        // compiler's sight: a known zero becomes the inline-constant
+        // accumulator input of the tile's first MFMAs, and with the
+        // accumulators in AGPRs that form gave wrong sums on this target (a
+        // one-wave check with VGPR accumulators gave the right ones)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                acc[x][y] = (v16f)(0.0f);
+                asm volatile("" : "+a"(acc[x][y]));
+            }
+        auto run = [&](u32 s, auto NEXT) {
+            const u32 sd = s + 4, nx = bnext(cur), pre = cur == 0 ? FP4P_NBUF - 1 : cur - 1;
+            const bool here = sd < nst;
+            stage(cur * FP4P_STAGE, nx * FP4P_STAGE, pre * FP4P_STAGE, here ? base : nbase,
+                  here ? sd : (more ? sd - nst : nst - 1), NEXT);
+            cur = nx;
+        };
+        for (u32 s = 0; s + 1 < nst; ++s) run(s, std::true_type{});
+        run(nst - 1, std::false_type{});
+        const bool mirror = a.triangle && tile.y != tile.z;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                fp4_epilogue<EPI>(a, acc[x][y], tile.y * WT + wm * 128 + 32 * x, tile.z * WT + wn * 128 + 32 * y, lane,
+                                  mirror, thr);
+                __builtin_amdgcn_sched_barrier(0);  // one block's registers at a time
+            }
+        if (!more) break;
+        // the next tile's stage 0 landed before the last stage's barrier and
+        // no DMA writes its buffer before that tile's second stage
+        frags(cur * FP4P_STAGE, lo0, F0);
+        tile = nt;
+        base = nbase;
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no DMA still writes this workgroup's LDS
+}
+
 __global__ void k_stage_popc(const u32 *P, u32 n, u32 stride, u32 stages, u32 *pop)
 {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1631,7 +1816,8 @@ void launch_wide8(int epi, const MfmaArgs &a, hipStream_t s)
 // after the launch) and runs the fp4 tiles.
 // mode 0: LDS staging through registers, 1: B operands from global, 2: LDS DMA,
 // 3: LDS DMA with 256 x 256 tiles, 4: 256 x 256 tiles of 4 software-pipelined waves,
-// 5: 256 x 128 tiles of 4 pipelined waves, two workgroups per CU
+// 5: 256 x 128 tiles of 4 pipelined waves, two workgroups per CU, 6: 256 x 256
+// tiles of 4 pipelined waves, one persistent workgroup per CU
 void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
 {
     MfmaArgs a = in;
@@ -1657,7 +1843,31 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
     const u32 grid = 8 * ((a.ntiles + 7) / 8);
     if ((mode == 4 || mode == 5) && (stages % 4 != 0 || stages < 4))
         throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: the pipelined tiles need a multiple of 4 stages");
-    if (mode == 5) {
+    if (mode == 6) {
+        // one workgroup per CU (the XCD mapping wants a multiple of 8)
+        static int cus = 0;
+        if (!cus) {
+            int dev = 0;
+            CASPHY_HIP(hipGetDevice(&dev));
+            CASPHY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        }
+        const u32 per_xcd = std::max(1u, (u32)cus / 8), per = (a.ntiles + 7) / 8;
+        const u32 pgrid = 8 * std::min(per_xcd, per);
+        if (epi == E_MASK)
+            hipLaunchKernelGGL(k_mfma_fp4p<E_MASK>, dim3(pgrid), dim3(256), 0, s, a);
+        else if (epi == E_SCORES)
+            hipLaunchKernelGGL(k_mfma_fp4p<E_SCORES>, dim3(pgrid), dim3(256), 0, s, a);
+        else if (epi == E_LOCALMAX)
+            hipLaunchKernelGGL(k_mfma_fp4p<E_LOCALMAX>, dim3(pgrid), dim3(256), 0, s, a);
+        else if (epi == E_TBE)
+            hipLaunchKernelGGL(k_mfma_fp4p<E_TBE>, dim3(pgrid), dim3(256), 0, s, a);
+        else if (epi == E_PAIRS)
+            hipLaunchKernelGGL(k_mfma_fp4p<E_PAIRS>, dim3(pgrid), dim3(256), 0, s, a);
+        else if (epi == E_ARGMAX)
+            hipLaunchKernelGGL(k_mfma_fp4p<E_ARGMAX>, dim3(pgrid), dim3(256), 0, s, a);
+        else
+            throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: unsupported epilogue");
+    } else if (mode == 5) {
         if (epi == E_MASK)
             hipLaunchKernelGGL(k_mfma_fp4y<E_MASK>, dim3(grid), dim3(256), 0, s, a);
         else if (epi == E_SCORES)
@@ -1766,18 +1976,27 @@ namespace {
 
 // Variants: LDS-staged with stage depth x prefetch, or register-resident ("reg").
 const char *const kVariants[] = {"128x1", "128x2", "128x4", "64x2", "64x4", "64x8", "reg", "wide", "wide8", "fp4",
-                                 "fp4b", "fp4d", "fp4w", "fp4x", "fp4y"};
-constexpr int kNumVariants = 15;
+                                 "fp4b", "fp4d", "fp4w", "fp4x", "fp4y", "fp4p"};
+constexpr int kNumVariants = 16;
 constexpr int kDefaultVariant = 3;
-constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14};  // what the autotuner times
+constexpr int kCandidates[] = {3, 0, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};  // what the autotuner times
 constexpr int kWide = 7, kWide8 = 8;  // radius-mask and score epilogues only
 constexpr int kFp4 = 9, kFp4b = 10, kFp4d = 11, kFp4w = 12, kFp4x = 13, kFp4y = 14;  // fp4y: mask, scores, localmax
+constexpr int kFp4p = 15;
 // super-tile edge for 128 x 128 tiles: 8 (8 x 8 tiles = 16 row blocks of 128);
 // CASPHY_MFMA_SUPER overrides, 0 keeps the plain column order
 u32 super_edge()
 {
     const char *e = std::getenv("CASPHY_MFMA_SUPER");
     return e ? (u32)std::atoi(e) : 8u;
+}
+
+// super-tile edge of the persistent 256 x 256 tiles; CASPHY_FP4P_SUPER
+// overrides, 0 keeps the plain column order
+u32 fp4p_super_edge()
+{
+    const char *e = std::getenv("CASPHY_FP4P_SUPER");
+    return e ? (u32)std::atoi(e) : 0u;
 }
 
 bool variant_supports(int v, int epi)
@@ -1799,7 +2018,12 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
 {
     MfmaArgs a = in;
     if (!variant_supports(v, epi)) v = kDefaultVariant;
-    const u32 T = v == 6 ? RT : (v == kWide || v == kWide8 || v == kFp4w || v == kFp4x || v == kFp4y) ? WT : TILE;
+    // the persistent tiles keep 32-bit DMA offsets (below 2^26 padded rows) and
+    // prefetch four stages ahead
+    if (v == kFp4p && (a.padA >= (1u << 26) || a.padB >= (1u << 26) || a.stages < 4)) v = kFp4;
+    const u32 T = v == 6 ? RT
+                         : (v == kWide || v == kWide8 || v == kFp4w || v == kFp4x || v == kFp4y || v == kFp4p) ? WT
+                                                                                                               : TILE;
     const u32 ntA = ceil_div(a.nA, T);
     a.ntA = ntA;
     a.ntB = ceil_div(a.nB, v == kFp4y ? (u32)TILE : T);  // fp4y: 256 A rows x 128 B rows
@@ -1813,7 +2037,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
         return;
     }
     // 128 x 128 tiles run in 8 x 8 super-tiles (launch slots without a tile exit)
-    a.sup = T == TILE ? super_edge() : 0;
+    a.sup = T == TILE ? super_edge() : v == kFp4p ? fp4p_super_edge() : 0;
     if (a.sup) {
         const u64 nSA = ceil_div(ntA, a.sup), nSB = ceil_div(a.ntB, a.sup);
         const u64 nsup = a.triangle ? nSA * (nSA + 1) / 2 : nSA * nSB;
@@ -1837,6 +2061,7 @@ void launch_one(int v, int epi, const MfmaArgs &in, hipStream_t s)
     case kFp4w: launch_fp4(epi, a, s, 3); break;
     case kFp4x: launch_fp4(epi, a, s, 4); break;
     case kFp4y: launch_fp4(epi, a, s, 5); break;
+    case kFp4p: launch_fp4(epi, a, s, 6); break;
     default: throw Error(CASPHY_EINTERNAL, "MFMA tiles: unknown variant");
     }
     CASPHY_HIP(hipGetLastError());

@@ -17,8 +17,8 @@ def rand_bits(rng, n, W, p):
 
 # ------------------------------------------------------------------ scores
 @pytest.fixture(params=["mfma-auto", "mfma-lds", "mfma-fp4", "mfma-fp4b", "mfma-fp4d", "mfma-fp4w", "mfma-fp4x",
-                        "mfma-fp4y", "mfma-wide", "mfma-wide8", "mfma-reg", "valu", "valu-k16x8", "valu-k32x4",
-                        "valu-k16x4"])
+                        "mfma-fp4y", "mfma-fp4p", "mfma-wide", "mfma-wide8", "mfma-reg", "valu", "valu-k16x8",
+                        "valu-k32x4", "valu-k16x4"])
 def bitpath(request, monkeypatch):
     """Runs a test on the autotuned int8 matrix-core path (every candidate
     variant runs on the first call of a shape class), on the fixed variants
@@ -41,7 +41,7 @@ def bitpath(request, monkeypatch):
     else:
         variant = {"mfma-reg": "reg", "mfma-wide": "wide", "mfma-wide8": "wide8", "mfma-fp4": "fp4",
                    "mfma-fp4b": "fp4b", "mfma-fp4d": "fp4d", "mfma-fp4w": "fp4w",
-                   "mfma-fp4x": "fp4x", "mfma-fp4y": "fp4y"}.get(request.param, "64x2")
+                   "mfma-fp4x": "fp4x", "mfma-fp4y": "fp4y", "mfma-fp4p": "fp4p"}.get(request.param, "64x2")
         monkeypatch.setenv("CASPHY_MFMA_VARIANT", variant)
     return request.param
 
@@ -52,6 +52,27 @@ def test_score_bit_matrix(hip, orc, bitpath, n, W):
     mat = rand_bits(rng, n, W, 0.15)
     S = hip.score_bit_matrix(mat)
     assert np.array_equal(S, orc.score_bit_matrix(mat).astype(np.int32))
+
+
+@pytest.mark.parametrize("sup", ["0", "3"])
+def test_fp4p_many_tiles(hip, orc, monkeypatch, sup):
+    """The persistent 256 x 256 fp4 tiles over many more tiles than
+    workgroups, so every workgroup runs several tiles and prefetches across
+    tile boundaries: 6,100 cells x 640 sites (5 stages, a ragged last row
+    block), scores and radius lists against the oracle, in plain and in 3 x 3
+    super-tile order."""
+    monkeypatch.setenv("CASPHY_MFMA", "1")
+    monkeypatch.setenv("CASPHY_MFMA_VARIANT", "fp4p")
+    monkeypatch.setenv("CASPHY_FP4P_SUPER", sup)
+    rng = np.random.default_rng(61)
+    mat = rand_bits(rng, 6100, 20, 0.2)
+    S = hip.score_bit_matrix(mat)
+    assert np.array_equal(S, orc.score_bit_matrix(mat).astype(np.int32))
+    del S
+    for r in (30, 45):
+        counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), r)
+        c2, i2 = orc.radius_bit(mat, r)
+        assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
 def test_score_bit_past_fp32_exact_sums(hip, monkeypatch):
@@ -155,16 +176,20 @@ def test_radius_bit_synthetic_config(hip, orc, bitpath):
         assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
-@pytest.fixture(params=["auto", "mfma", "valu", "valu-k48x8", "valu-k48x4"])
+@pytest.fixture(params=["auto", "mfma", "mfma-fp4p", "valu", "valu-k48x8", "valu-k48x4"])
 def dnaform(request, monkeypatch):
-    """ACGT data on the one-hot matrix-core form, on the 2-bit + mask VALU
-    form (autotuned, or with the 48-word stage in 128 x 128 or 128 x 64 tiles
+    """ACGT data on the one-hot matrix-core form (autotuned tiles, or the
+    persistent 256 x 256 tiles forced), on the 2-bit + mask VALU form
+    (autotuned, or with the 48-word stage in 128 x 128 or 128 x 64 tiles
     forced), or on the form the resident cells' tuner picks."""
     monkeypatch.setenv("CASPHY_MFMA", "1")
     monkeypatch.delenv("CASPHY_MFMA_VARIANT", raising=False)
     monkeypatch.delenv("CASPHY_VALU_VARIANT", raising=False)
     if request.param == "auto":
         monkeypatch.delenv("CASPHY_DNA_FORM", raising=False)
+    elif request.param == "mfma-fp4p":
+        monkeypatch.setenv("CASPHY_DNA_FORM", "mfma")
+        monkeypatch.setenv("CASPHY_MFMA_VARIANT", "fp4p")
     elif request.param.startswith("valu-"):
         monkeypatch.setenv("CASPHY_DNA_FORM", "valu")
         monkeypatch.setenv("CASPHY_VALU_VARIANT", request.param[5:])
