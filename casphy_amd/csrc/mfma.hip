@@ -1552,22 +1552,25 @@ __global__ __launch_bounds__(256, 1) void k_mfma_fp4p(MfmaArgs a)
         for (int y = 0; y < 4; ++y) F[4 + y] = *reinterpret_cast<const v4i *>(S + buf + fb0 + y * 32 * 64 + lo);
     };
     v16f acc[4][4];
-    auto mma = [&](const v4i(&F)[8], int b) {
+    // FIRST: the tile's first MFMA of every block, which takes its
+    // accumulator input from block (3, 3), zeroed before and itself last
+    auto mma = [&](const v4i(&F)[8], int b, auto FIRST) {
         const int x = b >> 2, y = b & 3;
         const v8i fa = v8i{F[x].x, F[x].y, F[x].z, F[x].w, 0, 0, 0, 0};
         const v8i fb = v8i{F[4 + y].x, F[4 + y].y, F[4 + y].z, F[4 + y].w, 0, 0, 0, 0};
-        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, acc[x][y], 4, 4, 0, 127, 0, 127);
+        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, decltype(FIRST)::value ? acc[3][3] : acc[x][y],
+                                                                  4, 4, 0, 127, 0, 127);
     };
     // one stage out of buffer `cur`; its DMA (base db, stage sd) goes into `pre`;
     // NEXT: the next stage's first fragments are read (the last stage of a
     // tile leaves them until after the epilogue, whose registers they would
     // crowd)
-    auto stage = [&](u32 cur, u32 nxt, u32 pre, const unsigned char *db, u32 sd, auto NEXT) {
+    auto stage = [&](u32 cur, u32 nxt, u32 pre, const unsigned char *db, u32 sd, auto NEXT, auto FIRST) {
         frags(cur, lo1, F1);
         dma(pre, db, sd);
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
-            mma(F0, b);
+            mma(F0, b, FIRST);
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
             if (b < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
             if (b >= 8) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // DMA
@@ -1581,7 +1584,7 @@ __global__ __launch_bounds__(256, 1) void k_mfma_fp4p(MfmaArgs a)
         if constexpr (decltype(NEXT)::value) frags(nxt, lo0, F0);
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
-            mma(F1, b);
+            mma(F1, b, std::false_type{});
             if (decltype(NEXT)::value && b < 8) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
                 __builtin_amdgcn_sched_group_barrier(0x100, 1, 1);
@@ -1612,27 +1615,23 @@ __global__ __launch_bounds__(256, 1) void k_mfma_fp4p(MfmaArgs a)
         // stage s prefetches stage s + 4 of this tile or, past its end, the
         // next tile's first stages (the last workgroup tile reloads its own
         // last stage into a buffer no one reads)
-        // the accumulators are zeroed in their registers, out of the
-        // compiler's sight: a known zero becomes the inline-constant
-        // accumulator input of the tile's first MFMAs, and with the
-        // accumulators in AGPRs that form gave wrong sums on this target (a
-        // one-wave check with VGPR accumulators gave the right ones)
-#pragma unroll
-        for (int x = 0; x < 4; ++x)
-#pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                acc[x][y] = (v16f)(0.0f);
-                asm volatile("" : "+a"(acc[x][y]));
-            }
-        auto run = [&](u32 s, auto NEXT) {
+        // block (3, 3) is zeroed in its registers, out of the compiler's
+        // sight: a known zero becomes the inline-constant accumulator input
+        // of the tile's first MFMAs, and with the accumulators in AGPRs that
+        // form gave wrong sums on this target (a one-wave check with VGPR
+        // accumulators gave the right ones)
+        acc[3][3] = (v16f)(0.0f);
+        asm volatile("" : "+a"(acc[3][3]));
+        auto run = [&](u32 s, auto NEXT, auto FIRST) {
             const u32 sd = s + 4, nx = bnext(cur), pre = cur == 0 ? FP4P_NBUF - 1 : cur - 1;
             const bool here = sd < nst;
             stage(cur * FP4P_STAGE, nx * FP4P_STAGE, pre * FP4P_STAGE, here ? base : nbase,
-                  here ? sd : (more ? sd - nst : nst - 1), NEXT);
+                  here ? sd : (more ? sd - nst : nst - 1), NEXT, FIRST);
             cur = nx;
         };
-        for (u32 s = 0; s + 1 < nst; ++s) run(s, std::true_type{});
-        run(nst - 1, std::false_type{});
+        run(0, std::true_type{}, std::true_type{});
+        for (u32 s = 1; s + 1 < nst; ++s) run(s, std::true_type{}, std::false_type{});
+        run(nst - 1, std::false_type{}, std::false_type{});
         const bool mirror = a.triangle && tile.y != tile.z;
 #pragma unroll
         for (int x = 0; x < 4; ++x)
