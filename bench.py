@@ -877,6 +877,77 @@ def serving_mix(cp, synth, njobs: int = 60, seed: int = 5, max_cells: int = 3000
             "capacity_speedup": el_p / el_s, "bit_exact_between_forms": True}
 
 
+def serving_poisson_segment(cp, synth, njobs: int, seed: int, max_cells: int, rate: float) -> dict:
+    """One segment of the serving configuration at its stated size: `njobs`
+    jobs of the mix above (sizes log-uniform in [1k, max_cells], its own seed)
+    through the served form only, Poisson arrivals at `rate` jobs/s.  Returns
+    every job's latency, so that segments of different seeds merge into one
+    distribution; every 25th job is run again through a one-worker plain
+    server and compared bit for bit."""
+    import threading
+    bmat = synth.binary_matrix(seed=seed, n=max_cells, nsite=4096, p=0.005, missing=0.05)
+    depth = int(np.ceil(np.log2(max_cells)))
+    imat = synth.indel_matrix(seed=seed, depth=depth, nsite=200)
+    jobs = synth.serving_jobs(seed, njobs, 1000, max_cells, bmat.shape[0], imat.shape[0])
+    print(f"# serving segment inputs ready ({njobs} jobs, seed {seed})", file=sys.stderr, flush=True)
+    srv = cp.Server(hbm_budget_bytes=64 << 30, workers=4)
+    mids = {"bit": srv.put_matrix(bmat, "bit"), "indel": srv.put_matrix(imat, "indel")}
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, len(jobs))
+    ids = [None] * len(jobs)
+
+    def feeder():
+        t0 = time.perf_counter()
+        due = 0.0
+        for k, (kind, data) in enumerate(jobs):
+            due += gaps[k]
+            d = t0 + due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids[k] = (srv.submit_transfer_trees(*data) if kind == "trees"
+                      else srv.submit_join_rows(mids[kind], data))
+    t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    done, lat = [], []
+    for k in range(len(jobs)):
+        while ids[k] is None:
+            time.sleep(0.001)
+        out, ms = srv.wait(ids[k])
+        done.append(out)
+        lat.append(ms)
+        if k % 50 == 49:
+            print(f"# segment {seed}: {k + 1} jobs done, {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    th.join()
+    el = time.perf_counter() - t0
+    srv.close()
+    plain = cp.Server(hbm_budget_bytes=64 << 30, workers=1)
+    checked = 0
+    for k in range(0, len(jobs), 25):
+        kind, data = jobs[k]
+        j = (plain.submit_transfer_trees(*data) if kind == "trees"
+             else plain.submit_join((bmat if kind == "bit" else imat)[data], kind))
+        ref = plain.wait(j)[0]
+        if isinstance(ref, tuple):
+            for x, y in zip(ref, done[k]):
+                if isinstance(x, list):
+                    assert all(np.array_equal(u, v) for u, v in zip(x, y))
+                else:
+                    assert np.array_equal(x, y)
+        else:
+            assert np.array_equal(ref, done[k])
+        checked += 1
+    plain.close()
+    sizes = np.array([d[4] if k == "trees" else d.size for k, d in jobs])
+    lat = np.array(lat)
+    return {"jobs": len(jobs), "seed": seed, "cells": f"log-uniform 1k..{max_cells // 1000}k",
+            "mean_cells": float(sizes.mean()), "largest": int(sizes.max()),
+            "tree_builds": sum(k != "trees" for k, _ in jobs), "tbe_jobs": sum(k == "trees" for k, _ in jobs),
+            "poisson_rate_per_s": rate, "elapsed_s": el, "jobs_per_s": len(jobs) / el,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "checked_against_plain": checked, "latency_ms": [round(float(x), 1) for x in lat],
+            "cells_per_job": [int(x) for x in sizes]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -887,8 +958,10 @@ def main():
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter scaling)")
     ap.add_argument("--serving-jobs", type=int, default=60, help="jobs of the serving mix")
     ap.add_argument("--serving-max", type=int, default=30000, help="largest job of the serving mix (cells)")
+    ap.add_argument("--serving-seed", type=int, default=11, help="seed of a serving segment")
+    ap.add_argument("--serving-rate", type=float, default=1.4, help="Poisson arrivals of a serving segment (jobs/s)")
     ap.add_argument("--only", choices=["tipmerge", "nodekeep", "voteroot", "fasta", "tbe", "base", "indel", "scan",
-                                       "dnavalu", "ceilings", "serving"],
+                                       "dnavalu", "ceilings", "serving", "serving-segment"],
                     help="run only one of the other configurations (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -933,6 +1006,10 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps(serving_mix(cp, synth, njobs=args.serving_jobs, max_cells=args.serving_max)))
+        return
+    if args.only == "serving-segment":
+        print(json.dumps(serving_poisson_segment(cp, synth, args.serving_jobs, args.serving_seed, args.serving_max,
+                                                 args.serving_rate)))
         return
     if args.only == "dnavalu":
         # the VALU 2-bit + mask tiles of the DNA configuration: one tryRadius

@@ -1228,6 +1228,27 @@ void tree_alloc(Tree &t, u32 n, u32 ldw, int rule)
 
 // The join loop in one launch per step (k_fstep), after k_join_init has put
 // the initial maxima into parity 0 of the per-slot state.
+
+thread_local u32 g_join_wg_cap = 0;
+
+// Workgroups of the one-launch step for n slots (slots_per_wg per pass).
+// Every workgroup repeats the selection over all workgroups' results of the
+// previous step, so past a few hundred workgroups that redundant work outweighs
+// the rescoring parallelism.  Measured single-join times (ms, binary / indel
+// codes): 10,000 cells 126 / 164 at 313 workgroups, 121 / 155 at 256; 30,000
+// cells 562 / 709 at 938, 464 / 568 at 256, 462 / 581 at 384; 100,000 cells
+// 2,835 / 2,994 at 1,024, 2,595 / 2,764 at 512, 3,101 / 3,033 at 256.  Four
+// concurrent 30,000-cell joins: 2.42 joins/s at 938, 4.95 at 256, 4.29 at 512.
+// CASPHY_JOIN_WG overrides both rules.
+u32 fstep_workgroups(u32 n, u32 slots_per_wg)
+{
+    const char *wenv = std::getenv("CASPHY_JOIN_WG");
+    u32 cap = 256 + n / 400;
+    if (g_join_wg_cap) cap = std::min(cap, g_join_wg_cap);
+    if (wenv) cap = (u32)std::max(1, std::atoi(wenv));
+    return std::max<u32>(1, std::min<u32>({ceil_div(n, slots_per_wg), cap, 1024u}));
+}
+
 void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hipStream_t s)
 {
     const u32 n = t.n;
@@ -1240,13 +1261,11 @@ void fstep_run(Tree &t, const Planes &planes, u32 min_size, float min_ratio, hip
         CASPHY_HIP(hipGetLastError());
     }
     const u32 ldw = t.codes ? t.cw : t.ldw;
-    const char *wenv = std::getenv("CASPHY_JOIN_WG");
-    const u32 cap = wenv ? std::max(1, std::atoi(wenv)) : 1024;
     // 8 lanes per slot, 32 slots per workgroup pass: measured faster than 16,
     // 32 or 64 lanes per slot for every row width (more slots in flight beat
     // fewer serial 16-byte loads per lane)
     const u32 lps = 8;
-    const u32 nwg = std::min<u32>(ceil_div(n, 256 / lps), cap);
+    const u32 nwg = fstep_workgroups(n, 256 / lps);
     DevBuf lm1((size_t)n * 4), li1((size_t)n * 4), sn1((size_t)n * 4), keys(2 * (size_t)nwg * 8),
         meta(2 * (size_t)nwg * sizeof(FMeta)), rkeys(2 * (size_t)nwg * 8), rslots(2 * (size_t)nwg * 4),
         ctl(2 * sizeof(FCtl));
@@ -1524,6 +1543,8 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
 void require(bool c, const char *msg) { CASPHY_REQUIRE(c, CASPHY_EARG, msg); }
 
 }  // namespace
+
+void set_join_workgroup_cap(u32 cap) { g_join_wg_cap = cap; }
 
 void join_bit_dev(const u32 *raw, u32 n, u32 W, int ancestral_type, u32 *edge_host, hipStream_t s)
 {

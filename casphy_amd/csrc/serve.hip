@@ -231,8 +231,15 @@ int casphy_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers
         const char *hol = std::getenv("CASPHY_SERVE_HOL_MS");
         if (hol) o.hol_ms = std::atof(hol);
         o.batch_max_jobs = BATCH_MAX_JOBS;
+        // with several workers, large joins run side by side: each takes at
+        // most 256 workgroups per step (the device fits several such steps)
         s->sched = std::make_unique<Scheduler>(
-            o, [device] { return casphy_set_device(device) == CASPHY_OK; }, run_group);
+            o,
+            [device, workers] {
+                set_join_workgroup_cap(workers > 1 ? 256 : 0);
+                return casphy_set_device(device) == CASPHY_OK;
+            },
+            run_group);
         *out = s.release();
     });
 }
