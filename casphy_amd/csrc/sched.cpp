@@ -129,11 +129,18 @@ std::vector<SchedJob *> Scheduler::pick(bool small_lane, size_t &bytes)
         return resident_ + running_ + extra + j->dev_bytes + head->dev_bytes <= opt_.budget;
     };
     auto first = queue_.end();
+    double best = 0;
     for (auto it = queue_.begin(); it != queue_.end(); ++it) {
         if (small_lane && !(*it)->batchable) continue;
-        if (admissible(*it, 0)) {
+        if (!admissible(*it, 0)) continue;
+        if (!opt_.shortest_first) {
             first = it;
             break;
+        }
+        const double key = (*it)->cost_ms - ms_between((*it)->t_submit, now);
+        if (first == queue_.end() || key < best) {  // ties: the older job
+            first = it;
+            best = key;
         }
     }
     if (first == queue_.end()) return group;

@@ -42,6 +42,7 @@ struct SchedJob {
     size_t dev_bytes = 0;    // estimated device footprint while it runs
     bool batchable = false;  // may share one launch with other jobs
     u64 batch_class = 0;     // jobs share a launch only within one class
+    double cost_ms = 0;      // estimated run time (shortest-first order)
     int status = 1;          // 1 pending, 0 done, < 0 CASPHY_E* code
     std::string err;
     sched_clock::time_point t_submit, t_done;
@@ -72,6 +73,11 @@ public:
         bool batching = true;
         double hol_ms = 500.0;
         size_t batch_max_jobs = 1024;
+        // a worker takes the admissible job with the least cost_ms minus
+        // time waited (short jobs pass long ones; a job's turn comes at the
+        // latest once it has waited its own cost longer than the others);
+        // false: oldest first
+        bool shortest_first = true;
     };
 
     Scheduler(const Options &o, WorkerInit init, Runner run);

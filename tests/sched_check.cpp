@@ -8,6 +8,9 @@
 //     no group exceeds the batch limit; with batching off every group is one job;
 //   * a job larger than the budget still runs (head-of-line reservation), and
 //     resident bytes count against the budget;
+//   * queued jobs run shortest first (least cost minus time waited), and a
+//     long job that has waited longer than its cost runs before a new short
+//     one; with shortest_first off, oldest first;
 //   * unknown ids and a second waiter on one job are refused;
 //   * the statistics count the successful jobs.
 // Exit status 0 on success; a message and status 1 on the first failure.
@@ -235,6 +238,64 @@ int bad_runner()
     return g_fail.load();
 }
 
+// One worker held by a first job; jobs queued behind it run in the order the
+// options ask for once it is released.
+std::vector<int> run_order(const std::vector<double> &costs, const std::vector<int> &delay_ms, bool shortest_first,
+                           int hold_ms)
+{
+    std::atomic<bool> release{false};
+    std::mutex mu;
+    std::vector<int> order;
+    Scheduler::Options o;
+    o.budget = 1000;
+    o.workers = 1;
+    o.batching = false;
+    o.shortest_first = shortest_first;
+    Scheduler sched(o, nullptr, [&](std::vector<SchedJob *> &g, std::string &) {
+        const StubJob *j = static_cast<const StubJob *>(g[0]);
+        if (j->input == 1000) {
+            while (!release.load()) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        } else {
+            std::lock_guard<std::mutex> lk(mu);
+            order.push_back((int)j->input);
+        }
+        return std::vector<int>(g.size(), 0);
+    });
+    auto blocker = std::make_unique<StubJob>();
+    blocker->input = 1000;
+    blocker->dev_bytes = 1;
+    std::vector<u64> ids{sched.submit(std::move(blocker))};
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));  // the worker holds the blocker
+    const auto t0 = std::chrono::steady_clock::now();
+    for (size_t k = 0; k < costs.size(); ++k) {
+        std::this_thread::sleep_until(t0 + std::chrono::milliseconds(delay_ms[k]));
+        auto j = std::make_unique<StubJob>();
+        j->input = k;
+        j->dev_bytes = 1;
+        j->cost_ms = costs[k];
+        ids.push_back(sched.submit(std::move(j)));
+    }
+    std::this_thread::sleep_until(t0 + std::chrono::milliseconds(hold_ms));
+    release = true;
+    for (u64 id : ids) check(sched.wait(id) != nullptr, "an ordered job did not complete");
+    return order;
+}
+
+int ordering()
+{
+    // all queued within a few milliseconds (cost plus queueing time 60, 21,
+    // 42, 3: at least 18 ms apart): shortest first
+    check(run_order({60, 20, 40, 0}, {0, 1, 2, 3}, true, 80) == std::vector<int>({3, 1, 2, 0}),
+          "queued jobs did not run shortest first");
+    check(run_order({60, 20, 40, 0}, {0, 1, 2, 3}, false, 80) == std::vector<int>({0, 1, 2, 3}),
+          "oldest-first order not kept with shortest_first off");
+    // a 100 ms job queued at 0 ms has waited 300 ms at the release (key -200);
+    // a 10 ms job queued at 250 ms has waited 50 ms (key -40): the long one first
+    check(run_order({100, 10}, {0, 250}, true, 300) == std::vector<int>({0, 1}),
+          "a long job that waited longer than its cost was passed");
+    return g_fail.load();
+}
+
 }  // namespace
 
 int main()
@@ -244,6 +305,7 @@ int main()
     if (run_mix(3, false, 5000, 0, 8, 100)) return 1;
     if (second_waiter()) return 1;
     if (bad_runner()) return 1;
+    if (ordering()) return 1;
     std::printf("sched_check ok\n");
     return 0;
 }
