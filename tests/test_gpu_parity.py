@@ -75,6 +75,31 @@ def test_fp4p_many_tiles(hip, orc, monkeypatch, sup):
         assert np.array_equal(counts, c2) and np.array_equal(idx, i2)
 
 
+def test_fp4p_other_epilogues(hip, orc, monkeypatch):
+    """The persistent 256 x 256 fp4 tiles forced for the epilogues they reach
+    only through the autotuner otherwise: transfer minima over a rectangle of
+    more tiles than workgroups (6,000 x 5,000 clade rows of 640 tips, 24 x 20
+    tiles), tipMerge pair counts (dense form, 512 sites) and the voteRoot2
+    argmax (3,000 nodes x 500 samples, 1,000 positions), against the oracle."""
+    monkeypatch.setenv("CASPHY_MFMA", "1")
+    monkeypatch.setenv("CASPHY_MFMA_VARIANT", "fp4p")
+    rng = np.random.default_rng(91)
+    ntip = 640
+    Y = synth.pack_bits(rng.random((6000, ntip)) < 0.3)
+    X = synth.pack_bits(rng.random((5000, ntip)) < rng.random((5000, 1)))
+    for g, w in zip(hip.transfer_dis(Y, X, ntip), orc.transfer(Y, X, ntip)):
+        assert np.array_equal(g, w)
+    monkeypatch.setenv("CASPHY_TIPMERGE", "dense")
+    mat = synth.binary_matrix(seed=1500, n=1500, nsite=512, p=0.01)
+    counts, idx = hip.radius_neighbor(np.ascontiguousarray(mat.T), 12)
+    for p in (0.0, 0.5, 0.8):
+        assert np.array_equal(hip.tip_merge(counts, idx, p), orc.tip_merge(counts, idx, p)), p
+    st, edge, vmat, vidx = synth.vote_inputs(3000, 3000, 500, 1000)
+    root, nsh = hip.vote_root(st, edge, vmat, vidx)
+    want_root, want_n = orc.vote_root(st, edge, vmat, vidx)
+    assert np.array_equal(nsh, want_n) and np.array_equal(root, want_root)
+
+
 def test_score_bit_past_fp32_exact_sums(hip, monkeypatch):
     """Rows of 2^24 + 1,024 sites (sums above 2^24, where fp32 sums stop being
     exact): the matrix-core tiles take the int8 form's int32 sums even when
