@@ -22,6 +22,7 @@
 // Scheduler (sched.hpp); this file holds the jobs and their device work.
 // Each job's output equals the corresponding plain entry point's.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <memory>
 #include <mutex>
@@ -124,6 +125,26 @@ struct casphy_server {
 
 namespace {
 
+// Large single joins running on each device (request-loop workers of every
+// server in the process).
+constexpr u32 LARGE_JOIN_CELLS = 8192;
+std::atomic<int> g_large_joins[64];
+
+struct LargeJoin {
+    int dev = -1, others = 0;
+    explicit LargeJoin(bool large)
+    {
+        const int d = thread_device();
+        if (!large || d < 0 || d >= 64) return;
+        dev = d;
+        others = g_large_joins[d].fetch_add(1);
+    }
+    ~LargeJoin()
+    {
+        if (dev >= 0) g_large_joins[dev].fetch_sub(1);
+    }
+};
+
 // The device work of one group (one job, or a batch of small joins of one
 // class): fills each job's `out`, returns each job's status.
 std::vector<int> run_group(std::vector<SchedJob *> &sgroup, std::string &err)
@@ -186,6 +207,11 @@ std::vector<int> run_group(std::vector<SchedJob *> &sgroup, std::string &err)
                }
                const bool bit = h->kind == CASPHY_KIND_BIT;
                if (group.size() == 1) {
+                   // a large join that starts while another runs takes at most 256
+                   // workgroups per step, so that the two share the device; alone
+                   // it takes the single-join count
+                   LargeJoin lj(h->n >= LARGE_JOIN_CELLS);
+                   set_join_workgroup_cap(lj.others > 0 ? 256 : 0);
                    if (bit) join_bit_dev(raw.as<u32>(), h->n, W, h->anc, h->out.data(), s);
                    else join_indel_dev(raw.as<int32_t>(), h->n, W, h->out.data(), s);
                    return;
@@ -250,15 +276,8 @@ int casphy_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers
         const char *hol = std::getenv("CASPHY_SERVE_HOL_MS");
         if (hol) o.hol_ms = std::atof(hol);
         o.batch_max_jobs = BATCH_MAX_JOBS;
-        // with several workers, large joins run side by side: each takes at
-        // most 256 workgroups per step (the device fits several such steps)
         s->sched = std::make_unique<Scheduler>(
-            o,
-            [device, workers] {
-                set_join_workgroup_cap(workers > 1 ? 256 : 0);
-                return casphy_set_device(device) == CASPHY_OK;
-            },
-            run_group);
+            o, [device] { return casphy_set_device(device) == CASPHY_OK; }, run_group);
         *out = s.release();
     });
 }
