@@ -391,23 +391,31 @@ def split_layer_step(cp, mat, cells, member, radius: int) -> dict:
 def indel_config(cp, synth, with_cpu: bool) -> dict:
     """The indel configuration (1,024 cells x 200 sites, seed 1): the full
     shared-mutation score matrix, one tree build, and 256 bootstrap
-    replicates (site resampling) in one persistent launch; the CPU path
-    (greedyJoinEdge2 and countApoFloat restated in the oracle) on 1 thread
-    and on all threads (replicates across threads, as R's fork cluster)."""
+    replicates (site resampling) in one persistent launch, from replicate
+    matrices built on the host and from the column indices alone (the
+    replicates built on the device); the CPU path (greedyJoinEdge2 and
+    countApoFloat restated in the oracle) on 1 thread and on all threads
+    (replicates across threads, as R's fork cluster)."""
     base = synth.indel_matrix(seed=1)
     n, ncol = base.shape
     cp.score_indel_matrix(base)
     sc_s, S = timed(lambda: cp.score_indel_matrix(base), 5)
     rng = np.random.default_rng(11)
-    reps = [base[:, rng.integers(0, ncol, ncol)] for _ in range(256)]
+    cols = rng.integers(0, ncol, (256, ncol)).astype(np.uint32)
+    reps = [np.ascontiguousarray(base[:, c]) for c in cols]
     cp.join_indel(reps[0])
     plain_s, e_plain = timed(lambda: cp.join_indel(reps[0]), 3)
     batch_s, e_batch = timed(lambda: cp.join_indel_batch(reps), 1)
+    cp.join_indel_bootstrap(base, cols[:4])
+    boot_s, e_boot = timed(lambda: cp.join_indel_bootstrap(base, cols), 3)
     assert np.array_equal(e_batch[0], e_plain[0])
+    assert all(np.array_equal(a, b) for a, b in zip(e_boot, e_batch)), "bootstrap / batch mismatch"
     out = {"cells": n, "sites": ncol,
            "score_matrix": {"ms": sc_s * 1e3, "cell_pair_scores_per_s": n * n / sc_s},
            "plain_jobs_per_s": 1 / plain_s, "batched_jobs_per_s": len(reps) / batch_s, "batched_jobs": len(reps),
-           "speedup": plain_s * len(reps) / batch_s}
+           "speedup": plain_s * len(reps) / batch_s,
+           "bootstrap_jobs_per_s": len(reps) / boot_s, "bootstrap_speedup": plain_s * len(reps) / boot_s,
+           "bootstrap_form": "the base matrix and 256 x 200 column indices in, replicates built on the device"}
     if with_cpu:
         from concurrent.futures import ThreadPoolExecutor
 

@@ -37,6 +37,7 @@ SIGNATURES = [
     ("casphy_cladewise", C.c_int, [vp, u32, vp, vp]),
     ("casphy_join_bit_batch", C.c_int, [vp, vp, u32, u32, vp, C.c_int, vp]),
     ("casphy_join_indel_batch", C.c_int, [vp, vp, u32, u32, vp, u32, vp, u32, f32, vp]),
+    ("casphy_join_indel_bootstrap", C.c_int, [vp, u32, u32, vp, u32, vp]),
     ("casphy_transfer", C.c_int, [vp, u32, vp, u32, u32, u32, vp, vp, vp, vp]),
     ("casphy_transfer_batch", C.c_int, [vp, u32, vp, vp, u32, u32, u32, vp, vp, vp, vp]),
     ("casphy_tip_merge", C.c_int, [vp, vp, u32, C.c_double, vp]),

@@ -31,7 +31,7 @@ __all__ = [
     "CasphyError", "score_bit_matrix", "score_indel_matrix",
     "radius_neighbor", "radius_neighbor_dna", "try_radius", "try_radius_dna",
     "try_radii", "try_radii_dna", "join_bit", "join_indel", "cladewise",
-    "join_bit_batch", "join_indel_batch", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
+    "join_bit_batch", "join_indel_batch", "join_indel_bootstrap", "transfer_dis", "transfer_dis_batch", "join_edge_bit_reverse", "join_edge_indel",
     "greedyJoinEdgeBitReverse", "greedyJoinEdge2", "radiusWeightedParallel",
     "radiusWeightedDNAParallel", "mfma_tuning", "tip_merge", "node_keep", "clade_bitsets", "transfer_trees",
     "vote_root", "find_root", "findRootAVX2Big", "fasta_encode", "fasta2bin", "read_newick", "tbe_support",
@@ -219,6 +219,20 @@ def join_indel_batch(mats, weight=None, tip_sizes=None, min_size: int = 0, min_r
         out.append(edges[pos:pos + 2 * s - 2])
         pos += 2 * s - 2
     return out
+
+
+def join_indel_bootstrap(mat_cellmajor, cols):
+    """Column-resample replicates of one indel matrix joined in one batched
+    launch: replicate r is mat[:, cols[r]] (cols: nrep x ncol indices), built
+    on the device; returns one raw edge array per replicate, each equal to
+    join_indel(mat[:, cols[r]])[0] with unit weights."""
+    m = _i32(mat_cellmajor)
+    n, ncol = m.shape
+    c = _u32(np.asarray(cols).reshape(-1, ncol))
+    nrep = c.shape[0]
+    edges = np.empty((nrep * (2 * n - 2), 2), dtype=np.uint32)
+    check(load().casphy_join_indel_bootstrap(_p(m), n, ncol, _p(c), nrep, _p(edges)))
+    return [edges[r * (2 * n - 2):(r + 1) * (2 * n - 2)] for r in range(nrep)]
 
 
 def join_edge_bit_reverse(tbmat, weight=None, ancestral_type: int = 1):

@@ -1185,6 +1185,18 @@ __global__ void k_indel_codes(const int32_t *mat, u32 n, u32 ncol, u32 *out, u32
     }
 }
 
+// Replicate r of a column resample: rows r * n .. r * n + n - 1 of out are the
+// n rows of mat at the columns cols[r * ncol ..] (a bootstrap replicate).
+__global__ void k_gather_cols(const int32_t *mat, u32 n, u32 ncol, const u32 *cols, u32 nrep, int32_t *out)
+{
+    const u64 total = (u64)nrep * n * ncol;
+    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
+        const u64 row = x / ncol;
+        const u32 c = (u32)(x - row * ncol), r = (u32)(row / n), i = (u32)(row - (u64)r * n);
+        out[x] = mat[(size_t)i * ncol + cols[(size_t)r * ncol + c]];
+    }
+}
+
 // Byte codes of `count` rows back to states (-1 missing).
 __global__ void k_codes_decode(const u32 *codes, u32 count, u32 cw, u32 ncol, int32_t *out)
 {
@@ -1515,6 +1527,14 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
     // lanes per pair: the fewest whose 16-byte loads of a row are at most four a lane
     const u32 lps = ldw <= 64 ? 4u : ldw <= 128 ? 8u : ldw <= 256 ? 16u : ldw <= 512 ? 32u : 64u;
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), 0, s, b); };
+    const char *st_env = std::getenv("CASPHY_JOIN_STATS");
+    const bool stats = st_env && st_env[0] == '1';
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (stats) {
+        CASPHY_HIP(hipEventCreate(&ev[0]));
+        CASPHY_HIP(hipEventCreate(&ev[1]));
+        CASPHY_HIP(hipEventRecord(ev[0], s));
+    }
     if (codes) {
         switch (lps) {
         case 4: go(k_join_batch<FMT_CODES, 4>); break;
@@ -1533,6 +1553,16 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
         }
     }
     CASPHY_HIP(hipGetLastError());
+    if (stats) {
+        CASPHY_HIP(hipEventRecord(ev[1], s));
+        CASPHY_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0;
+        CASPHY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        std::fprintf(stderr, "join stats: batch of %u joins, %u workgroups of 1024, %u lanes per pair%s: %.3f ms\n",
+                     njobs, grid, lps, codes ? ", byte codes" : "", ms);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
     std::vector<u32> st(njobs);
     d2h(st.data(), status.p, (size_t)njobs * 4, s);
     for (u32 k = 0; k < njobs; ++k)
@@ -1541,6 +1571,7 @@ void batch_run(DevBuf &rows, const std::vector<u32> &offs, u32 ldw, int rule, co
 }
 
 void require(bool c, const char *msg) { CASPHY_REQUIRE(c, CASPHY_EARG, msg); }
+
 
 }  // namespace
 
@@ -1580,6 +1611,39 @@ bool unit_weights(const double *weight, u32 nstate, u32 ncol)
     for (u64 i = 0; i < (u64)nstate * ncol; ++i)
         if (weight[i] != 1.0) return false;
     return true;
+}
+
+// Batched indel joins over jobs whose rows are already on the device (dmat:
+// offs.back() x ncol allele codes): byte codes when the weights are unit and
+// every site has at most 254 alleles, else one-hot planes.
+static void indel_batch_dev(DevBuf &dmat, const std::vector<u32> &offs, u32 ncol, const double *weight, u32 nstate,
+                     const u32 *tip_size, u32 min_size, float min_ratio, u32 *edge, hipStream_t s)
+{
+    const u32 total = offs.back();
+    // one layout for every job: per-site allele groups sized by the maximum over all jobs
+    IndelLayout L = indel_layout(dmat.as<int32_t>(), total, ncol, s);
+    Planes planes = indel_planes(L, weight, nstate, ncol);
+    DevBuf dts;
+    if (tip_size) {
+        dts.alloc((size_t)total * 4);
+        h2d(dts.p, tip_size, (size_t)total * 4, s);
+    }
+    if (unit_weights(weight, nstate, ncol) && codes_fit(L) && !no_codes_env()) {  // byte codes
+        const u32 cw = round_up(ceil_div(ncol, 4), 4u);
+        DevBuf crows((size_t)total * cw * 4);
+        hipLaunchKernelGGL(k_indel_codes, dim3(grid_for((u64)total * cw)), dim3(256), 0, s, dmat.as<int32_t>(), total,
+                           ncol, crows.as<u32>(), cw);
+        CASPHY_HIP(hipGetLastError());
+        batch_run(crows, offs, cw, 2, Planes{}, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge, s,
+                  true);
+        return;
+    }
+    DevBuf rows((size_t)total * 2 * L.ldw * 4);
+    launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
+    hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, dmat.as<int32_t>(), total,
+                       ncol, L.dloci.as<u32>(), L.dwidth.as<u32>(), rows.as<u32>(), L.ldw);
+    CASPHY_HIP(hipGetLastError());
+    batch_run(rows, offs, L.ldw, 2, planes, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge, s);
 }
 
 void join_indel_dev(const int32_t *raw, u32 n, u32 ncol, u32 *edge_host, hipStream_t s)
@@ -1793,30 +1857,30 @@ int casphy_join_indel_batch(const int32_t *mat, const uint32_t *offsets, uint32_
         const u32 total = offs.back();
         DevBuf dmat((size_t)total * ncol * 4);
         h2d(dmat.p, mat, (size_t)total * ncol * 4, s);
-        // one layout for every job: per-site allele groups sized by the maximum over all jobs
-        IndelLayout L = indel_layout(dmat.as<int32_t>(), total, ncol, s);
-        Planes planes = indel_planes(L, weight, nstate, ncol);
-        DevBuf dts;
-        if (tip_size) {
-            dts.alloc((size_t)total * 4);
-            h2d(dts.p, tip_size, (size_t)total * 4, s);
-        }
-        if (unit_weights(weight, nstate, ncol) && codes_fit(L) && !no_codes_env()) {  // byte codes
-            const u32 cw = round_up(ceil_div(ncol, 4), 4u);
-            DevBuf crows((size_t)total * cw * 4);
-            hipLaunchKernelGGL(k_indel_codes, dim3(grid_for((u64)total * cw)), dim3(256), 0, s, dmat.as<int32_t>(),
-                               total, ncol, crows.as<u32>(), cw);
-            CASPHY_HIP(hipGetLastError());
-            batch_run(crows, offs, cw, 2, Planes{}, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge,
-                      s, true);
-            return;
-        }
-        DevBuf rows((size_t)total * 2 * L.ldw * 4);
-        launch_fill_u32(rows.as<u32>(), (u64)total * 2 * L.ldw, 0, s);
-        hipLaunchKernelGGL(k_indel_encode, dim3(grid_for((u64)total * ncol)), dim3(256), 0, s, dmat.as<int32_t>(),
-                           total, ncol, L.dloci.as<u32>(), L.dwidth.as<u32>(), rows.as<u32>(), L.ldw);
+        indel_batch_dev(dmat, offs, ncol, weight, nstate, tip_size, min_size, min_ratio, edge, s);
+    });
+}
+
+int casphy_join_indel_bootstrap(const int32_t *mat_cellmajor, uint32_t n, uint32_t ncol, const uint32_t *cols,
+                                uint32_t nrep, uint32_t *edge)
+{
+    return guarded([&] {
+        require(mat_cellmajor && cols && edge && n >= 2 && ncol > 0, "casphy_join_indel_bootstrap: bad arguments");
+        if (nrep == 0) return;
+        CASPHY_REQUIRE((u64)nrep * n < (1ull << 31), CASPHY_EARG, "casphy_join_indel_bootstrap: too many rows");
+        for (u64 k = 0; k < (u64)nrep * ncol; ++k)
+            CASPHY_REQUIRE(cols[k] < ncol, CASPHY_EARG,
+                           "casphy_join_indel_bootstrap: column index " + std::to_string(cols[k]) + " >= ncol");
+        hipStream_t s = thread_stream();
+        DevBuf base((size_t)n * ncol * 4), dcols((size_t)nrep * ncol * 4), dmat((size_t)nrep * n * ncol * 4);
+        h2d(base.p, mat_cellmajor, (size_t)n * ncol * 4, s);
+        h2d(dcols.p, cols, (size_t)nrep * ncol * 4, s);
+        hipLaunchKernelGGL(k_gather_cols, dim3(grid_for((u64)nrep * n * ncol)), dim3(256), 0, s, base.as<int32_t>(), n,
+                           ncol, dcols.as<u32>(), nrep, dmat.as<int32_t>());
         CASPHY_HIP(hipGetLastError());
-        batch_run(rows, offs, L.ldw, 2, planes, tip_size ? dts.as<u32>() : nullptr, min_size, min_ratio, edge, s);
+        std::vector<u32> offs(nrep + 1);
+        for (u32 r = 0; r <= nrep; ++r) offs[r] = r * n;
+        indel_batch_dev(dmat, offs, ncol, nullptr, 0, nullptr, 0, 2.0f, edge, s);
     });
 }
 

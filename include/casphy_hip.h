@@ -181,6 +181,16 @@ int casphy_join_indel_batch(const int32_t *mat_cellmajor, const uint32_t *offset
                             uint32_t ncol, const double *weight, uint32_t nstate,
                             const uint32_t *tip_size, uint32_t min_size, float min_ratio,
                             uint32_t *edge);
+/* Bootstrap replicates of one indel matrix as one batched launch: replicate r
+ * is the n rows of mat_cellmajor (n x ncol allele codes, -1 missing) at the
+ * columns cols[r*ncol .. r*ncol + ncol - 1] (every index below ncol); unit
+ * weights.  The replicates are built on the device from the matrix and the
+ * indices, so only those travel.  edge: nrep x (2n-2) x 2, replicate after
+ * replicate, each equal to casphy_join_indel on that replicate's matrix.
+ * No counterpart in the reference (a caller resamples the columns and runs
+ * one join per replicate); the batched form of casphy_join_indel_batch. */
+int casphy_join_indel_bootstrap(const int32_t *mat_cellmajor, uint32_t n, uint32_t ncol, const uint32_t *cols,
+                                uint32_t nrep, uint32_t *edge);
 
 /* ------------------------------------------------------------------ transfer */
 /* Transfer distance between bipartition bitsets (row-major [nrow][ncol] u32,

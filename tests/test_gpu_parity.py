@@ -528,6 +528,28 @@ def test_join_indel_batch_bootstrap(hip, orc, monkeypatch, rows):
         assert np.array_equal(e, e2)
 
 
+def test_join_indel_bootstrap(hip, orc):
+    """Column-resample replicates built on the device equal the batched joins
+    of the same replicates built on the host, and the oracle's joins of two of
+    them: byte codes (the simulator's 200 sites) and one-hot planes (a site of
+    300 alleles), replicates of 256 and 37 cells."""
+    rng = np.random.default_rng(44)
+    for base in (synth.indel_matrix(seed=5, depth=8), synth.indel_matrix(seed=6, depth=8)[:37, :60].copy()):
+        if base.shape[0] == 37:
+            base[:, 7] = rng.integers(-1, 301, base.shape[0])  # more alleles than a byte code holds
+        n, ncol = base.shape
+        cols = rng.integers(0, ncol, (5, ncol)).astype(np.uint32)
+        cols[0] = np.arange(ncol)
+        got = hip.join_indel_bootstrap(base, cols)
+        reps = [np.ascontiguousarray(base[:, c]) for c in cols]
+        want = hip.join_indel_batch(reps)
+        assert all(np.array_equal(g, w) for g, w in zip(got, want))
+        for r in (0, 3):
+            w = np.ones((max(1, int(reps[r].max())), ncol), np.float32)
+            e2, _, _ = orc.join_indel(reps[r], w)
+            assert np.array_equal(got[r], e2)
+
+
 # ------------------------------------------------------------------ transfer
 @pytest.mark.parametrize("ntip,ny,nx", [(64, 62, 62), (75, 20, 300), (1000, 130, 129), (32, 1, 1)])
 def test_transfer(hip, orc, bitpath, ntip, ny, nx):
