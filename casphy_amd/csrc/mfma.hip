@@ -23,6 +23,7 @@
 // hardware orders k inside an instruction.
 #include "mfma.hpp"
 
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -1843,12 +1844,16 @@ void launch_fp4(int epi, const MfmaArgs &in, hipStream_t s, int mode)
     if ((mode == 4 || mode == 5) && (stages % 4 != 0 || stages < 4))
         throw Error(CASPHY_EINTERNAL, "MFMA fp4 tiles: the pipelined tiles need a multiple of 4 stages");
     if (mode == 6) {
-        // one workgroup per CU (the XCD mapping wants a multiple of 8)
-        static int cus = 0;
+        // one workgroup per CU (the XCD mapping wants a multiple of 8); the CU
+        // count of each device is looked up once (any thread may race to it,
+        // all store the same value)
+        static std::atomic<int> cu_count[64];
+        int dev = 0;
+        CASPHY_HIP(hipGetDevice(&dev));
+        int cus = dev >= 0 && dev < 64 ? cu_count[dev].load(std::memory_order_relaxed) : 0;
         if (!cus) {
-            int dev = 0;
-            CASPHY_HIP(hipGetDevice(&dev));
             CASPHY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            if (dev >= 0 && dev < 64) cu_count[dev].store(cus, std::memory_order_relaxed);
         }
         const u32 per_xcd = std::max(1u, (u32)cus / 8), per = (a.ntiles + 7) / 8;
         const u32 pgrid = 8 * std::min(per_xcd, per);
