@@ -51,7 +51,7 @@ VALU_SPEC_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz
 MFMA_I8_PEAK_TOPS = 5000.0                # MI355X dense int8 matrix-core peak, 5.0 POP/s
 MFMA_FP4_PEAK_TOPS = 10100.0              # MI355X dense fp4 matrix-core peak, 10.1 PFLOP/s
 HBM_PEAK_GBS = 8000.0
-CPU_THREADS_MAX = 16                      # the GPU box's CPU share
+CPU_THREADS_MAX = 16                      # host threads of the CPU baselines, at most
 
 
 @functools.lru_cache(maxsize=None)
@@ -973,7 +973,7 @@ def main():
                     help="run only one of the other configurations (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     import casphy_amd as cp
