@@ -387,13 +387,17 @@ __global__ void k_bit_slots(const u32 *mat, u32 n, u32 W, u32 *rows, u32 ldw)
 }
 
 // Indel: per-site maximum allele (states -1 / 0 / k).
+// Per-site maximum allele: a thread owns one column over the rows y, y +
+// gridDim.y, ... (coalesced across the threads of a row) and issues one
+// atomicMax at the end (one per site and workgroup row, not one per entry:
+// with a few hundred sites the entries' atomics all met on those addresses).
 __global__ void k_site_max(const int32_t *mat, u32 n, u32 ncol, int32_t *kmax)
 {
-    const u64 total = (u64)n * ncol;
-    for (u64 x = blockIdx.x * (u64)blockDim.x + threadIdx.x; x < total; x += (u64)gridDim.x * blockDim.x) {
-        const int32_t v = mat[x];
-        if (v > 0) atomicMax(&kmax[x % ncol], v);
-    }
+    const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    int32_t m = 0;
+    for (u32 r = blockIdx.y; r < n; r += gridDim.y) m = max(m, mat[(size_t)r * ncol + c]);
+    if (m > 0) atomicMax(&kmax[c], m);
 }
 
 // Indel: one-hot encode into the two planes (rows pre-zeroed).
@@ -1123,7 +1127,7 @@ IndelLayout indel_layout(const int32_t *dmat, u32 n, u32 ncol, hipStream_t s)
     IndelLayout L;
     DevBuf dk((size_t)ncol * 4);
     launch_fill_u32(dk.as<u32>(), ncol, 0, s);
-    hipLaunchKernelGGL(k_site_max, dim3(grid_for((u64)n * ncol)), dim3(256), 0, s, dmat, n, ncol,
+    hipLaunchKernelGGL(k_site_max, dim3(ceil_div(ncol, 256u), std::max<u32>(1, std::min<u32>(n, 1024))), dim3(256), 0, s, dmat, n, ncol,
                        dk.as<int32_t>());
     CASPHY_HIP(hipGetLastError());
     L.kmax.resize(ncol);
